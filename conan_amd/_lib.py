@@ -1,0 +1,117 @@
+"""ctypes binding of libconanpm.so (the C ABI declared in include/conanpm.h).
+
+The library is loaded from this package directory only.  There is no
+fallback: when the shared object is missing or no GPU is present the calls
+raise, so a test or benchmark can never silently run on something else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libconanpm.so")
+
+PM_FP64 = 0
+PM_FP32 = 1
+PM_FLAG_WRAP_GATHER = 1
+PM_NPHASES = 9
+PHASE_NAMES = ["keys+sort", "deposit", "fft rows fwd", "fft y fwd", "green (z fwd*G*bwd)",
+               "fft y bwd", "fft rows bwd", "gather+kick(+drift)", "drift"]
+
+# Every exported entry point of include/conanpm.h.
+EXPORTS = [
+    "pm_last_error", "pm_create", "pm_destroy", "pm_init_zeldovich", "pm_drift", "pm_kick",
+    "pm_steps", "pm_get_state", "pm_set_state", "pm_get_times", "pm_deposit", "pm_potential",
+    "pm_forces", "pm_gather_mesh", "pm_poisson_mesh", "pm_power_spectrum", "pm_diagnostics", "pm_profile", "pm_sync",
+    "pm_ic_scale_free", "pm_ic_potential", "pm_ic_displacement", "pm_fft_r2c", "pm_fft_c2r",
+    "pm_cic",
+]
+
+
+class PMError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class pm_config(C.Structure):
+    _fields_ = [("dim", C.c_int), ("Nm", C.c_int), ("Nf", C.c_int), ("L", C.c_double),
+                ("H0", C.c_double), ("Omega_m", C.c_double), ("Omega_L", C.c_double),
+                ("precision", C.c_int), ("flags", C.c_uint)]
+
+
+class pm_ic_params(C.Structure):
+    _fields_ = [("dim", C.c_int), ("mbits", C.c_int), ("L", C.c_double), ("ns", C.c_double),
+                ("smooth", C.c_int), ("scale", C.c_double), ("sigma", C.c_double)]
+
+
+class pm_diag(C.Structure):
+    _fields_ = [("mass", C.c_double), ("momentum", C.c_double * 3), ("p2", C.c_double)]
+
+
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run `python -m conan_amd.build` first")
+    lib = C.CDLL(LIB_PATH)
+    V = C.c_void_p
+    sig = {
+        "pm_last_error": (C.c_char_p, [V]),
+        "pm_create": (C.c_int, [C.POINTER(pm_config), C.POINTER(V)]),
+        "pm_destroy": (None, [V]),
+        "pm_init_zeldovich": (C.c_int, [V, _dp, C.c_double, C.c_double]),
+        "pm_drift": (C.c_int, [V, C.c_double, C.c_double]),
+        "pm_kick": (C.c_int, [V, C.c_double, C.c_double]),
+        "pm_steps": (C.c_int, [V, C.c_double, C.c_double, C.c_int, _dp]),
+        "pm_get_state": (C.c_int, [V, _dp, _dp]),
+        "pm_set_state": (C.c_int, [V, _dp, _dp]),
+        "pm_get_times": (C.c_int, [V, _dp, _dp]),
+        "pm_deposit": (C.c_int, [V, _dp, C.POINTER(C.c_int32)]),
+        "pm_potential": (C.c_int, [V, C.c_double, _dp]),
+        "pm_forces": (C.c_int, [V, C.c_double, _dp]),
+        "pm_gather_mesh": (C.c_int, [V, _dp, _dp]),
+        "pm_poisson_mesh": (C.c_int, [V, _dp, C.c_double, _dp]),
+        "pm_power_spectrum": (C.c_int, [V, C.c_int, _dp, _dp, _dp, _dp]),
+        "pm_diagnostics": (C.c_int, [V, C.POINTER(pm_diag)]),
+        "pm_profile": (C.c_int, [V, C.c_int, _dp, C.POINTER(C.c_longlong)]),
+        "pm_sync": (C.c_int, [V]),
+        "pm_ic_scale_free": (C.c_int, [C.POINTER(pm_ic_params), _dp, _dp, _dp]),
+        "pm_ic_potential": (C.c_int, [C.c_int, C.c_int, C.c_double, _dp]),
+        "pm_ic_displacement": (C.c_int, [C.c_int, C.c_int, C.c_double, _dp, _dp]),
+        "pm_fft_r2c": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp]),
+        "pm_fft_c2r": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp]),
+        "pm_cic": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_dp)
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(C.POINTER(C.c_int32))
+    raise TypeError(a.dtype)
+
+
+def check(rc: int, h=None) -> None:
+    if rc != 0:
+        msg = load().pm_last_error(h)
+        raise PMError(rc, msg.decode() if msg else "unknown error")

@@ -1,0 +1,918 @@
+// pm_api.hip -- the C ABI of include/conanpm.h: a PM solver handle that owns
+// the device state of one run, and the stand-alone ic / cic / FFT operations.
+//
+// The handle is the MI355X counterpart of Gravity<R> (src/run/nbody.hh:53-219):
+// it keeps positions and momenta resident in HBM across steps, and one kick is
+//   keys -> stable (key, id) sort -> row bounds -> deposit (rho)
+//   -> rows r2c -> [y fwd] -> last axis fwd * Green * bwd -> [y bwd] -> rows c2r (phi)
+//   -> gather + kick (+ the next step's drift).
+// Scalars that the reference computes per call (adot, G, the drift and kick
+// factors, the Poisson scale) are computed here on the host with the
+// reference's expressions, so both sides see the same doubles.
+#include "../../include/conanpm.h"
+#include "pm_fft_api.hpp"
+#include "pm_particles.hpp"
+#include "pm_sort.hpp"
+#include "pm_spectral.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace cpm {
+
+// src/run/cosmology.hh:6-28
+struct Cosmo {
+    double H0, Om, OL, Ok, G;
+    Cosmo(double H0_, double m_, double L_) : H0(H0_), Om(m_), OL(L_), Ok(1 - m_ - L_), G(3. / 2 * m_ * H0_ * H0_) {}
+    double adot(double a) const { return H0 * a * sqrt(OL + Om * pow(a, -3) + Ok * pow(a, -2)); }
+};
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n_) { alloc(n_); }
+    void alloc(size_t n_) {
+        release();
+        n = n_;
+        if (n) CPM_HIP(hipMalloc(&p, sizeof(T) * n));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+};
+
+static void check_finite(const double* a, size_t n, const char* what) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(a[i])) throw Error(PM_ENONFINITE, std::string("non-finite value in ") + what);
+}
+
+static int ilog2_exact(long n, const char* what) {
+    if (n < 1 || (n & (n - 1))) throw Error(PM_EUNSUP, std::string(what) + " must be a power of two");
+    return ilog2(n);
+}
+
+struct Prof {
+    bool on = false;
+    hipEvent_t b[PM_NPHASES] = {}, e[PM_NPHASES] = {};
+    double ms[PM_NPHASES] = {};
+    long long cnt[PM_NPHASES] = {};
+    void enable(bool v) {
+        if (v && !b[0])
+            for (int i = 0; i < PM_NPHASES; ++i) {
+                CPM_HIP(hipEventCreate(&b[i]));
+                CPM_HIP(hipEventCreate(&e[i]));
+            }
+        if (v)
+            for (int i = 0; i < PM_NPHASES; ++i) {
+                ms[i] = 0;
+                cnt[i] = 0;
+            }
+        on = v;
+    }
+    void begin(int ph, hipStream_t st) {
+        if (on) CPM_HIP(hipEventRecord(b[ph], st));
+    }
+    void end(int ph, hipStream_t st) {
+        if (!on) return;
+        CPM_HIP(hipEventRecord(e[ph], st));
+        CPM_HIP(hipEventSynchronize(e[ph]));
+        float t = 0;
+        CPM_HIP(hipEventElapsedTime(&t, b[ph], e[ph]));
+        ms[ph] += t;
+        ++cnt[ph];
+    }
+    ~Prof() {
+        for (int i = 0; i < PM_NPHASES; ++i) {
+            if (b[i]) (void)hipEventDestroy(b[i]);
+            if (e[i]) (void)hipEventDestroy(e[i]);
+        }
+    }
+};
+
+struct ImplBase {
+    virtual ~ImplBase() = default;
+    virtual void init_zeldovich(const double* phi, double a_pos, double a_mom) = 0;
+    virtual void drift(double a, double da) = 0;
+    virtual void kick(double a, double da) = 0;
+    virtual double steps(double a, double da, int n) = 0;
+    virtual void get_state(double* X, double* P) = 0;
+    virtual void set_state(const double* X, const double* P) = 0;
+    virtual void set_positions_device(const double* Xsoa) = 0;
+    virtual void deposit(double* rho, int32_t* keys, double mass_override) = 0;
+    virtual void potential(double a, double* phi) = 0;
+    virtual void forces(double a, double* F) = 0;
+    virtual void poisson_mesh(const double* rho, double a, double* phi) = 0;
+    virtual void gather_mesh(const double* phi, double* F) = 0;
+    virtual void power_spectrum(int nbins, const double* edges, double* k, double* pk, double* cnt) = 0;
+    virtual void diagnostics(pm_diag* d) = 0;
+    double last_apos = 0, last_avel = 0;
+    bool have_state = false;
+    Prof prof;
+    hipStream_t st = nullptr;
+};
+
+template <typename PP>
+struct Impl : ImplBase {
+    using Pos = typename PP::Pos;
+    using Mesh = typename PP::Mesh;
+    static constexpr bool FX = sizeof(Mesh) == 4;
+
+    pm_config cfg;
+    int R, lgNf;
+    long Np, M, nrows;
+    Cosmo cos;
+    double mass;
+    PosArgs pa;
+    FFTPlan<Mesh> fft;
+    RadixSorter sorter;
+    DevBuf<Pos> X;
+    DevBuf<double> P;
+    DevBuf<uint32_t> k0, v0, k1, v1, row_start;
+    DevBuf<Mesh> mesh;
+    DevBuf<cx<Mesh>> spec, nyq;
+    DevBuf<double> row_partial, red, scal;
+    const uint32_t* skeys = nullptr;
+    const uint32_t* spids = nullptr;
+    bool have_deposit = false;
+
+    static long ipow(long n, int r) {
+        long s = 1;
+        for (int i = 0; i < r; ++i) s *= n;
+        return s;
+    }
+
+    explicit Impl(const pm_config& c)
+        : cfg(c), R(c.dim), lgNf(ilog2_exact(c.Nf, "Nf")), Np(ipow(c.Nm, c.dim)), M(ipow(c.Nf, c.dim)),
+          nrows(M / c.Nf), cos(c.H0, c.Omega_m, c.Omega_L), fft(c.dim, c.Nf), sorter(Np, c.dim * lgNf) {
+        mass = pow(double(c.Nf) / c.Nm, R);  // src/run/nbody.hh:85
+        pa.L = c.L;
+        pa.res = c.L / c.Nf;
+        pa.wrap = (c.flags & PM_FLAG_WRAP_GATHER) ? 1 : 0;
+        pa.fshift = 32 - lgNf;
+        pa.fmask = (uint32_t)((1ull << pa.fshift) - 1);
+        pa.fscale = (float)ldexp(1.0, -pa.fshift);
+        CPM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        X.alloc((size_t)R * Np);
+        P.alloc((size_t)R * Np);
+        k0.alloc(Np);
+        v0.alloc(Np);
+        k1.alloc(Np);
+        v1.alloc(Np);
+        row_start.alloc(nrows + 1);
+        mesh.alloc(M);
+        spec.alloc(fft.spec_elems);
+        nyq.alloc(fft.nyq_elems);
+        row_partial.alloc(nrows);
+        red.alloc(4096);
+        scal.alloc(16);
+    }
+    ~Impl() override {
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
+    }
+
+    void need_state() const {
+        if (!have_state) throw Error(PM_ESTATE, "no particle state: call pm_init_zeldovich or pm_set_state first");
+    }
+    double drift_factor(double a, double da) const {
+        double adot = cos.adot(a);
+        double c = da / (a * a * adot);  // src/run/nbody.hh:124,129
+        return FX ? c * (4294967296.0 / cfg.L) : c;
+    }
+
+    void init_zeldovich(const double* phi, double a_pos, double a_mom) override {
+        const long npm = Np;
+        if (cfg.Nm < 4 || (cfg.Nm & (cfg.Nm - 1))) throw Error(PM_EUNSUP, "Zel'dovich start needs a power-of-two Nm >= 4");
+        check_finite(phi, npm, "potential");
+        DevBuf<double> d(npm);
+        CPM_HIP(hipMemcpyAsync(d.p, phi, sizeof(double) * npm, hipMemcpyHostToDevice, st));
+        ParticleOps<PP>::zeldovich(R, d.p, ilog2(cfg.Nm), cfg.L, a_pos, a_mom, X.p, P.p, st);
+        CPM_HIP(hipStreamSynchronize(st));
+        last_apos = a_pos;
+        last_avel = a_mom;
+        have_state = true;
+        have_deposit = false;
+    }
+
+    void do_deposit(Mesh m) {
+        prof.begin(0, st);
+        ParticleOps<PP>::keys(R, X.p, Np, lgNf, pa, k0.p, v0.p, st);
+        const int where = sorter.sort(k0.p, v0.p, k1.p, v1.p, st);
+        skeys = where ? k1.p : k0.p;
+        spids = where ? v1.p : v0.p;
+        row_bounds(skeys, Np, lgNf, nrows, row_start.p, st);
+        prof.end(0, st);
+        prof.begin(1, st);
+        ParticleOps<PP>::deposit(R, row_start.p, skeys, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p, nrows,
+                                 st);
+        prof.end(1, st);
+        have_deposit = true;
+    }
+
+    // mesh (rho) -> mesh (phi) at scale factor a.
+    // FX: scal[0] must hold the FP64 mean of the mesh (mean_ready), or the
+    // row partials of the last deposit provide it.
+    void do_poisson(double a, bool mean_ready = false) {
+        const double s = fft.M / cos.G * a;  // real_part(size / G * a), src/run/nbody.hh:110
+        const double kf = 2 * M_PI / cfg.L;
+        const double* mean_ptr = nullptr;
+        Mesh shift = (Mesh)1.0;  // delta = rho - 1 (src/run/nbody.hh:103-104)
+        if (FX) {
+            // FP64 mean of the FP32 deposit, subtracted on the device
+            if (!mean_ready) sum_partials(row_partial.p, nrows, 1.0 / (double)M, scal.p, st);
+            mean_ptr = scal.p;
+        }
+        for (int stage = 0; stage < 5; ++stage) {
+            prof.begin(2 + stage, st);
+            fft.poisson_stage(stage, mesh.p, mesh.p, spec.p, nyq.p, shift, mean_ptr, kf, s, st);
+            prof.end(2 + stage, st);
+        }
+    }
+
+    void drift(double a, double da) override {
+        need_state();
+        prof.begin(8, st);
+        ParticleOps<PP>::drift(X.p, P.p, (long)R * Np, drift_factor(a, da), st);
+        prof.end(8, st);
+        CPM_HIP(hipStreamSynchronize(st));
+        last_apos = a;
+    }
+
+    void kick_impl(double a, double da, bool fuse, double a_next, double da_next) {
+        do_deposit((Mesh)mass);
+        do_poisson(a);
+        const double adot = cos.adot(a);
+        const double ck = da / adot;  // src/run/nbody.hh:112-116
+        const double cd = fuse ? drift_factor(a_next, da_next) : 0.0;
+        prof.begin(7, st);
+        ParticleOps<PP>::gather_kick(R, X.p, P.p, Np, mesh.p, lgNf, pa, ck, cd, fuse ? 1 : 0, nullptr, st);
+        prof.end(7, st);
+        last_avel = a;
+        if (fuse) last_apos = a_next;
+    }
+
+    void kick(double a, double da) override {
+        need_state();
+        kick_impl(a, da, false, 0, 0);
+        CPM_HIP(hipStreamSynchronize(st));
+    }
+
+    // LeapFrog::run's loop body n times (src/run/leapfrog.hh:38-44).
+    double steps(double a, double da, int n) override {
+        need_state();
+        if (n <= 0) return a;
+        prof.begin(8, st);
+        ParticleOps<PP>::drift(X.p, P.p, (long)R * Np, drift_factor(a, da), st);
+        prof.end(8, st);
+        last_apos = a;
+        for (int i = 0; i < n; ++i) {
+            const double ak = a + da / 2;
+            const bool more = i + 1 < n;
+            const double anext = a + da;
+            kick_impl(ak, da, more, anext, da);
+            a += da;
+        }
+        CPM_HIP(hipStreamSynchronize(st));
+        return a;
+    }
+
+    void get_state(double* Xh, double* Ph) override {
+        need_state();
+        std::vector<double> tmp((size_t)R * Np);
+        if (Xh) {
+            if (FX) {
+                DevBuf<double> d((size_t)R * Np);
+                fx_to_f64(reinterpret_cast<const uint32_t*>(X.p), (long)R * Np, cfg.L, d.p, st);
+                CPM_HIP(hipMemcpyAsync(tmp.data(), d.p, sizeof(double) * R * Np, hipMemcpyDeviceToHost, st));
+                CPM_HIP(hipStreamSynchronize(st));
+            } else {
+                CPM_HIP(hipMemcpyAsync(tmp.data(), X.p, sizeof(double) * R * Np, hipMemcpyDeviceToHost, st));
+                CPM_HIP(hipStreamSynchronize(st));
+            }
+            for (long i = 0; i < Np; ++i)
+                for (int k = 0; k < R; ++k) Xh[i * R + k] = tmp[(size_t)k * Np + i];
+        }
+        if (Ph) {
+            CPM_HIP(hipMemcpyAsync(tmp.data(), P.p, sizeof(double) * R * Np, hipMemcpyDeviceToHost, st));
+            CPM_HIP(hipStreamSynchronize(st));
+            for (long i = 0; i < Np; ++i)
+                for (int k = 0; k < R; ++k) Ph[i * R + k] = tmp[(size_t)k * Np + i];
+        }
+    }
+
+    void upload_positions_soa(const double* soa_host_or_dev, bool device) {
+        if (FX) {
+            DevBuf<double> d;
+            const double* src = soa_host_or_dev;
+            if (!device) {
+                d.alloc((size_t)R * Np);
+                CPM_HIP(hipMemcpyAsync(d.p, soa_host_or_dev, sizeof(double) * R * Np, hipMemcpyHostToDevice, st));
+                src = d.p;
+            }
+            f64_to_fx(src, (long)R * Np, cfg.L, reinterpret_cast<uint32_t*>(X.p), st);
+            CPM_HIP(hipStreamSynchronize(st));
+        } else {
+            CPM_HIP(hipMemcpyAsync(X.p, soa_host_or_dev, sizeof(double) * R * Np,
+                                   device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+            CPM_HIP(hipStreamSynchronize(st));
+        }
+    }
+
+    void set_state(const double* Xh, const double* Ph) override {
+        std::vector<double> tmp((size_t)R * Np);
+        if (Xh) {
+            check_finite(Xh, (size_t)R * Np, "positions");
+            for (long i = 0; i < Np; ++i)
+                for (int k = 0; k < R; ++k) tmp[(size_t)k * Np + i] = Xh[i * R + k];
+            upload_positions_soa(tmp.data(), false);
+        }
+        if (Ph) {
+            check_finite(Ph, (size_t)R * Np, "momenta");
+            for (long i = 0; i < Np; ++i)
+                for (int k = 0; k < R; ++k) tmp[(size_t)k * Np + i] = Ph[i * R + k];
+            CPM_HIP(hipMemcpyAsync(P.p, tmp.data(), sizeof(double) * R * Np, hipMemcpyHostToDevice, st));
+            CPM_HIP(hipStreamSynchronize(st));
+        } else if (!have_state) {
+            CPM_HIP(hipMemsetAsync(P.p, 0, sizeof(double) * R * Np, st));
+            CPM_HIP(hipStreamSynchronize(st));
+        }
+        if (Xh) have_state = true;
+        have_deposit = false;
+    }
+
+    void set_positions_device(const double* Xsoa) override {
+        upload_positions_soa(Xsoa, true);
+        CPM_HIP(hipMemsetAsync(P.p, 0, sizeof(double) * R * Np, st));
+        CPM_HIP(hipStreamSynchronize(st));
+        have_state = true;
+    }
+
+    void mesh_to_host(double* out) {
+        if (FX) {
+            std::vector<float> t(M);
+            CPM_HIP(hipMemcpyAsync(t.data(), mesh.p, sizeof(float) * M, hipMemcpyDeviceToHost, st));
+            CPM_HIP(hipStreamSynchronize(st));
+            for (long i = 0; i < M; ++i) out[i] = t[i];
+        } else {
+            CPM_HIP(hipMemcpyAsync(out, mesh.p, sizeof(double) * M, hipMemcpyDeviceToHost, st));
+            CPM_HIP(hipStreamSynchronize(st));
+        }
+    }
+    void mesh_from_host(const double* in) {
+        if (FX) {
+            std::vector<float> t(M);
+            for (long i = 0; i < M; ++i) t[i] = (float)in[i];
+            CPM_HIP(hipMemcpyAsync(mesh.p, t.data(), sizeof(float) * M, hipMemcpyHostToDevice, st));
+            CPM_HIP(hipStreamSynchronize(st));
+        } else {
+            CPM_HIP(hipMemcpyAsync(mesh.p, in, sizeof(double) * M, hipMemcpyHostToDevice, st));
+            CPM_HIP(hipStreamSynchronize(st));
+        }
+    }
+
+    void deposit(double* rho, int32_t* keys, double mass_override) override {
+        need_state();
+        do_deposit((Mesh)(mass_override > 0 ? mass_override : mass));
+        if (rho) mesh_to_host(rho);
+        if (keys) {
+            ParticleOps<PP>::keys(R, X.p, Np, lgNf, pa, k1.p, v1.p, st);
+            CPM_HIP(hipMemcpyAsync(keys, k1.p, sizeof(uint32_t) * Np, hipMemcpyDeviceToHost, st));
+            have_deposit = false;  // k1 was reused: the sorted keys are gone
+        }
+        CPM_HIP(hipStreamSynchronize(st));
+    }
+
+    void potential(double a, double* phi) override {
+        need_state();
+        do_deposit((Mesh)mass);
+        do_poisson(a);
+        mesh_to_host(phi);
+    }
+
+    void forces(double a, double* F) override {
+        need_state();
+        do_deposit((Mesh)mass);
+        do_poisson(a);
+        gather_to_host(F);
+    }
+
+    void gather_mesh(const double* phi, double* F) override {
+        need_state();
+        check_finite(phi, M, "potential");
+        mesh_from_host(phi);
+        have_deposit = false;
+        gather_to_host(F);
+    }
+
+    void gather_to_host(double* F) {
+        DevBuf<double> d((size_t)R * Np);
+        ParticleOps<PP>::gather_kick(R, X.p, P.p, Np, mesh.p, lgNf, pa, 0.0, 0.0, 0, d.p, st);
+        std::vector<double> tmp((size_t)R * Np);
+        CPM_HIP(hipMemcpyAsync(tmp.data(), d.p, sizeof(double) * R * Np, hipMemcpyDeviceToHost, st));
+        CPM_HIP(hipStreamSynchronize(st));
+        for (long i = 0; i < Np; ++i)
+            for (int k = 0; k < R; ++k) F[i * R + k] = tmp[(size_t)k * Np + i];
+    }
+
+    void poisson_mesh(const double* rho, double a, double* phi) override {
+        check_finite(rho, M, "density");
+        mesh_from_host(rho);
+        if (FX) reduce_sum(mesh.p, M, 0, 1.0 / (double)M, red.p, 4096, scal.p, st);
+        do_poisson(a, true);
+        mesh_to_host(phi);
+    }
+
+    void power_spectrum(int nbins, const double* edges, double* kmean, double* pk, double* cnt) override {
+        need_state();
+        if (nbins < 1) throw Error(PM_EINVAL, "nbins must be positive");
+        do_deposit((Mesh)mass);
+        std::vector<double> e(nbins + 1);
+        const double kf = 2 * M_PI / cfg.L, kn = M_PI * cfg.Nf / cfg.L;
+        if (edges) {
+            for (int b = 0; b <= nbins; ++b) e[b] = edges[b];
+        } else {
+            for (int b = 0; b <= nbins; ++b) e[b] = kf * pow(kn / kf, (double)b / nbins);
+        }
+        const double* mean_ptr = nullptr;
+        if (FX) {
+            sum_partials(row_partial.p, nrows, 1.0 / (double)M, scal.p, st);
+            mean_ptr = scal.p;
+        }
+        fft.forward(mesh.p, spec.p, nyq.p, (Mesh)1.0, mean_ptr, st);
+        DevBuf<double> de(nbins + 1), part(pk_partial_elems(fft.spec_elems, fft.nyq_elems, nbins)), out(3 * nbins);
+        CPM_HIP(hipMemcpyAsync(de.p, e.data(), sizeof(double) * (nbins + 1), hipMemcpyHostToDevice, st));
+        const double norm = pow(cfg.L, R) / ((double)M * (double)M);
+        cpm::power_spectrum<Mesh>(spec.p, nyq.p, (long)fft.spec_elems, (long)fft.nyq_elems, R, lgNf, kf, de.p, nbins,
+                                  norm, part.p, out.p, out.p + nbins, out.p + 2 * nbins, st);
+        std::vector<double> h(3 * nbins);
+        CPM_HIP(hipMemcpyAsync(h.data(), out.p, sizeof(double) * 3 * nbins, hipMemcpyDeviceToHost, st));
+        CPM_HIP(hipStreamSynchronize(st));
+        for (int b = 0; b < nbins; ++b) {
+            if (kmean) kmean[b] = h[b];
+            if (pk) pk[b] = h[nbins + b];
+            if (cnt) cnt[b] = h[2 * nbins + b];
+        }
+        have_deposit = false;  // the mesh was consumed
+    }
+
+    void diagnostics(pm_diag* d) override {
+        need_state();
+        std::memset(d, 0, sizeof(*d));
+        double h[8] = {0};
+        if (have_deposit) sum_partials(row_partial.p, nrows, 1.0, scal.p + 2, st);
+        for (int k = 0; k < R; ++k) reduce_sum(P.p + (size_t)k * Np, Np, 0, 1.0, red.p, 4096, scal.p + 3 + k, st);
+        reduce_sum(P.p, (long)R * Np, 1, 1.0, red.p, 4096, scal.p + 6, st);
+        CPM_HIP(hipMemcpyAsync(h, scal.p + 2, sizeof(double) * 5, hipMemcpyDeviceToHost, st));
+        CPM_HIP(hipStreamSynchronize(st));
+        d->mass = have_deposit ? h[0] : 0.0;
+        for (int k = 0; k < R; ++k) d->momentum[k] = h[1 + k];
+        d->p2 = h[4];
+    }
+};
+
+}  // namespace cpm
+
+// ====================================================================== C ABI
+struct pm_handle {
+    std::unique_ptr<cpm::ImplBase> impl;
+    pm_config cfg;
+    std::string err;
+};
+
+static thread_local std::string g_err;
+
+#define PM_TRY try {
+#define PM_CATCH(H)                                           \
+    }                                                         \
+    catch (const cpm::Error& e) {                             \
+        if (H) (H)->err = e.what(); else g_err = e.what();     \
+        return e.code;                                        \
+    }                                                         \
+    catch (const std::exception& e) {                         \
+        if (H) (H)->err = e.what(); else g_err = e.what();     \
+        return PM_EDEVICE;                                    \
+    }                                                         \
+    return PM_OK;
+
+#define PM_NEED(H)                                     \
+    if (!(H)) {                                        \
+        g_err = "null handle";                         \
+        return PM_EINVAL;                              \
+    }
+
+namespace cpm {
+template <typename T>
+void fft_test(int dim, int N, const double* in, double* spec, bool forward) {
+    cpm::FFTPlan<T> f(dim, N);
+    hipStream_t st;
+    CPM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    cpm::DevBuf<T> m(f.M);
+    cpm::DevBuf<cpm::cx<T>> s(f.spec_elems + f.nyq_elems);
+    const size_t ns = f.spec_elems + f.nyq_elems;
+    if (forward) {
+        std::vector<T> t(f.M);
+        for (long i = 0; i < f.M; ++i) t[i] = (T)in[i];
+        CPM_HIP(hipMemcpyAsync(m.p, t.data(), sizeof(T) * f.M, hipMemcpyHostToDevice, st));
+        f.forward(m.p, s.p, s.p + f.spec_elems, (T)0, nullptr, st);
+        std::vector<cpm::cx<T>> o(ns);
+        CPM_HIP(hipMemcpyAsync(o.data(), s.p, sizeof(cpm::cx<T>) * ns, hipMemcpyDeviceToHost, st));
+        CPM_HIP(hipStreamSynchronize(st));
+        for (size_t i = 0; i < ns; ++i) {
+            spec[2 * i] = o[i].re;
+            spec[2 * i + 1] = o[i].im;
+        }
+    } else {
+        std::vector<cpm::cx<T>> o(ns);
+        for (size_t i = 0; i < ns; ++i) o[i] = cpm::mk<T>((T)in[2 * i], (T)in[2 * i + 1]);
+        CPM_HIP(hipMemcpyAsync(s.p, o.data(), sizeof(cpm::cx<T>) * ns, hipMemcpyHostToDevice, st));
+        f.backward(s.p, s.p + f.spec_elems, m.p, 1.0, st);
+        std::vector<T> t(f.M);
+        CPM_HIP(hipMemcpyAsync(t.data(), m.p, sizeof(T) * f.M, hipMemcpyDeviceToHost, st));
+        CPM_HIP(hipStreamSynchronize(st));
+        for (long i = 0; i < f.M; ++i) spec[i] = t[i];
+    }
+    (void)hipStreamDestroy(st);
+}
+
+}  // namespace cpm
+
+extern "C" {
+
+const char* pm_last_error(const pm_handle* h) { return h ? h->err.c_str() : g_err.c_str(); }
+
+static void validate(const pm_config* c) {
+    if (!c) throw cpm::Error(PM_EINVAL, "null config");
+    if (c->dim != 2 && c->dim != 3) throw cpm::Error(PM_EUNSUP, "dim must be 2 or 3");
+    if (c->Nm < 1) throw cpm::Error(PM_EINVAL, "Nm must be positive");
+    if (c->Nf < 4 || c->Nf > 1024 || (c->Nf & (c->Nf - 1)))
+        throw cpm::Error(PM_EUNSUP, "Nf must be a power of two in [4, 1024]");
+    if (!(c->L > 0) || !std::isfinite(c->L)) throw cpm::Error(PM_EINVAL, "L must be positive");
+    if (c->precision != PM_FP64 && c->precision != PM_FP32) throw cpm::Error(PM_EINVAL, "unknown precision");
+    double np = std::pow((double)c->Nm, c->dim);
+    if (np > 2147483647.0) throw cpm::Error(PM_EUNSUP, "too many particles");
+    int dev = 0;
+    hipError_t e = hipGetDeviceCount(&dev);
+    if (e != hipSuccess || dev < 1) throw cpm::Error(PM_EDEVICE, "no HIP device available");
+}
+
+int pm_create(const pm_config* cfg, pm_handle** out) {
+    pm_handle* h = nullptr;
+    PM_TRY
+    if (!out) throw cpm::Error(PM_EINVAL, "null output pointer");
+    *out = nullptr;
+    validate(cfg);
+    std::unique_ptr<pm_handle> hh(new pm_handle);
+    hh->cfg = *cfg;
+    if (cfg->precision == PM_FP64)
+        hh->impl.reset(new cpm::Impl<cpm::PosF64>(*cfg));
+    else
+        hh->impl.reset(new cpm::Impl<cpm::PosFX>(*cfg));
+    *out = hh.release();
+    PM_CATCH(h)
+}
+
+void pm_destroy(pm_handle* h) { delete h; }
+
+int pm_init_zeldovich(pm_handle* h, const double* phi, double a_pos, double a_mom) {
+    PM_NEED(h)
+    PM_TRY
+    if (!phi) throw cpm::Error(PM_EINVAL, "null potential");
+    h->impl->init_zeldovich(phi, a_pos, a_mom);
+    PM_CATCH(h)
+}
+
+int pm_drift(pm_handle* h, double a, double da) {
+    PM_NEED(h)
+    PM_TRY
+    h->impl->drift(a, da);
+    PM_CATCH(h)
+}
+
+int pm_kick(pm_handle* h, double a, double da) {
+    PM_NEED(h)
+    PM_TRY
+    h->impl->kick(a, da);
+    PM_CATCH(h)
+}
+
+int pm_steps(pm_handle* h, double a, double da, int n, double* a_out) {
+    PM_NEED(h)
+    PM_TRY
+    double r = h->impl->steps(a, da, n);
+    if (a_out) *a_out = r;
+    PM_CATCH(h)
+}
+
+int pm_get_state(pm_handle* h, double* X, double* P) {
+    PM_NEED(h)
+    PM_TRY
+    h->impl->get_state(X, P);
+    PM_CATCH(h)
+}
+
+int pm_set_state(pm_handle* h, const double* X, const double* P) {
+    PM_NEED(h)
+    PM_TRY
+    if (!X && !h->impl->have_state) throw cpm::Error(PM_EINVAL, "positions required");
+    h->impl->set_state(X, P);
+    PM_CATCH(h)
+}
+
+int pm_get_times(const pm_handle* h, double* a_pos, double* a_mom) {
+    if (!h) return PM_EINVAL;
+    if (a_pos) *a_pos = h->impl->last_apos;
+    if (a_mom) *a_mom = h->impl->last_avel;
+    return PM_OK;
+}
+
+int pm_deposit(pm_handle* h, double* rho, int32_t* keys) {
+    PM_NEED(h)
+    PM_TRY
+    h->impl->deposit(rho, keys, 0.0);
+    PM_CATCH(h)
+}
+
+int pm_potential(pm_handle* h, double a, double* phi) {
+    PM_NEED(h)
+    PM_TRY
+    if (!phi) throw cpm::Error(PM_EINVAL, "null output");
+    h->impl->potential(a, phi);
+    PM_CATCH(h)
+}
+
+int pm_forces(pm_handle* h, double a, double* F) {
+    PM_NEED(h)
+    PM_TRY
+    if (!F) throw cpm::Error(PM_EINVAL, "null output");
+    h->impl->forces(a, F);
+    PM_CATCH(h)
+}
+
+int pm_gather_mesh(pm_handle* h, const double* phi, double* F) {
+    PM_NEED(h)
+    PM_TRY
+    if (!phi || !F) throw cpm::Error(PM_EINVAL, "null argument");
+    h->impl->gather_mesh(phi, F);
+    PM_CATCH(h)
+}
+
+int pm_poisson_mesh(pm_handle* h, const double* rho, double a, double* phi) {
+    PM_NEED(h)
+    PM_TRY
+    if (!rho || !phi) throw cpm::Error(PM_EINVAL, "null mesh");
+    h->impl->poisson_mesh(rho, a, phi);
+    PM_CATCH(h)
+}
+
+int pm_power_spectrum(pm_handle* h, int nbins, const double* edges, double* kmean, double* pk, double* count) {
+    PM_NEED(h)
+    PM_TRY
+    h->impl->power_spectrum(nbins, edges, kmean, pk, count);
+    PM_CATCH(h)
+}
+
+int pm_diagnostics(pm_handle* h, pm_diag* d) {
+    PM_NEED(h)
+    PM_TRY
+    if (!d) throw cpm::Error(PM_EINVAL, "null output");
+    h->impl->diagnostics(d);
+    PM_CATCH(h)
+}
+
+int pm_profile(pm_handle* h, int enable, double* ms, long long* counts) {
+    PM_NEED(h)
+    PM_TRY
+    if (ms)
+        for (int i = 0; i < PM_NPHASES; ++i) ms[i] = h->impl->prof.ms[i];
+    if (counts)
+        for (int i = 0; i < PM_NPHASES; ++i) counts[i] = h->impl->prof.cnt[i];
+    if (enable >= 0) h->impl->prof.enable(enable != 0);
+    PM_CATCH(h)
+}
+
+int pm_sync(pm_handle* h) {
+    PM_NEED(h)
+    PM_TRY
+    CPM_HIP(hipStreamSynchronize(h->impl->st));
+    PM_CATCH(h)
+}
+
+// ---------------------------------------------------------------- ic
+struct IcCtx {
+    int dim, N;
+    long M;
+    cpm::FFTPlan<double> fft;
+    cpm::DevBuf<double> mesh, red, scal;
+    cpm::DevBuf<cpm::cx<double>> spec, nyq, spec2, nyq2;
+    hipStream_t st = nullptr;
+    IcCtx(int d, int mbits) : dim(d), N(1 << mbits), fft(d, 1 << mbits) {
+        M = fft.M;
+        mesh.alloc(M);
+        red.alloc(4096);
+        scal.alloc(4);
+        spec.alloc(fft.spec_elems);
+        nyq.alloc(fft.nyq_elems);
+        CPM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    }
+    ~IcCtx() {
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
+    }
+    void upload(const double* a) {
+        CPM_HIP(hipMemcpyAsync(mesh.p, a, sizeof(double) * M, hipMemcpyHostToDevice, st));
+    }
+    void download(double* a) {
+        CPM_HIP(hipMemcpyAsync(a, mesh.p, sizeof(double) * M, hipMemcpyDeviceToHost, st));
+        CPM_HIP(hipStreamSynchronize(st));
+    }
+    cpm::FilterArgs fargs(int kind, double kscale) const {
+        cpm::FilterArgs f{};
+        f.kind = kind;
+        f.dim = dim;
+        f.lgN = cpm::ilog2(N);
+        f.kscale = kscale;
+        return f;
+    }
+    // _compute_potential on the mesh (ic.cc:100-117)
+    void potential(double L) {
+        fft.forward(mesh.p, spec.p, nyq.p, 0.0, nullptr, st);
+        cpm::apply_filter(spec.p, nyq.p, (long)fft.spec_elems, (long)fft.nyq_elems, fargs(cpm::FILT_POT, 2 * M_PI / L), st);
+        fft.backward(spec.p, nyq.p, mesh.p, (double)M, st);
+    }
+};
+
+static void ic_check(int dim, int mbits) {
+    if (dim != 2 && dim != 3) throw cpm::Error(PM_EUNSUP, "only 2 and 3 dimensions supported.");
+    if (mbits < 2 || mbits > 10) throw cpm::Error(PM_EUNSUP, "mbits must lie in [2, 10]");
+    int dev = 0;
+    if (hipGetDeviceCount(&dev) != hipSuccess || dev < 1) throw cpm::Error(PM_EDEVICE, "no HIP device available");
+}
+
+int pm_ic_scale_free(const pm_ic_params* p, const double* noise, double* density, double* potential) {
+    pm_handle* h = nullptr;
+    PM_TRY
+    if (!p || !noise) throw cpm::Error(PM_EINVAL, "null argument");
+    ic_check(p->dim, p->mbits);
+    IcCtx c(p->dim, p->mbits);
+    cpm::check_finite(noise, c.M, "noise");
+    const size_t N = size_t(1) << p->mbits;
+    c.spec2.alloc(c.fft.spec_elems);
+    c.nyq2.alloc(c.fft.nyq_elems);
+    c.upload(noise);
+    c.fft.forward(c.mesh.p, c.spec.p, c.nyq.p, 0.0, nullptr, c.st);
+    CPM_HIP(hipMemcpyAsync(c.spec2.p, c.spec.p, sizeof(cpm::cx<double>) * c.fft.spec_elems, hipMemcpyDeviceToDevice, c.st));
+    CPM_HIP(hipMemcpyAsync(c.nyq2.p, c.nyq.p, sizeof(cpm::cx<double>) * c.fft.nyq_elems, hipMemcpyDeviceToDevice, c.st));
+    cpm::FilterArgs fa = c.fargs(cpm::FILT_PS, 2 * M_PI / (double)N);  // kspace<R>(N, N)
+    fa.ns = p->ns;
+    fa.t = p->scale * N / p->L;  // Fourier::scale<R>(sigma * N/L), ic.cc:63
+    cpm::apply_filter(c.spec.p, c.nyq.p, (long)c.fft.spec_elems, (long)c.fft.nyq_elems, fa, c.st);
+    c.fft.backward(c.spec.p, c.nyq.p, c.mesh.p, (double)c.M, c.st);
+    cpm::reduce_sum(c.mesh.p, c.M, 1, 1.0 / (double)c.M, c.red.p, 4096, c.scal.p, c.st);
+    double var = 0;
+    CPM_HIP(hipMemcpyAsync(&var, c.scal.p, sizeof(double), hipMemcpyDeviceToHost, c.st));
+    CPM_HIP(hipStreamSynchronize(c.st));
+    fa.kind = p->smooth ? cpm::FILT_PS : cpm::FILT_P;
+    cpm::apply_filter(c.spec2.p, c.nyq2.p, (long)c.fft.spec_elems, (long)c.fft.nyq_elems, fa, c.st);
+    const double s = c.M * sqrt(var) / p->sigma;  // real_part(size * sqrt(var) / norm), ic.cc:80
+    c.fft.backward(c.spec2.p, c.nyq2.p, c.mesh.p, s, c.st);
+    if (density) c.download(density);
+    if (potential) {
+        c.potential(p->L);
+        c.download(potential);
+    }
+    CPM_HIP(hipStreamSynchronize(c.st));
+    PM_CATCH(h)
+}
+
+int pm_ic_cdm(int dim, int mbits, double L, double ns, double Omega0, double H0, const double* noise,
+              double* density) {
+    pm_handle* h = nullptr;
+    PM_TRY
+    if (!noise || !density) throw cpm::Error(PM_EINVAL, "null argument");
+    ic_check(dim, mbits);
+    IcCtx c(dim, mbits);
+    cpm::check_finite(noise, c.M, "noise");
+    const size_t N = size_t(1) << mbits;
+    const double res = L / N;
+    c.upload(noise);
+    c.fft.forward(c.mesh.p, c.spec.p, c.nyq.p, 0.0, nullptr, c.st);
+    cpm::FilterArgs fa = c.fargs(cpm::FILT_CDM, 2 * M_PI / L);  // kspace<R>(N, L), ic.cc:84
+    fa.ns = ns;
+    fa.omega0 = Omega0;
+    fa.h = H0 / 100.0;
+    cpm::apply_filter(c.spec.p, c.nyq.p, (long)c.fft.spec_elems, (long)c.fft.nyq_elems, fa, c.st);
+    c.fft.backward(c.spec.p, c.nyq.p, c.mesh.p, c.M * pow(res, 0.5 * dim), c.st);  // ic.cc:92
+    c.download(density);
+    PM_CATCH(h)
+}
+
+int pm_ic_potential(int dim, int mbits, double L, double* field) {
+    pm_handle* h = nullptr;
+    PM_TRY
+    if (!field) throw cpm::Error(PM_EINVAL, "null argument");
+    ic_check(dim, mbits);
+    IcCtx c(dim, mbits);
+    cpm::check_finite(field, c.M, "density");
+    c.upload(field);
+    c.potential(L);
+    c.download(field);
+    PM_CATCH(h)
+}
+
+int pm_ic_displacement(int dim, int mbits, double L, const double* potential, double* psi) {
+    pm_handle* h = nullptr;
+    PM_TRY
+    if (!potential || !psi) throw cpm::Error(PM_EINVAL, "null argument");
+    ic_check(dim, mbits);
+    IcCtx c(dim, mbits);
+    cpm::check_finite(potential, c.M, "potential");
+    c.spec2.alloc(c.fft.spec_elems);
+    c.nyq2.alloc(c.fft.nyq_elems);
+    c.upload(potential);
+    c.fft.forward(c.mesh.p, c.spec2.p, c.nyq2.p, 0.0, nullptr, c.st);
+    const int N = 1 << mbits;
+    std::vector<double> comp(c.M);
+    for (int k = 0; k < dim; ++k) {
+        CPM_HIP(hipMemcpyAsync(c.spec.p, c.spec2.p, sizeof(cpm::cx<double>) * c.fft.spec_elems, hipMemcpyDeviceToDevice, c.st));
+        CPM_HIP(hipMemcpyAsync(c.nyq.p, c.nyq2.p, sizeof(cpm::cx<double>) * c.fft.nyq_elems, hipMemcpyDeviceToDevice, c.st));
+        cpm::FilterArgs fa = c.fargs(cpm::FILT_DERIV, 2 * M_PI / (double)N);
+        fa.axis = k;
+        cpm::apply_filter(c.spec.p, c.nyq.p, (long)c.fft.spec_elems, (long)c.fft.nyq_elems, fa, c.st);
+        c.fft.backward(c.spec.p, c.nyq.p, c.mesh.p, c.M / L * N, c.st);  // real_part(size / L * N), ic.cc:142
+        c.download(comp.data());
+        for (long i = 0; i < c.M; ++i) psi[i * dim + k] = comp[i];
+    }
+    PM_CATCH(h)
+}
+
+int pm_fft_r2c(int dim, int N, int precision, const double* in, double* spec) {
+    pm_handle* h = nullptr;
+    PM_TRY
+    if (!in || !spec) throw cpm::Error(PM_EINVAL, "null argument");
+    ic_check(dim, cpm::ilog2(N));
+    if (precision == PM_FP64) cpm::fft_test<double>(dim, N, in, spec, true);
+    else cpm::fft_test<float>(dim, N, in, spec, true);
+    PM_CATCH(h)
+}
+
+int pm_fft_c2r(int dim, int N, int precision, const double* spec, double* out) {
+    pm_handle* h = nullptr;
+    PM_TRY
+    if (!spec || !out) throw cpm::Error(PM_EINVAL, "null argument");
+    ic_check(dim, cpm::ilog2(N));
+    if (precision == PM_FP64) cpm::fft_test<double>(dim, N, spec, out, false);
+    else cpm::fft_test<float>(dim, N, spec, out, false);
+    PM_CATCH(h)
+}
+
+// ---------------------------------------------------------------- cic
+int pm_cic(int dim, int Nm, int Nf, double L, int s, const double* X, double* rho) {
+    pm_handle* h = nullptr;
+    PM_TRY
+    if (!X || !rho) throw cpm::Error(PM_EINVAL, "null argument");
+    if (s < 1) throw cpm::Error(PM_EINVAL, "sub-sample must be >= 1");
+    pm_config c{};
+    c.dim = dim;
+    c.Nm = Nm * s;
+    c.Nf = Nf;
+    c.L = L;
+    c.H0 = 1;
+    c.Omega_m = 1;
+    c.precision = PM_FP64;
+    validate(&c);
+    cpm::Impl<cpm::PosF64> im(c);
+    if (s == 1) {
+        im.set_state(X, nullptr);
+        im.deposit(rho, nullptr, 1.0);  // md_cic_T with unit mass (cic.hh:59-63)
+    } else {
+        long npm = 1;
+        for (int k = 0; k < dim; ++k) npm *= Nm;
+        std::vector<double> soa((size_t)dim * npm);
+        cpm::check_finite(X, (size_t)dim * npm, "positions");
+        for (long i = 0; i < npm; ++i)
+            for (int k = 0; k < dim; ++k) soa[(size_t)k * npm + i] = X[i * dim + k];
+        cpm::DevBuf<double> dx(soa.size()), dy((size_t)dim * im.Np);
+        CPM_HIP(hipMemcpyAsync(dx.p, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice, im.st));
+        cpm::subsample(dim, dx.p, Nm, s, L, dy.p, im.st);
+        im.set_positions_device(dy.p);
+        const double a = pow(double(s * Nm) / double(Nf), -float(dim));  // cic.hh:76
+        im.deposit(rho, nullptr, a);
+    }
+    PM_CATCH(h)
+}
+
+}  // extern "C"

@@ -1,0 +1,477 @@
+// pm_fft.hip -- real-to-complex / complex-to-real 2-D and 3-D FFTs on LDS pencils,
+// and the fused k-space passes of the Poisson solve.
+//
+// Replaces the FFTW c2c plans of the reference (src/base/fft.cc:15-38, used at
+// src/run/nbody.hh:105,109 and src/ic/ic.cc:66-115).  Data layout in HBM:
+//   real mesh   T[N^R], axis 0 fastest (src/base/box.hh:51-53);
+//   spectrum    main  cx<T>[H * N^(R-1)], H = N/2, modes kx in [0, H), kx fastest;
+//               nyq   cx<T>[N^(R-1)], the kx = N/2 plane, ky fastest.
+// Keeping the Nyquist plane apart leaves every pencil of the main block a
+// power of two long and aligned.
+//
+// Passes of one Poisson solve (2R-1 read+write sweeps of the half spectrum):
+//   rows r2c (fused: delta = rho - mean) -> [3-D: y forward] ->
+//   last axis forward * Green's function * last axis backward (one pass) ->
+//   [3-D: y backward] -> rows c2r (fused: phi = x / s).
+#include "pm_fft.hpp"
+#include "pm_fft_api.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace cpm {
+
+// LDS slot of element n of pencil p: pencils are N apart, and the low bits
+// of n are XOR-swizzled with the pencil number so that both the column-wise
+// global<->LDS copies and the in-pencil Stockham accesses spread over banks.
+template <int N>
+__device__ __forceinline__ int lidx(int p, int n) {
+    constexpr int SW = (N < 32 ? N : 32) - 1;
+    return p * N + (n ^ (p & SW));
+}
+
+template <typename T, int N, int E, int NT, int DIR, int R>
+__device__ __forceinline__ void lds_stage(cx<T>* sh, int Ns, const cx<T>* __restrict__ tw, int tws,
+                                          int tid) {
+    constexpr int NB = E / R;
+    constexpr int BPP = N / R;
+    cx<T> v[E];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int b = tid + i * NT;
+        const int p = b / BPP, j = b % BPP;
+#pragma unroll
+        for (int t = 0; t < R; ++t) v[i * R + t] = sh[lidx<N>(p, j + t * BPP)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int b = tid + i * NT;
+        const int p = b / BPP, j = b % BPP;
+        butterfly<T, N, R, DIR>(&v[i * R], j, Ns, tw, tws);
+#pragma unroll
+        for (int t = 0; t < R; ++t) sh[lidx<N>(p, stock_out<N, R>(j, Ns, t))] = v[i * R + t];
+    }
+    __syncthreads();
+}
+
+// In-LDS transform of all pencils of the tile (E*NT elements, pencils of N).
+// The caller has synchronised after filling LDS; on return LDS holds the
+// result and all threads have synchronised.
+template <typename T, int N, int E, int NT, int DIR>
+__device__ __forceinline__ void lds_fft(cx<T>* sh, const cx<T>* __restrict__ tw, int tws, int tid) {
+    using S = Sched<N>;
+    int Ns = 1;
+    if constexpr (S::lead != 0) {
+        lds_stage<T, N, E, NT, DIR, S::lead>(sh, Ns, tw, tws, tid);
+        Ns *= S::lead;
+    }
+    if constexpr (S::n8 > 0) {
+#pragma unroll
+        for (int s = 0; s < S::n8; ++s) {
+            lds_stage<T, N, E, NT, DIR, 8>(sh, Ns, tw, tws, tid);
+            Ns *= 8;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ rows
+template <typename T, int N>
+struct RowCfg {
+    static constexpr int H = N / 2;
+    static constexpr int E = H >= 8 ? 8 : H;
+    static constexpr int P = (2048 / H) > 0 ? 2048 / H : 1;
+    static constexpr int NT = P * H / E;
+};
+
+// rows of N reals (minus the shift) -> H complex modes + the Nyquist mode.
+template <typename T, int N>
+__global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_r2c(
+    const T* __restrict__ in, cx<T>* __restrict__ spec, cx<T>* __restrict__ nyq, long nrows,
+    T shift, const double* __restrict__ mean_ptr, const cx<T>* __restrict__ tw) {
+    using C = RowCfg<T, N>;
+    constexpr int H = C::H, P = C::P, NT = C::NT;
+    __shared__ cx<T> sh[P * H];
+    const int tid = threadIdx.x;
+    const long row0 = (long)blockIdx.x * P;
+    const T sft = mean_ptr ? (T)(*mean_ptr) : shift;
+    for (int e = tid; e < P * H; e += NT) {
+        const int p = e / H, n = e % H;
+        const long row = row0 + p;
+        cx<T> z = mk<T>(0, 0);
+        if (row < nrows) {
+            cx<T> pr = *reinterpret_cast<const cx<T>*>(in + row * N + 2 * n);
+            z = mk<T>(pr.re - sft, pr.im - sft);
+        }
+        sh[lidx<H>(p, n)] = z;
+    }
+    __syncthreads();
+    lds_fft<T, H, C::E, NT, -1>(sh, tw, 2, tid);
+    for (int e = tid; e < P * H; e += NT) {
+        const int p = e / H, k = e % H;
+        const long row = row0 + p;
+        if (row >= nrows) continue;
+        const cx<T> Zk = sh[lidx<H>(p, k)];
+        const cx<T> Zm = conj(sh[lidx<H>(p, (H - k) & (H - 1))]);
+        const cx<T> A = scale(Zk + Zm, (T)0.5);
+        const cx<T> WB = tw[k] * (Zk - Zm);
+        spec[row * H + k] = mk<T>(A.re + (T)0.5 * WB.im, A.im - (T)0.5 * WB.re);
+        if (k == 0) nyq[row] = mk<T>(Zk.re - Zk.im, (T)0);
+    }
+}
+
+// H complex modes + Nyquist mode per row -> N reals, divided by s.
+template <typename T, int N>
+__global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_c2r(
+    const cx<T>* __restrict__ spec, const cx<T>* __restrict__ nyq, T* __restrict__ out, long nrows,
+    double s, const cx<T>* __restrict__ tw) {
+    using C = RowCfg<T, N>;
+    constexpr int H = C::H, P = C::P, NT = C::NT, E = C::E;
+    __shared__ cx<T> sh[P * H];
+    __shared__ cx<T> shn[P];
+    const int tid = threadIdx.x;
+    const long row0 = (long)blockIdx.x * P;
+    for (int e = tid; e < P * H; e += NT) {
+        const int p = e / H, n = e % H;
+        const long row = row0 + p;
+        sh[lidx<H>(p, n)] = row < nrows ? spec[row * H + n] : mk<T>(0, 0);
+    }
+    for (int p = tid; p < P; p += NT) shn[p] = (row0 + p < nrows) ? nyq[row0 + p] : mk<T>(0, 0);
+    __syncthreads();
+    cx<T> z[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const int e = tid + i * NT;
+        const int p = e / H, k = e % H;
+        const cx<T> Xk = sh[lidx<H>(p, k)];
+        const cx<T> Xm = conj(k == 0 ? shn[p] : sh[lidx<H>(p, H - k)]);
+        const cx<T> Ep = Xk + Xm;
+        const cx<T> wO = conj(tw[k]) * (Xk - Xm);
+        z[i] = mk<T>(Ep.re - wO.im, Ep.im + wO.re);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const int e = tid + i * NT;
+        sh[lidx<H>(e / H, e % H)] = z[i];
+    }
+    __syncthreads();
+    lds_fft<T, H, E, NT, +1>(sh, tw, 2, tid);
+    const T inv = (T)(1.0 / s);
+    for (int e = tid; e < P * H; e += NT) {
+        const int p = e / H, n = e % H;
+        const long row = row0 + p;
+        if (row >= nrows) continue;
+        const cx<T> v = sh[lidx<H>(p, n)];
+        cx<T> o;
+        if constexpr (sizeof(T) == 8) {
+            o = mk<T>(v.re / (T)s, v.im / (T)s);  // real_part(s): Re(z) / s (fourier.hh:31-37)
+        } else {
+            o = mk<T>(v.re * inv, v.im * inv);
+        }
+        *reinterpret_cast<cx<T>*>(out + row * N + 2 * n) = o;
+    }
+}
+
+// --------------------------------------------------------------- columns
+template <typename T, int N>
+struct ColCfg {
+    static constexpr int CS = (int)sizeof(cx<T>);
+    static constexpr int Ct = sizeof(T) == 4 ? 16 : 8;
+    static constexpr int C = (Ct * N * CS > 65536) ? (65536 / (N * CS)) : Ct;
+    static constexpr int NT = N >= 8 ? ((C * N / 8 < 256) ? C * N / 8 : 256) : C;
+    static constexpr int E = C * N / NT;
+};
+
+template <typename T, int N>
+__device__ __forceinline__ void col_load(cx<T>* sh, const cx<T>* __restrict__ data, const ColGeom& g,
+                                         long c0, int tid) {
+    using C = ColCfg<T, N>;
+    const long bbase = (long)blockIdx.y * g.bs;
+    for (int e = tid; e < C::C * N; e += C::NT) {
+        const int c = e % C::C, n = e / C::C;
+        const long col = c0 + c;
+        sh[lidx<N>(c, n)] = col < g.nc ? data[bbase + col * g.cs + (long)n * g.es] : mk<T>(0, 0);
+    }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void col_store(const cx<T>* sh, cx<T>* __restrict__ data, const ColGeom& g,
+                                          long c0, int tid) {
+    using C = ColCfg<T, N>;
+    const long bbase = (long)blockIdx.y * g.bs;
+    for (int e = tid; e < C::C * N; e += C::NT) {
+        const int c = e % C::C, n = e / C::C;
+        const long col = c0 + c;
+        if (col < g.nc) data[bbase + col * g.cs + (long)n * g.es] = sh[lidx<N>(c, n)];
+    }
+}
+
+template <typename T, int N, int DIR>
+__global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols(cx<T>* __restrict__ data, ColGeom g,
+                                                           const cx<T>* __restrict__ tw) {
+    using C = ColCfg<T, N>;
+    __shared__ cx<T> sh[C::C * N];
+    const int tid = threadIdx.x;
+    const long c0 = (long)blockIdx.x * C::C;
+    col_load<T, N>(sh, data, g, c0, tid);
+    __syncthreads();
+    lds_fft<T, N, C::E, C::NT, DIR>(sh, tw, 1, tid);
+    col_store<T, N>(sh, data, g, c0, tid);
+}
+
+// Signed mode number of index n on an N mesh: n > N/2 maps to n - N, and
+// N/2 stays +N/2 (src/base/fourier.hh:52).
+__device__ __forceinline__ int signed_mode(int n, int N) { return n > N / 2 ? n - N : n; }
+
+// Green's function -1/|k|^2 with physical k = n * 2 pi / L (src/base/fourier.hh:102-109;
+// applied at src/run/nbody.hh:106-108), zero at k = 0.
+template <typename T>
+__device__ __forceinline__ T green_value(int ix, int iy, int iz, int dim, int N, T kf) {
+    if (ix == 0 && iy == 0 && iz == 0) return (T)0;
+    T kx = (T)signed_mode(ix, N) * kf;
+    T ky = (T)signed_mode(iy, N) * kf;
+    T v = (T)0;
+    v += kx * kx;
+    v += ky * ky;
+    if (dim == 3) {
+        T kz = (T)signed_mode(iz, N) * kf;
+        v += kz * kz;
+    }
+    return (T)-1. / v;
+}
+
+// Last-axis forward transform, multiplication by the Green's function and
+// last-axis backward transform, in one read and one write of the spectrum.
+template <typename T, int N>
+__global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols_green(cx<T>* __restrict__ data, ColGeom g,
+                                                                 GreenGeom gg,
+                                                                 const cx<T>* __restrict__ tw) {
+    using C = ColCfg<T, N>;
+    __shared__ cx<T> sh[C::C * N];
+    const int tid = threadIdx.x;
+    const long c0 = (long)blockIdx.x * C::C;
+    col_load<T, N>(sh, data, g, c0, tid);
+    __syncthreads();
+    lds_fft<T, N, C::E, C::NT, -1>(sh, tw, 1, tid);
+    const int H = N / 2;
+    const T kf = (T)gg.kf;
+    for (int e = tid; e < C::C * N; e += C::NT) {
+        const int c = e / N, n = e % N;
+        const long col = c0 + c;
+        int ix, iy, iz = 0;
+        if (gg.dim == 3) {
+            ix = gg.nyq ? H : (int)(col & (H - 1));
+            iy = gg.nyq ? (int)col : (int)(col / H);
+            iz = n;
+        } else {
+            ix = gg.nyq ? H : (int)col;
+            iy = n;
+        }
+        const T gv = green_value<T>(ix, iy, iz, gg.dim, N, kf);
+        const int li = lidx<N>(c, n);
+        sh[li] = (ix == 0 && iy == 0 && iz == 0) ? mk<T>(0, 0) : scale(sh[li], gv);
+    }
+    __syncthreads();
+    lds_fft<T, N, C::E, C::NT, +1>(sh, tw, 1, tid);
+    col_store<T, N>(sh, data, g, c0, tid);
+}
+
+// ------------------------------------------------------------ dispatch
+#define CPM_FOR_N(MACRO) \
+    MACRO(4) MACRO(8) MACRO(16) MACRO(32) MACRO(64) MACRO(128) MACRO(256) MACRO(512) MACRO(1024)
+
+template <typename T>
+void launch_rows_r2c(int N, const T* in, cx<T>* spec, cx<T>* nyq, long nrows, T shift,
+                     const double* mean_ptr, const cx<T>* tw, hipStream_t st) {
+    switch (N) {
+#define CASE(n)                                                                        \
+    case n: {                                                                          \
+        using C = RowCfg<T, n>;                                                        \
+        dim3 grid((unsigned)((nrows + C::P - 1) / C::P));                              \
+        hipLaunchKernelGGL((k_rows_r2c<T, n>), grid, dim3(C::NT), 0, st, in, spec, nyq, \
+                           nrows, shift, mean_ptr, tw);                                \
+        break;                                                                         \
+    }
+        CPM_FOR_N(CASE)
+#undef CASE
+        default:
+            throw Error(-2, "unsupported FFT length " + std::to_string(N));
+    }
+    CPM_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_rows_c2r(int N, const cx<T>* spec, const cx<T>* nyq, T* out, long nrows, double s,
+                     const cx<T>* tw, hipStream_t st) {
+    switch (N) {
+#define CASE(n)                                                                               \
+    case n: {                                                                                 \
+        using C = RowCfg<T, n>;                                                               \
+        dim3 grid((unsigned)((nrows + C::P - 1) / C::P));                                     \
+        hipLaunchKernelGGL((k_rows_c2r<T, n>), grid, dim3(C::NT), 0, st, spec, nyq, out, nrows, \
+                           s, tw);                                                            \
+        break;                                                                                \
+    }
+        CPM_FOR_N(CASE)
+#undef CASE
+        default:
+            throw Error(-2, "unsupported FFT length " + std::to_string(N));
+    }
+    CPM_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_cols(int N, int dir, cx<T>* data, const ColGeom& g, const cx<T>* tw, hipStream_t st) {
+    switch (N) {
+#define CASE(n)                                                                              \
+    case n: {                                                                                \
+        using C = ColCfg<T, n>;                                                              \
+        dim3 grid((unsigned)((g.nc + C::C - 1) / C::C), (unsigned)g.nb);                     \
+        if (dir < 0)                                                                         \
+            hipLaunchKernelGGL((k_cols<T, n, -1>), grid, dim3(C::NT), 0, st, data, g, tw);   \
+        else                                                                                 \
+            hipLaunchKernelGGL((k_cols<T, n, +1>), grid, dim3(C::NT), 0, st, data, g, tw);   \
+        break;                                                                               \
+    }
+        CPM_FOR_N(CASE)
+#undef CASE
+        default:
+            throw Error(-2, "unsupported FFT length " + std::to_string(N));
+    }
+    CPM_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_cols_green(int N, cx<T>* data, const ColGeom& g, const GreenGeom& gg, const cx<T>* tw,
+                       hipStream_t st) {
+    switch (N) {
+#define CASE(n)                                                                                  \
+    case n: {                                                                                    \
+        using C = ColCfg<T, n>;                                                                  \
+        dim3 grid((unsigned)((g.nc + C::C - 1) / C::C), (unsigned)g.nb);                         \
+        hipLaunchKernelGGL((k_cols_green<T, n>), grid, dim3(C::NT), 0, st, data, g, gg, tw);     \
+        break;                                                                                   \
+    }
+        CPM_FOR_N(CASE)
+#undef CASE
+        default:
+            throw Error(-2, "unsupported FFT length " + std::to_string(N));
+    }
+    CPM_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------ the plan
+template <typename T>
+FFTPlan<T>::FFTPlan(int dim_, int N_) : dim(dim_), N(N_), tw(nullptr) {
+    if (dim != 2 && dim != 3) throw Error(-2, "FFT supports 2-D and 3-D meshes only");
+    if (N < 4 || N > 1024 || (N & (N - 1))) throw Error(-2, "mesh size must be a power of two in [4, 1024]");
+    H = N / 2;
+    nrows = 1;
+    for (int k = 1; k < dim; ++k) nrows *= N;
+    M = nrows * N;
+    spec_elems = (size_t)H * nrows;
+    nyq_elems = (size_t)nrows;
+    std::vector<cx<T>> h(N);
+    const long double twopi = 6.283185307179586476925286766559005768L;
+    for (int m = 0; m < N; ++m) {
+        long double a = twopi * (long double)m / (long double)N;
+        h[m] = mk<T>((T)cosl(a), (T)(-sinl(a)));
+    }
+    CPM_HIP(hipMalloc(&tw, sizeof(cx<T>) * N));
+    CPM_HIP(hipMemcpy(tw, h.data(), sizeof(cx<T>) * N, hipMemcpyHostToDevice));
+}
+
+template <typename T>
+FFTPlan<T>::~FFTPlan() {
+    if (tw) (void)hipFree(tw);
+}
+
+template <typename T>
+void FFTPlan<T>::geoms(ColGeom* ymain, ColGeom* ynyq, ColGeom* zmain, ColGeom* znyq) const {
+    if (dim == 3) {
+        // main: index kx + H*(ky + N*kz); nyq: ky + N*kz
+        *ymain = ColGeom{(long)H, 1, (long)H, (long)H * N, N};
+        *ynyq = ColGeom{1, (long)N, (long)N, 0, 1};
+        *zmain = ColGeom{(long)H * N, 1, (long)H * N, 0, 1};
+        *znyq = ColGeom{(long)N, 1, (long)N, 0, 1};
+    } else {
+        // main: kx + H*ky; nyq: ky.  The y pass is the last axis.
+        *ymain = ColGeom{(long)H, 1, (long)H, 0, 1};
+        *ynyq = ColGeom{1, (long)N, 1, 0, 1};
+        *zmain = *ymain;
+        *znyq = *ynyq;
+    }
+}
+
+template <typename T>
+void FFTPlan<T>::forward(const T* in, cx<T>* spec, cx<T>* nyq, T shift, const double* mean_ptr,
+                         hipStream_t st) const {
+    launch_rows_r2c<T>(N, in, spec, nyq, nrows, shift, mean_ptr, tw, st);
+    ColGeom ym, yn, zm, zn;
+    geoms(&ym, &yn, &zm, &zn);
+    launch_cols<T>(N, -1, spec, ym, tw, st);
+    launch_cols<T>(N, -1, nyq, yn, tw, st);
+    if (dim == 3) {
+        launch_cols<T>(N, -1, spec, zm, tw, st);
+        launch_cols<T>(N, -1, nyq, zn, tw, st);
+    }
+}
+
+template <typename T>
+void FFTPlan<T>::backward(cx<T>* spec, cx<T>* nyq, T* out, double s, hipStream_t st) const {
+    ColGeom ym, yn, zm, zn;
+    geoms(&ym, &yn, &zm, &zn);
+    if (dim == 3) {
+        launch_cols<T>(N, +1, spec, zm, tw, st);
+        launch_cols<T>(N, +1, nyq, zn, tw, st);
+    }
+    launch_cols<T>(N, +1, spec, ym, tw, st);
+    launch_cols<T>(N, +1, nyq, yn, tw, st);
+    launch_rows_c2r<T>(N, spec, nyq, out, nrows, s, tw, st);
+}
+
+template <typename T>
+void FFTPlan<T>::poisson_stage(int stage, const T* rho, T* phi, cx<T>* spec, cx<T>* nyq, T shift,
+                               const double* mean_ptr, double kf, double s, hipStream_t st) const {
+    ColGeom ym, yn, zm, zn;
+    geoms(&ym, &yn, &zm, &zn);
+    const GreenGeom gmain{dim, 0, kf}, gnyq{dim, 1, kf};
+    switch (stage) {
+        case 0:
+            launch_rows_r2c<T>(N, rho, spec, nyq, nrows, shift, mean_ptr, tw, st);
+            break;
+        case 1:
+            if (dim == 3) {
+                launch_cols<T>(N, -1, spec, ym, tw, st);
+                launch_cols<T>(N, -1, nyq, yn, tw, st);
+            }
+            break;
+        case 2:
+            launch_cols_green<T>(N, spec, dim == 3 ? zm : ym, gmain, tw, st);
+            launch_cols_green<T>(N, nyq, dim == 3 ? zn : yn, gnyq, tw, st);
+            break;
+        case 3:
+            if (dim == 3) {
+                launch_cols<T>(N, +1, spec, ym, tw, st);
+                launch_cols<T>(N, +1, nyq, yn, tw, st);
+            }
+            break;
+        case 4:
+            launch_rows_c2r<T>(N, spec, nyq, phi, nrows, s, tw, st);
+            break;
+        default:
+            throw Error(-1, "bad Poisson stage");
+    }
+}
+
+template <typename T>
+void FFTPlan<T>::poisson(const T* rho, T* phi, cx<T>* spec, cx<T>* nyq, T shift,
+                         const double* mean_ptr, double kf, double s, hipStream_t st) const {
+    for (int stage = 0; stage < 5; ++stage) poisson_stage(stage, rho, phi, spec, nyq, shift, mean_ptr, kf, s, st);
+}
+
+template struct FFTPlan<float>;
+template struct FFTPlan<double>;
+
+}  // namespace cpm

@@ -1,0 +1,521 @@
+// pm_particles.hip -- per-particle kernels of the PM step on gfx950:
+// cell keys, the deterministic CIC deposit, the fused FD-gradient gather +
+// kick (+ drift), the standalone drift, the Zel'dovich start, and the FP64
+// reductions.
+//
+// Two forms share the kernels through a position policy:
+//   F64: positions are IEEE doubles, unwrapped, exactly as the reference keeps
+//        them (src/run/nbody.hh:65); every per-particle operation repeats the
+//        reference's arithmetic in its order, so keys, weights, the gather,
+//        the kick and the drift are bit-identical to the reference for equal
+//        inputs.
+//   FX:  positions are unsigned 32-bit fractions of the box (x = q * L / 2^32),
+//        so the periodic wrap is exact integer overflow; mesh and gather in
+//        FP32, momenta and reductions in FP64.
+// Particle arrays are SoA: component k of particle i at [k * np + i].
+#include "pm_particles.hpp"
+
+namespace cpm {
+
+// ------------------------------------------------------------ keys
+template <typename PP, int R>
+__global__ __launch_bounds__(256) void k_keys(const typename PP::Pos* __restrict__ X, long np, int lgN,
+                                               PosArgs pa, uint32_t* __restrict__ keys,
+                                               uint32_t* __restrict__ pids) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const int mask = (1 << lgN) - 1;
+    uint32_t key = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        int o;
+        typename PP::Mesh f;
+        PP::dep(X[k * np + i], pa, o, f);
+        key |= (uint32_t)(o & mask) << (k * lgN);  // Box::idx of the origin (box.hh:75-81)
+    }
+    keys[i] = key;
+    pids[i] = (uint32_t)i;
+}
+
+// row_start[r] = first sorted position whose key lies in mesh row r or later.
+__global__ __launch_bounds__(256) void k_row_bounds(const uint32_t* __restrict__ keys, long np, int lgN,
+                                                    long nrows, uint32_t* __restrict__ row_start) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > np) return;
+    const long rp = (s == 0) ? -1 : (long)(keys[s - 1] >> lgN);
+    const long rc = (s == np) ? nrows : (long)(keys[s] >> lgN);
+    for (long r = rp + 1; r <= rc; ++r) row_start[r] = (uint32_t)s;
+}
+
+// ------------------------------------------------------------ deposit
+// One workgroup per mesh row.  Cell c receives, from each origin cell
+// c - block[j], the weight m * prod_k A[bit_k(j)][k] of every particle there
+// (src/run/numeric.hh:124-137).  The 2^R per-origin lists are each sorted by
+// particle id, and they are merged by id, so each cell sums its contributions
+// in ascending particle order: the order of the reference's serial loop.
+static constexpr int DEP_NT = 256;
+
+template <typename PP, int R>
+__global__ __launch_bounds__(DEP_NT) void k_deposit(const uint32_t* __restrict__ row_start,
+                                                     const uint32_t* __restrict__ keys,
+                                                     const uint32_t* __restrict__ pids,
+                                                     const typename PP::Pos* __restrict__ X, long np,
+                                                     int lgN, PosArgs pa, typename PP::Mesh mass,
+                                                     typename PP::Mesh* __restrict__ mesh,
+                                                     double* __restrict__ row_partial) {
+    using M = typename PP::Mesh;
+    constexpr int NL = 1 << (R - 1);
+    constexpr int NC = 1 << R;
+    __shared__ uint32_t cs[NL][1025];
+    __shared__ double red[DEP_NT];
+    const int N = 1 << lgN, msk = N - 1;
+    const long row = blockIdx.x;
+    const int cy = (int)(row & msk);
+    const int cz = (R == 3) ? (int)(row >> lgN) : 0;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        const int by = l & 1, bz = (l >> 1) & 1;
+        long orow = (long)((cy - by) & msk);
+        if (R == 3) orow += (long)((cz - bz) & msk) << lgN;
+        const uint32_t s0 = row_start[orow], s1 = row_start[orow + 1];
+        for (uint32_t s = s0 + tid; s <= s1; s += DEP_NT) {
+            const int cp = (s == s0) ? -1 : (int)(keys[s - 1] & msk);
+            const int cc = (s == s1) ? N : (int)(keys[s] & msk);
+            for (int c = cp + 1; c <= cc; ++c) cs[l][c] = s;
+        }
+    }
+    __syncthreads();
+    double rsum = 0;
+    for (int cx = tid; cx < N; cx += DEP_NT) {
+        uint32_t cur[NC], end[NC], hp[NC];
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+            const int ox = (cx - (j & 1)) & msk;
+            cur[j] = cs[j >> 1][ox];
+            end[j] = cs[j >> 1][ox + 1];
+            hp[j] = cur[j] < end[j] ? pids[cur[j]] : 0xFFFFFFFFu;
+        }
+        M acc = 0;
+        while (true) {
+            uint32_t m = 0xFFFFFFFFu;
+            int jm = -1;
+#pragma unroll
+            for (int j = 0; j < NC; ++j)
+                if (hp[j] < m) {
+                    m = hp[j];
+                    jm = j;
+                }
+            if (jm < 0) break;
+            M z = mass;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                int o;
+                M f;
+                PP::dep(X[k * np + m], pa, o, f);
+                z *= ((jm >> k) & 1) ? f : (M)1 - f;
+            }
+            acc += z;
+#pragma unroll
+            for (int j = 0; j < NC; ++j)
+                if (j == jm) {
+                    ++cur[j];
+                    hp[j] = cur[j] < end[j] ? pids[cur[j]] : 0xFFFFFFFFu;
+                }
+        }
+        mesh[row * N + cx] = acc;
+        rsum += (double)acc;
+    }
+    red[tid] = rsum;
+    __syncthreads();
+    for (int off = DEP_NT / 2; off > 0; off >>= 1) {
+        if (tid < off) red[tid] += red[tid + off];
+        __syncthreads();
+    }
+    if (tid == 0) row_partial[row] = red[0];
+}
+
+// ------------------------------------------------------------ gather + kick (+ drift)
+// P -= grad(phi)(X) * ck with the trilinear interpolation of the 4-point FD
+// gradient at the 2^R surrounding nodes (src/run/nbody.hh:112-117,
+// src/run/numeric.hh:17-44,86-102); then, when cd_on, the next drift
+// X += P * cd (src/run/nbody.hh:126-130) on the updated momentum.
+template <typename T>
+__device__ __forceinline__ T load_mesh(const T* __restrict__ phi, long idx) {
+    return phi[idx];
+}
+
+template <typename PP, int R>
+__global__ __launch_bounds__(256) void k_gather_kick(typename PP::Pos* __restrict__ X, double* __restrict__ P,
+                                                      long np, const typename PP::Mesh* __restrict__ phi,
+                                                      int lgN, PosArgs pa, double ck, double cd, int cd_on,
+                                                      double* __restrict__ Fout) {
+    using M = typename PP::Mesh;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const int N = 1 << lgN, msk = N - 1;
+    int o[R];
+    M a1[R];
+    typename PP::Pos xs[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        xs[k] = X[k * np + i];
+        PP::gat(xs[k], pa, o[k], a1[k]);
+    }
+    // corner weights z_j = prod_k A[bit_k(j)][k] in the reference's order
+    constexpr int NC = 1 << R;
+    M z[NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+        M w = (M)1;
+#pragma unroll
+        for (int k = 0; k < R; ++k) w *= ((j >> k) & 1) ? a1[k] : (M)1 - a1[k];
+        z[j] = w;
+    }
+    // wrapped node coordinates o + t - 2, t = 0..5
+    int w0[6], w1[6], w2[6];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        w0[t] = (o[0] + t - 2) & msk;
+        w1[t] = (o[1] + t - 2) & msk;
+        w2[t] = (R == 3) ? ((o[2] + t - 2) & msk) : 0;
+    }
+    const long sN = N, sNN = (long)N * N;
+    M resv;
+    if constexpr (sizeof(M) == 8) resv = (M)pa.res;
+    else resv = (M)(1.0 / (12.0 * pa.res));
+    M F[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) F[k] = 0;
+    // axis 0: lines along x through the (y, z) corners
+#pragma unroll
+    for (int l = 0; l < NC / 2; ++l) {
+        const int by = l & 1, bz = (l >> 1) & 1;
+        const long base = w1[2 + by] * sN + (R == 3 ? w2[2 + bz] * sNN : 0);
+        M f[6];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) f[t] = load_mesh(phi, base + w0[t]);
+#pragma unroll
+        for (int bx = 0; bx < 2; ++bx) {
+            const int j = bx | (by << 1) | (bz << 2);
+            const M g = fd4(f[bx], f[bx + 1], f[bx + 3], f[bx + 4], resv);
+            F[0] += g * z[j];
+        }
+    }
+    // axes 1 and 2: gradients per corner first, then accumulated in corner
+    // order so that every component sums j = 0 .. 2^R-1 like the reference.
+    {
+        M g1[NC];
+#pragma unroll
+        for (int l = 0; l < NC / 2; ++l) {
+            const int bx = l & 1, bz = (l >> 1) & 1;
+            const long base = w0[2 + bx] + (R == 3 ? w2[2 + bz] * sNN : 0);
+            M f[6];
+#pragma unroll
+            for (int t = 0; t < 6; ++t) f[t] = load_mesh(phi, base + w1[t] * sN);
+#pragma unroll
+            for (int by = 0; by < 2; ++by) g1[bx | (by << 1) | (bz << 2)] = fd4(f[by], f[by + 1], f[by + 3], f[by + 4], resv);
+        }
+#pragma unroll
+        for (int j = 0; j < NC; ++j) F[1] += g1[j] * z[j];
+    }
+    if constexpr (R == 3) {
+        M g2[NC];
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            const int bx = l & 1, by = (l >> 1) & 1;
+            const long base = w0[2 + bx] + w1[2 + by] * sN;
+            M f[6];
+#pragma unroll
+            for (int t = 0; t < 6; ++t) f[t] = load_mesh(phi, base + w2[t] * sNN);
+#pragma unroll
+            for (int bz = 0; bz < 2; ++bz) g2[bx | (by << 1) | (bz << 2)] = fd4(f[bz], f[bz + 1], f[bz + 3], f[bz + 4], resv);
+        }
+#pragma unroll
+        for (int j = 0; j < NC; ++j) F[2] += g2[j] * z[j];
+    }
+    if (Fout) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) Fout[k * np + i] = (double)F[k];
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const double p = P[k * np + i] - (double)F[k] * ck;
+        P[k * np + i] = p;
+        if (cd_on) {
+            if constexpr (sizeof(M) == 8) {
+                X[k * np + i] = xs[k] + p * cd;
+            } else {
+                X[k * np + i] = xs[k] + (uint32_t)__double2ll_rn(p * cd);
+            }
+        }
+    }
+}
+
+// X += P * cd (src/run/nbody.hh:122-133); FX: cd is in units of L / 2^32.
+template <typename PP>
+__global__ __launch_bounds__(256) void k_drift(typename PP::Pos* __restrict__ X, const double* __restrict__ P,
+                                                long n, double cd) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if constexpr (sizeof(typename PP::Mesh) == 8) {
+        X[i] = X[i] + P[i] * cd;
+    } else {
+        X[i] = X[i] + (uint32_t)__double2ll_rn(P[i] * cd);
+    }
+}
+
+// ------------------------------------------------------------ Zel'dovich start
+// u = FD gradient of phi on the mass grid (src/run/numeric.hh:48-70);
+// X = q * (L/N) - u * a_pos, P = u * -a_mom (src/run/nbody.hh:40-50).
+// FX stores x / L as a 32-bit fraction of the box.
+template <typename PP, int R>
+__global__ __launch_bounds__(256) void k_zeldovich(const double* __restrict__ phi, int lgN, double L,
+                                                    double a_pos, double a_mom,
+                                                    typename PP::Pos* __restrict__ X, double* __restrict__ P) {
+    const int N = 1 << lgN, msk = N - 1;
+    long np = 1;
+#pragma unroll
+    for (int k = 0; k < R; ++k) np <<= lgN;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const double res = L / N;
+    int q[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) q[k] = (int)((i >> (k * lgN)) & msk);
+    long stride[3] = {1, (long)N, (long)N * N};
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        long base = i - (long)q[k] * stride[k];
+        double f[4];
+        const int off[4] = {-2, -1, 1, 2};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) f[s] = phi[base + (long)((q[k] + off[s]) & msk) * stride[k]];
+        const double u = fd4(f[0], f[1], f[2], f[3], res);
+        const double G = q[k] * (L / N);
+        const double x = G - u * a_pos;
+        P[k * np + i] = u * -a_mom;
+        if constexpr (sizeof(typename PP::Mesh) == 8) {
+            X[k * np + i] = x;
+        } else {
+            double t = x / L;
+            t = t - floor(t);
+            X[k * np + i] = (uint32_t)(unsigned long long)(t * 4294967296.0);
+        }
+    }
+}
+
+// ------------------------------------------------------------ FP64 reductions
+static constexpr int RED_NT = 256;
+
+template <typename T>
+__global__ __launch_bounds__(RED_NT) void k_partial_sum(const T* __restrict__ a, long n, long chunk,
+                                                         int square, double* __restrict__ partials) {
+    __shared__ double red[RED_NT];
+    const long b0 = (long)blockIdx.x * chunk;
+    const long b1 = b0 + chunk < n ? b0 + chunk : n;
+    double s = 0;
+    for (long i = b0 + threadIdx.x; i < b1; i += RED_NT) {
+        const double v = (double)a[i];
+        s += square ? v * v : v;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = RED_NT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(RED_NT) void k_final_sum(const double* __restrict__ partials, long n,
+                                                      double scale, double* __restrict__ out) {
+    __shared__ double red[RED_NT];
+    double s = 0;
+    for (long i = threadIdx.x; i < n; i += RED_NT) s += partials[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = RED_NT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0] * scale;
+}
+
+// FX positions -> doubles in [0, L)
+__global__ __launch_bounds__(256) void k_fx_to_f64(const uint32_t* __restrict__ q, long n, double L,
+                                                   double* __restrict__ x) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = (double)q[i] * (L / 4294967296.0);
+}
+__global__ __launch_bounds__(256) void k_f64_to_fx(const double* __restrict__ x, long n, double L,
+                                                   uint32_t* __restrict__ q) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double t = x[i] / L;
+    t = t - floor(t);
+    q[i] = (uint32_t)(unsigned long long)(t * 4294967296.0);
+}
+
+// ------------------------------------------------------------ conan cic -s
+// Sub-sampled Lagrangian grid of cic_run<R> (src/cic/cic.hh:64-77): the
+// displacement D = X - q L/Nm on the mass grid is interpolated at the points
+// y of an (s Nm)^R grid (LinearInterpolation, numeric.hh:17-44; y >= 0 so no
+// truncation issue), giving positions y + D(y).
+template <int R>
+__global__ __launch_bounds__(256) void k_subsample(const double* __restrict__ X, long npm, int Nm, int Ns,
+                                                    double L, double* __restrict__ Y, long nps) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nps) return;
+    const double res_m = L / Nm;
+    int qs[R];
+    long t = i;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        qs[k] = (int)(t % Ns);
+        t /= Ns;
+    }
+    double y[R], a1[R];
+    int o[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        y[k] = qs[k] * (L / Ns);
+        const double x = y[k] / res_m;
+        o[k] = (int)(x);
+        a1[k] = x - o[k];
+    }
+    double v[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = 0;
+#pragma unroll
+    for (int j = 0; j < (1 << R); ++j) {
+        double z = 1;
+        long idx = 0, stride = 1;
+        int q[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int b = (j >> k) & 1;
+            z *= b ? a1[k] : 1 - a1[k];
+            q[k] = (o[k] + b) % Nm;
+            idx += (long)q[k] * stride;
+            stride *= Nm;
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const double d = X[k * npm + idx] - q[k] * (L / Nm);
+            v[k] += d * z;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) Y[k * nps + i] = y[k] + v[k];
+}
+
+void subsample(int R, const double* X, int Nm, int s, double L, double* Y, hipStream_t st) {
+    long npm = 1, nps = 1;
+    for (int k = 0; k < R; ++k) {
+        npm *= Nm;
+        nps *= (long)Nm * s;
+    }
+    const unsigned nb = (unsigned)((nps + 255) / 256);
+    if (R == 3)
+        hipLaunchKernelGGL(k_subsample<3>, dim3(nb), dim3(256), 0, st, X, npm, Nm, Nm * s, L, Y, nps);
+    else if (R == 2)
+        hipLaunchKernelGGL(k_subsample<2>, dim3(nb), dim3(256), 0, st, X, npm, Nm, Nm * s, L, Y, nps);
+    else
+        throw Error(-2, "only 2-D and 3-D meshes are supported");
+    CPM_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------ launchers
+static inline unsigned nblk(long n, int nt) { return (unsigned)((n + nt - 1) / nt); }
+
+#define CPM_R_DISPATCH(R, ...)                          \
+    do {                                                \
+        if ((R) == 3) { constexpr int RR = 3; __VA_ARGS__; } \
+        else if ((R) == 2) { constexpr int RR = 2; __VA_ARGS__; } \
+        else throw Error(-2, "only 2-D and 3-D runs are supported"); \
+    } while (0)
+
+template <typename PP>
+void ParticleOps<PP>::keys(int R, const Pos* X, long np, int lgN, const PosArgs& pa, uint32_t* keys,
+                           uint32_t* pids, hipStream_t st) {
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_keys<PP, RR>), dim3(nblk(np, 256)), dim3(256), 0, st, X, np,
+                                         lgN, pa, keys, pids));
+    CPM_LAUNCH_CHECK();
+}
+
+void row_bounds(const uint32_t* keys, long np, int lgN, long nrows, uint32_t* row_start, hipStream_t st) {
+    hipLaunchKernelGGL(k_row_bounds, dim3(nblk(np + 1, 256)), dim3(256), 0, st, keys, np, lgN, nrows,
+                       row_start);
+    CPM_LAUNCH_CHECK();
+}
+
+template <typename PP>
+void ParticleOps<PP>::deposit(int R, const uint32_t* row_start, const uint32_t* keys, const uint32_t* pids,
+                              const Pos* X, long np, int lgN, const PosArgs& pa, Mesh mass, Mesh* mesh,
+                              double* row_partial, long nrows, hipStream_t st) {
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit<PP, RR>), dim3((unsigned)nrows), dim3(DEP_NT), 0, st,
+                                         row_start, keys, pids, X, np, lgN, pa, mass, mesh, row_partial));
+    CPM_LAUNCH_CHECK();
+}
+
+template <typename PP>
+void ParticleOps<PP>::gather_kick(int R, Pos* X, double* P, long np, const Mesh* phi, int lgN,
+                                  const PosArgs& pa, double ck, double cd, int cd_on, double* Fout,
+                                  hipStream_t st) {
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_kick<PP, RR>), dim3(nblk(np, 256)), dim3(256), 0, st, X,
+                                         P, np, phi, lgN, pa, ck, cd, cd_on, Fout));
+    CPM_LAUNCH_CHECK();
+}
+
+template <typename PP>
+void ParticleOps<PP>::drift(Pos* X, const double* P, long n, double cd, hipStream_t st) {
+    hipLaunchKernelGGL((k_drift<PP>), dim3(nblk(n, 256)), dim3(256), 0, st, X, P, n, cd);
+    CPM_LAUNCH_CHECK();
+}
+
+template <typename PP>
+void ParticleOps<PP>::zeldovich(int R, const double* phi, int lgNm, double L, double a_pos, double a_mom,
+                                Pos* X, double* P, hipStream_t st) {
+    const long np = 1L << (R * lgNm);
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_zeldovich<PP, RR>), dim3(nblk(np, 256)), dim3(256), 0, st, phi,
+                                         lgNm, L, a_pos, a_mom, X, P));
+    CPM_LAUNCH_CHECK();
+}
+
+template <typename T>
+void reduce_sum(const T* a, long n, int square, double scale, double* partials, long max_partials,
+                double* out, hipStream_t st) {
+    long nb = max_partials < 1 ? 1 : max_partials;
+    long chunk = (n + nb - 1) / nb;
+    if (chunk < RED_NT) chunk = RED_NT;
+    nb = (n + chunk - 1) / chunk;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL((k_partial_sum<T>), dim3((unsigned)nb), dim3(RED_NT), 0, st, a, n, chunk, square,
+                       partials);
+    CPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_NT), 0, st, (const double*)partials, nb, scale, out);
+    CPM_LAUNCH_CHECK();
+}
+
+void sum_partials(const double* partials, long n, double scale, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_NT), 0, st, partials, n, scale, out);
+    CPM_LAUNCH_CHECK();
+}
+
+void fx_to_f64(const uint32_t* q, long n, double L, double* x, hipStream_t st) {
+    hipLaunchKernelGGL(k_fx_to_f64, dim3(nblk(n, 256)), dim3(256), 0, st, q, n, L, x);
+    CPM_LAUNCH_CHECK();
+}
+void f64_to_fx(const double* x, long n, double L, uint32_t* q, hipStream_t st) {
+    hipLaunchKernelGGL(k_f64_to_fx, dim3(nblk(n, 256)), dim3(256), 0, st, x, n, L, q);
+    CPM_LAUNCH_CHECK();
+}
+
+template struct ParticleOps<PosF64>;
+template struct ParticleOps<PosFX>;
+template void reduce_sum<float>(const float*, long, int, double, double*, long, double*, hipStream_t);
+template void reduce_sum<double>(const double*, long, int, double, double*, long, double*, hipStream_t);
+
+}  // namespace cpm

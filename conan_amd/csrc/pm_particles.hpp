@@ -1,0 +1,89 @@
+// pm_particles.hpp -- particle kernels of the PM step: host interface and the
+// two position forms.
+#pragma once
+#include "pm_common.hpp"
+
+namespace cpm {
+
+// Parameters of the position -> (cell, fraction) maps.
+struct PosArgs {
+    double L, res;   // box size and mesh spacing (F64)
+    int wrap;        // F64 gather: 0 = truncate the unwrapped coordinate (reference), 1 = fold first
+    int fshift;      // FX: 32 - log2 N
+    uint32_t fmask;  // FX: (1 << fshift) - 1
+    float fscale;    // FX: 2^-fshift
+};
+
+// src/base/misc.hh:144-148
+__device__ __forceinline__ double modulus_d(double a, double b) {
+    if (a < 0) return a - (double)(int)(a / b - 1) * b;
+    return a - (double)(int)(a / b) * b;
+}
+
+// The 4-point derivative of the reference (src/run/numeric.hh:98,101):
+// ((f[-2]/12 - 2 f[-1]/3) + 2 f[+1]/3) - f[+2]/12, then / res.
+__device__ __forceinline__ double fd4(double fm2, double fm1, double fp1, double fp2, double res) {
+    double v = fm2 / 12. - 2 * fm1 / 3. + 2 * fp1 / 3. - fp2 / 12.;
+    return v / res;
+}
+// Same operator in FP32, arranged for fewer roundings: (f[-2] - f[+2] + 8 (f[+1] - f[-1])) / (12 res).
+__device__ __forceinline__ float fd4(float fm2, float fm1, float fp1, float fp2, float inv12res) {
+    return ((fm2 - fp2) + 8.f * (fp1 - fm1)) * inv12res;
+}
+
+// ------------------------------------------------------------ position policies
+struct PosF64 {
+    using Pos = double;
+    using Mesh = double;
+    // deposit coordinate: wrapped into [0, L) first (src/run/numeric.hh:114-122)
+    static __device__ __forceinline__ void dep(Pos x, const PosArgs& a, int& o, double& f) {
+        double p0 = modulus_d(x, a.L) / a.res;
+        o = (int)(p0);
+        f = p0 - o;
+    }
+    // gather coordinate: the reference truncates the unwrapped coordinate
+    // (src/run/numeric.hh:19,26-28); wrap != 0 folds it like the deposit.
+    static __device__ __forceinline__ void gat(Pos x, const PosArgs& a, int& o, double& f) {
+        double p = (a.wrap ? modulus_d(x, a.L) : x) / a.res;
+        o = (int)(p);
+        f = p - o;
+    }
+};
+
+struct PosFX {
+    using Pos = uint32_t;
+    using Mesh = float;
+    static __device__ __forceinline__ void dep(Pos x, const PosArgs& a, int& o, float& f) {
+        o = (int)(x >> a.fshift);
+        f = (float)(x & a.fmask) * a.fscale;
+    }
+    static __device__ __forceinline__ void gat(Pos x, const PosArgs& a, int& o, float& f) { dep(x, a, o, f); }
+};
+
+
+template <typename PP>
+struct ParticleOps {
+    using Pos = typename PP::Pos;
+    using Mesh = typename PP::Mesh;
+    static void keys(int R, const Pos* X, long np, int lgN, const PosArgs& pa, uint32_t* keys, uint32_t* pids,
+                     hipStream_t st);
+    static void deposit(int R, const uint32_t* row_start, const uint32_t* keys, const uint32_t* pids,
+                        const Pos* X, long np, int lgN, const PosArgs& pa, Mesh mass, Mesh* mesh,
+                        double* row_partial, long nrows, hipStream_t st);
+    static void gather_kick(int R, Pos* X, double* P, long np, const Mesh* phi, int lgN, const PosArgs& pa,
+                            double ck, double cd, int cd_on, double* Fout, hipStream_t st);
+    static void drift(Pos* X, const double* P, long n, double cd, hipStream_t st);
+    static void zeldovich(int R, const double* phi, int lgNm, double L, double a_pos, double a_mom, Pos* X,
+                          double* P, hipStream_t st);
+};
+
+void row_bounds(const uint32_t* keys, long np, int lgN, long nrows, uint32_t* row_start, hipStream_t st);
+template <typename T>
+void reduce_sum(const T* a, long n, int square, double scale, double* partials, long max_partials,
+                double* out, hipStream_t st);
+void sum_partials(const double* partials, long n, double scale, double* out, hipStream_t st);
+void subsample(int R, const double* X, int Nm, int s, double L, double* Y, hipStream_t st);
+void fx_to_f64(const uint32_t* q, long n, double L, double* x, hipStream_t st);
+void f64_to_fx(const double* x, long n, double L, uint32_t* q, hipStream_t st);
+
+}  // namespace cpm

@@ -1,0 +1,210 @@
+// pm_sort.hip -- stable LSD radix sort of (cell key, particle id) pairs and a
+// device-wide exclusive scan, both deterministic (no floating point, no
+// order-dependent results).
+//
+// The deposit needs particles grouped by cell with ascending particle id
+// inside each cell; sorting (key, id) pairs that start in id order with a
+// stable LSD sort gives exactly that.
+#include "pm_sort.hpp"
+
+namespace cpm {
+
+static constexpr int SORT_NT = 256;
+static constexpr int SORT_ITEMS = 16;
+static constexpr int SORT_TILE = SORT_NT * SORT_ITEMS;
+static constexpr int SORT_WAVES = SORT_NT / 64;
+static constexpr int MAX_RADIX = 512;
+
+__global__ __launch_bounds__(SORT_NT) void k_rs_hist(const uint32_t* __restrict__ keys, long n, int shift,
+                                                     int dbits, uint32_t* __restrict__ counts, int nblocks) {
+    __shared__ uint32_t hist[MAX_RADIX];
+    const int radix = 1 << dbits;
+    const uint32_t dmask = (uint32_t)radix - 1;
+    for (int d = threadIdx.x; d < radix; d += SORT_NT) hist[d] = 0;
+    __syncthreads();
+    const long tile0 = (long)blockIdx.x * SORT_TILE;
+#pragma unroll 4
+    for (int r = 0; r < SORT_ITEMS; ++r) {
+        long e = tile0 + (long)r * SORT_NT + threadIdx.x;
+        if (e < n) atomicAdd(&hist[(keys[e] >> shift) & dmask], 1u);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < radix; d += SORT_NT) counts[(long)d * nblocks + blockIdx.x] = hist[d];
+}
+
+// Each tile is walked in index order (round r, then thread); the rank of an
+// element among equal digits = count in earlier rounds + count in earlier
+// waves of this round + count in earlier lanes of its wave.
+__global__ __launch_bounds__(SORT_NT) void k_rs_scatter(const uint32_t* __restrict__ kin,
+                                                        const uint32_t* __restrict__ vin,
+                                                        uint32_t* __restrict__ kout,
+                                                        uint32_t* __restrict__ vout, long n, int shift,
+                                                        int dbits, const uint32_t* __restrict__ offsets,
+                                                        int nblocks) {
+    __shared__ uint32_t run[MAX_RADIX];
+    __shared__ uint32_t wcnt[SORT_WAVES][MAX_RADIX];
+    const int radix = 1 << dbits;
+    const uint32_t dmask = (uint32_t)radix - 1;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int d = tid; d < radix; d += SORT_NT) {
+        run[d] = offsets[(long)d * nblocks + blockIdx.x];
+#pragma unroll
+        for (int w = 0; w < SORT_WAVES; ++w) wcnt[w][d] = 0;
+    }
+    __syncthreads();
+    const long tile0 = (long)blockIdx.x * SORT_TILE;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int r = 0; r < SORT_ITEMS; ++r) {
+        const long e = tile0 + (long)r * SORT_NT + tid;
+        const bool valid = e < n;
+        const uint32_t key = valid ? kin[e] : 0u;
+        const uint32_t d = (key >> shift) & dmask;
+        uint64_t mask = __ballot(valid);
+        for (int b = 0; b < dbits; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
+            mask &= bit ? bb : ~bb;
+        }
+        const int rank = __popcll(mask & below);
+        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(mask);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[d];
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+            pos += (uint32_t)rank;
+            kout[pos] = key;
+            vout[pos] = vin[e];
+        }
+        __syncthreads();
+        for (int dd = tid; dd < radix; dd += SORT_NT) {
+            uint32_t s = 0;
+#pragma unroll
+            for (int w = 0; w < SORT_WAVES; ++w) {
+                s += wcnt[w][dd];
+                wcnt[w][dd] = 0;
+            }
+            run[dd] += s;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ scan
+static constexpr int SCAN_NT = 256;
+static constexpr int SCAN_ITEMS = 16;
+static constexpr int SCAN_TILE = SCAN_NT * SCAN_ITEMS;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
+    // sh: SCAN_NT entries
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < SCAN_NT; off <<= 1) {
+        uint32_t t = tid >= off ? sh[tid - off] : 0u;
+        __syncthreads();
+        sh[tid] += t;
+        __syncthreads();
+    }
+    uint32_t incl = sh[tid];
+    *total = sh[SCAN_NT - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+// Exclusive scan of each tile; tile totals to sums (if non-null).
+__global__ __launch_bounds__(SCAN_NT) void k_scan_tiles(const uint32_t* __restrict__ in,
+                                                        uint32_t* __restrict__ out, long n,
+                                                        uint32_t* __restrict__ sums) {
+    __shared__ uint32_t sh[SCAN_NT];
+    const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t loc = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        v[i] = (base + i < n) ? in[base + i] : 0u;
+        loc += v[i];
+    }
+    uint32_t tot;
+    uint32_t pre = block_exclusive_scan(loc, sh, &tot);
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        if (base + i < n) out[base + i] = pre;
+        pre += v[i];
+    }
+    if (sums && threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_NT) void k_scan_add(uint32_t* __restrict__ out, long n,
+                                                      const uint32_t* __restrict__ sums) {
+    const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
+    const uint32_t add = sums[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+        if (base + i < n) out[base + i] += add;
+}
+
+size_t scan_scratch_elems(long n) {
+    size_t tot = 0;
+    long m = n;
+    while (m > SCAN_TILE) {
+        m = (m + SCAN_TILE - 1) / SCAN_TILE;
+        tot += (size_t)m + (size_t)m;  // sums + scanned sums
+    }
+    return tot + 1;
+}
+
+void exclusive_scan(const uint32_t* in, uint32_t* out, long n, uint32_t* scratch, hipStream_t st) {
+    const long nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nt <= 1) {
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_NT), 0, st, in, out, n, (uint32_t*)nullptr);
+        CPM_LAUNCH_CHECK();
+        return;
+    }
+    uint32_t* sums = scratch;
+    uint32_t* ssc = scratch + nt;
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)nt), dim3(SCAN_NT), 0, st, in, out, n, sums);
+    CPM_LAUNCH_CHECK();
+    exclusive_scan(sums, ssc, nt, scratch + 2 * nt, st);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nt), dim3(SCAN_NT), 0, st, out, n, (const uint32_t*)ssc);
+    CPM_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------ sort driver
+RadixSorter::RadixSorter(long n_, int keybits_) : n(n_), keybits(keybits_) {
+    passes = (keybits + 8) / 9;
+    if (passes < 1) passes = 1;
+    dbits = (keybits + passes - 1) / passes;
+    nblocks = (int)((n + SORT_TILE - 1) / SORT_TILE);
+    if (nblocks < 1) nblocks = 1;
+    ncounts = (long)(1 << dbits) * nblocks;
+    CPM_HIP(hipMalloc(&counts, sizeof(uint32_t) * (ncounts + 1)));
+    CPM_HIP(hipMalloc(&scratch, sizeof(uint32_t) * scan_scratch_elems(ncounts)));
+}
+
+RadixSorter::~RadixSorter() {
+    if (counts) (void)hipFree(counts);
+    if (scratch) (void)hipFree(scratch);
+}
+
+int RadixSorter::sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, hipStream_t st) {
+    uint32_t *ki = k0, *vi = v0, *ko = k1, *vo = v1;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = p * dbits;
+        int db = dbits;
+        if (shift + db > keybits) db = keybits - shift;
+        if (db <= 0) break;
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(SORT_NT), 0, st, (const uint32_t*)ki, n, shift, db,
+                           counts, nblocks);
+        CPM_LAUNCH_CHECK();
+        const long nc = (long)(1 << db) * nblocks;
+        exclusive_scan(counts, counts, nc, scratch, st);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(SORT_NT), 0, st, (const uint32_t*)ki,
+                           (const uint32_t*)vi, ko, vo, n, shift, db, (const uint32_t*)counts, nblocks);
+        CPM_LAUNCH_CHECK();
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+    }
+    return ki == k0 ? 0 : 1;
+}
+
+}  // namespace cpm

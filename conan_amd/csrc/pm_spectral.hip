@@ -1,0 +1,210 @@
+// pm_spectral.hip -- element-wise k-space filters of the initial conditions and
+// the binned power-spectrum estimator, on the half spectrum of pm_fft.hip.
+//
+// Filters (src/base/fourier.hh:59-128, applied by src/ic/ic.cc:64-115):
+//   PS    sqrt(P(|k|)) * S(k), P = |k|^ns, S = prod_i exp(t (cos k_i - 1)), k in grid units
+//   P     sqrt(P(|k|))
+//   POT   -1/|k|^2, k physical
+//   DERIV i sin(k_axis), k in grid units
+//   CDM   sqrt(A k^ns T0(k)^2), the BBKS-like transfer function of CDM<R>
+//         (ic.cc:14-35), k physical
+// The first three zero the k = 0 mode as the reference does after filtering
+// (`fft.in[0] = 0`, ic.cc:68,78,113).
+#include "pm_spectral.hpp"
+
+namespace cpm {
+
+struct ModeIdx {
+    int ix, iy, iz;
+};
+
+// Element e of the half spectrum: main block first, then the Nyquist plane.
+__device__ __forceinline__ ModeIdx mode_of(long e, long nmain, int dim, int lgN) {
+    const int N = 1 << lgN, H = N / 2, msk = N - 1;
+    ModeIdx m;
+    if (e < nmain) {
+        m.ix = (int)(e & (H - 1));
+        const long r = e >> (lgN - 1);
+        m.iy = (int)(r & msk);
+        m.iz = dim == 3 ? (int)(r >> lgN) : 0;
+    } else {
+        const long r = e - nmain;
+        m.ix = H;
+        m.iy = (int)(r & msk);
+        m.iz = dim == 3 ? (int)(r >> lgN) : 0;
+    }
+    return m;
+}
+
+__device__ __forceinline__ double kcomp(int n, int N, double kscale) {
+    return (double)(n > N / 2 ? n - N : n) * kscale;
+}
+
+__global__ __launch_bounds__(256) void k_filter(cx<double>* __restrict__ spec, cx<double>* __restrict__ nyq,
+                                                long nmain, long nnyq, FilterArgs fa) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nmain + nnyq) return;
+    const int N = 1 << fa.lgN;
+    const ModeIdx m = mode_of(e, nmain, fa.dim, fa.lgN);
+    double K[3];
+    K[0] = kcomp(m.ix, N, fa.kscale);
+    K[1] = kcomp(m.iy, N, fa.kscale);
+    K[2] = fa.dim == 3 ? kcomp(m.iz, N, fa.kscale) : 0.0;
+    cx<double>* p = e < nmain ? spec + e : nyq + (e - nmain);
+    cx<double> z = *p;
+    const bool dc = (m.ix == 0 && m.iy == 0 && m.iz == 0);
+    if (fa.kind == FILT_DERIV) {
+        const double s = sin(K[fa.axis]);
+        *p = mk<double>(-z.im * s, z.re * s);
+        return;
+    }
+    if (dc) {
+        *p = mk<double>(0, 0);
+        return;
+    }
+    double k2 = 0;
+    for (int k = 0; k < fa.dim; ++k) k2 += K[k] * K[k];
+    double f;
+    if (fa.kind == FILT_POT) {
+        f = -1. / k2;
+    } else if (fa.kind == FILT_CDM) {
+        const double e = exp(1.), Theta = 2.7255 / 2.7, A = 1122670;
+        const double k = sqrt(k2);
+        const double q = k * pow(Theta, 2) / (fa.omega0 * fa.h);
+        const double L0 = log(2 * e + 1.8 * q);
+        const double C0 = 14.2 + 731.0 / (1 + 62.5 * q);
+        const double T0 = L0 / (L0 + C0 * pow(q, 2));
+        f = sqrt(A * pow(k, fa.ns) * pow(T0, 2));
+    } else {
+        f = sqrt(pow(sqrt(k2), fa.ns));
+        if (fa.kind == FILT_PS) {
+            double v = 1.0;
+            for (int k = 0; k < fa.dim; ++k) v *= exp(fa.t * (cos(K[k]) - 1));
+            f = f * v;
+        }
+    }
+    *p = mk<double>(z.re * f, z.im * f);
+}
+
+void apply_filter(cx<double>* spec, cx<double>* nyq, long nmain, long nnyq, const FilterArgs& fa,
+                  hipStream_t st) {
+    const long n = nmain + nnyq;
+    hipLaunchKernelGGL(k_filter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, spec, nyq, nmain, nnyq,
+                       fa);
+    CPM_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------ power spectrum
+// Each block stages PK_CH modes (bin, weighted |delta_k|^2, weighted |k|,
+// weight) in LDS; thread b then sums bin b over the chunk in index order.
+// Per-block partials are summed over blocks in block order: the result does
+// not depend on scheduling.  Interior modes 0 < kx < N/2 stand for k and -k
+// (weight 2); the kx = 0 and kx = N/2 planes are complete (weight 1).
+static constexpr int PK_NT = 256;
+static constexpr int PK_CH = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(PK_NT) void k_pk_partial(const cx<T>* __restrict__ spec, const cx<T>* __restrict__ nyq,
+                                                      long nmain, long nnyq, int dim, int lgN, double kf,
+                                                      const double* __restrict__ edges, int nbins,
+                                                      double* __restrict__ part) {
+    __shared__ short sb[PK_CH];
+    __shared__ double sp[PK_CH];
+    __shared__ double sk[PK_CH];
+    __shared__ float sw[PK_CH];
+    const int N = 1 << lgN;
+    const long n = nmain + nnyq;
+    const long e0 = (long)blockIdx.x * PK_CH;
+    for (int t = threadIdx.x; t < PK_CH; t += PK_NT) {
+        const long e = e0 + t;
+        short bin = -1;
+        double p2 = 0, kk = 0;
+        float w = 0;
+        if (e < n) {
+            const ModeIdx m = mode_of(e, nmain, dim, lgN);
+            double K[3];
+            K[0] = kcomp(m.ix, N, kf);
+            K[1] = kcomp(m.iy, N, kf);
+            K[2] = dim == 3 ? kcomp(m.iz, N, kf) : 0.0;
+            double k2 = 0;
+            for (int k = 0; k < dim; ++k) k2 += K[k] * K[k];
+            const double k = sqrt(k2);
+            const bool dc = (m.ix == 0 && m.iy == 0 && m.iz == 0);
+            if (!dc && k >= edges[0] && k <= edges[nbins]) {
+                int lo = 0, hi = nbins - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (edges[mid] <= k) lo = mid;
+                    else hi = mid - 1;
+                }
+                bin = (short)lo;
+                const cx<T> z = e < nmain ? spec[e] : nyq[e - nmain];
+                w = (m.ix == 0 || m.ix == N / 2) ? 1.f : 2.f;
+                const double zr = (double)z.re, zi = (double)z.im;
+                p2 = (double)w * (zr * zr + zi * zi);
+                kk = (double)w * k;
+            }
+        }
+        sb[t] = bin;
+        sp[t] = p2;
+        sk[t] = kk;
+        sw[t] = w;
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbins; b += PK_NT) {
+        double P = 0, Kk = 0, W = 0;
+        for (int t = 0; t < PK_CH; ++t)
+            if (sb[t] == b) {
+                P += sp[t];
+                Kk += sk[t];
+                W += (double)sw[t];
+            }
+        double* o = part + ((long)blockIdx.x * nbins + b) * 3;
+        o[0] = P;
+        o[1] = Kk;
+        o[2] = W;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_pk_final(const double* __restrict__ part, long nblocks, int nbins,
+                                                 double norm, double* __restrict__ kmean, double* __restrict__ pk,
+                                                 double* __restrict__ cnt) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbins) return;
+    double P = 0, K = 0, W = 0;
+    for (long i = 0; i < nblocks; ++i) {
+        const double* o = part + (i * nbins + b) * 3;
+        P += o[0];
+        K += o[1];
+        W += o[2];
+    }
+    cnt[b] = W;
+    kmean[b] = W > 0 ? K / W : 0.0;
+    pk[b] = W > 0 ? P / W * norm : 0.0;
+}
+
+long pk_partial_elems(long nmain, long nnyq, int nbins) {
+    const long nb = (nmain + nnyq + PK_CH - 1) / PK_CH;
+    return nb * nbins * 3;
+}
+
+template <typename T>
+void power_spectrum(const cx<T>* spec, const cx<T>* nyq, long nmain, long nnyq, int dim, int lgN, double kf,
+                    const double* edges, int nbins, double norm, double* part, double* kmean, double* pk,
+                    double* cnt, hipStream_t st) {
+    if (nbins > 32767) throw Error(-2, "too many power-spectrum bins");
+    const long nb = (nmain + nnyq + PK_CH - 1) / PK_CH;
+    hipLaunchKernelGGL((k_pk_partial<T>), dim3((unsigned)nb), dim3(PK_NT), 0, st, spec, nyq, nmain, nnyq, dim,
+                       lgN, kf, edges, nbins, part);
+    CPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_pk_final, dim3((unsigned)((nbins + 63) / 64)), dim3(64), 0, st, (const double*)part,
+                       nb, nbins, norm, kmean, pk, cnt);
+    CPM_LAUNCH_CHECK();
+}
+
+template void power_spectrum<float>(const cx<float>*, const cx<float>*, long, long, int, int, double,
+                                    const double*, int, double, double*, double*, double*, double*, hipStream_t);
+template void power_spectrum<double>(const cx<double>*, const cx<double>*, long, long, int, int, double,
+                                     const double*, int, double, double*, double*, double*, double*, hipStream_t);
+
+}  // namespace cpm

@@ -1,0 +1,30 @@
+// pm_spectral.hpp -- host interface of the k-space filters and the P(k) estimator.
+#pragma once
+#include "pm_common.hpp"
+
+namespace cpm {
+
+enum FilterKind { FILT_PS = 0, FILT_P = 1, FILT_POT = 2, FILT_DERIV = 3, FILT_CDM = 4 };
+
+struct FilterArgs {
+    int kind;
+    int dim;
+    int lgN;
+    double kscale;  // 2 pi / N (grid units) or 2 pi / L (physical)
+    double ns;      // power-law slope
+    double t;       // smoothing: scale * N / L
+    int axis;       // FILT_DERIV axis
+    double omega0;  // FILT_CDM: matter density
+    double h;       // FILT_CDM: H0 / 100
+};
+
+void apply_filter(cx<double>* spec, cx<double>* nyq, long nmain, long nnyq, const FilterArgs& fa,
+                  hipStream_t st);
+
+long pk_partial_elems(long nmain, long nnyq, int nbins);
+template <typename T>
+void power_spectrum(const cx<T>* spec, const cx<T>* nyq, long nmain, long nnyq, int dim, int lgN, double kf,
+                    const double* edges, int nbins, double norm, double* part, double* kmean, double* pk,
+                    double* cnt, hipStream_t st);
+
+}  // namespace cpm

@@ -1,0 +1,190 @@
+/* conanpm.h -- C ABI of the MI355X particle-mesh library (libconanpm.so).
+ *
+ * Drop-in boundary for conan's PM path.  The reference has no FFI: its seams
+ * are the in-process Solver interface driven by the leapfrog integrator
+ *   src/run/leapfrog.hh:6-21   Solver { init, kick, drift, save_snapshot }
+ * and the ic / run / cic command lines with their .conan files.  The entry
+ * points below replace the Solver implementation Gravity<R>
+ * (src/run/nbody.hh:53-219) and the numeric kernels it calls; the conan CLI
+ * of this package calls them.  Plain C types only: host pointers in, host
+ * pointers out (the library copies; it never keeps a caller's pointer).
+ *
+ * Conventions: particle arrays are AoS of dim doubles per particle in the
+ * reference's order (mass-grid index, axis 0 fastest: src/base/box.hh:51-53);
+ * meshes are N^dim doubles, axis 0 fastest.  Every call returns 0 on success
+ * and a negative code on failure; pm_last_error() then holds the message.
+ * A handle owns one HIP stream and its device buffers; calls on one handle are
+ * stream-ordered and must come from one host thread.
+ */
+#ifndef CONANPM_H
+#define CONANPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define PM_OK 0
+#define PM_EINVAL -1   /* bad argument or configuration */
+#define PM_EUNSUP -2   /* unsupported size or mode */
+#define PM_EDEVICE -3  /* HIP error (no device, allocation failure, kernel fault) */
+#define PM_ESTATE -4   /* call out of order (e.g. kick before init) */
+#define PM_ENONFINITE -5 /* non-finite values in an input */
+
+/* precision forms */
+#define PM_FP64 0 /* plain form: FP64 everywhere, reference arithmetic (the drop-in) */
+#define PM_FP32 1 /* mixed form: 32-bit fixed-point positions, FP32 mesh/FFT/gather, FP64 momenta */
+
+/* flags */
+#define PM_FLAG_WRAP_GATHER 1u /* FP64: fold positions into [0, L) before the gather, as the
+                                  deposit does; the reference truncates the unwrapped
+                                  coordinate (src/run/numeric.hh:19,26-28) */
+
+/* Configuration of a run: Gravity<R>(mbox, fbox, cosmos) (src/run/nbody.hh:75-86),
+ * Cosmology(H0, Omega_m, Omega_L) (src/run/cosmology.hh:11-14). */
+typedef struct pm_config {
+    int dim;         /* 2 or 3 */
+    int Nm;          /* mass grid per side: particles = Nm^dim */
+    int Nf;          /* force mesh per side (f * Nm in conan run) */
+    double L;        /* box size */
+    double H0;       /* Hubble parameter */
+    double Omega_m;
+    double Omega_L;
+    int precision;   /* PM_FP64 or PM_FP32 */
+    unsigned flags;  /* PM_FLAG_* */
+} pm_config;
+
+typedef struct pm_handle pm_handle;
+
+/* Message of the last failed call on h; h == NULL: of the calling thread's
+ * last failed handle-less call (pm_create, pm_ic_*). */
+const char* pm_last_error(const pm_handle* h);
+
+/* new Gravity<R> / ~Gravity (src/run/nbody.hh:75-86, 221-231) */
+int pm_create(const pm_config* cfg, pm_handle** out);
+void pm_destroy(pm_handle* h);
+
+/* Solver::init: Zel'dovich start from the potential on the mass grid
+ * (src/run/nbody.hh:88-95, 26-51).  phi: Nm^dim doubles.  LeapFrog passes
+ * a_pos = a0 and a_mom = a0 + da/2 (src/run/leapfrog.hh:35). */
+int pm_init_zeldovich(pm_handle* h, const double* phi, double a_pos, double a_mom);
+
+/* Solver::drift: X += P * da / (a^2 adot(a))  (src/run/nbody.hh:122-133) */
+int pm_drift(pm_handle* h, double a, double da);
+
+/* Solver::kick: deposit, Poisson solve at a, gather, P -= F * da / adot(a)
+ * (src/run/nbody.hh:97-120) */
+int pm_kick(pm_handle* h, double a, double da);
+
+/* n iterations of LeapFrog::run's loop body (src/run/leapfrog.hh:38-44)
+ * starting at scale factor a: drift(a, da); kick(a + da/2, da); a += da.
+ * Kick i and drift i+1 run as one fused kernel; results equal the unfused
+ * sequence bit for bit.  a_out (nullable) receives the final a. */
+int pm_steps(pm_handle* h, double a, double da, int n, double* a_out);
+
+/* State in the reference layout (Array<dVector<R>> X, P; src/run/nbody.hh:65).
+ * PM_FP32 returns positions folded into [0, L). */
+int pm_get_state(pm_handle* h, double* X, double* P);
+int pm_set_state(pm_handle* h, const double* X, const double* P);
+
+/* Scale factors of the last drift / kick (save_conan's a-pos / a-mom,
+ * src/run/nbody.hh:174-175). */
+int pm_get_times(const pm_handle* h, double* a_pos, double* a_mom);
+
+/* CIC deposit of the current positions with particle mass (Nf/Nm)^dim
+ * (md_cic, src/run/numeric.hh:106-139): rho (Nf^dim doubles, nullable),
+ * keys (Nm^dim cell indices idx(origin), nullable). */
+int pm_deposit(pm_handle* h, double* rho, int32_t* keys);
+
+/* Deposit + Poisson solve at scale factor a (src/run/nbody.hh:99-110):
+ * the potential phi (Nf^dim doubles). */
+int pm_potential(pm_handle* h, double a, double* phi);
+
+/* Deposit + Poisson + gather at a: the interpolated FD gradient F at every
+ * particle (LinearInterpolation<Gradient<R>>, src/run/numeric.hh:6-45,72-103),
+ * AoS Nm^dim x dim doubles.  State is left unchanged. */
+int pm_forces(pm_handle* h, double a, double* F);
+
+/* Gather of a given potential mesh phi (Nf^dim doubles) at the current
+ * positions: F as in pm_forces, with no deposit or Poisson solve.  Lets the
+ * interpolation be compared with the reference independent of FFT rounding. */
+int pm_gather_mesh(pm_handle* h, const double* phi, double* F);
+
+/* Poisson solve of a given density mesh (rho - 1 -> phi at a). */
+int pm_poisson_mesh(pm_handle* h, const double* rho, double a, double* phi);
+
+/* Power spectrum of delta = rho - 1 of the current deposit.  edges: nbins+1
+ * ascending |k| (physical, 2 pi / L units); NULL = nbins logarithmic bins
+ * from 2 pi / L to pi Nf / L.  Outputs per bin: mean |k|, P(k) =
+ * L^dim / Nf^(2 dim) <|FFT(delta)_k|^2>, and the mode count. */
+int pm_power_spectrum(pm_handle* h, int nbins, const double* edges, double* kmean, double* pk,
+                      double* count);
+
+/* FP64 reductions of the state and the last deposit. */
+typedef struct pm_diag {
+    double mass;          /* sum of the last deposit (= Nm^dim * (Nf/Nm)^dim) */
+    double momentum[3];   /* sum_i P_i */
+    double p2;            /* sum_i |P_i|^2 */
+} pm_diag;
+int pm_diagnostics(pm_handle* h, pm_diag* d);
+
+/* Per-phase device time (HIP events on the handle's stream), accumulated
+ * while enabled.  Phases: 0 keys+sort, 1 deposit, 2 FFT rows forward,
+ * 3 FFT y forward, 4 Green pass, 5 FFT y backward, 6 FFT rows backward,
+ * 7 gather+kick(+drift), 8 drift. */
+#define PM_NPHASES 9
+int pm_profile(pm_handle* h, int enable, double* ms, long long* counts);
+
+/* Synchronise the handle's stream. */
+int pm_sync(pm_handle* h);
+
+/* ---------------------------------------------------------------- conan ic
+ * _generate_random_field<R> scale-free branch + _compute_potential
+ * (src/ic/ic.cc:37-83, 100-117), on the GPU in FP64.  noise: N^dim white
+ * noise in linear order (Gaussian_white_noise, src/base/misc.hh:179-188).
+ * density (nullable), potential (nullable): N^dim doubles. */
+typedef struct pm_ic_params {
+    int dim;
+    int mbits;        /* N = 2^mbits */
+    double L;         /* "size" */
+    double ns;        /* power-law slope */
+    int smooth;       /* apply the smoothing to the final field too */
+    double scale;     /* smoothing scale (Mpc/h) */
+    double sigma;     /* normalisation */
+} pm_ic_params;
+int pm_ic_scale_free(const pm_ic_params* p, const double* noise, double* density, double* potential);
+
+/* _generate_random_field<R>, CDM branch (src/ic/ic.cc:84-97 with CDM<R>,
+ * ic.cc:14-35): noise filtered by sqrt(P_CDM(k)), physical k, divided by
+ * N^dim * res^(dim/2).  density: N^dim doubles. */
+int pm_ic_cdm(int dim, int mbits, double L, double ns, double Omega0, double H0, const double* noise,
+              double* density);
+
+/* _compute_potential<R> alone (src/ic/ic.cc:100-117): in-place. */
+int pm_ic_potential(int dim, int mbits, double L, double* field);
+
+/* _compute_displacement<R> (src/ic/ic.cc:119-146): psi AoS N^dim x dim. */
+int pm_ic_displacement(int dim, int mbits, double L, const double* potential, double* psi);
+
+/* Unnormalised real-to-complex forward FFT of an N^dim mesh in the library's
+ * half-spectrum layout, as interleaved doubles: main block (N/2 x N^(dim-1)
+ * modes, kx fastest) followed by the kx = N/2 plane (N^(dim-1) modes).
+ * precision: PM_FP64 or PM_FP32 (computed in that precision). */
+int pm_fft_r2c(int dim, int N, int precision, const double* in, double* spec);
+/* Inverse of pm_fft_r2c, unnormalised (output = N^dim * input field). */
+int pm_fft_c2r(int dim, int N, int precision, const double* spec, double* out);
+
+/* ---------------------------------------------------------------- conan cic
+ * cic_run<R> (src/cic/cic.hh:49-81): unit-mass deposit of snapshot positions
+ * X (Nm^dim particles) on the Nf mesh; s > 1 first sub-samples the Lagrangian
+ * grid s times per axis by interpolating displacements.  rho: Nf^dim doubles. */
+int pm_cic(int dim, int Nm, int Nf, double L, int s, const double* X, double* rho);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CONANPM_H */

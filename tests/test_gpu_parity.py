@@ -1,0 +1,254 @@
+"""GPU parity of the HIP path against the oracle (CPU FP64 restatement).
+
+Every comparison calls libconanpm.so through its C ABI.  Integer results
+(cell keys) are compared bit for bit; FP64 quantities that the GPU computes
+with the reference's own operation order (deposit, gather, Zel'dovich start)
+are compared bit for bit too; quantities that pass through the FFT are
+compared within the stated relative tolerances, because the reference's
+FFTW and this library's Stockham FFT round differently.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(12345)
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+# ------------------------------------------------------------------- FFT
+@pytest.mark.parametrize("dim,N", [(2, 8), (2, 64), (2, 256), (3, 4), (3, 8), (3, 16), (3, 32), (3, 64)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
+def test_fft_r2c_matches_numpy(pm, dim, N, prec, tol):
+    say("fft r2c", dim, N, prec)
+    x = RNG.standard_normal(N ** dim)
+    got = pm.fft_r2c(x, dim, prec)
+    # numpy with axis 0 fastest: reshape reversed, rfft along the last numpy axis
+    ref = np.fft.rfftn(x.reshape((N,) * dim))  # numpy axes (z,y,x) -> last axis is x
+    ref = ref.reshape(-1, N // 2 + 1)
+    assert rel(got, ref) < tol
+    back = pm.fft_c2r(got, dim, N, prec) / N ** dim
+    assert rel(back, x) < tol
+
+
+# ------------------------------------------------------------------- keys / deposit
+def _positions(dim, Nm, L, disp, special=True):
+    q = np.stack(np.meshgrid(*([np.arange(Nm) * (L / Nm)] * dim), indexing="ij"), -1)
+    q = q.reshape(-1, dim)[:, ::-1].copy()  # axis 0 fastest
+    X = q + disp * RNG.standard_normal(q.shape)
+    if special:
+        n = min(64, X.shape[0])
+        cases = np.array([0.0, -0.0, L, -L, 2 * L, L - 1e-13, -1e-300, 1e-300, L * (1 - 2 ** -52),
+                          -L / 3, 7 * L / 3, L / 2, -2 * L - 1e-9, 0.5 * L / Nm, 3.0, -3.0])
+        X[:n] = RNG.choice(cases, size=(n, dim))
+    return X
+
+
+@pytest.mark.parametrize("dim,Nm,Nf", [(2, 16, 16), (2, 32, 64), (3, 8, 8), (3, 16, 32), (3, 32, 32)])
+def test_keys_and_deposit_bit_exact_fp64(pm, orc, dim, Nm, Nf):
+    L = 100.0
+    X = _positions(dim, Nm, L, 3.0 * L / Nm)
+    say("deposit fp64", dim, Nm, Nf)
+    with pm.Gravity(dim, Nm, Nf, L) as g:
+        g.set_state(X)
+        rho, keys = g.deposit(keys=True)
+    m = (Nf / Nm) ** dim
+    ko = orc.cic_keys(dim, Nf, L, X)
+    ro = orc.md_cic(dim, Nf, L, X, m)
+    assert np.array_equal(keys, ko)
+    assert np.array_equal(rho, ro)  # same weights, summed in the same particle order
+
+
+@pytest.mark.parametrize("dim,Nm,Nf", [(2, 32, 32), (3, 16, 32), (3, 32, 32)])
+def test_deposit_clustered_bit_exact_fp64(pm, orc, dim, Nm, Nf):
+    """Many particles per cell (late-time clustering): the merge order still
+    reproduces the serial sum exactly."""
+    L = 10.0
+    X = L * 0.5 + 0.02 * L * RNG.standard_normal((Nm ** dim, dim))
+    say("deposit clustered", dim, Nm, Nf)
+    with pm.Gravity(dim, Nm, Nf, L) as g:
+        g.set_state(X)
+        rho = g.deposit()
+    ro = orc.md_cic(dim, Nf, L, X, (Nf / Nm) ** dim)
+    assert np.array_equal(rho, ro)
+    assert abs(rho.sum() - Nm ** dim * (Nf / Nm) ** dim) < 1e-9 * Nm ** dim
+
+
+def test_deposit_fp32_mass_and_density(pm, orc):
+    dim, Nm, Nf, L = 3, 32, 64, 100.0
+    X = _positions(dim, Nm, L, 2.0, special=False)
+    say("deposit fp32")
+    with pm.Gravity(dim, Nm, Nf, L, precision="fp32") as g:
+        g.set_state(X)
+        rho, keys = g.deposit(keys=True)
+    Xw = np.mod(X, L)
+    ro = orc.md_cic(dim, Nf, L, Xw, (Nf / Nm) ** dim)
+    assert abs(rho.sum() / ro.sum() - 1) < 1e-6
+    assert np.max(np.abs(rho - ro)) < 1e-5 * max(1.0, ro.max())
+
+
+# ------------------------------------------------------------------- gather
+@pytest.mark.parametrize("wrap", [0, 1])
+@pytest.mark.parametrize("dim,Nm,Nf", [(2, 32, 32), (3, 16, 32)])
+def test_gather_bit_exact_fp64(pm, orc, dim, Nm, Nf, wrap):
+    L = 50.0
+    X = _positions(dim, Nm, L, 2 * L / Nm)
+    phi = RNG.standard_normal(Nf ** dim)
+    say("gather", dim, Nm, Nf, wrap)
+    with pm.Gravity(dim, Nm, Nf, L, wrap_gather=bool(wrap)) as g:
+        g.set_state(X)
+        F = g.gather(phi)
+    Fo = orc.gather(dim, Nf, L, phi, X, wrap)
+    assert np.array_equal(F, Fo)
+
+
+# ------------------------------------------------------------------- Poisson
+@pytest.mark.parametrize("dim,N", [(2, 64), (3, 32), (3, 64)])
+def test_poisson_fp64(pm, orc, dim, N):
+    L, a = 100.0, 0.37
+    rho = 1.0 + 0.3 * RNG.standard_normal(N ** dim)
+    say("poisson", dim, N)
+    with pm.Gravity(dim, N, N, L) as g:
+        phi = g.poisson(rho, a)
+    phio = orc.poisson(dim, N, L, rho, a)
+    assert rel(phi, phio) < 1e-12
+
+
+def test_poisson_fp32(pm, orc):
+    dim, N, L, a = 3, 64, 100.0, 0.5
+    rho = 1.0 + 0.3 * RNG.standard_normal(N ** dim)
+    say("poisson fp32")
+    with pm.Gravity(dim, N, N, L, precision="fp32") as g:
+        phi = g.poisson(rho, a)
+    phio = orc.poisson(dim, N, L, rho, a)
+    assert rel(phi, phio) < 1e-5
+
+
+# ------------------------------------------------------------------- Zel'dovich + IC
+@pytest.mark.parametrize("dim,mbits", [(2, 6), (3, 4), (3, 5)])
+def test_ic_and_zeldovich(pm, orc, dim, mbits):
+    L, ns = 100.0, -2.0
+    N = 1 << mbits
+    noise = orc.white_noise(7, N ** dim)
+    say("ic", dim, mbits)
+    dens, phi = pm.ic_scale_free(dim, mbits, L, ns, noise)
+    dref = orc.ic_scale_free(dim, mbits, L, ns, noise)
+    pref = orc.compute_potential(dim, mbits, L, dref)
+    assert rel(dens, dref) < 1e-12
+    assert rel(phi, pref) < 1e-12
+    # Zel'dovich start from the oracle's potential: bit-exact
+    a0, da = 0.02, 0.0098
+    with pm.Gravity(dim, N, N, L) as g:
+        g.init(pref, a0, a0 + da / 2)
+        X, P = g.get_state()
+    Xo, Po = orc.zeldovich(dim, N, L, pref, a0, a0 + da / 2)
+    assert np.array_equal(X, Xo)
+    assert np.array_equal(P, Po)
+
+
+def test_ic_displacement(pm, orc):
+    dim, mbits, L = 3, 4, 100.0
+    N = 1 << mbits
+    phi = RNG.standard_normal(N ** dim)
+    say("ic displacement")
+    psi = pm.ic_displacement(dim, mbits, L, phi)
+    ref = orc.compute_displacement(dim, mbits, L, phi)
+    assert rel(psi, ref) < 1e-12
+
+
+# ------------------------------------------------------------------- leapfrog
+@pytest.mark.parametrize("dim,Nm,Nf,nsteps", [(2, 32, 32, 10), (3, 16, 16, 10), (3, 16, 32, 5)])
+def test_leapfrog_fp64_vs_oracle(pm, orc, dim, Nm, Nf, nsteps):
+    L, a0, da = 100.0, 0.02, 0.0098
+    mbits = int(np.log2(Nm))
+    noise = orc.white_noise(3, Nm ** dim)
+    dens = orc.ic_scale_free(dim, mbits, L, -2.0, noise)
+    phi = orc.compute_potential(dim, mbits, L, dens)
+    say("leapfrog", dim, Nm, Nf, nsteps)
+    sim = orc.Sim(dim, Nm, Nf, L)
+    sim.init(phi, a0, da)
+    sim.steps(nsteps)
+    Xo, Po, t = sim.state()
+    with pm.Gravity(dim, Nm, Nf, L) as g:
+        g.init(phi, a0, a0 + da / 2)
+        a = g.steps(a0, da, nsteps)
+        X, P = g.get_state()
+        apos, amom = g.times()
+    assert a == t[2]
+    assert (apos, amom) == (t[0], t[1])
+    # the potential passes through the FFT each step: rounding-level drift
+    assert np.max(np.abs(X - Xo)) < 1e-9 * L
+    assert rel(P, Po) < 1e-9
+
+
+def test_steps_fused_equals_unfused(pm, orc):
+    dim, Nm, Nf, L, a0, da = 3, 16, 16, 100.0, 0.02, 0.0098
+    noise = orc.white_noise(5, Nm ** dim)
+    phi = orc.compute_potential(dim, 4, L, orc.ic_scale_free(dim, 4, L, -2.0, noise))
+    say("fused vs unfused")
+    with pm.Gravity(dim, Nm, Nf, L) as g1, pm.Gravity(dim, Nm, Nf, L) as g2:
+        g1.init(phi, a0, a0 + da / 2)
+        g2.init(phi, a0, a0 + da / 2)
+        g1.steps(a0, da, 4)
+        a = a0
+        for _ in range(4):
+            g2.drift(a, da)
+            g2.kick(a + da / 2, da)
+            a += da
+        X1, P1 = g1.get_state()
+        X2, P2 = g2.get_state()
+    assert np.array_equal(X1, X2) and np.array_equal(P1, P2)
+
+
+def test_fp32_form_tracks_fp64(pm, orc):
+    dim, Nm, Nf, L, a0, da, n = 3, 32, 32, 100.0, 0.02, 0.0098, 20
+    noise = orc.white_noise(11, Nm ** dim)
+    phi = orc.compute_potential(dim, 5, L, orc.ic_scale_free(dim, 5, L, -2.0, noise))
+    say("fp32 vs fp64")
+    out = {}
+    for prec in ("fp64", "fp32"):
+        with pm.Gravity(dim, Nm, Nf, L, precision=prec, wrap_gather=True) as g:
+            g.init(phi, a0, a0 + da / 2)
+            g.steps(a0, da, n)
+            X, P = g.get_state()
+            out[prec] = (np.mod(X, L), P, g.deposit())
+    X64, P64, r64 = out["fp64"]
+    X32, P32, r32 = out["fp32"]
+    dx = np.abs(X32 - X64)
+    dx = np.minimum(dx, L - dx)
+    assert np.sqrt(np.mean(dx ** 2)) < 1e-4 * L / Nm
+    assert rel(P32, P64) < 1e-3
+    assert np.sqrt(np.mean((r32 - r64) ** 2)) < 1e-3
+
+
+def test_cic_tool(pm, orc):
+    dim, Nm, Nf, L = 3, 16, 32, 100.0
+    X = _positions(dim, Nm, L, 1.0, special=False)
+    say("cic tool")
+    r1 = pm.cic(dim, Nm, Nf, L, X, 1)
+    assert np.array_equal(r1, orc.cic_tool(dim, Nm, Nf, L, 1, X))
+    r2 = pm.cic(dim, Nm, Nf, L, X, 2)
+    ref2 = orc.cic_tool(dim, Nm, Nf, L, 2, X)
+    assert rel(r2, ref2) < 1e-12
+
+
+def test_power_spectrum(pm, orc):
+    dim, Nm, Nf, L = 3, 32, 32, 100.0
+    X = _positions(dim, Nm, L, 2.0, special=False)
+    edges = orc.log_edges(Nf, L, 16)
+    say("power spectrum")
+    with pm.Gravity(dim, Nm, Nf, L) as g:
+        g.set_state(X)
+        rho = g.deposit()
+        k, p, c = g.power_spectrum(16, edges)
+    ko, po, co = orc.power_spectrum(dim, Nf, L, rho - 1.0, edges)
+    assert np.array_equal(c.astype(np.int64), co)
+    assert rel(k, ko) < 1e-12
+    assert rel(p, po) < 1e-10

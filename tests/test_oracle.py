@@ -1,0 +1,399 @@
+"""Pins of the oracle (CPU FP64 restatement of the reference) on the CPU.
+
+The reference's own tests hold no golden vectors (every test prints and
+returns true), so each oracle function is pinned here by closed forms,
+invariants, brute force on tiny inputs and independent libraries (numpy.fft).
+Each check is chosen so that a plausible mistake (dropped term, wrong sign,
+wrong index or axis order) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+RNG = np.random.default_rng(2024)
+
+
+def grid(dim, N, L):
+    q = np.stack(np.meshgrid(*([np.arange(N) * (L / N)] * dim), indexing="ij"), -1).reshape(-1, dim)
+    return q[:, ::-1].copy()  # axis 0 fastest
+
+
+# ------------------------------------------------------------------ noise
+def test_mt19937_known_answer(orc):
+    # C++11 [rand.predef]: the 10000th invocation of a default-constructed mt19937 yields 4123659995
+    assert orc.mt19937_nth(5489, 10000) == 4123659995
+
+
+def test_white_noise_statistics_and_determinism(orc):
+    a = orc.white_noise(42, 200000)
+    b = orc.white_noise(42, 200000)
+    assert np.array_equal(a, b)
+    assert abs(a.mean()) < 0.01 and abs(a.var() - 1) < 0.01
+    assert not np.array_equal(a[:100], orc.white_noise(43, 100))
+
+
+# ------------------------------------------------------------------ modulus / keys
+def test_modulus_int_exhaustive(orc):
+    for b in range(1, 9):
+        for a in range(-4 * b, 4 * b + 1):
+            m = orc.modulus_i(a, b)
+            math_mod = a % b  # Python: mathematical modulo
+            if a < 0 and a % b == 0:
+                assert m == b  # the reference's quirk for negative multiples
+            else:
+                assert m == math_mod
+
+
+@pytest.mark.parametrize("a", [0.0, -0.0, 1e-300, -1e-300, 99.99999999999999, 100.0, -100.0, 250.5,
+                               -250.5, 1e6 + 0.25, -3.0, 199.99999999999997])
+def test_modulus_double_special_cases(orc, a):
+    b = 100.0
+    m = orc.modulus_d(a, b)
+    # within rounding of the mathematical result, and in [0, b] up to one ulp of a
+    ref = math.fmod(a, b)
+    if ref < 0:
+        ref += b
+    assert abs(m - ref) < 1e-9 or abs(abs(m - ref) - b) < 1e-9
+    assert -1e-9 <= m <= b
+
+
+def test_cic_keys_brute_force(orc):
+    dim, N, L = 3, 8, 10.0
+    h = L / N
+    X = RNG.uniform(-2 * L, 3 * L, size=(500, dim))
+    k = orc.cic_keys(dim, N, L, X)
+    for i in range(X.shape[0]):
+        o = [int(math.floor(np.mod(X[i, d], L) / h + 1e-12)) % N for d in range(dim)]
+        # skip points within rounding of a cell face
+        if any(abs(np.mod(X[i, d], L) / h - round(np.mod(X[i, d], L) / h)) < 1e-9 for d in range(dim)):
+            continue
+        assert k[i] == o[0] + N * (o[1] + N * o[2])
+
+
+# ------------------------------------------------------------------ FFT
+@pytest.mark.parametrize("dim,N", [(1, 16), (2, 8), (2, 32), (3, 8), (3, 16)])
+def test_fft_matches_numpy(orc, dim, N):
+    x = RNG.standard_normal(N ** dim) + 1j * RNG.standard_normal(N ** dim)
+    got = orc.fft(x, dim, -1)
+    ref = np.fft.fftn(x.reshape((N,) * dim)).ravel()
+    assert np.max(np.abs(got - ref)) < 1e-12 * np.max(np.abs(ref))
+    back = orc.fft(got, dim, +1) / N ** dim
+    assert np.max(np.abs(back - x)) < 1e-13
+
+
+def test_fft_closed_forms(orc):
+    dim, N = 3, 8
+    d = np.zeros(N ** dim, complex)
+    d[0] = 1
+    assert np.allclose(orc.fft(d, dim, -1), 1.0, atol=1e-15)
+    # plane wave e^{2 pi i (m . x)/N} -> N^dim at mode m (axis 0 fastest)
+    m = (3, 1, 5)
+    idx = np.arange(N ** dim)
+    x = [idx % N, (idx // N) % N, idx // (N * N)]
+    w = np.exp(2j * np.pi * sum(m[k] * x[k] for k in range(3)) / N)
+    F = orc.fft(w, dim, -1)
+    k = m[0] + N * (m[1] + N * m[2])
+    assert abs(F[k] - N ** dim) < 1e-10
+    F[k] = 0
+    assert np.max(np.abs(F)) < 1e-10
+
+
+# ------------------------------------------------------------------ deposit
+def test_deposit_hand_weights(orc):
+    dim, N, L = 2, 4, 4.0  # res = 1
+    X = np.array([[0.25, 0.5]])
+    rho = orc.md_cic(dim, N, L, X, 1.0).reshape(N, N)  # [y][x]
+    assert rho[0, 0] == 0.75 * 0.5 and rho[0, 1] == 0.25 * 0.5
+    assert rho[1, 0] == 0.75 * 0.5 and rho[1, 1] == 0.25 * 0.5
+    assert rho.sum() == 1.0
+    # periodic wrap across the upper face, and a negative coordinate
+    X = np.array([[3.5, -0.25]])
+    rho = orc.md_cic(dim, N, L, X, 2.0).reshape(N, N)
+    # x: 3.5 -> cells 3 (0.5), 0 (0.5); y: -0.25 -> 3.75 -> cells 3 (0.25), 0 (0.75)
+    assert rho[3, 3] == 2 * 0.5 * 0.25 and rho[3, 0] == 2 * 0.5 * 0.25
+    assert rho[0, 3] == 2 * 0.5 * 0.75 and rho[0, 0] == 2 * 0.5 * 0.75
+
+
+def test_deposit_invariants(orc):
+    dim, N, L = 3, 16, 50.0
+    X = RNG.uniform(-L, 2 * L, size=(3000, dim))
+    rho = orc.md_cic(dim, N, L, X, 1.0)
+    assert abs(rho.sum() - 3000) < 1e-9
+    assert rho.min() >= 0
+    # integer-cell translation commutes with the deposit
+    h = L / N
+    sh = np.array([2, 5, 1])
+    rho2 = orc.md_cic(dim, N, L, X + sh * h, 1.0).reshape(N, N, N)
+    ref = np.roll(rho.reshape(N, N, N), shift=tuple(sh[::-1]), axis=(0, 1, 2))
+    assert np.max(np.abs(rho2 - ref)) < 1e-12
+    # a particle on a node deposits into one cell only
+    one = orc.md_cic(dim, N, L, np.array([[3 * h, 7 * h, 11 * h]]), 1.0)
+    assert np.count_nonzero(one) == 1 and one[3 + N * (7 + N * 11)] == 1.0
+
+
+def test_cic_unit_test_scenario(orc):
+    # the reference's only hot-path test (src/run/cic-test.cc:10-40): 256 points
+    # from Gaussian_white_noise(0) + 2.5 on Box<2>(64, 5.0); now with assertions.
+    noise = orc.white_noise(0, 512)
+    X = np.empty((256, 2))
+    X[:, 0] = noise[0::2] + 2.5
+    X[:, 1] = noise[1::2] + 2.5
+    rho = orc.md_cic(2, 64, 5.0, X, 1.0)
+    assert abs(rho.sum() - 256) < 1e-12
+    assert rho.min() >= 0 and np.count_nonzero(rho) <= 4 * 256
+
+
+# ------------------------------------------------------------------ gradient / Poisson / gather
+def fd_symbol(k, h):
+    return (8 * np.sin(k * h) - np.sin(2 * k * h)) / (6 * h)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_fd_gradient_plane_wave(orc, axis):
+    dim, N, L = 3, 16, 30.0
+    h = L / N
+    q = grid(dim, N, L)
+    k = 2 * np.pi * 3 / L
+    f = np.cos(k * q[:, axis] + 0.3)
+    g = orc.gradient_2nd_order(dim, N, L, f)
+    ref = -fd_symbol(k, h) * np.sin(k * q[:, axis] + 0.3)
+    assert np.max(np.abs(g[:, axis] - ref)) < 1e-12
+    others = [d for d in range(dim) if d != axis]
+    assert np.max(np.abs(g[:, others])) < 1e-12
+
+
+def test_poisson_plane_wave(orc):
+    dim, N, L, a = 3, 16, 40.0, 0.3
+    H0, Om = 68.5, 1.0
+    G = 1.5 * Om * H0 * H0
+    q = grid(dim, N, L)
+    kv = 2 * np.pi * np.array([2, -1, 3]) / L
+    eps = 0.01
+    delta = eps * np.cos(q @ kv)
+    phi = orc.poisson(dim, N, L, 1.0 + delta, a, H0, Om, 0.0)
+    ref = -(G / a) * delta / (kv @ kv)
+    assert np.max(np.abs(phi - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+def test_gather_interpolates_fd_gradient(orc):
+    dim, N, L = 3, 16, 32.0
+    h = L / N
+    q = grid(dim, N, L)
+    k = 2 * np.pi / L
+    phi = np.sin(k * q[:, 0]) + 0.5 * np.cos(2 * k * q[:, 2])
+    X = RNG.uniform(0, L, size=(400, dim))
+    F = orc.gather(dim, N, L, phi, X, 0)
+    # trilinear interpolation of the nodal FD gradient = linear interpolation
+    # along the axis the component varies on
+    gx = lambda x: fd_symbol(k, h) * np.cos(k * x)  # noqa: E731
+    gz = lambda z: -0.5 * fd_symbol(2 * k, h) * np.sin(2 * k * z)  # noqa: E731
+
+    def lin(g, x):
+        i = np.floor(x / h)
+        t = x / h - i
+        return (1 - t) * g(i * h) + t * g((i + 1) * h)
+
+    assert np.max(np.abs(F[:, 0] - lin(gx, X[:, 0]))) < 1e-12
+    assert np.max(np.abs(F[:, 1])) < 1e-12
+    assert np.max(np.abs(F[:, 2] - lin(gz, X[:, 2]))) < 1e-12
+
+
+def test_gather_truncation_quirk(orc):
+    """Q1: the gather truncates the unwrapped coordinate (numeric.hh:19,26-28):
+    at x = -0.25 h the origin is node 0 and the weights extrapolate."""
+    dim, N, L = 2, 8, 8.0  # h = 1
+    q = grid(dim, N, L)
+    phi = np.sin(2 * np.pi * q[:, 0] / L)
+    g = orc.gradient_2nd_order(dim, N, L, phi)[:, 0]
+    X = np.array([[-0.25, 0.0]])
+    F = orc.gather(dim, N, L, phi, X, 0)
+    # A1 = -0.25, A0 = 1.25: F = 1.25 g(0) - 0.25 g(1)
+    assert abs(F[0, 0] - (1.25 * g[0] - 0.25 * g[1])) < 1e-14
+    Fw = orc.gather(dim, N, L, phi, X, 1)
+    # folded: x = 7.75 -> 0.25 g(7) + 0.75 g(0)
+    assert abs(Fw[0, 0] - (0.25 * g[7] + 0.75 * g[0])) < 1e-14
+
+
+def test_deposit_gather_adjoint(orc):
+    """<deposit(x), grad phi> = sum_i gather(phi, x_i) (unit mass, folded coordinates)."""
+    dim, N, L = 3, 8, 16.0
+    X = RNG.uniform(0, L, size=(300, dim))
+    phi = RNG.standard_normal(N ** dim)
+    rho = orc.md_cic(dim, N, L, X, 1.0)
+    grad = orc.gradient_2nd_order(dim, N, L, phi)
+    lhs = rho @ grad
+    rhs = orc.gather(dim, N, L, phi, X, 1).sum(axis=0)
+    assert np.max(np.abs(lhs - rhs)) < 1e-10 * np.max(np.abs(lhs))
+
+
+# ------------------------------------------------------------------ Zel'dovich / leapfrog
+def test_zeldovich_plane_wave(orc):
+    dim, N, L = 3, 16, 64.0
+    h = L / N
+    q = grid(dim, N, L)
+    k = 2 * np.pi * 2 / L
+    phi = 0.3 * np.cos(k * q[:, 1])
+    X, P = orc.zeldovich(dim, N, L, phi, 0.02, 0.025)
+    u = -0.3 * fd_symbol(k, h) * np.sin(k * q[:, 1])
+    assert np.max(np.abs(X[:, 1] - (q[:, 1] - 0.02 * u))) < 1e-13
+    assert np.max(np.abs(P[:, 1] + 0.025 * u)) < 1e-15
+    # the reference's stencil ((f/12 - 2f/3) + 2f/3) - f/12 of a constant
+    # line rounds to ~1e-17 f, not 0: the other components are tiny, not zero
+    assert np.max(np.abs(X[:, [0, 2]] - q[:, [0, 2]])) < 1e-15
+    assert np.max(np.abs(P[:, [0, 2]])) < 1e-15
+
+
+def test_unperturbed_lattice_stays_at_rest(orc):
+    dim, Nm, Nf, L = 3, 8, 16, 10.0
+    sim = orc.Sim(dim, Nm, Nf, L)
+    sim.init(np.zeros(Nm ** dim), 0.02, 0.01)
+    sim.steps(5)
+    X, P, t = sim.state()
+    assert np.array_equal(X, grid(dim, Nm, L))
+    assert np.max(np.abs(P)) < 1e-9
+    assert abs(t[2] - 0.07) < 1e-15
+
+
+def test_drift_constant_momentum(orc):
+    dim, Nm, L = 2, 8, 10.0
+    sim = orc.Sim(dim, Nm, Nm, L)
+    sim.init(np.zeros(Nm ** dim), 0.1, 0.01)
+    q = grid(dim, Nm, L)
+    P = np.tile([1e-3, -2e-3], (Nm ** dim, 1))
+    sim.set_state(q, P)
+    sim.drift(0.1, 0.01)
+    X, _, _ = sim.state()
+    c = 0.01 / (0.1 * 0.1 * orc.adot(68.5, 1.0, 0.0, 0.1))
+    assert np.max(np.abs(X - (q + P * c))) < 1e-15
+
+
+def test_adot_closed_form(orc):
+    H0, a = 68.5, 0.3
+    assert abs(orc.adot(H0, 1.0, 0.0, a) - H0 / math.sqrt(a)) < 1e-12
+    assert abs(orc.adot(H0, 0.31, 0.69, a) - H0 * a * math.sqrt(0.69 + 0.31 * a ** -3)) < 1e-10
+
+
+def test_single_mode_growth_matches_ode(orc):
+    """A long plane wave of small amplitude: each sheet obeys the linear ODE
+    ds/da = P/(a^2 adot), dP/da = (G/(a adot)) s_k where s_k is the
+    displacement convolved with the PM force response of that mode; for k much
+    smaller than the mesh Nyquist the response is 1 up to O((kh)^2).  The
+    leapfrog must follow the ODE solution from the same initial state."""
+    from scipy.integrate import solve_ivp
+
+    # kh = 2 pi / 64: the CIC deposit/interpolation response 1 - (kh)^2/6 and
+    # the FD symbol bias stay well under the 1 % tolerance
+    dim, N, L, H0 = 2, 64, 100.0, 68.5
+    G = 1.5 * H0 * H0
+    q = grid(dim, N, L)
+    k = 2 * np.pi / L
+    amp = 1e-4
+    phi = amp * np.cos(k * q[:, 0])
+    a0, da, n = 0.02, 0.001, 300
+    sim = orc.Sim(dim, N, N, L)
+    sim.init(phi, a0, da)
+    sim.steps(n)
+    X, P, t = sim.state()
+    h = L / N
+    u0 = -amp * fd_symbol(k, h)  # u(q) = u0 sin(kq); s = -a u, P = -a' u
+    adot = lambda a: H0 / math.sqrt(a)  # noqa: E731
+
+    def rhs(a, y):
+        s, p = y
+        return [p / (a * a * adot(a)), G / (a * adot(a)) * s]
+
+    sol = solve_ivp(rhs, (a0, t[2]), [-a0 * u0, -(a0 + da / 2) * u0], rtol=1e-10, atol=1e-16)
+    s_ode = sol.y[0, -1]
+    disp = X[:, 0] - q[:, 0]
+    sel = np.abs(np.sin(k * q[:, 0])) > 0.5
+    ratio = disp[sel] / np.sin(k * q[sel, 0])
+    s_pm = np.median(ratio)
+    assert s_ode / (-a0 * u0) > 3  # the mode did grow
+    assert abs(s_pm / s_ode - 1) < 0.01
+
+
+# ------------------------------------------------------------------ initial conditions
+def test_ic_smoothed_field_has_unit_normalisation(orc):
+    dim, mbits, L, sigma = 3, 4, 100.0, 0.83
+    noise = orc.white_noise(9, 16 ** 3)
+    dens = orc.ic_scale_free(dim, mbits, L, -1.0, noise, smooth=True, sigma=sigma)
+    assert abs(dens.mean()) < 1e-15
+    assert abs(np.mean(dens ** 2) - sigma ** 2) < 1e-12
+
+
+def test_ic_filter_shape(orc):
+    dim, mbits, L, ns = 2, 5, 100.0, -1.5
+    N = 1 << mbits
+    noise = orc.white_noise(10, N ** dim)
+    dens = orc.ic_scale_free(dim, mbits, L, ns, noise, smooth=False)
+    W = np.fft.fft2(noise.reshape(N, N))
+    D = np.fft.fft2(dens.reshape(N, N))
+    kk = np.fft.fftfreq(N) * 2 * np.pi
+    ky, kx = np.meshgrid(kk, kk, indexing="ij")
+    # Nyquist index maps to +N/2 in the reference; |k| is symmetric so it doesn't matter here
+    kabs = np.sqrt(kx ** 2 + ky ** 2)
+    m = kabs > 0
+    ratio = (D[m] / W[m]).real / np.sqrt(kabs[m] ** ns)
+    assert np.max(np.abs((D[m] / W[m]).imag)) < 1e-9
+    assert np.max(np.abs(ratio / ratio.mean() - 1)) < 1e-9
+    assert abs(D[0, 0]) < 1e-12
+
+
+def test_ic_potential_solves_poisson(orc):
+    dim, mbits, L = 3, 4, 50.0
+    N = 1 << mbits
+    dens = orc.white_noise(11, N ** dim)
+    dens -= dens.mean()
+    phi = orc.compute_potential(dim, mbits, L, dens)
+    Fp = np.fft.fftn(phi.reshape(N, N, N))
+    Fd = np.fft.fftn(dens.reshape(N, N, N))
+    k = np.fft.fftfreq(N) * N * 2 * np.pi / L
+    k[N // 2] = abs(k[N // 2])
+    kz, ky, kx = np.meshgrid(k, k, k, indexing="ij")
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    m = k2 > 0
+    assert np.max(np.abs(-k2[m] * Fp[m] - Fd[m])) < 1e-10 * np.max(np.abs(Fd))
+
+
+def test_ic_displacement_single_mode(orc):
+    dim, mbits, L = 3, 4, 50.0
+    N = 1 << mbits
+    q = grid(dim, N, L)
+    m = 3
+    phi = np.cos(2 * np.pi * m * q[:, 2] / L)
+    psi = orc.compute_displacement(dim, mbits, L, phi)
+    kg = 2 * np.pi * m / N
+    # Re IFFT[i sin(k) phi_k] / (size N / L) * size  ->  -sin(kg) sin(k z) * L / N
+    ref = -np.sin(kg) * np.sin(2 * np.pi * m * q[:, 2] / L) * L / N
+    assert np.max(np.abs(psi[:, 2] - ref)) < 1e-12
+    assert np.max(np.abs(psi[:, :2])) < 1e-12
+
+
+# ------------------------------------------------------------------ cic tool / power spectrum
+def test_cic_tool_mass(orc):
+    dim, Nm, Nf, L = 3, 8, 16, 10.0
+    X = grid(dim, Nm, L) + 0.3 * RNG.standard_normal((Nm ** dim, dim))
+    r1 = orc.cic_tool(dim, Nm, Nf, L, 1, X)
+    assert abs(r1.sum() - Nm ** dim) < 1e-9
+    r2 = orc.cic_tool(dim, Nm, Nf, L, 2, X)
+    assert abs(r2.sum() - (2 * Nm) ** dim * (2 * Nm / Nf) ** -dim) < 1e-9
+    # an undisplaced lattice sub-sampled is the finer lattice
+    r3 = orc.cic_tool(dim, Nm, Nf, L, 2, grid(dim, Nm, L))
+    assert np.max(np.abs(r3 - 1.0)) < 1e-12
+
+
+def test_power_spectrum_plane_wave(orc):
+    dim, N, L = 3, 16, 100.0
+    q = grid(dim, N, L)
+    A = 0.2
+    kv = 2 * np.pi * np.array([3, 0, 0]) / L
+    delta = A * np.cos(q @ kv)
+    edges = orc.log_edges(N, L, 16)
+    k, p, c = orc.power_spectrum(dim, N, L, delta, edges)
+    b = np.nonzero(c)[0]
+    hit = [i for i in b if p[i] > 1e-20]
+    assert len(hit) == 1
+    i = hit[0]
+    # the two modes +-k0 carry L^3 A^2 / 4 each; the bin averages over c[i] modes
+    assert abs(p[i] * c[i] - 2 * L ** 3 * A ** 2 / 4) < 1e-9 * L ** 3 * A ** 2
