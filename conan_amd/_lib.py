@@ -26,7 +26,7 @@ EXPORTS = [
     "pm_last_error", "pm_create", "pm_destroy", "pm_init_zeldovich", "pm_drift", "pm_kick",
     "pm_steps", "pm_get_state", "pm_set_state", "pm_get_times", "pm_deposit", "pm_potential",
     "pm_forces", "pm_gather_mesh", "pm_poisson_mesh", "pm_power_spectrum", "pm_diagnostics", "pm_profile", "pm_sync",
-    "pm_ic_scale_free", "pm_ic_potential", "pm_ic_displacement", "pm_fft_r2c", "pm_fft_c2r",
+    "pm_ic_scale_free", "pm_ic_cdm", "pm_ic_potential", "pm_ic_displacement", "pm_fft_r2c", "pm_fft_c2r",
     "pm_cic",
 ]
 
@@ -86,6 +86,7 @@ def load() -> C.CDLL:
         "pm_profile": (C.c_int, [V, C.c_int, _dp, C.POINTER(C.c_longlong)]),
         "pm_sync": (C.c_int, [V]),
         "pm_ic_scale_free": (C.c_int, [C.POINTER(pm_ic_params), _dp, _dp, _dp]),
+        "pm_ic_cdm": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp]),
         "pm_ic_potential": (C.c_int, [C.c_int, C.c_int, C.c_double, _dp]),
         "pm_ic_displacement": (C.c_int, [C.c_int, C.c_int, C.c_double, _dp, _dp]),
         "pm_fft_r2c": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp]),
