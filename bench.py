@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X particle-mesh step (conan's `run` hot loop).
+
+Workload (the configuration the headline metric is quoted on): 3-D PM,
+256^3 particles on a 512^3 mesh, box L = 100, Einstein-de Sitter, scale-free
+P(k) ~ k^-1.5 initial conditions from seeded white noise, Zel'dovich start at
+a = 0.02, leapfrog in a with da = 0.0049 (200 steps reach a = 0.99).
+
+A step is one leapfrog iteration (drift + deposit + Poisson solve + gather +
+kick).  The timed region runs --steps steps of the FP32 form (32-bit
+fixed-point positions, FP32 mesh/FFT/gather, FP64 momenta) with particles and
+meshes resident in HBM; the same is measured for the FP64 plain form.  One JSON
+line goes to stdout; progress goes to stderr.
+
+Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
+METRIC = "particle-steps/s & %HBM peak at 256³/512³; FP32 vs FP64; grad/fwd time ratio"
+
+DIM, MBITS, F, L, NS = 3, 8, 2, 100.0, -1.5
+A0, DA = 0.02, 0.0049
+NM = 1 << MBITS
+NF = F * NM
+NP = NM ** DIM
+M = NF ** DIM
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def phase_bytes(prec: str) -> dict:
+    """Algorithmic HBM bytes of one launch of each phase (reads + writes the
+    step needs, not what an implementation moves).  Sum over one step:
+    FP32 84 Np + 48 M, FP64 120 Np + 96 M."""
+    b_pos = 4 if prec == "fp32" else 8  # position component
+    b_mesh = 4 if prec == "fp32" else 8
+    nrows = M // NF
+    spec = 2 * b_mesh * (M // 2)  # one read + one write of the main half spectrum
+    return {
+        1: DIM * b_pos * NP + b_mesh * M,  # deposit: positions in, mesh out
+        2: b_mesh * M + b_mesh * M + 2 * b_mesh * nrows,  # rows r2c: real in, half spectrum out
+        3: spec,
+        4: spec,
+        5: spec,
+        6: b_mesh * M + b_mesh * M + 2 * b_mesh * nrows,  # rows c2r
+        7: 2 * DIM * (b_pos + 8) * NP + b_mesh * M,  # gather+kick+drift: X, P in and out, phi in
+    }
+
+
+def step_bytes(prec: str) -> int:
+    return (84 * NP + 48 * M) if prec == "fp32" else (120 * NP + 96 * M)
+
+
+def read_traffic(kernel_tag: str):
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None when absent)."""
+    path = os.path.join(ROOT, "profiles", "traffic_pm_step_c3.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 2 and parts[0] == kernel_tag:
+            try:
+                return float(parts[1])
+            except ValueError:
+                return None
+    return None
+
+
+def run_form(pm, phi, prec, steps, warmup, wrap=False, profile=True, pk_bins=64):
+    g = pm.Gravity(DIM, NM, NF, L, precision=prec, wrap_gather=wrap)
+    try:
+        if warmup > 0:
+            g.init(phi, A0, A0 + DA / 2)
+            g.steps(A0, DA, warmup)
+        g.init(phi, A0, A0 + DA / 2)
+        g.sync()
+        if profile:
+            g.profile(1)
+        t0 = time.perf_counter()
+        a = g.steps(A0, DA, steps)  # returns after the stream is synchronised
+        t1 = time.perf_counter()
+        ms, cnt = g.profile(0) if profile else (None, None)
+        k, p, c = g.power_spectrum(pk_bins)
+        diag = g.diagnostics()
+        return {"seconds": t1 - t0, "a": a, "phase_ms": ms, "phase_cnt": cnt, "pk": (k, p, c), "diag": diag}
+    finally:
+        g.close()
+
+
+def cpu_baseline(phi, nsteps: int, threads: int):
+    """The oracle (FP64 CPU restatement of the reference, OpenMP loops and the
+    per-add critical section of the reference's deposit) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle as orc
+
+    sim = orc.Sim(DIM, NM, NF, L, nthreads=threads)
+    sim.init(phi, A0, DA)
+    secs = sim.steps(nsteps)  # steady_clock around the leapfrog loop only
+    return {"value": NP * nsteps / secs, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{nsteps} leapfrog step(s) (drift+deposit+FFT Poisson+gather+kick) of the same "
+                      f"256^3/512^3 FP64 run from the same potential, {threads} OpenMP threads, "
+                      f"{secs:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every box here has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from conan_amd import pm
+
+    log(f"IC: {NM}^3 white noise (seed 1), scale-free ns={NS}, FP64 on the GPU")
+    noise = np.random.default_rng(1).standard_normal(NP)
+    _, phi = pm.ic_scale_free(DIM, MBITS, L, NS, noise)
+    del noise
+
+    log(f"FP32 form: {args.warmup} warmup + {args.steps} timed steps")
+    r32 = run_form(pm, phi, "fp32", args.steps, args.warmup)
+    log(f"  {r32['seconds']:.3f} s -> {r32['seconds'] / args.steps * 1e3:.3f} ms/step")
+    r64 = None
+    if not args.no_fp64:
+        log(f"FP64 plain form: {args.warmup} warmup + {args.steps} timed steps")
+        r64 = run_form(pm, phi, "fp64", args.steps, args.warmup)
+        log(f"  {r64['seconds']:.3f} s -> {r64['seconds'] / args.steps * 1e3:.3f} ms/step")
+
+    from conan_amd._lib import PHASE_NAMES
+
+    def phase_report(r, prec):
+        pb = phase_bytes(prec)
+        out = {}
+        for i, name in enumerate(PHASE_NAMES):
+            if r["phase_cnt"][i] == 0:
+                continue
+            avg = r["phase_ms"][i] / r["phase_cnt"][i] * 1e-3
+            e = {"ms_per_launch": round(avg * 1e3, 4), "launches": int(r["phase_cnt"][i])}
+            if i in pb:
+                e["GBps"] = round(pb[i] / avg / 1e9, 1)
+                e["frac_hbm"] = round(pb[i] / avg / PEAK_HBM, 4)
+            out[name] = e
+        return out
+
+    ph32 = phase_report(r32, "fp32")
+    for k, v in ph32.items():
+        log(f"  fp32 {k:24s} {v}")
+
+    # dominant single-launch kernel (phases 1..7) by total time
+    pb = phase_bytes("fp32")
+    dom = max(pb, key=lambda i: r32["phase_ms"][i])
+    dom_avg = r32["phase_ms"][dom] / r32["phase_cnt"][dom] * 1e-3
+    tags = {1: "k_deposit", 2: "k_rows_r2c", 3: "k_cols_fwd", 4: "k_cols_green", 5: "k_cols_bwd", 6: "k_rows_c2r",
+            7: "k_gather_kick"}
+    achieved = pb[dom] / dom_avg
+    roofline = {"bound": "hbm", "kernel": f"{tags[dom]} ({PHASE_NAMES[dom]}), FP32, 512^3",
+                "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM, 4), "traffic": read_traffic(tags[dom]),
+                "algorithmic_bytes_per_launch": pb[dom], "avg_launch_ms": round(dom_avg * 1e3, 4)}
+
+    t32 = r32["seconds"] / args.steps
+    val32 = NP / t32
+    step_roof = {"bytes_per_step": step_bytes("fp32"), "achieved_GBps": round(step_bytes("fp32") / t32 / 1e9, 1),
+                 "frac": round(step_bytes("fp32") / t32 / PEAK_HBM, 4)}
+    out = {
+        "metric": METRIC, "value": round(val32, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t32 * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic: seeded Gaussian white noise -> scale-free GRF (ns=-1.5) -> Zel'dovich start",
+        "config": {"workload": "3-D PM N-body, 256^3 particles on a 512^3 mesh, L=100, EdS, "
+                               f"a=0.02 + {args.steps} x 0.0049, FP32 form (fixed-point positions, "
+                               "FP64 momenta)",
+                   "particles": NP, "mesh": NF, "form": "fp32"},
+        "roofline": roofline,
+        "step_roofline": step_roof,
+        "phases_fp32": ph32,
+    }
+    if r64 is not None:
+        t64 = r64["seconds"] / args.steps
+        out["fp64"] = {"value": round(NP / t64, 1), "ms_per_step": round(t64 * 1e3, 4),
+                       "step_roofline_frac": round(step_bytes("fp64") / t64 / PEAK_HBM, 4),
+                       "phases": phase_report(r64, "fp64")}
+        out["fp32_over_fp64"] = round(t64 / t32, 3)
+        k32, p32, c32 = r32["pk"]
+        k64, p64, c64 = r64["pk"]
+        m = (c32 > 0) & (p64 > 0)
+        out["pk_fp32_vs_fp64_plain_max_rel"] = float(np.max(np.abs(p32[m] / p64[m] - 1)))
+        # the FP32 form wraps positions exactly; compare with the FP64 form
+        # run the same way (gather on folded coordinates), untimed
+        log("FP64 form with folded gather coordinates (untimed, for the P(k) comparison)")
+        rw = run_form(pm, phi, "fp64", args.steps, 0, wrap=True, profile=False)
+        kw, pw, cw = rw["pk"]
+        m = (c32 > 0) & (pw > 0)
+        rel = np.abs(p32[m] / pw[m] - 1)
+        out["pk_fp32_vs_fp64"] = {"a": rw["a"], "bins": int(m.sum()), "max_rel": float(rel.max()),
+                                  "median_rel": float(np.median(rel))}
+    out["grad_fwd_ratio"] = None
+    out["final_a"] = r32["a"]
+    out["momentum_sum_fp32"] = r32["diag"]["momentum"]
+
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"CPU baseline: oracle, {args.cpu_steps} step(s), {threads} threads")
+        out["cpu_baseline"] = cpu_baseline(phi, args.cpu_steps, threads)
+        log(f"  {out['cpu_baseline']['sample']}")
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -17,9 +17,9 @@ LIB_PATH = os.path.join(HERE, "libconanpm.so")
 PM_FP64 = 0
 PM_FP32 = 1
 PM_FLAG_WRAP_GATHER = 1
-PM_NPHASES = 9
+PM_NPHASES = 10
 PHASE_NAMES = ["keys+sort", "deposit", "fft rows fwd", "fft y fwd", "green (z fwd*G*bwd)",
-               "fft y bwd", "fft rows bwd", "gather+kick(+drift)", "drift"]
+               "fft y bwd", "fft rows bwd", "gather+kick(+drift)", "drift", "nyquist-plane passes"]
 
 # Every exported entry point of include/conanpm.h.
 EXPORTS = [

@@ -61,40 +61,61 @@ static int ilog2_exact(long n, const char* what) {
     return ilog2(n);
 }
 
+// Per-phase device time from HIP event pairs recorded on the handle's
+// stream.  Recording does not synchronise: pairs queue up in a pool and are
+// resolved when the totals are read, so profiling can stay on in a timed run.
 struct Prof {
     bool on = false;
-    hipEvent_t b[PM_NPHASES] = {}, e[PM_NPHASES] = {};
+    std::vector<hipEvent_t> evb, eve;
+    std::vector<int> phase;
+    size_t used = 0;
     double ms[PM_NPHASES] = {};
     long long cnt[PM_NPHASES] = {};
     void enable(bool v) {
-        if (v && !b[0])
-            for (int i = 0; i < PM_NPHASES; ++i) {
-                CPM_HIP(hipEventCreate(&b[i]));
-                CPM_HIP(hipEventCreate(&e[i]));
-            }
-        if (v)
+        if (v) {
             for (int i = 0; i < PM_NPHASES; ++i) {
                 ms[i] = 0;
                 cnt[i] = 0;
             }
+            used = 0;
+        }
         on = v;
     }
     void begin(int ph, hipStream_t st) {
-        if (on) CPM_HIP(hipEventRecord(b[ph], st));
+        if (!on) return;
+        if (used == evb.size()) {
+            hipEvent_t a, b;
+            CPM_HIP(hipEventCreate(&a));
+            CPM_HIP(hipEventCreate(&b));
+            evb.push_back(a);
+            eve.push_back(b);
+            phase.push_back(ph);
+        }
+        phase[used] = ph;
+        CPM_HIP(hipEventRecord(evb[used], st));
     }
     void end(int ph, hipStream_t st) {
         if (!on) return;
-        CPM_HIP(hipEventRecord(e[ph], st));
-        CPM_HIP(hipEventSynchronize(e[ph]));
-        float t = 0;
-        CPM_HIP(hipEventElapsedTime(&t, b[ph], e[ph]));
-        ms[ph] += t;
-        ++cnt[ph];
+        CPM_HIP(hipEventRecord(eve[used], st));
+        ++used;
+        (void)ph;
+    }
+    // Folds the queued pairs into the totals (synchronises on the last event).
+    void collect() {
+        if (used == 0) return;
+        CPM_HIP(hipEventSynchronize(eve[used - 1]));
+        for (size_t i = 0; i < used; ++i) {
+            float t = 0;
+            CPM_HIP(hipEventElapsedTime(&t, evb[i], eve[i]));
+            ms[phase[i]] += t;
+            ++cnt[phase[i]];
+        }
+        used = 0;
     }
     ~Prof() {
-        for (int i = 0; i < PM_NPHASES; ++i) {
-            if (b[i]) (void)hipEventDestroy(b[i]);
-            if (e[i]) (void)hipEventDestroy(e[i]);
+        for (size_t i = 0; i < evb.size(); ++i) {
+            (void)hipEventDestroy(evb[i]);
+            (void)hipEventDestroy(eve[i]);
         }
     }
 };
@@ -236,8 +257,13 @@ struct Impl : ImplBase {
         }
         for (int stage = 0; stage < 5; ++stage) {
             prof.begin(2 + stage, st);
-            fft.poisson_stage(stage, mesh.p, mesh.p, spec.p, nyq.p, shift, mean_ptr, kf, s, st);
+            fft.poisson_stage(stage, 0, mesh.p, mesh.p, spec.p, nyq.p, shift, mean_ptr, kf, s, st);
             prof.end(2 + stage, st);
+            if (stage >= 1 && stage <= 3) {
+                prof.begin(9, st);
+                fft.poisson_stage(stage, 1, mesh.p, mesh.p, spec.p, nyq.p, shift, mean_ptr, kf, s, st);
+                prof.end(9, st);
+            }
         }
     }
 
@@ -694,6 +720,7 @@ int pm_diagnostics(pm_handle* h, pm_diag* d) {
 int pm_profile(pm_handle* h, int enable, double* ms, long long* counts) {
     PM_NEED(h)
     PM_TRY
+    h->impl->prof.collect();
     if (ms)
         for (int i = 0; i < PM_NPHASES; ++i) ms[i] = h->impl->prof.ms[i];
     if (counts)

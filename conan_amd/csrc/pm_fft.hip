@@ -432,33 +432,33 @@ void FFTPlan<T>::backward(cx<T>* spec, cx<T>* nyq, T* out, double s, hipStream_t
 }
 
 template <typename T>
-void FFTPlan<T>::poisson_stage(int stage, const T* rho, T* phi, cx<T>* spec, cx<T>* nyq, T shift,
+void FFTPlan<T>::poisson_stage(int stage, int part, const T* rho, T* phi, cx<T>* spec, cx<T>* nyq, T shift,
                                const double* mean_ptr, double kf, double s, hipStream_t st) const {
     ColGeom ym, yn, zm, zn;
     geoms(&ym, &yn, &zm, &zn);
     const GreenGeom gmain{dim, 0, kf}, gnyq{dim, 1, kf};
     switch (stage) {
         case 0:
-            launch_rows_r2c<T>(N, rho, spec, nyq, nrows, shift, mean_ptr, tw, st);
+            if (part == 0) launch_rows_r2c<T>(N, rho, spec, nyq, nrows, shift, mean_ptr, tw, st);
             break;
         case 1:
             if (dim == 3) {
-                launch_cols<T>(N, -1, spec, ym, tw, st);
-                launch_cols<T>(N, -1, nyq, yn, tw, st);
+                if (part == 0) launch_cols<T>(N, -1, spec, ym, tw, st);
+                else launch_cols<T>(N, -1, nyq, yn, tw, st);
             }
             break;
         case 2:
-            launch_cols_green<T>(N, spec, dim == 3 ? zm : ym, gmain, tw, st);
-            launch_cols_green<T>(N, nyq, dim == 3 ? zn : yn, gnyq, tw, st);
+            if (part == 0) launch_cols_green<T>(N, spec, dim == 3 ? zm : ym, gmain, tw, st);
+            else launch_cols_green<T>(N, nyq, dim == 3 ? zn : yn, gnyq, tw, st);
             break;
         case 3:
             if (dim == 3) {
-                launch_cols<T>(N, +1, spec, ym, tw, st);
-                launch_cols<T>(N, +1, nyq, yn, tw, st);
+                if (part == 0) launch_cols<T>(N, +1, spec, ym, tw, st);
+                else launch_cols<T>(N, +1, nyq, yn, tw, st);
             }
             break;
         case 4:
-            launch_rows_c2r<T>(N, spec, nyq, phi, nrows, s, tw, st);
+            if (part == 0) launch_rows_c2r<T>(N, spec, nyq, phi, nrows, s, tw, st);
             break;
         default:
             throw Error(-1, "bad Poisson stage");
@@ -468,7 +468,9 @@ void FFTPlan<T>::poisson_stage(int stage, const T* rho, T* phi, cx<T>* spec, cx<
 template <typename T>
 void FFTPlan<T>::poisson(const T* rho, T* phi, cx<T>* spec, cx<T>* nyq, T shift,
                          const double* mean_ptr, double kf, double s, hipStream_t st) const {
-    for (int stage = 0; stage < 5; ++stage) poisson_stage(stage, rho, phi, spec, nyq, shift, mean_ptr, kf, s, st);
+    for (int stage = 0; stage < 5; ++stage)
+        for (int part = 0; part < 2; ++part)
+            poisson_stage(stage, part, rho, phi, spec, nyq, shift, mean_ptr, kf, s, st);
 }
 
 template struct FFTPlan<float>;

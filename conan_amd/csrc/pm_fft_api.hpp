@@ -44,8 +44,9 @@ struct FFTPlan {
     void backward(cx<T>* spec, cx<T>* nyq, T* out, double s, hipStream_t st) const;
     // phi = IFFT[ FFT(rho - shift) * (-1/|k|^2) ] / s, in 2*dim - 1 sweeps.
     // One sweep of poisson(): 0 rows r2c, 1 y forward (3-D), 2 last axis with
-    // the Green's function, 3 y backward (3-D), 4 rows c2r.
-    void poisson_stage(int stage, const T* rho, T* phi, cx<T>* spec, cx<T>* nyq, T shift,
+    // the Green's function, 3 y backward (3-D), 4 rows c2r; part 0 launches the
+    // main block (and the rows), part 1 the kx = N/2 plane.
+    void poisson_stage(int stage, int part, const T* rho, T* phi, cx<T>* spec, cx<T>* nyq, T shift,
                        const double* mean_ptr, double kf, double s, hipStream_t st) const;
     void poisson(const T* rho, T* phi, cx<T>* spec, cx<T>* nyq, T shift, const double* mean_ptr,
                  double kf, double s, hipStream_t st) const;
