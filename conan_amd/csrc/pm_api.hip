@@ -156,8 +156,11 @@ struct Impl : ImplBase {
     PosArgs pa;
     FFTPlan<Mesh> fft;
     RadixSorter sorter;
-    DevBuf<Pos> X;
-    DevBuf<double> P;
+    DevBuf<Pos> X, X2;
+    DevBuf<double> P, P2;
+    // FX keeps particles in the cell order of the last deposit; id[s] is the
+    // mass-grid index of the particle stored at s (X2/P2/id2: the next order).
+    DevBuf<uint32_t> id, id2;
     DevBuf<uint32_t> k0, v0, k1, v1, row_start;
     DevBuf<Mesh> mesh;
     DevBuf<cx<Mesh>> spec, nyq;
@@ -165,6 +168,7 @@ struct Impl : ImplBase {
     const uint32_t* skeys = nullptr;
     const uint32_t* spids = nullptr;
     bool have_deposit = false;
+    long npartials = 0;  // FP64 mesh-sum partials written by the last deposit
 
     static long ipow(long n, int r) {
         long s = 1;
@@ -174,7 +178,9 @@ struct Impl : ImplBase {
 
     explicit Impl(const pm_config& c)
         : cfg(c), R(c.dim), lgNf(ilog2_exact(c.Nf, "Nf")), Np(ipow(c.Nm, c.dim)), M(ipow(c.Nf, c.dim)),
-          nrows(M / c.Nf), cos(c.H0, c.Omega_m, c.Omega_L), fft(c.dim, c.Nf), sorter(Np, c.dim * lgNf) {
+          nrows(M / c.Nf), cos(c.H0, c.Omega_m, c.Omega_L), fft(c.dim, c.Nf),
+          // FP64: (cell, id) pairs sorted by the full cell index; FX: by mesh row
+          sorter(Np, (FX ? c.dim - 1 : c.dim) * lgNf) {
         mass = pow(double(c.Nf) / c.Nm, R);  // src/run/nbody.hh:85
         pa.L = c.L;
         pa.res = c.L / c.Nf;
@@ -182,9 +188,16 @@ struct Impl : ImplBase {
         pa.fshift = 32 - lgNf;
         pa.fmask = (uint32_t)((1ull << pa.fshift) - 1);
         pa.fscale = (float)ldexp(1.0, -pa.fshift);
+        pa.kshift = FX ? lgNf : 0;
         CPM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         X.alloc((size_t)R * Np);
         P.alloc((size_t)R * Np);
+        if (FX) {
+            X2.alloc((size_t)R * Np);
+            P2.alloc((size_t)R * Np);
+            id.alloc(Np);
+            id2.alloc(Np);
+        }
         k0.alloc(Np);
         v0.alloc(Np);
         k1.alloc(Np);
@@ -207,6 +220,31 @@ struct Impl : ImplBase {
     void need_state() const {
         if (!have_state) throw Error(PM_ESTATE, "no particle state: call pm_init_zeldovich or pm_set_state first");
     }
+
+    // Storage order = mass-grid order (set after ZA / set_state).
+    void reset_ids() {
+        if (!FX) return;
+        std::vector<uint32_t> h(Np);
+        for (long i = 0; i < Np; ++i) h[i] = (uint32_t)i;
+        CPM_HIP(hipMemcpyAsync(id.p, h.data(), sizeof(uint32_t) * Np, hipMemcpyHostToDevice, st));
+        CPM_HIP(hipStreamSynchronize(st));
+    }
+    // Mass-grid index of each stored particle.
+    std::vector<uint32_t> host_ids() {
+        std::vector<uint32_t> h(Np);
+        if (FX) {
+            CPM_HIP(hipMemcpyAsync(h.data(), id.p, sizeof(uint32_t) * Np, hipMemcpyDeviceToHost, st));
+            CPM_HIP(hipStreamSynchronize(st));
+        } else {
+            for (long i = 0; i < Np; ++i) h[i] = (uint32_t)i;
+        }
+        return h;
+    }
+    // SoA in storage order -> AoS in mass-grid order.
+    void soa_to_aos(const std::vector<double>& soa, const std::vector<uint32_t>& ids, double* out) const {
+        for (long s = 0; s < Np; ++s)
+            for (int k = 0; k < R; ++k) out[(size_t)ids[s] * R + k] = soa[(size_t)k * Np + s];
+    }
     double drift_factor(double a, double da) const {
         double adot = cos.adot(a);
         double c = da / (a * a * adot);  // src/run/nbody.hh:124,129
@@ -221,6 +259,7 @@ struct Impl : ImplBase {
         CPM_HIP(hipMemcpyAsync(d.p, phi, sizeof(double) * npm, hipMemcpyHostToDevice, st));
         ParticleOps<PP>::zeldovich(R, d.p, ilog2(cfg.Nm), cfg.L, a_pos, a_mom, X.p, P.p, st);
         CPM_HIP(hipStreamSynchronize(st));
+        reset_ids();
         last_apos = a_pos;
         last_avel = a_mom;
         have_state = true;
@@ -233,11 +272,17 @@ struct Impl : ImplBase {
         const int where = sorter.sort(k0.p, v0.p, k1.p, v1.p, st);
         skeys = where ? k1.p : k0.p;
         spids = where ? v1.p : v0.p;
-        row_bounds(skeys, Np, lgNf, nrows, row_start.p, st);
+        row_bounds(skeys, Np, FX ? 0 : lgNf, nrows, row_start.p, st);
         prof.end(0, st);
         prof.begin(1, st);
-        ParticleOps<PP>::deposit(R, row_start.p, skeys, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p, nrows,
-                                 st);
+        if constexpr (FX) {
+            deposit_fx_push(R, row_start.p, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p, st);
+            npartials = deposit_fx_partials(R, lgNf);
+        } else {
+            ParticleOps<PP>::deposit(R, row_start.p, skeys, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p,
+                                     nrows, st);
+            npartials = nrows;
+        }
         prof.end(1, st);
         have_deposit = true;
     }
@@ -252,7 +297,7 @@ struct Impl : ImplBase {
         Mesh shift = (Mesh)1.0;  // delta = rho - 1 (src/run/nbody.hh:103-104)
         if (FX) {
             // FP64 mean of the FP32 deposit, subtracted on the device
-            if (!mean_ready) sum_partials(row_partial.p, nrows, 1.0 / (double)M, scal.p, st);
+            if (!mean_ready) sum_partials(row_partial.p, npartials, 1.0 / (double)M, red.p, 4096, scal.p, st);
             mean_ptr = scal.p;
         }
         for (int stage = 0; stage < 5; ++stage) {
@@ -283,7 +328,22 @@ struct Impl : ImplBase {
         const double ck = da / adot;  // src/run/nbody.hh:112-116
         const double cd = fuse ? drift_factor(a_next, da_next) : 0.0;
         prof.begin(7, st);
-        ParticleOps<PP>::gather_kick(R, X.p, P.p, Np, mesh.p, lgNf, pa, ck, cd, fuse ? 1 : 0, nullptr, st);
+        if constexpr (FX) {
+            // kick (+ drift) and move the particles into the deposit's cell order
+            Reorder<PP> ro;
+            ro.perm = spids;
+            ro.Xo = X2.p;
+            ro.Po = P2.p;
+            ro.idi = id.p;
+            ro.ido = id2.p;
+            ParticleOps<PP>::gather_kick(R, X.p, P.p, Np, mesh.p, lgNf, pa, ck, cd, fuse ? 1 : 0, nullptr, ro, st);
+            std::swap(X.p, X2.p);
+            std::swap(P.p, P2.p);
+            std::swap(id.p, id2.p);
+        } else {
+            ParticleOps<PP>::gather_kick(R, X.p, P.p, Np, mesh.p, lgNf, pa, ck, cd, fuse ? 1 : 0, nullptr,
+                                         Reorder<PP>(), st);
+        }
         prof.end(7, st);
         last_avel = a;
         if (fuse) last_apos = a_next;
@@ -317,6 +377,7 @@ struct Impl : ImplBase {
     void get_state(double* Xh, double* Ph) override {
         need_state();
         std::vector<double> tmp((size_t)R * Np);
+        const std::vector<uint32_t> ids = host_ids();
         if (Xh) {
             if (FX) {
                 DevBuf<double> d((size_t)R * Np);
@@ -327,14 +388,12 @@ struct Impl : ImplBase {
                 CPM_HIP(hipMemcpyAsync(tmp.data(), X.p, sizeof(double) * R * Np, hipMemcpyDeviceToHost, st));
                 CPM_HIP(hipStreamSynchronize(st));
             }
-            for (long i = 0; i < Np; ++i)
-                for (int k = 0; k < R; ++k) Xh[i * R + k] = tmp[(size_t)k * Np + i];
+            soa_to_aos(tmp, ids, Xh);
         }
         if (Ph) {
             CPM_HIP(hipMemcpyAsync(tmp.data(), P.p, sizeof(double) * R * Np, hipMemcpyDeviceToHost, st));
             CPM_HIP(hipStreamSynchronize(st));
-            for (long i = 0; i < Np; ++i)
-                for (int k = 0; k < R; ++k) Ph[i * R + k] = tmp[(size_t)k * Np + i];
+            soa_to_aos(tmp, ids, Ph);
         }
     }
 
@@ -363,11 +422,13 @@ struct Impl : ImplBase {
             for (long i = 0; i < Np; ++i)
                 for (int k = 0; k < R; ++k) tmp[(size_t)k * Np + i] = Xh[i * R + k];
             upload_positions_soa(tmp.data(), false);
+            reset_ids();
         }
         if (Ph) {
             check_finite(Ph, (size_t)R * Np, "momenta");
-            for (long i = 0; i < Np; ++i)
-                for (int k = 0; k < R; ++k) tmp[(size_t)k * Np + i] = Ph[i * R + k];
+            const std::vector<uint32_t> ids = host_ids();  // momenta follow the stored order
+            for (long s = 0; s < Np; ++s)
+                for (int k = 0; k < R; ++k) tmp[(size_t)k * Np + s] = Ph[(size_t)ids[s] * R + k];
             CPM_HIP(hipMemcpyAsync(P.p, tmp.data(), sizeof(double) * R * Np, hipMemcpyHostToDevice, st));
             CPM_HIP(hipStreamSynchronize(st));
         } else if (!have_state) {
@@ -380,6 +441,7 @@ struct Impl : ImplBase {
 
     void set_positions_device(const double* Xsoa) override {
         upload_positions_soa(Xsoa, true);
+        reset_ids();
         CPM_HIP(hipMemsetAsync(P.p, 0, sizeof(double) * R * Np, st));
         CPM_HIP(hipStreamSynchronize(st));
         have_state = true;
@@ -413,9 +475,14 @@ struct Impl : ImplBase {
         do_deposit((Mesh)(mass_override > 0 ? mass_override : mass));
         if (rho) mesh_to_host(rho);
         if (keys) {
-            ParticleOps<PP>::keys(R, X.p, Np, lgNf, pa, k1.p, v1.p, st);
-            CPM_HIP(hipMemcpyAsync(keys, k1.p, sizeof(uint32_t) * Np, hipMemcpyDeviceToHost, st));
-            have_deposit = false;  // k1 was reused: the sorted keys are gone
+            PosArgs pk = pa;
+            pk.kshift = 0;  // the full cell index idx(origin)
+            ParticleOps<PP>::keys(R, X.p, Np, lgNf, pk, k1.p, v1.p, st);
+            std::vector<uint32_t> k(Np);
+            CPM_HIP(hipMemcpyAsync(k.data(), k1.p, sizeof(uint32_t) * Np, hipMemcpyDeviceToHost, st));
+            CPM_HIP(hipStreamSynchronize(st));
+            const std::vector<uint32_t> ids = host_ids();
+            for (long s = 0; s < Np; ++s) keys[ids[s]] = (int32_t)k[s];
         }
         CPM_HIP(hipStreamSynchronize(st));
     }
@@ -444,12 +511,11 @@ struct Impl : ImplBase {
 
     void gather_to_host(double* F) {
         DevBuf<double> d((size_t)R * Np);
-        ParticleOps<PP>::gather_kick(R, X.p, P.p, Np, mesh.p, lgNf, pa, 0.0, 0.0, 0, d.p, st);
+        ParticleOps<PP>::gather_kick(R, X.p, P.p, Np, mesh.p, lgNf, pa, 0.0, 0.0, 0, d.p, Reorder<PP>(), st);
         std::vector<double> tmp((size_t)R * Np);
         CPM_HIP(hipMemcpyAsync(tmp.data(), d.p, sizeof(double) * R * Np, hipMemcpyDeviceToHost, st));
         CPM_HIP(hipStreamSynchronize(st));
-        for (long i = 0; i < Np; ++i)
-            for (int k = 0; k < R; ++k) F[i * R + k] = tmp[(size_t)k * Np + i];
+        soa_to_aos(tmp, host_ids(), F);
     }
 
     void poisson_mesh(const double* rho, double a, double* phi) override {
@@ -473,7 +539,7 @@ struct Impl : ImplBase {
         }
         const double* mean_ptr = nullptr;
         if (FX) {
-            sum_partials(row_partial.p, nrows, 1.0 / (double)M, scal.p, st);
+            sum_partials(row_partial.p, npartials, 1.0 / (double)M, red.p, 4096, scal.p, st);
             mean_ptr = scal.p;
         }
         fft.forward(mesh.p, spec.p, nyq.p, (Mesh)1.0, mean_ptr, st);
@@ -497,7 +563,7 @@ struct Impl : ImplBase {
         need_state();
         std::memset(d, 0, sizeof(*d));
         double h[8] = {0};
-        if (have_deposit) sum_partials(row_partial.p, nrows, 1.0, scal.p + 2, st);
+        if (have_deposit) sum_partials(row_partial.p, npartials, 1.0, red.p, 4096, scal.p + 2, st);
         for (int k = 0; k < R; ++k) reduce_sum(P.p + (size_t)k * Np, Np, 0, 1.0, red.p, 4096, scal.p + 3 + k, st);
         reduce_sum(P.p, (long)R * Np, 1, 1.0, red.p, 4096, scal.p + 6, st);
         CPM_HIP(hipMemcpyAsync(h, scal.p + 2, sizeof(double) * 5, hipMemcpyDeviceToHost, st));

@@ -92,9 +92,11 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_r2c(
     using C = RowCfg<T, N>;
     constexpr int H = C::H, P = C::P, NT = C::NT;
     __shared__ cx<T> sh[P * H];
+    __shared__ cx<T> stw[N];
     const int tid = threadIdx.x;
     const long row0 = (long)blockIdx.x * P;
     const T sft = mean_ptr ? (T)(*mean_ptr) : shift;
+    for (int m = tid; m < N; m += NT) stw[m] = tw[m];
     for (int e = tid; e < P * H; e += NT) {
         const int p = e / H, n = e % H;
         const long row = row0 + p;
@@ -106,7 +108,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_r2c(
         sh[lidx<H>(p, n)] = z;
     }
     __syncthreads();
-    lds_fft<T, H, C::E, NT, -1>(sh, tw, 2, tid);
+    lds_fft<T, H, C::E, NT, -1>(sh, stw, 2, tid);
     for (int e = tid; e < P * H; e += NT) {
         const int p = e / H, k = e % H;
         const long row = row0 + p;
@@ -114,7 +116,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_r2c(
         const cx<T> Zk = sh[lidx<H>(p, k)];
         const cx<T> Zm = conj(sh[lidx<H>(p, (H - k) & (H - 1))]);
         const cx<T> A = scale(Zk + Zm, (T)0.5);
-        const cx<T> WB = tw[k] * (Zk - Zm);
+        const cx<T> WB = stw[k] * (Zk - Zm);
         spec[row * H + k] = mk<T>(A.re + (T)0.5 * WB.im, A.im - (T)0.5 * WB.re);
         if (k == 0) nyq[row] = mk<T>(Zk.re - Zk.im, (T)0);
     }
@@ -129,8 +131,10 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_c2r(
     constexpr int H = C::H, P = C::P, NT = C::NT, E = C::E;
     __shared__ cx<T> sh[P * H];
     __shared__ cx<T> shn[P];
+    __shared__ cx<T> stw[N];
     const int tid = threadIdx.x;
     const long row0 = (long)blockIdx.x * P;
+    for (int m = tid; m < N; m += NT) stw[m] = tw[m];
     for (int e = tid; e < P * H; e += NT) {
         const int p = e / H, n = e % H;
         const long row = row0 + p;
@@ -146,7 +150,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_c2r(
         const cx<T> Xk = sh[lidx<H>(p, k)];
         const cx<T> Xm = conj(k == 0 ? shn[p] : sh[lidx<H>(p, H - k)]);
         const cx<T> Ep = Xk + Xm;
-        const cx<T> wO = conj(tw[k]) * (Xk - Xm);
+        const cx<T> wO = conj(stw[k]) * (Xk - Xm);
         z[i] = mk<T>(Ep.re - wO.im, Ep.im + wO.re);
     }
     __syncthreads();
@@ -156,7 +160,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_c2r(
         sh[lidx<H>(e / H, e % H)] = z[i];
     }
     __syncthreads();
-    lds_fft<T, H, E, NT, +1>(sh, tw, 2, tid);
+    lds_fft<T, H, E, NT, +1>(sh, stw, 2, tid);
     const T inv = (T)(1.0 / s);
     for (int e = tid; e < P * H; e += NT) {
         const int p = e / H, n = e % H;
@@ -212,11 +216,13 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols(cx<T>* __restrict__
                                                            const cx<T>* __restrict__ tw) {
     using C = ColCfg<T, N>;
     __shared__ cx<T> sh[C::C * N];
+    __shared__ cx<T> stw[N];
     const int tid = threadIdx.x;
     const long c0 = (long)blockIdx.x * C::C;
+    for (int m = tid; m < N; m += C::NT) stw[m] = tw[m];
     col_load<T, N>(sh, data, g, c0, tid);
     __syncthreads();
-    lds_fft<T, N, C::E, C::NT, DIR>(sh, tw, 1, tid);
+    lds_fft<T, N, C::E, C::NT, DIR>(sh, stw, 1, tid);
     col_store<T, N>(sh, data, g, c0, tid);
 }
 
@@ -249,11 +255,13 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols_green(cx<T>* __rest
                                                                  const cx<T>* __restrict__ tw) {
     using C = ColCfg<T, N>;
     __shared__ cx<T> sh[C::C * N];
+    __shared__ cx<T> stw[N];
     const int tid = threadIdx.x;
     const long c0 = (long)blockIdx.x * C::C;
+    for (int m = tid; m < N; m += C::NT) stw[m] = tw[m];
     col_load<T, N>(sh, data, g, c0, tid);
     __syncthreads();
-    lds_fft<T, N, C::E, C::NT, -1>(sh, tw, 1, tid);
+    lds_fft<T, N, C::E, C::NT, -1>(sh, stw, 1, tid);
     const int H = N / 2;
     const T kf = (T)gg.kf;
     for (int e = tid; e < C::C * N; e += C::NT) {
@@ -273,7 +281,7 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols_green(cx<T>* __rest
         sh[li] = (ix == 0 && iy == 0 && iz == 0) ? mk<T>(0, 0) : scale(sh[li], gv);
     }
     __syncthreads();
-    lds_fft<T, N, C::E, C::NT, +1>(sh, tw, 1, tid);
+    lds_fft<T, N, C::E, C::NT, +1>(sh, stw, 1, tid);
     col_store<T, N>(sh, data, g, c0, tid);
 }
 

@@ -33,7 +33,7 @@ __global__ __launch_bounds__(256) void k_keys(const typename PP::Pos* __restrict
         PP::dep(X[k * np + i], pa, o, f);
         key |= (uint32_t)(o & mask) << (k * lgN);  // Box::idx of the origin (box.hh:75-81)
     }
-    keys[i] = key;
+    keys[i] = key >> pa.kshift;  // kshift = log2 N: the mesh row of the origin
     pids[i] = (uint32_t)i;
 }
 
@@ -135,6 +135,85 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit(const uint32_t* __restrict__
     if (tid == 0) row_partial[row] = red[0];
 }
 
+// ------------------------------------------------------------ FX: persistent row order
+// The FX form keeps its particle arrays grouped by mesh row (y, z) in the
+// order of the previous deposit: `perm` maps sorted position s -> storage
+// index, and particles move less than a cell per step, so perm is close to
+// the identity and reads through it are nearly contiguous.
+//
+// One workgroup owns a tile of FX_TY x FX_TZ mesh rows and pushes the CIC
+// weights of every particle whose origin row lies in the tile or its lower
+// halo into an LDS copy of the tile.  The accumulators are 64-bit fixed point
+// (2^-40 mass units): integer addition is associative, so the result does not
+// depend on the order of the LDS atomics and the deposit is deterministic; it
+// is rounded once to FP32 on the way out.
+static constexpr int FX_TY = 2, FX_TZ = 2, FX_NT = 512;
+static constexpr double FX_ONE = 1099511627776.0;  // 2^40
+
+template <int R>
+__global__ __launch_bounds__(FX_NT) void k_deposit_fx_push(const uint32_t* __restrict__ row_start,
+                                                           const uint32_t* __restrict__ perm,
+                                                           const uint32_t* __restrict__ X, long np, int lgN,
+                                                           PosArgs pa, float mass, float* __restrict__ mesh,
+                                                           double* __restrict__ partial) {
+    constexpr int TZ = (R == 3) ? FX_TZ : 1;
+    constexpr int TILE = FX_TY * TZ * 1024;  // cells of the largest mesh (N <= 1024)
+    __shared__ unsigned long long acc[TILE];
+    __shared__ double red[FX_NT];
+    const int N = 1 << lgN, msk = N - 1;
+    const int y0 = blockIdx.x * FX_TY;
+    const int z0 = (R == 3) ? (int)blockIdx.y * TZ : 0;
+    const int tid = threadIdx.x;
+    const int ncell = FX_TY * TZ * N;
+    for (int e = tid; e < ncell; e += FX_NT) acc[e] = 0ull;
+    __syncthreads();
+    // origin rows y0-1 .. y0+TY-1 (and z0-1 .. z0+TZ-1)
+    for (int dz = 0; dz <= (R == 3 ? TZ : 0); ++dz) {
+        for (int dy = 0; dy <= FX_TY; ++dy) {
+            long orow = (long)((y0 - 1 + dy) & msk);
+            if (R == 3) orow += (long)((z0 - 1 + dz) & msk) << lgN;
+            const uint32_t s0 = row_start[orow], s1 = row_start[orow + 1];
+            for (uint32_t s = s0 + tid; s < s1; s += FX_NT) {
+                const uint32_t i = perm[s];
+                int o[R];
+                float f[R];
+#pragma unroll
+                for (int k = 0; k < R; ++k) PosFX::dep(X[k * np + i], pa, o[k], f[k]);
+#pragma unroll
+                for (int j = 0; j < (1 << R); ++j) {
+                    const int bx = j & 1, by = (j >> 1) & 1, bz = (j >> 2) & 1;
+                    const int ty = dy - 1 + by, tz = (R == 3) ? dz - 1 + bz : 0;
+                    if (ty < 0 || ty >= FX_TY || tz < 0 || tz >= TZ) continue;
+                    float w = mass;
+                    w *= bx ? f[0] : 1.f - f[0];
+                    w *= by ? f[1] : 1.f - f[1];
+                    if (R == 3) w *= bz ? f[2] : 1.f - f[2];
+                    const int tx = (o[0] + bx) & msk;
+                    const long long q = __double2ll_rn((double)w * FX_ONE);
+                    atomicAdd(&acc[(tz * FX_TY + ty) * N + tx], (unsigned long long)q);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    double rsum = 0;
+    for (int e = tid; e < ncell; e += FX_NT) {
+        const int tx = e & msk, t = e >> lgN;
+        const int ty = t % FX_TY, tz = t / FX_TY;
+        const float v = (float)((double)(long long)acc[e] * (1.0 / FX_ONE));
+        const long row = (long)((y0 + ty) & msk) + (R == 3 ? (long)((z0 + tz) & msk) << lgN : 0);
+        mesh[row * N + tx] = v;
+        rsum += (double)v;
+    }
+    red[tid] = rsum;
+    __syncthreads();
+    for (int off = FX_NT / 2; off > 0; off >>= 1) {
+        if (tid < off) red[tid] += red[tid + off];
+        __syncthreads();
+    }
+    if (tid == 0) partial[(long)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
 // ------------------------------------------------------------ gather + kick (+ drift)
 // P -= grad(phi)(X) * ck with the trilinear interpolation of the 4-point FD
 // gradient at the 2^R surrounding nodes (src/run/nbody.hh:112-117,
@@ -145,14 +224,22 @@ __device__ __forceinline__ T load_mesh(const T* __restrict__ phi, long idx) {
     return phi[idx];
 }
 
+// With ro.perm set, thread s reads particle perm[s] and writes the updated
+// particle (and its id) to position s of the output arrays: the kick also
+// moves the particles into the cell order of the deposit.
 template <typename PP, int R>
 __global__ __launch_bounds__(256) void k_gather_kick(typename PP::Pos* __restrict__ X, double* __restrict__ P,
                                                       long np, const typename PP::Mesh* __restrict__ phi,
                                                       int lgN, PosArgs pa, double ck, double cd, int cd_on,
-                                                      double* __restrict__ Fout) {
+                                                      double* __restrict__ Fout, Reorder<PP> ro) {
     using M = typename PP::Mesh;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np) return;
+    const long i = ro.perm ? (long)ro.perm[s] : s;  // input index
+    const long w = ro.perm ? s : i;                   // output index
+    typename PP::Pos* __restrict__ Xo = ro.perm ? ro.Xo : X;
+    double* __restrict__ Po = ro.perm ? ro.Po : P;
+    if (ro.perm) ro.ido[s] = ro.idi[i];
     const int N = 1 << lgN, msk = N - 1;
     int o[R];
     M a1[R];
@@ -242,14 +329,16 @@ __global__ __launch_bounds__(256) void k_gather_kick(typename PP::Pos* __restric
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const double p = P[k * np + i] - (double)F[k] * ck;
-        P[k * np + i] = p;
+        Po[k * np + w] = p;
+        typename PP::Pos xn = xs[k];
         if (cd_on) {
             if constexpr (sizeof(M) == 8) {
-                X[k * np + i] = xs[k] + p * cd;
+                xn = xs[k] + p * cd;
             } else {
-                X[k * np + i] = xs[k] + (uint32_t)__double2ll_rn(p * cd);
+                xn = xs[k] + (uint32_t)__double2ll_rn(p * cd);
             }
         }
+        if (cd_on || ro.perm) Xo[k * np + w] = xn;
     }
 }
 
@@ -463,9 +552,24 @@ void ParticleOps<PP>::deposit(int R, const uint32_t* row_start, const uint32_t* 
 template <typename PP>
 void ParticleOps<PP>::gather_kick(int R, Pos* X, double* P, long np, const Mesh* phi, int lgN,
                                   const PosArgs& pa, double ck, double cd, int cd_on, double* Fout,
-                                  hipStream_t st) {
+                                  const Reorder<PP>& ro, hipStream_t st) {
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_kick<PP, RR>), dim3(nblk(np, 256)), dim3(256), 0, st, X,
-                                         P, np, phi, lgN, pa, ck, cd, cd_on, Fout));
+                                         P, np, phi, lgN, pa, ck, cd, cd_on, Fout, ro));
+    CPM_LAUNCH_CHECK();
+}
+
+long deposit_fx_partials(int R, int lgN) {
+    const long n = 1L << lgN;
+    return R == 3 ? (n / FX_TY) * (n / FX_TZ) : n / FX_TY;
+}
+
+void deposit_fx_push(int R, const uint32_t* row_start, const uint32_t* perm, const uint32_t* X, long np, int lgN,
+                     const PosArgs& pa, float mass, float* mesh, double* partial, hipStream_t st) {
+    const unsigned n = 1u << lgN;
+    if (n % FX_TY) throw Error(-2, "mesh too small for the deposit tile");
+    const dim3 grid(n / FX_TY, R == 3 ? n / FX_TZ : 1);
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit_fx_push<RR>), grid, dim3(FX_NT), 0, st, row_start, perm, X,
+                                         np, lgN, pa, mass, mesh, partial));
     CPM_LAUNCH_CHECK();
 }
 
@@ -499,9 +603,9 @@ void reduce_sum(const T* a, long n, int square, double scale, double* partials, 
     CPM_LAUNCH_CHECK();
 }
 
-void sum_partials(const double* partials, long n, double scale, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_NT), 0, st, partials, n, scale, out);
-    CPM_LAUNCH_CHECK();
+void sum_partials(const double* partials, long n, double scale, double* scratch, long scratch_n, double* out,
+                  hipStream_t st) {
+    reduce_sum<double>(partials, n, 0, scale, scratch, scratch_n, out, st);
 }
 
 void fx_to_f64(const uint32_t* q, long n, double L, double* x, hipStream_t st) {

@@ -12,6 +12,7 @@ struct PosArgs {
     int fshift;      // FX: 32 - log2 N
     uint32_t fmask;  // FX: (1 << fshift) - 1
     float fscale;    // FX: 2^-fshift
+    int kshift;      // sort keys: cell index >> kshift (0: cell, log2 N: row)
 };
 
 // src/base/misc.hh:144-148
@@ -60,6 +61,16 @@ struct PosFX {
     static __device__ __forceinline__ void gat(Pos x, const PosArgs& a, int& o, float& f) { dep(x, a, o, f); }
 };
 
+// Optional reordering of the gather + kick: read particle perm[s], write it
+// (and its id) to position s of the output arrays.  perm == nullptr: in place.
+template <typename PP>
+struct Reorder {
+    const uint32_t* perm = nullptr;
+    typename PP::Pos* Xo = nullptr;
+    double* Po = nullptr;
+    const uint32_t* idi = nullptr;
+    uint32_t* ido = nullptr;
+};
 
 template <typename PP>
 struct ParticleOps {
@@ -71,17 +82,24 @@ struct ParticleOps {
                         const Pos* X, long np, int lgN, const PosArgs& pa, Mesh mass, Mesh* mesh,
                         double* row_partial, long nrows, hipStream_t st);
     static void gather_kick(int R, Pos* X, double* P, long np, const Mesh* phi, int lgN, const PosArgs& pa,
-                            double ck, double cd, int cd_on, double* Fout, hipStream_t st);
+                            double ck, double cd, int cd_on, double* Fout, const Reorder<PP>& ro, hipStream_t st);
     static void drift(Pos* X, const double* P, long n, double cd, hipStream_t st);
     static void zeldovich(int R, const double* phi, int lgNm, double L, double a_pos, double a_mom, Pos* X,
                           double* P, hipStream_t st);
 };
 
 void row_bounds(const uint32_t* keys, long np, int lgN, long nrows, uint32_t* row_start, hipStream_t st);
+// FX deposit from particles grouped by mesh row (perm: sorted position ->
+// storage index); writes deposit_fx_partials(R, lgN) FP64 partial sums.
+long deposit_fx_partials(int R, int lgN);
+void deposit_fx_push(int R, const uint32_t* row_start, const uint32_t* perm, const uint32_t* X, long np, int lgN,
+                     const PosArgs& pa, float mass, float* mesh, double* partial, hipStream_t st);
 template <typename T>
 void reduce_sum(const T* a, long n, int square, double scale, double* partials, long max_partials,
                 double* out, hipStream_t st);
-void sum_partials(const double* partials, long n, double scale, double* out, hipStream_t st);
+// Deterministic two-level sum of n partials (scratch: scratch_n doubles, not aliasing).
+void sum_partials(const double* partials, long n, double scale, double* scratch, long scratch_n, double* out,
+                  hipStream_t st);
 void subsample(int R, const double* X, int Nm, int s, double L, double* Y, hipStream_t st);
 void fx_to_f64(const uint32_t* q, long n, double L, double* x, hipStream_t st);
 void f64_to_fx(const double* x, long n, double L, uint32_t* q, hipStream_t st);

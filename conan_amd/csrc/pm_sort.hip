@@ -34,30 +34,67 @@ __global__ __launch_bounds__(SORT_NT) void k_rs_hist(const uint32_t* __restrict_
 
 // Each tile is walked in index order (round r, then thread); the rank of an
 // element among equal digits = count in earlier rounds + count in earlier
-// waves of this round + count in earlier lanes of its wave.
+// waves of this round + count in earlier lanes of its wave.  Elements are
+// first placed in LDS in (digit, rank) order and then written out so that
+// consecutive threads write consecutive addresses of one digit's run.
 __global__ __launch_bounds__(SORT_NT) void k_rs_scatter(const uint32_t* __restrict__ kin,
                                                         const uint32_t* __restrict__ vin,
                                                         uint32_t* __restrict__ kout,
                                                         uint32_t* __restrict__ vout, long n, int shift,
                                                         int dbits, const uint32_t* __restrict__ offsets,
                                                         int nblocks) {
-    __shared__ uint32_t run[MAX_RADIX];
+    __shared__ uint32_t run[MAX_RADIX];   // next local position per digit
+    __shared__ uint32_t loff[MAX_RADIX];  // local start of each digit's run
+    __shared__ uint32_t goff[MAX_RADIX];  // global start of each digit's run
     __shared__ uint32_t wcnt[SORT_WAVES][MAX_RADIX];
+    __shared__ uint32_t lk[SORT_TILE], lv[SORT_TILE];
     const int radix = 1 << dbits;
     const uint32_t dmask = (uint32_t)radix - 1;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const long tile0 = (long)blockIdx.x * SORT_TILE;
+    const int tn = (int)((n - tile0) < SORT_TILE ? (n - tile0) : SORT_TILE);
+    // local histogram of the tile -> local offsets
     for (int d = tid; d < radix; d += SORT_NT) {
-        run[d] = offsets[(long)d * nblocks + blockIdx.x];
+        run[d] = 0;
+        goff[d] = offsets[(long)d * nblocks + blockIdx.x];
 #pragma unroll
         for (int w = 0; w < SORT_WAVES; ++w) wcnt[w][d] = 0;
     }
     __syncthreads();
-    const long tile0 = (long)blockIdx.x * SORT_TILE;
+    uint32_t kr[SORT_ITEMS];
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; ++r) {
+        const int le = r * SORT_NT + tid;
+        kr[r] = le < tn ? kin[tile0 + le] : 0u;
+        if (le < tn) atomicAdd(&run[(kr[r] >> shift) & dmask], 1u);
+    }
+    __syncthreads();
+    {
+        // exclusive scan of the radix counts: thread t owns digits 2t, 2t+1
+        uint32_t* sc = wcnt[0];
+        const int d0 = 2 * tid;
+        const uint32_t c0 = d0 < radix ? run[d0] : 0u, c1 = d0 + 1 < radix ? run[d0 + 1] : 0u;
+        sc[tid] = c0 + c1;
+        __syncthreads();
+        for (int off = 1; off < SORT_NT; off <<= 1) {
+            const uint32_t t = tid >= off ? sc[tid - off] : 0u;
+            __syncthreads();
+            sc[tid] += t;
+            __syncthreads();
+        }
+        const uint32_t excl = sc[tid] - (c0 + c1);
+        __syncthreads();
+        if (d0 < radix) loff[d0] = run[d0] = excl;
+        if (d0 + 1 < radix) loff[d0 + 1] = run[d0 + 1] = excl + c0;
+        sc[tid] = 0;
+        if (SORT_NT + tid < MAX_RADIX) sc[SORT_NT + tid] = 0;
+    }
+    __syncthreads();
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int r = 0; r < SORT_ITEMS; ++r) {
-        const long e = tile0 + (long)r * SORT_NT + tid;
-        const bool valid = e < n;
-        const uint32_t key = valid ? kin[e] : 0u;
+        const int le = r * SORT_NT + tid;
+        const bool valid = le < tn;
+        const uint32_t key = kr[r];
         const uint32_t d = (key >> shift) & dmask;
         uint64_t mask = __ballot(valid);
         for (int b = 0; b < dbits; ++b) {
@@ -72,8 +109,8 @@ __global__ __launch_bounds__(SORT_NT) void k_rs_scatter(const uint32_t* __restri
             uint32_t pos = run[d];
             for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
             pos += (uint32_t)rank;
-            kout[pos] = key;
-            vout[pos] = vin[e];
+            lk[pos] = key;
+            lv[pos] = vin[tile0 + le];
         }
         __syncthreads();
         for (int dd = tid; dd < radix; dd += SORT_NT) {
@@ -86,6 +123,13 @@ __global__ __launch_bounds__(SORT_NT) void k_rs_scatter(const uint32_t* __restri
             run[dd] += s;
         }
         __syncthreads();
+    }
+    for (int p = tid; p < tn; p += SORT_NT) {
+        const uint32_t key = lk[p];
+        const uint32_t d = (key >> shift) & dmask;
+        const uint32_t pos = goff[d] + ((uint32_t)p - loff[d]);
+        kout[pos] = key;
+        vout[pos] = lv[p];
     }
 }
 

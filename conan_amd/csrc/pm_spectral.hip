@@ -166,21 +166,36 @@ __global__ __launch_bounds__(PK_NT) void k_pk_partial(const cx<T>* __restrict__ 
     }
 }
 
-__global__ __launch_bounds__(64) void k_pk_final(const double* __restrict__ part, long nblocks, int nbins,
-                                                 double norm, double* __restrict__ kmean, double* __restrict__ pk,
-                                                 double* __restrict__ cnt) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nbins) return;
+// One block per bin: fixed strided per-thread sums, then a fixed LDS tree.
+__global__ __launch_bounds__(PK_NT) void k_pk_final(const double* __restrict__ part, long nblocks, int nbins,
+                                                    double norm, double* __restrict__ kmean,
+                                                    double* __restrict__ pk, double* __restrict__ cnt) {
+    __shared__ double s[3][PK_NT];
+    const int b = blockIdx.x;
     double P = 0, K = 0, W = 0;
-    for (long i = 0; i < nblocks; ++i) {
+    for (long i = threadIdx.x; i < nblocks; i += PK_NT) {
         const double* o = part + (i * nbins + b) * 3;
         P += o[0];
         K += o[1];
         W += o[2];
     }
-    cnt[b] = W;
-    kmean[b] = W > 0 ? K / W : 0.0;
-    pk[b] = W > 0 ? P / W * norm : 0.0;
+    s[0][threadIdx.x] = P;
+    s[1][threadIdx.x] = K;
+    s[2][threadIdx.x] = W;
+    __syncthreads();
+    for (int off = PK_NT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            for (int q = 0; q < 3; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        P = s[0][0];
+        K = s[1][0];
+        W = s[2][0];
+        cnt[b] = W;
+        kmean[b] = W > 0 ? K / W : 0.0;
+        pk[b] = W > 0 ? P / W * norm : 0.0;
+    }
 }
 
 long pk_partial_elems(long nmain, long nnyq, int nbins) {
@@ -197,7 +212,7 @@ void power_spectrum(const cx<T>* spec, const cx<T>* nyq, long nmain, long nnyq, 
     hipLaunchKernelGGL((k_pk_partial<T>), dim3((unsigned)nb), dim3(PK_NT), 0, st, spec, nyq, nmain, nnyq, dim,
                        lgN, kf, edges, nbins, part);
     CPM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_pk_final, dim3((unsigned)((nbins + 63) / 64)), dim3(64), 0, st, (const double*)part,
+    hipLaunchKernelGGL(k_pk_final, dim3((unsigned)nbins), dim3(PK_NT), 0, st, (const double*)part,
                        nb, nbins, norm, kmean, pk, cnt);
     CPM_LAUNCH_CHECK();
 }

@@ -252,3 +252,23 @@ def test_power_spectrum(pm, orc):
     assert np.array_equal(c.astype(np.int64), co)
     assert rel(k, ko) < 1e-12
     assert rel(p, po) < 1e-10
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_runs_are_bit_reproducible(pm, orc, prec):
+    """The deposit has no order-dependent floating-point sums: two runs from
+    the same state agree bit for bit (the reference's omp critical deposit
+    does not have this property)."""
+    dim, Nm, Nf, L, a0, da = 3, 32, 64, 100.0, 0.02, 0.0098
+    noise = orc.white_noise(21, Nm ** dim)
+    phi = orc.compute_potential(dim, 5, L, orc.ic_scale_free(dim, 5, L, -2.0, noise))
+    say("reproducible", prec)
+    outs = []
+    for _ in range(2):
+        with pm.Gravity(dim, Nm, Nf, L, precision=prec) as g:
+            g.init(phi, a0, a0 + da / 2)
+            g.steps(a0, da, 8)
+            X, P = g.get_state()
+            outs.append((X, P, g.deposit()))
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
