@@ -49,7 +49,7 @@ def phase_bytes(prec: str) -> dict:
     b_pos = 4 if prec == "fp32" else 8  # position component
     b_mesh = 4 if prec == "fp32" else 8
     nrows = M // NF
-    spec = 2 * b_mesh * (M // 2)  # one read + one write of the main half spectrum
+    spec = 2 * (2 * b_mesh) * (M // 2)  # one read + one write of the M/2 complex modes
     return {
         1: DIM * b_pos * NP + b_mesh * M,  # deposit: positions in, mesh out
         2: b_mesh * M + b_mesh * M + 2 * b_mesh * nrows,  # rows r2c: real in, half spectrum out

@@ -168,7 +168,14 @@ struct Impl : ImplBase {
     const uint32_t* skeys = nullptr;
     const uint32_t* spids = nullptr;
     bool have_deposit = false;
-    long npartials = 0;  // FP64 mesh-sum partials written by the last deposit
+    long npartials = 0;     // FP64 mesh-sum partials written by the last deposit
+    long nsort_groups = 0;  // rows (FP64) or bricks (FX) delimited by row_start
+
+    static int fx_keybits(int dim, int lgN) {
+        PosArgs t{};
+        brick_layout(dim, lgN, t);
+        return t.lnb[0] + t.lnb[1] + t.lnb[2] + dim;  // brick, then face class
+    }
 
     static long ipow(long n, int r) {
         long s = 1;
@@ -179,8 +186,8 @@ struct Impl : ImplBase {
     explicit Impl(const pm_config& c)
         : cfg(c), R(c.dim), lgNf(ilog2_exact(c.Nf, "Nf")), Np(ipow(c.Nm, c.dim)), M(ipow(c.Nf, c.dim)),
           nrows(M / c.Nf), cos(c.H0, c.Omega_m, c.Omega_L), fft(c.dim, c.Nf),
-          // FP64: (cell, id) pairs sorted by the full cell index; FX: by mesh row
-          sorter(Np, (FX ? c.dim - 1 : c.dim) * lgNf) {
+          // FP64: (cell, id) pairs sorted by the full cell index; FX: by brick
+          sorter(Np, FX ? fx_keybits(c.dim, lgNf) : c.dim * lgNf) {
         mass = pow(double(c.Nf) / c.Nm, R);  // src/run/nbody.hh:85
         pa.L = c.L;
         pa.res = c.L / c.Nf;
@@ -188,7 +195,10 @@ struct Impl : ImplBase {
         pa.fshift = 32 - lgNf;
         pa.fmask = (uint32_t)((1ull << pa.fshift) - 1);
         pa.fscale = (float)ldexp(1.0, -pa.fshift);
-        pa.kshift = FX ? lgNf : 0;
+        pa.brick = 0;
+        for (int k = 0; k < 3; ++k) pa.lb[k] = pa.lnb[k] = 0;
+        if (FX) brick_layout(R, lgNf, pa);
+        nsort_groups = FX ? (brick_count(pa) << R) : nrows;
         CPM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         X.alloc((size_t)R * Np);
         P.alloc((size_t)R * Np);
@@ -202,11 +212,11 @@ struct Impl : ImplBase {
         v0.alloc(Np);
         k1.alloc(Np);
         v1.alloc(Np);
-        row_start.alloc(nrows + 1);
+        row_start.alloc((nsort_groups > nrows ? nsort_groups : nrows) + 1);
         mesh.alloc(M);
         spec.alloc(fft.spec_elems);
         nyq.alloc(fft.nyq_elems);
-        row_partial.alloc(nrows);
+        row_partial.alloc(FX && brick_count(pa) > nrows ? brick_count(pa) : nrows);
         red.alloc(4096);
         scal.alloc(16);
     }
@@ -272,12 +282,12 @@ struct Impl : ImplBase {
         const int where = sorter.sort(k0.p, v0.p, k1.p, v1.p, st);
         skeys = where ? k1.p : k0.p;
         spids = where ? v1.p : v0.p;
-        row_bounds(skeys, Np, FX ? 0 : lgNf, nrows, row_start.p, st);
+        row_bounds(skeys, Np, FX ? 0 : lgNf, nsort_groups, row_start.p, st);
         prof.end(0, st);
         prof.begin(1, st);
         if constexpr (FX) {
-            deposit_fx_push(R, row_start.p, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p, st);
-            npartials = deposit_fx_partials(R, lgNf);
+            deposit_fx_brick(R, row_start.p, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p, st);
+            npartials = brick_count(pa);
         } else {
             ParticleOps<PP>::deposit(R, row_start.p, skeys, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p,
                                      nrows, st);
@@ -329,14 +339,10 @@ struct Impl : ImplBase {
         const double cd = fuse ? drift_factor(a_next, da_next) : 0.0;
         prof.begin(7, st);
         if constexpr (FX) {
-            // kick (+ drift) and move the particles into the deposit's cell order
-            Reorder<PP> ro;
-            ro.perm = spids;
-            ro.Xo = X2.p;
-            ro.Po = P2.p;
-            ro.idi = id.p;
-            ro.ido = id2.p;
-            ParticleOps<PP>::gather_kick(R, X.p, P.p, Np, mesh.p, lgNf, pa, ck, cd, fuse ? 1 : 0, nullptr, ro, st);
+            // kick (+ drift) and move the particles into the deposit's brick order
+            gather_kick_fx_brick(R, row_start.p, spids, reinterpret_cast<const uint32_t*>(X.p), P.p, id.p,
+                                 reinterpret_cast<uint32_t*>(X2.p), P2.p, id2.p, Np,
+                                 reinterpret_cast<const float*>(mesh.p), lgNf, pa, ck, cd, fuse ? 1 : 0, st);
             std::swap(X.p, X2.p);
             std::swap(P.p, P2.p);
             std::swap(id.p, id2.p);
@@ -476,7 +482,7 @@ struct Impl : ImplBase {
         if (rho) mesh_to_host(rho);
         if (keys) {
             PosArgs pk = pa;
-            pk.kshift = 0;  // the full cell index idx(origin)
+            pk.brick = 0;  // the full cell index idx(origin)
             ParticleOps<PP>::keys(R, X.p, Np, lgNf, pk, k1.p, v1.p, st);
             std::vector<uint32_t> k(Np);
             CPM_HIP(hipMemcpyAsync(k.data(), k1.p, sizeof(uint32_t) * Np, hipMemcpyDeviceToHost, st));

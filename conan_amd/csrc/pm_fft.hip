@@ -178,12 +178,26 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_c2r(
 }
 
 // --------------------------------------------------------------- columns
+// Column tiles: C adjacent pencils (C * sizeof(cx<T>) contiguous bytes per
+// row of the tile), NT threads; LDS holds the C x N tile.
+#ifndef CPM_COL_F32
+#define CPM_COL_F32 16
+#endif
+#ifndef CPM_COL_F64
+#define CPM_COL_F64 8
+#endif
+#ifndef CPM_COL_NT
+#define CPM_COL_NT 1024
+#endif
+#ifndef CPM_COL_LDS
+#define CPM_COL_LDS 65536
+#endif
 template <typename T, int N>
 struct ColCfg {
     static constexpr int CS = (int)sizeof(cx<T>);
-    static constexpr int Ct = sizeof(T) == 4 ? 16 : 8;
-    static constexpr int C = (Ct * N * CS > 65536) ? (65536 / (N * CS)) : Ct;
-    static constexpr int NT = N >= 8 ? ((C * N / 8 < 256) ? C * N / 8 : 256) : C;
+    static constexpr int Ct = sizeof(T) == 4 ? CPM_COL_F32 : CPM_COL_F64;
+    static constexpr int C = (Ct * N * CS > CPM_COL_LDS) ? (CPM_COL_LDS / (N * CS)) : Ct;
+    static constexpr int NT = N >= 8 ? ((C * N / 8 < CPM_COL_NT) ? C * N / 8 : CPM_COL_NT) : C;
     static constexpr int E = C * N / NT;
 };
 

@@ -26,14 +26,16 @@ __global__ __launch_bounds__(256) void k_keys(const typename PP::Pos* __restrict
     if (i >= np) return;
     const int mask = (1 << lgN) - 1;
     uint32_t key = 0;
+    int oc[3] = {0, 0, 0};
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         int o;
         typename PP::Mesh f;
         PP::dep(X[k * np + i], pa, o, f);
-        key |= (uint32_t)(o & mask) << (k * lgN);  // Box::idx of the origin (box.hh:75-81)
+        oc[k] = o & mask;
+        key |= (uint32_t)oc[k] << (k * lgN);  // Box::idx of the origin (box.hh:75-81)
     }
-    keys[i] = key >> pa.kshift;  // kshift = log2 N: the mesh row of the origin
+    keys[i] = pa.brick ? brick_key(oc[0], oc[1], oc[2], R, pa) : key;
     pids[i] = (uint32_t)i;
 }
 
@@ -135,83 +137,260 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit(const uint32_t* __restrict__
     if (tid == 0) row_partial[row] = red[0];
 }
 
-// ------------------------------------------------------------ FX: persistent row order
-// The FX form keeps its particle arrays grouped by mesh row (y, z) in the
-// order of the previous deposit: `perm` maps sorted position s -> storage
-// index, and particles move less than a cell per step, so perm is close to
-// the identity and reads through it are nearly contiguous.
-//
-// One workgroup owns a tile of FX_TY x FX_TZ mesh rows and pushes the CIC
-// weights of every particle whose origin row lies in the tile or its lower
-// halo into an LDS copy of the tile.  The accumulators are 64-bit fixed point
-// (2^-40 mass units): integer addition is associative, so the result does not
-// depend on the order of the LDS atomics and the deposit is deterministic; it
-// is rounded once to FP32 on the way out.
-static constexpr int FX_TY = 2, FX_TZ = 2, FX_NT = 512;
+// ------------------------------------------------------------ FX: persistent brick order
+// The FX form keeps its particle arrays grouped by brick (FXB_X x FXB_Y x
+// FXB_Z cells, bricks in x-fastest order) in the order of the previous step:
+// `perm` maps sorted position s -> storage index, and particles move less
+// than a cell per step, so perm is close to the identity and reads through
+// it are nearly contiguous.  bstart[b] is the first sorted position of brick b.
+static constexpr int FXB_NT = 256;
+#ifndef CPM_FXG_NT
+#define CPM_FXG_NT 512
+#endif
+static constexpr int FXG_NT = CPM_FXG_NT;  // gather workgroup size
+static constexpr int FXB_CELLS = FXB_X * FXB_Y * FXB_Z;
+static constexpr int FXB_TILE = (FXB_X + 5) * (FXB_Y + 5) * (FXB_Z + 5);
 static constexpr double FX_ONE = 1099511627776.0;  // 2^40
 
+struct BrickCoord {
+    int x0[3], B[3], nb[3], bi[3];
+};
+
+__device__ __forceinline__ BrickCoord brick_coord(uint32_t b, const PosArgs& pa) {
+    BrickCoord c;
+    uint32_t t = b;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        c.nb[k] = 1 << pa.lnb[k];
+        c.bi[k] = (int)(t & (uint32_t)(c.nb[k] - 1));
+        t >>= pa.lnb[k];
+        c.B[k] = 1 << pa.lb[k];
+        c.x0[k] = c.bi[k] << pa.lb[k];
+    }
+    return c;
+}
+
+// Deposit, one workgroup per brick.  The brick's cells receive the CIC
+// weights of the particles of the brick and of its lower neighbours (a
+// particle reaches its origin cell and the next one along each axis); only
+// weights that land inside the brick are added, into an LDS copy of the
+// brick.  The accumulators are 64-bit fixed point (2^-40 mass units):
+// integer addition is associative, so the sum does not depend on the order of
+// the LDS atomics and the deposit is deterministic; each cell is rounded once
+// to FP32 on the way out and written by exactly one workgroup.
 template <int R>
-__global__ __launch_bounds__(FX_NT) void k_deposit_fx_push(const uint32_t* __restrict__ row_start,
-                                                           const uint32_t* __restrict__ perm,
-                                                           const uint32_t* __restrict__ X, long np, int lgN,
-                                                           PosArgs pa, float mass, float* __restrict__ mesh,
-                                                           double* __restrict__ partial) {
-    constexpr int TZ = (R == 3) ? FX_TZ : 1;
-    constexpr int TILE = FX_TY * TZ * 1024;  // cells of the largest mesh (N <= 1024)
-    __shared__ unsigned long long acc[TILE];
-    __shared__ double red[FX_NT];
+__global__ __launch_bounds__(FXB_NT) void k_deposit_fx_brick(const uint32_t* __restrict__ bstart,
+                                                             const uint32_t* __restrict__ perm,
+                                                             const uint32_t* __restrict__ X, long np, int lgN,
+                                                             PosArgs pa, float mass, float* __restrict__ mesh,
+                                                             double* __restrict__ partial) {
+    __shared__ unsigned long long acc[FXB_CELLS];
+    __shared__ double red[FXB_NT];
     const int N = 1 << lgN, msk = N - 1;
-    const int y0 = blockIdx.x * FX_TY;
-    const int z0 = (R == 3) ? (int)blockIdx.y * TZ : 0;
+    const BrickCoord bc = brick_coord(blockIdx.x, pa);
+    const int ncell = bc.B[0] * bc.B[1] * bc.B[2];
     const int tid = threadIdx.x;
-    const int ncell = FX_TY * TZ * N;
-    for (int e = tid; e < ncell; e += FX_NT) acc[e] = 0ull;
+    for (int e = tid; e < ncell; e += FXB_NT) acc[e] = 0ull;
+    // The particle ranges that reach this brick: the whole own brick, and of
+    // each lower neighbour only the face classes towards this brick.  The
+    // first wave resolves all of them at once (lane = neighbour x class), so
+    // the workgroup waits for one round of offset loads, then walks the
+    // concatenated ranges.
+    __shared__ uint32_t rs0[64], rlen[64], roff[65];
+    __shared__ int nrange;
+    if (tid < 64) {
+        const int nbr = tid >> R, c = tid & ((1 << R) - 1);
+        const int d[3] = {nbr & 1, (nbr >> 1) & 1, (nbr >> 2) & 1};
+        bool ok = nbr < (1 << R);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) ok &= !(d[k] && bc.nb[k] == 1);  // that brick would be this one
+        ok &= (nbr == 0) ? (c == 0) : ((c & nbr) == nbr);
+        uint32_t s0 = 0, s1 = 0;
+        if (ok) {
+            uint32_t nb = 0;
+            int sh = 0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                nb |= (uint32_t)((bc.bi[k] - d[k]) & (bc.nb[k] - 1)) << sh;
+                sh += pa.lnb[k];
+            }
+            const uint32_t g0 = (nb << R) + (nbr == 0 ? 0 : c);
+            s0 = bstart[g0];
+            s1 = bstart[nbr == 0 ? g0 + (1u << R) : g0 + 1];
+        }
+        const bool have = ok && s1 > s0;
+        const uint64_t m = __ballot(have);
+        if (have) {
+            const int slot = __popcll(m & ((tid == 0) ? 0ull : (~0ull >> (64 - tid))));
+            rs0[slot] = s0;
+            rlen[slot] = s1 - s0;
+        }
+        if (tid == 0) nrange = __popcll(m);
+    }
     __syncthreads();
-    // origin rows y0-1 .. y0+TY-1 (and z0-1 .. z0+TZ-1)
-    for (int dz = 0; dz <= (R == 3 ? TZ : 0); ++dz) {
-        for (int dy = 0; dy <= FX_TY; ++dy) {
-            long orow = (long)((y0 - 1 + dy) & msk);
-            if (R == 3) orow += (long)((z0 - 1 + dz) & msk) << lgN;
-            const uint32_t s0 = row_start[orow], s1 = row_start[orow + 1];
-            for (uint32_t s = s0 + tid; s < s1; s += FX_NT) {
-                const uint32_t i = perm[s];
-                int o[R];
-                float f[R];
+    if (tid == 0) {
+        uint32_t a = 0;
+        for (int r = 0; r < nrange; ++r) {
+            roff[r] = a;
+            a += rlen[r];
+        }
+        roff[nrange] = a;
+    }
+    __syncthreads();
+    {
+        const uint32_t total = roff[nrange];
+        for (uint32_t t = tid; t < total; t += FXB_NT) {
+            int r = 0;
+            while (roff[r + 1] <= t) ++r;
+            const uint32_t s = rs0[r] + (t - roff[r]);
+            const uint32_t i = perm[s];
+            int o[3] = {0, 0, 0};
+            float f[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-                for (int k = 0; k < R; ++k) PosFX::dep(X[k * np + i], pa, o[k], f[k]);
+            for (int k = 0; k < R; ++k) PosFX::dep(X[k * np + i], pa, o[k], f[k]);
 #pragma unroll
-                for (int j = 0; j < (1 << R); ++j) {
-                    const int bx = j & 1, by = (j >> 1) & 1, bz = (j >> 2) & 1;
-                    const int ty = dy - 1 + by, tz = (R == 3) ? dz - 1 + bz : 0;
-                    if (ty < 0 || ty >= FX_TY || tz < 0 || tz >= TZ) continue;
-                    float w = mass;
-                    w *= bx ? f[0] : 1.f - f[0];
-                    w *= by ? f[1] : 1.f - f[1];
-                    if (R == 3) w *= bz ? f[2] : 1.f - f[2];
-                    const int tx = (o[0] + bx) & msk;
-                    const long long q = __double2ll_rn((double)w * FX_ONE);
-                    atomicAdd(&acc[(tz * FX_TY + ty) * N + tx], (unsigned long long)q);
+            for (int j = 0; j < (1 << R); ++j) {
+                int l[3];
+                bool in = true;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    l[k] = (k < R) ? (((o[k] + ((j >> k) & 1)) - bc.x0[k]) & msk) : 0;
+                    in &= l[k] < bc.B[k];
                 }
+                if (!in) continue;
+                float w = mass;
+#pragma unroll
+                for (int k = 0; k < R; ++k) w *= ((j >> k) & 1) ? f[k] : 1.f - f[k];
+                const long long q = __double2ll_rn((double)w * FX_ONE);
+                atomicAdd(&acc[l[0] + bc.B[0] * (l[1] + bc.B[1] * l[2])], (unsigned long long)q);
             }
         }
     }
     __syncthreads();
     double rsum = 0;
-    for (int e = tid; e < ncell; e += FX_NT) {
-        const int tx = e & msk, t = e >> lgN;
-        const int ty = t % FX_TY, tz = t / FX_TY;
+    for (int e = tid; e < ncell; e += FXB_NT) {
+        const int lx = e & (bc.B[0] - 1), ly = (e >> pa.lb[0]) & (bc.B[1] - 1), lz = e >> (pa.lb[0] + pa.lb[1]);
         const float v = (float)((double)(long long)acc[e] * (1.0 / FX_ONE));
-        const long row = (long)((y0 + ty) & msk) + (R == 3 ? (long)((z0 + tz) & msk) << lgN : 0);
-        mesh[row * N + tx] = v;
+        const long idx = (long)(bc.x0[0] + lx) + (long)N * ((bc.x0[1] + ly) + (long)N * (bc.x0[2] + lz));
+        mesh[idx] = v;
         rsum += (double)v;
     }
     red[tid] = rsum;
     __syncthreads();
-    for (int off = FX_NT / 2; off > 0; off >>= 1) {
+    for (int off = FXB_NT / 2; off > 0; off >>= 1) {
         if (tid < off) red[tid] += red[tid + off];
         __syncthreads();
     }
-    if (tid == 0) partial[(long)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+    if (tid == 0) partial[blockIdx.x] = red[0];
+}
+
+// Gather + kick (+ drift), one workgroup per brick: the potential of the
+// brick and a halo of 2 cells below / 3 above along each axis (the 4-point
+// stencil of the 2^R corners) is staged in LDS once; every particle of the
+// brick then reads its 56 stencil values from LDS.  Particle s (storage
+// perm[s]) is written to position s of the output arrays.
+template <int R>
+__global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
+    const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ X,
+    const double* __restrict__ P, const uint32_t* __restrict__ idi, uint32_t* __restrict__ Xo,
+    double* __restrict__ Po, uint32_t* __restrict__ ido, long np, const float* __restrict__ phi, int lgN,
+    PosArgs pa, double ck, double cd, int cd_on) {
+    __shared__ float tile[FXB_TILE];
+    const int N = 1 << lgN, msk = N - 1;
+    const BrickCoord bc = brick_coord(blockIdx.x, pa);
+    const int TX = bc.B[0] + 5, TY = bc.B[1] + 5, TZ = (R == 3) ? bc.B[2] + 5 : 1;
+    const int tid = threadIdx.x;
+    {
+        // walk the tile in steps of FXG_NT elements, carrying (tx, ty, tz)
+        int tx = tid % TX, r = tid / TX;
+        int ty = r % TY, tz = r / TY;
+        const int dtx = FXG_NT % TX, dr = FXG_NT / TX;
+        for (int e = tid; e < TX * TY * TZ; e += FXG_NT) {
+            const int gx = (bc.x0[0] - 2 + tx) & msk, gy = (bc.x0[1] - 2 + ty) & msk;
+            const int gz = (R == 3) ? ((bc.x0[2] - 2 + tz) & msk) : 0;
+            tile[e] = phi[gx + (long)N * (gy + (long)N * gz)];
+            tx += dtx;
+            int carry = dr;
+            if (tx >= TX) {
+                tx -= TX;
+                ++carry;
+            }
+            ty += carry;
+            while (ty >= TY) {
+                ty -= TY;
+                ++tz;
+            }
+        }
+    }
+    __syncthreads();
+    const float inv12res = (float)(1.0 / (12.0 * pa.res));
+    const uint32_t s0 = bstart[blockIdx.x << R], s1 = bstart[(blockIdx.x + 1) << R];
+    for (uint32_t s = s0 + tid; s < s1; s += FXG_NT) {
+        const uint32_t i = perm[s];
+        ido[s] = idi[i];
+        int o[3] = {0, 0, 0};
+        float a1[3] = {0.f, 0.f, 0.f};
+        uint32_t xs[3];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            xs[k] = X[k * np + i];
+            PosFX::gat(xs[k], pa, o[k], a1[k]);
+        }
+        // tile coordinates of the stencil origin (cell o - 2 along each axis)
+        const int lx = (o[0] - bc.x0[0]) & msk, ly = (o[1] - bc.x0[1]) & msk;
+        const int lz = (R == 3) ? ((o[2] - bc.x0[2]) & msk) : 0;
+        constexpr int NC = 1 << R;
+        float z[NC];
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+            float w = 1.f;
+#pragma unroll
+            for (int k = 0; k < R; ++k) w *= ((j >> k) & 1) ? a1[k] : 1.f - a1[k];
+            z[j] = w;
+        }
+        const int zc = (R == 3) ? 2 : 0;  // tile z of the origin's own plane offset
+        auto T = [&](int tx, int ty, int tz) -> float { return tile[tx + TX * (ty + TY * tz)]; };
+        float F[3] = {0.f, 0.f, 0.f};
+        // axis 0: lines along x through the (y, z) corners
+#pragma unroll
+        for (int l = 0; l < NC / 2; ++l) {
+            const int by = l & 1, bz = (l >> 1) & 1;
+            float f[6];
+#pragma unroll
+            for (int t = 0; t < 6; ++t) f[t] = T(lx + t, ly + 2 + by, lz + zc + bz);
+#pragma unroll
+            for (int bx = 0; bx < 2; ++bx)
+                F[0] += fd4(f[bx], f[bx + 1], f[bx + 3], f[bx + 4], inv12res) * z[bx | (by << 1) | (bz << 2)];
+        }
+#pragma unroll
+        for (int l = 0; l < NC / 2; ++l) {
+            const int bx = l & 1, bz = (l >> 1) & 1;
+            float f[6];
+#pragma unroll
+            for (int t = 0; t < 6; ++t) f[t] = T(lx + 2 + bx, ly + t, lz + zc + bz);
+#pragma unroll
+            for (int by = 0; by < 2; ++by)
+                F[1] += fd4(f[by], f[by + 1], f[by + 3], f[by + 4], inv12res) * z[bx | (by << 1) | (bz << 2)];
+        }
+        if constexpr (R == 3) {
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                const int bx = l & 1, by = (l >> 1) & 1;
+                float f[6];
+#pragma unroll
+                for (int t = 0; t < 6; ++t) f[t] = T(lx + 2 + bx, ly + 2 + by, lz + t);
+#pragma unroll
+                for (int bz = 0; bz < 2; ++bz)
+                    F[2] += fd4(f[bz], f[bz + 1], f[bz + 3], f[bz + 4], inv12res) * z[bx | (by << 1) | (bz << 2)];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const double p = P[k * np + i] - (double)F[k] * ck;
+            Po[k * np + s] = p;
+            Xo[k * np + s] = cd_on ? xs[k] + (uint32_t)__double2ll_rn(p * cd) : xs[k];
+        }
+    }
 }
 
 // ------------------------------------------------------------ gather + kick (+ drift)
@@ -558,18 +737,32 @@ void ParticleOps<PP>::gather_kick(int R, Pos* X, double* P, long np, const Mesh*
     CPM_LAUNCH_CHECK();
 }
 
-long deposit_fx_partials(int R, int lgN) {
-    const long n = 1L << lgN;
-    return R == 3 ? (n / FX_TY) * (n / FX_TZ) : n / FX_TY;
+void brick_layout(int R, int lgN, PosArgs& pa) {
+    const int want[3] = {ilog2c(FXB_X), R == 3 ? ilog2c(FXB_Y) : ilog2c(FXB_X), R == 3 ? ilog2c(FXB_Z) : 0};
+    for (int k = 0; k < 3; ++k) {
+        const int lg = (k < R) ? lgN : 0;
+        pa.lb[k] = want[k] < lg ? want[k] : lg;
+        pa.lnb[k] = lg - pa.lb[k];
+    }
+    pa.brick = 1;
 }
 
-void deposit_fx_push(int R, const uint32_t* row_start, const uint32_t* perm, const uint32_t* X, long np, int lgN,
-                     const PosArgs& pa, float mass, float* mesh, double* partial, hipStream_t st) {
-    const unsigned n = 1u << lgN;
-    if (n % FX_TY) throw Error(-2, "mesh too small for the deposit tile");
-    const dim3 grid(n / FX_TY, R == 3 ? n / FX_TZ : 1);
-    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit_fx_push<RR>), grid, dim3(FX_NT), 0, st, row_start, perm, X,
+long brick_count(const PosArgs& pa) { return 1L << (pa.lnb[0] + pa.lnb[1] + pa.lnb[2]); }
+
+void deposit_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, long np, int lgN,
+                      const PosArgs& pa, float mass, float* mesh, double* partial, hipStream_t st) {
+    const unsigned nb = (unsigned)brick_count(pa);
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit_fx_brick<RR>), dim3(nb), dim3(FXB_NT), 0, st, bstart, perm, X,
                                          np, lgN, pa, mass, mesh, partial));
+    CPM_LAUNCH_CHECK();
+}
+
+void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, const double* P,
+                          const uint32_t* idi, uint32_t* Xo, double* Po, uint32_t* ido, long np, const float* phi,
+                          int lgN, const PosArgs& pa, double ck, double cd, int cd_on, hipStream_t st) {
+    const unsigned nb = (unsigned)brick_count(pa);
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_kick_fx_brick<RR>), dim3(nb), dim3(FXG_NT), 0, st, bstart, perm,
+                                         X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck, cd, cd_on));
     CPM_LAUNCH_CHECK();
 }
 

@@ -12,8 +12,25 @@ struct PosArgs {
     int fshift;      // FX: 32 - log2 N
     uint32_t fmask;  // FX: (1 << fshift) - 1
     float fscale;    // FX: 2^-fshift
-    int kshift;      // sort keys: cell index >> kshift (0: cell, log2 N: row)
+    int brick;       // sort keys: 0 = cell index idx(origin), 1 = brick index (FX)
+    int lb[3];       // log2 brick size per axis
+    int lnb[3];      // log2 bricks per axis
 };
+
+// Brick of the cell (x, y, z): axis 0 fastest.
+__host__ __device__ __forceinline__ uint32_t brick_of(int x, int y, int z, const PosArgs& a) {
+    return (uint32_t)(x >> a.lb[0]) | ((uint32_t)(y >> a.lb[1]) << a.lnb[0]) |
+           ((uint32_t)(z >> a.lb[2]) << (a.lnb[0] + a.lnb[1]));
+}
+// Sort key of an FX particle: brick, then the face class (bit k set when the
+// origin cell lies on the brick's upper face along axis k, i.e. when its CIC
+// cloud reaches into the next brick along k).
+__host__ __device__ __forceinline__ uint32_t brick_key(int x, int y, int z, int R, const PosArgs& a) {
+    const uint32_t c = (uint32_t)(((x & ((1 << a.lb[0]) - 1)) == (1 << a.lb[0]) - 1)) |
+                       ((uint32_t)(((y & ((1 << a.lb[1]) - 1)) == (1 << a.lb[1]) - 1)) << 1) |
+                       ((uint32_t)(R == 3 && ((z & ((1 << a.lb[2]) - 1)) == (1 << a.lb[2]) - 1)) << 2);
+    return (brick_of(x, y, z, a) << R) | c;
+}
 
 // src/base/misc.hh:144-148
 __device__ __forceinline__ double modulus_d(double a, double b) {
@@ -89,11 +106,28 @@ struct ParticleOps {
 };
 
 void row_bounds(const uint32_t* keys, long np, int lgN, long nrows, uint32_t* row_start, hipStream_t st);
-// FX deposit from particles grouped by mesh row (perm: sorted position ->
-// storage index); writes deposit_fx_partials(R, lgN) FP64 partial sums.
-long deposit_fx_partials(int R, int lgN);
-void deposit_fx_push(int R, const uint32_t* row_start, const uint32_t* perm, const uint32_t* X, long np, int lgN,
-                     const PosArgs& pa, float mass, float* mesh, double* partial, hipStream_t st);
+// FX brick order: bricks of FXB_X x FXB_Y x FXB_Z cells (2-D: FXB_X x FXB_X),
+// clamped to the mesh.  brick_layout() fills pa.brick / lb / lnb.
+#ifndef CPM_FXB_X
+#define CPM_FXB_X 32
+#endif
+#ifndef CPM_FXB_Y
+#define CPM_FXB_Y 16
+#endif
+#ifndef CPM_FXB_Z
+#define CPM_FXB_Z 8
+#endif
+constexpr int FXB_X = CPM_FXB_X, FXB_Y = CPM_FXB_Y, FXB_Z = CPM_FXB_Z;
+void brick_layout(int R, int lgN, PosArgs& pa);
+long brick_count(const PosArgs& pa);
+// FX deposit / gather+kick(+drift) over particles grouped by brick (perm:
+// sorted position -> storage index; bstart: brick_count + 1 offsets).  The
+// deposit writes one FP64 partial sum per brick.
+void deposit_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, long np, int lgN,
+                      const PosArgs& pa, float mass, float* mesh, double* partial, hipStream_t st);
+void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, const double* P,
+                          const uint32_t* idi, uint32_t* Xo, double* Po, uint32_t* ido, long np, const float* phi,
+                          int lgN, const PosArgs& pa, double ck, double cd, int cd_on, hipStream_t st);
 template <typename T>
 void reduce_sum(const T* a, long n, int square, double scale, double* partials, long max_partials,
                 double* out, hipStream_t st);
