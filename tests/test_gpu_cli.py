@@ -1,0 +1,117 @@
+"""The conan CLI end to end on the GPU: ic -> run -> cic with the reference's
+options and .conan files, checked against the oracle run from the same stored
+white noise."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CONAN = os.path.join(ROOT, "conan_amd", "bin", "conan")
+
+
+def run(args, cwd):
+    print("[case] conan", " ".join(args), flush=True)
+    r = subprocess.run([CONAN, *args], cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    return r
+
+
+@pytest.fixture(scope="module")
+def sim_dir(tmp_path_factory, orc):
+    from conan_amd import conanfile
+
+    d = tmp_path_factory.mktemp("cli")
+    dim, mbits, L = 3, 4, 100.0
+    noise = orc.white_noise(5, (1 << mbits) ** dim)
+    conanfile.write_noise(str(d / "noise.conan"), noise, dim, mbits)
+    run(["ic", "-i", "t", "-d", "3", "-b", str(mbits), "-L", "100", "--scale-free", "-n", "-2", "-p",
+         "--noise", "noise.conan"], d)
+    return d, noise
+
+
+def test_ic_file_matches_oracle(sim_dir, orc):
+    from conan_amd import conanfile
+
+    d, noise = sim_dir
+    h, hist, rec = conanfile.read(str(d / "t.density.init.conan"))
+    # every option is in the header (Header << Argv), plus N (ic/main.cc:74)
+    for k in ("dim", "mbits", "size", "seed", "ns", "scale-free", "potential", "N", "id", "H0", "Omega0"):
+        assert k in h
+    assert h["N"] == "16" and h["scale-free"] == "1" and h["smooth"] == "false"
+    assert len(hist) == 1 and hist[0][1].startswith("ic ")
+    dens = orc.ic_scale_free(3, 4, 100.0, -2.0, noise)
+    phi = orc.compute_potential(3, 4, 100.0, dens)
+    assert rec["density"][0]["dtype"] == "float64"
+    assert np.max(np.abs(rec["density"][1] - dens)) < 1e-12 * np.max(np.abs(dens))
+    assert np.max(np.abs(rec["potential"][1] - phi)) < 1e-12 * np.max(np.abs(phi))
+
+
+def test_run_snapshots_match_oracle(sim_dir, orc):
+    from conan_amd import conanfile
+
+    d, noise = sim_dir
+    run(["run", "-i", "t", "-f", "2", "--a0", "0.02", "--da", "0.0098", "-n", "20"], d)
+    assert (d / "t.pos.1.conan").exists() and (d / "t.pos.2.conan").exists()
+    h, hist, rec = conanfile.read(str(d / "t.pos.2.conan"))
+    # save_conan's header and records (src/run/nbody.hh:168-183)
+    assert sorted(h) == ["L", "N", "a-mom", "a-pos", "dim", "id"]
+    assert h["N"] == "16" and h["L"] == "100" and h["dim"] == "3" and h["id"] == "t"
+    assert hist[0][1] == "ran nbody"
+    assert rec["pos"][0]["dtype"] == "3f8" and rec["pos"][1].shape == (16 ** 3, 3)
+    _, _, irec = conanfile.read(str(d / "t.density.init.conan"))
+    sim = orc.Sim(3, 16, 32, 100.0)
+    sim.init(irec["potential"][1], 0.02, 0.0098)
+    sim.steps(20)
+    Xo, Po, t = sim.state()
+    assert float(h["a-pos"]) == pytest.approx(t[0], rel=1e-5)
+    assert float(h["a-mom"]) == pytest.approx(t[1], rel=1e-5)
+    assert np.max(np.abs(rec["pos"][1] - Xo)) < 1e-9 * 100
+    assert np.max(np.abs(rec["mom"][1] - Po)) < 1e-9 * np.max(np.abs(Po))
+
+
+def test_cic_tool_matches_oracle(sim_dir, orc):
+    from conan_amd import conanfile
+
+    d, _ = sim_dir
+    if not (d / "t.pos.2.conan").exists():
+        run(["run", "-i", "t", "-f", "2", "--a0", "0.02", "--da", "0.0098", "-n", "20"], d)
+    run(["cic", "-i", "t", "-m", "2", "-f", "2"], d)
+    h, hist, rec = conanfile.read(str(d / "t.rho.2.conan"))
+    assert h["N"] == "32" and h["sub-sample"] == "1"
+    assert [c for _, c in hist] == ["ran nbody", "cic -i t -m 2 -f 2"]
+    _, _, prec = conanfile.read(str(d / "t.pos.2.conan"))
+    ref = orc.cic_tool(3, 16, 32, 100.0, 1, prec["pos"][1])
+    assert np.array_equal(rec["density"][1], ref)
+    run(["cic", "-i", "t", "-m", "2", "-f", "2", "-s", "2"], d)
+    _, _, rec2 = conanfile.read(str(d / "t.rho.2.conan"))
+    ref2 = orc.cic_tool(3, 16, 32, 100.0, 2, prec["pos"][1])
+    assert np.max(np.abs(rec2["density"][1] - ref2)) < 1e-12 * np.max(ref2)
+
+
+def test_ifrit_and_ascii_outputs(sim_dir):
+    import struct
+
+    d, _ = sim_dir
+    run(["run", "-i", "t", "-f", "1", "--da", "0.0098", "-n", "10", "--ifrit"], d)
+    raw = (d / "t.pos.1.bin").read_bytes()
+    n = struct.unpack("<I", raw[:4])[0]
+    assert n == 4 and struct.unpack("<I", raw[4:8])[0] == 4096  # uint32 framing, particle count
+    assert len(raw) == (4 + 4 + 4) + (4 + 24 + 4) + 6 * (4 + 4096 * 4 + 4)
+    run(["run", "-i", "t", "-f", "1", "--da", "0.0098", "-n", "10", "--ascii"], d)
+    lines = (d / "t.pos.1.txt").read_text().splitlines()
+    assert len(lines) == 4096 and len(lines[0].split()) == 6
+
+
+def test_fp32_precision_flag(sim_dir):
+    from conan_amd import conanfile
+
+    d, _ = sim_dir
+    run(["run", "-i", "t", "-f", "2", "--a0", "0.02", "--da", "0.0098", "-n", "10", "--precision", "fp32"], d)
+    h, _, rec = conanfile.read(str(d / "t.pos.1.conan"))
+    X = rec["pos"][1]
+    assert X.min() >= 0 and X.max() < 100.0  # the fixed-point form keeps positions folded
