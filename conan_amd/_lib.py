@@ -27,7 +27,8 @@ EXPORTS = [
     "pm_steps", "pm_get_state", "pm_set_state", "pm_get_times", "pm_deposit", "pm_potential",
     "pm_forces", "pm_gather_mesh", "pm_poisson_mesh", "pm_power_spectrum", "pm_diagnostics", "pm_profile", "pm_sync",
     "pm_ic_scale_free", "pm_ic_cdm", "pm_ic_potential", "pm_ic_displacement", "pm_fft_r2c", "pm_fft_c2r",
-    "pm_cic",
+    "pm_cic", "pm_diff_create", "pm_diff_destroy", "pm_diff_last_error", "pm_diff_forward", "pm_diff_backward",
+    "pm_diff_get_stats",
 ]
 
 
@@ -52,6 +53,18 @@ class pm_diag(C.Structure):
     _fields_ = [("mass", C.c_double), ("momentum", C.c_double * 3), ("p2", C.c_double)]
 
 
+class pm_diff_params(C.Structure):
+    _fields_ = [("dim", C.c_int), ("mbits", C.c_int), ("f", C.c_int), ("L", C.c_double), ("ns", C.c_double),
+                ("smooth", C.c_int), ("scale", C.c_double), ("sigma", C.c_double), ("H0", C.c_double),
+                ("Omega_m", C.c_double), ("Omega_L", C.c_double), ("a0", C.c_double), ("da", C.c_double),
+                ("nsteps", C.c_int), ("ckpt_every", C.c_int), ("precision", C.c_int)]
+
+
+class pm_diff_stats(C.Structure):
+    _fields_ = [("forward_ms", C.c_double), ("backward_ms", C.c_double), ("device_bytes", C.c_double),
+                ("ncheckpoints", C.c_int), ("recon_pos", C.c_longlong), ("recon_mom", C.c_double)]
+
+
 _lib = None
 
 _dp = C.POINTER(C.c_double)
@@ -63,6 +76,14 @@ def load() -> C.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is not built: run `python -m conan_amd.build` first")
+    # torch ships its own HIP runtime under the same soname (libamdhip64.so.7).
+    # Loaded first, it is the one the library binds to, so torch tensors and
+    # the library share one runtime; loaded after the library, torch would
+    # bring a second runtime into the process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     V = C.c_void_p
     sig = {
@@ -92,6 +113,12 @@ def load() -> C.CDLL:
         "pm_fft_r2c": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp]),
         "pm_fft_c2r": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp]),
         "pm_cic": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp, _dp]),
+        "pm_diff_create": (C.c_int, [C.POINTER(pm_diff_params), C.POINTER(V)]),
+        "pm_diff_destroy": (None, [V]),
+        "pm_diff_last_error": (C.c_char_p, [V]),
+        "pm_diff_forward": (C.c_int, [V, V, V, C.c_int]),
+        "pm_diff_backward": (C.c_int, [V, V, V, C.c_int]),
+        "pm_diff_get_stats": (C.c_int, [V, C.POINTER(pm_diff_stats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

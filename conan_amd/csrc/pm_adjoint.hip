@@ -1,0 +1,524 @@
+// pm_adjoint.hip -- adjoint kernels of the FX (fixed-point position) PM step.
+//
+// The forward step, per particle p with position Y_p (after the drift) and
+// momentum P_p:
+//   rho(c)  = m sum_p W(Y_p, c)                    CIC deposit
+//   phi     = G rho                                Poisson (symmetric, DC = 0)
+//   F_p     = sum_c W(Y_p, c) (D phi)(c)           FD gradient, CIC gather
+//   P_p'    = P_p - ck F_p,   Y_p' = Y_p + cd' P_p'
+// Its adjoint, given lY' = dL/dY', lP' = dL/dP':
+//   lP~     = lP' + cd' lY'                        drift
+//   s_k(c)  = sum_p lP~_pk W(Y_p, c)               deposit of lP~ (R meshes)
+//   chi     = G (-sum_k D_k s_k)                   D^T = -D, G^T = G
+//   lY_pk   = lY'_pk - ck (sum_i lP~_pi dF_pi/dY_pk + m sum_c dW(Y_p, c)/dY_pk chi(c))
+//   lP_p    = lP~_p
+// The deposit's adjoint is a gather (of chi, with the derivative of the CIC
+// weights) and the gather's adjoint is a deposit (of lP~): each kernel here
+// is the transpose of a forward kernel in pm_particles.hip, over the same
+// brick order.  The reverse pass also reconstructs the state of the previous
+// step: the drift is undone exactly in integer arithmetic, and the kick is
+// undone with the force recomputed from the same positions (the FX deposit
+// is order independent, so it reproduces the forward's mesh bit for bit).
+#include "pm_adjoint.hpp"
+#include "pm_brick.hpp"
+
+namespace cpm {
+
+// ------------------------------------------------------------ reverse drift + keys
+template <int R>
+__global__ __launch_bounds__(256) void k_rev_drift_keys(uint32_t* __restrict__ Y, const double* __restrict__ P,
+                                                         const double* __restrict__ lY, double* __restrict__ lP,
+                                                         long np, double cd_fx, double cd, int lgN, PosArgs pa,
+                                                         uint32_t* __restrict__ keys, uint32_t* __restrict__ pids,
+                                                         uint32_t* __restrict__ lmax) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    float mx = 0.f;
+    if (i < np) {
+        const int mask = (1 << lgN) - 1;
+        int oc[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            uint32_t y = Y[k * np + i];
+            if (cd_fx != 0.0) {
+                y -= (uint32_t)__double2ll_rn(P[k * np + i] * cd_fx);
+                Y[k * np + i] = y;
+            }
+            const double lp = lP[k * np + i] + cd * lY[k * np + i];
+            lP[k * np + i] = lp;
+            mx = fmaxf(mx, fabsf((float)lp));
+            oc[k] = (int)(y >> pa.fshift) & mask;
+        }
+        keys[i] = brick_key(oc[0], oc[1], oc[2], R, pa);
+        pids[i] = (uint32_t)i;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+    if ((threadIdx.x & 63) == 0 && mx > 0.f) atomicMax(lmax, __float_as_uint(mx));
+}
+
+// ------------------------------------------------------------ vector deposit
+// The brick deposit of pm_particles.hip with R value channels instead of the
+// mass: 64-bit fixed-point LDS accumulators, scaled so that the largest
+// |value| maps below 2^40 (headroom for 2^23 contributions per cell).
+template <int R>
+__global__ __launch_bounds__(FXB_NT) void k_deposit_fx_vec(const uint32_t* __restrict__ bstart,
+                                                           const uint32_t* __restrict__ perm,
+                                                           const uint32_t* __restrict__ Y,
+                                                           const double* __restrict__ lP, long np, int lgN,
+                                                           PosArgs pa, const uint32_t* __restrict__ lmax,
+                                                           float* __restrict__ s, long M) {
+    __shared__ unsigned long long acc[R][FXB_CELLS];
+    __shared__ BrickRanges rg;
+    const int N = 1 << lgN, msk = N - 1;
+    const BrickCoord bc = brick_coord(blockIdx.x, pa);
+    const int ncell = bc.B[0] * bc.B[1] * bc.B[2];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < ncell; e += FXB_NT)
+#pragma unroll
+        for (int c = 0; c < R; ++c) acc[c][e] = 0ull;
+    const float mx = __uint_as_float(*lmax);
+    double scale = 0.0;
+    if (mx > 0.f) {
+        int ex = 0;
+        (void)frexpf(mx, &ex);  // mx < 2^ex
+        scale = ldexp(1.0, 40 - ex);
+    }
+    brick_ranges<R>(rg, bstart, bc, pa, tid);
+    const uint32_t total = rg.roff[rg.n];
+    for (uint32_t t = tid; t < total; t += FXB_NT) {
+        const uint32_t sp = brick_range_pos(rg, t);
+        const uint32_t i = perm[sp];
+        int o[3] = {0, 0, 0};
+        float f[3] = {0.f, 0.f, 0.f};
+        double lam[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            PosFX::dep(Y[k * np + i], pa, o[k], f[k]);
+            lam[k] = lP[k * np + i] * scale;
+        }
+#pragma unroll
+        for (int j = 0; j < (1 << R); ++j) {
+            int l[3];
+            bool in = true;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                l[k] = (k < R) ? (((o[k] + ((j >> k) & 1)) - bc.x0[k]) & msk) : 0;
+                in &= l[k] < bc.B[k];
+            }
+            if (!in) continue;
+            float w = 1.f;
+#pragma unroll
+            for (int k = 0; k < R; ++k) w *= ((j >> k) & 1) ? f[k] : 1.f - f[k];
+            const int cell = l[0] + bc.B[0] * (l[1] + bc.B[1] * l[2]);
+#pragma unroll
+            for (int c = 0; c < R; ++c)
+                atomicAdd(&acc[c][cell], (unsigned long long)__double2ll_rn((double)w * lam[c]));
+        }
+    }
+    __syncthreads();
+    const double inv = scale > 0 ? 1.0 / scale : 0.0;  // exact: scale is a power of two
+    for (int e = tid; e < ncell; e += FXB_NT) {
+        const int lx = e & (bc.B[0] - 1), ly = (e >> pa.lb[0]) & (bc.B[1] - 1), lz = e >> (pa.lb[0] + pa.lb[1]);
+        const long idx = (long)(bc.x0[0] + lx) + (long)N * ((bc.x0[1] + ly) + (long)N * (bc.x0[2] + lz));
+#pragma unroll
+        for (int c = 0; c < R; ++c) s[c * M + idx] = (float)((double)(long long)acc[c][e] * inv);
+    }
+}
+
+// ------------------------------------------------------------ -div (FP32 mesh)
+template <int R>
+__global__ __launch_bounds__(256) void k_neg_div_f32(const float* __restrict__ s, int lgN, long M,
+                                                      float inv12res, float* __restrict__ psi) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= M) return;
+    const int msk = (1 << lgN) - 1;
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const long stride = 1L << (k * lgN);
+        const int q = (int)((c >> (k * lgN)) & msk);
+        const long base = c - (long)q * stride;
+        const float* sk = s + k * M;
+        const float fm2 = sk[base + (long)((q - 2) & msk) * stride];
+        const float fm1 = sk[base + (long)((q - 1) & msk) * stride];
+        const float fp1 = sk[base + (long)((q + 1) & msk) * stride];
+        const float fp2 = sk[base + (long)((q + 2) & msk) * stride];
+        acc += fd4(fm2, fm1, fp1, fp2, inv12res);
+    }
+    psi[c] = -acc;
+}
+
+// ------------------------------------------------------------ fused adjoint gather
+// One workgroup per brick.  phi (brick + halo -2..+3) and chi (brick + 1) are
+// staged in LDS.  Per particle: the forward's FD-gradient gather F (same
+// operations in the same order as k_gather_kick_fx_brick, so the force is
+// bit-identical), its per-corner gradients g_i(c), and the derivative of the
+// CIC weights dW(c)/dY_k = +-1/res * prod_{m != k} A_m(c).
+static constexpr int FXB_CTILE = (FXB_X + 1) * (FXB_Y + 1) * (FXB_Z + 1);
+
+template <int R>
+__global__ __launch_bounds__(FXG_NT) void k_adj_gather_fx_brick(
+    const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ Y,
+    const double* __restrict__ P, const double* __restrict__ lY, const double* __restrict__ lP,
+    const uint32_t* __restrict__ idi, uint32_t* __restrict__ Yo, double* __restrict__ Po,
+    double* __restrict__ lYo, double* __restrict__ lPo, uint32_t* __restrict__ ido, long np,
+    const float* __restrict__ phi, const float* __restrict__ chi, int lgN, PosArgs pa, AdjCoef cf) {
+    __shared__ float tile[FXB_TILE];
+    __shared__ float ctile[FXB_CTILE];
+    const int N = 1 << lgN, msk = N - 1;
+    const BrickCoord bc = brick_coord(blockIdx.x, pa);
+    const int TX = bc.B[0] + 5, TY = bc.B[1] + 5, TZ = (R == 3) ? bc.B[2] + 5 : 1;
+    const int CX = bc.B[0] + 1, CY = bc.B[1] + 1, CZ = (R == 3) ? bc.B[2] + 1 : 1;
+    const int tid = threadIdx.x;
+    const bool hp = phi != nullptr, hc = chi != nullptr;
+    if (hp) {
+        for (int e = tid; e < TX * TY * TZ; e += FXG_NT) {
+            const int tx = e % TX, r = e / TX, ty = r % TY, tz = r / TY;
+            const int gx = (bc.x0[0] - 2 + tx) & msk, gy = (bc.x0[1] - 2 + ty) & msk;
+            const int gz = (R == 3) ? ((bc.x0[2] - 2 + tz) & msk) : 0;
+            tile[e] = phi[gx + (long)N * (gy + (long)N * gz)];
+        }
+    }
+    if (hc) {
+        for (int e = tid; e < CX * CY * CZ; e += FXG_NT) {
+            const int tx = e % CX, r = e / CX, ty = r % CY, tz = r / CY;
+            const int gx = (bc.x0[0] + tx) & msk, gy = (bc.x0[1] + ty) & msk;
+            const int gz = (R == 3) ? ((bc.x0[2] + tz) & msk) : 0;
+            ctile[e] = chi[gx + (long)N * (gy + (long)N * gz)];
+        }
+    }
+    __syncthreads();
+    const float inv12res = (float)(1.0 / (12.0 * pa.res));
+    const double ires = 1.0 / pa.res;
+    constexpr int NC = 1 << R;
+    const uint32_t s0 = bstart[blockIdx.x << R], s1 = bstart[(blockIdx.x + 1) << R];
+    for (uint32_t s = s0 + tid; s < s1; s += FXG_NT) {
+        const uint32_t i = perm[s];
+        ido[s] = idi[i];
+        int o[3] = {0, 0, 0};
+        float a1[3] = {0.f, 0.f, 0.f};
+        uint32_t ys[3];
+        double lam[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            ys[k] = Y[k * np + i];
+            PosFX::gat(ys[k], pa, o[k], a1[k]);
+            lam[k] = lP[k * np + i];
+        }
+        const int lx = (o[0] - bc.x0[0]) & msk, ly = (o[1] - bc.x0[1]) & msk;
+        const int lz = (R == 3) ? ((o[2] - bc.x0[2]) & msk) : 0;
+        float z[NC];
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+            float w = 1.f;
+#pragma unroll
+            for (int k = 0; k < R; ++k) w *= ((j >> k) & 1) ? a1[k] : 1.f - a1[k];
+            z[j] = w;
+        }
+        // dW(c_j)/dY_k
+        double dz[3][NC];
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+#pragma unroll
+            for (int j = 0; j < NC; ++j) {
+                double w = ((j >> k) & 1) ? ires : -ires;
+#pragma unroll
+                for (int m = 0; m < R; ++m)
+                    if (m != k) w *= ((j >> m) & 1) ? (double)a1[m] : 1.0 - (double)a1[m];
+                dz[k][j] = w;
+            }
+        float F[3] = {0.f, 0.f, 0.f};
+        double T[3] = {0, 0, 0}, U[3] = {0, 0, 0};
+        if (hp) {
+            float g[3][NC];
+            const int zc = (R == 3) ? 2 : 0;
+            auto Tt = [&](int tx, int ty, int tz) -> float { return tile[tx + TX * (ty + TY * tz)]; };
+#pragma unroll
+            for (int l = 0; l < NC / 2; ++l) {
+                const int by = l & 1, bz = (l >> 1) & 1;
+                float f[6];
+#pragma unroll
+                for (int t = 0; t < 6; ++t) f[t] = Tt(lx + t, ly + 2 + by, lz + zc + bz);
+#pragma unroll
+                for (int bx = 0; bx < 2; ++bx) {
+                    const int j = bx | (by << 1) | (bz << 2);
+                    g[0][j] = fd4(f[bx], f[bx + 1], f[bx + 3], f[bx + 4], inv12res);
+                    F[0] += g[0][j] * z[j];
+                }
+            }
+#pragma unroll
+            for (int l = 0; l < NC / 2; ++l) {
+                const int bx = l & 1, bz = (l >> 1) & 1;
+                float f[6];
+#pragma unroll
+                for (int t = 0; t < 6; ++t) f[t] = Tt(lx + 2 + bx, ly + t, lz + zc + bz);
+#pragma unroll
+                for (int by = 0; by < 2; ++by) {
+                    const int j = bx | (by << 1) | (bz << 2);
+                    g[1][j] = fd4(f[by], f[by + 1], f[by + 3], f[by + 4], inv12res);
+                    F[1] += g[1][j] * z[j];
+                }
+            }
+            if constexpr (R == 3) {
+#pragma unroll
+                for (int l = 0; l < 4; ++l) {
+                    const int bx = l & 1, by = (l >> 1) & 1;
+                    float f[6];
+#pragma unroll
+                    for (int t = 0; t < 6; ++t) f[t] = Tt(lx + 2 + bx, ly + 2 + by, lz + t);
+#pragma unroll
+                    for (int bz = 0; bz < 2; ++bz) {
+                        const int j = bx | (by << 1) | (bz << 2);
+                        g[2][j] = fd4(f[bz], f[bz + 1], f[bz + 3], f[bz + 4], inv12res);
+                        F[2] += g[2][j] * z[j];
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NC; ++j) {
+                double h = 0;
+#pragma unroll
+                for (int q = 0; q < R; ++q) h += lam[q] * (double)g[q][j];
+#pragma unroll
+                for (int k = 0; k < R; ++k) T[k] += dz[k][j] * h;
+            }
+        }
+        if (hc) {
+#pragma unroll
+            for (int j = 0; j < NC; ++j) {
+                const int bx = j & 1, by = (j >> 1) & 1, bz = (j >> 2) & 1;
+                const double cv = (double)ctile[(lx + bx) + CX * ((ly + by) + CY * (lz + (R == 3 ? bz : 0)))];
+#pragma unroll
+                for (int k = 0; k < R; ++k) U[k] += dz[k][j] * cv;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            Yo[k * np + s] = ys[k];
+            Po[k * np + s] = P[k * np + i] + (double)F[k] * cf.cF;
+            lYo[k * np + s] = lY[k * np + i] + cf.cT * T[k] + cf.cU * U[k];
+            lPo[k * np + s] = lam[k];
+        }
+    }
+}
+
+// ------------------------------------------------------------ permutations by particle id
+template <typename T>
+__global__ __launch_bounds__(256) void k_scatter_by_id(const T* __restrict__ src, const uint32_t* __restrict__ id,
+                                                        long np, int nc, T* __restrict__ dst) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np) return;
+    const long d = id[s];
+    for (int k = 0; k < nc; ++k) dst[k * np + d] = src[k * np + s];
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_by_id(const T* __restrict__ src, const uint32_t* __restrict__ id,
+                                                       long np, int nc, T* __restrict__ dst) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np) return;
+    const long d = id[s];
+    for (int k = 0; k < nc; ++k) dst[k * np + s] = src[k * np + d];
+}
+
+// ------------------------------------------------------------ -div (FP64 mass grid)
+template <int R>
+__global__ __launch_bounds__(256) void k_neg_div_f64(const double* __restrict__ u, int lgN, long n, double res,
+                                                      double* __restrict__ psi) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int msk = (1 << lgN) - 1;
+    double acc = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const long stride = 1L << (k * lgN);
+        const int q = (int)((c >> (k * lgN)) & msk);
+        const long base = c - (long)q * stride;
+        const double* uk = u + k * n;
+        acc += fd4(uk[base + (long)((q - 2) & msk) * stride], uk[base + (long)((q - 1) & msk) * stride],
+                   uk[base + (long)((q + 1) & msk) * stride], uk[base + (long)((q + 2) & msk) * stride], res);
+    }
+    psi[c] = -acc;
+}
+
+// ------------------------------------------------------------ small vector kernels
+__global__ __launch_bounds__(256) void k_axpby(long n, double a, const double* __restrict__ x, double b,
+                                               const double* __restrict__ y, double* __restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a * x[i] + b * y[i];
+}
+
+static constexpr int DOT_NT = 256;
+__global__ __launch_bounds__(DOT_NT) void k_dot_partial(const double* __restrict__ a, const double* __restrict__ b,
+                                                        long n, long chunk, double* __restrict__ partials) {
+    __shared__ double red[DOT_NT];
+    const long b0 = (long)blockIdx.x * chunk;
+    const long b1 = b0 + chunk < n ? b0 + chunk : n;
+    double s = 0;
+    for (long i = b0 + threadIdx.x; i < b1; i += DOT_NT) s += a[i] * b[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = DOT_NT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+__global__ __launch_bounds__(DOT_NT) void k_dot_final(const double* __restrict__ partials, long n,
+                                                      double* __restrict__ out) {
+    __shared__ double red[DOT_NT];
+    double s = 0;
+    for (long i = threadIdx.x; i < n; i += DOT_NT) s += partials[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = DOT_NT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_f32_to_f64(const float* __restrict__ a, long n, double* __restrict__ b) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+__global__ __launch_bounds__(256) void k_f64_to_f32(const double* __restrict__ a, long n, float* __restrict__ b) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+__global__ __launch_bounds__(256) void k_iota_u32(uint32_t* __restrict__ p, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (uint32_t)i;
+}
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, long n, uint32_t v) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_state_diff(const uint32_t* __restrict__ Ya, const uint32_t* __restrict__ Yb,
+                                                    const double* __restrict__ Pa, const double* __restrict__ Pb,
+                                                    long n, double* __restrict__ partials) {
+    __shared__ double r0[256], r1[256], r2[256];
+    double cnt = 0, mdp = 0, mp = 0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        cnt += (Ya[i] != Yb[i]) ? 1.0 : 0.0;
+        mdp = fmax(mdp, fabs(Pa[i] - Pb[i]));
+        mp = fmax(mp, fabs(Pb[i]));
+    }
+    r0[threadIdx.x] = cnt;
+    r1[threadIdx.x] = mdp;
+    r2[threadIdx.x] = mp;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            r0[threadIdx.x] += r0[threadIdx.x + off];
+            r1[threadIdx.x] = fmax(r1[threadIdx.x], r1[threadIdx.x + off]);
+            r2[threadIdx.x] = fmax(r2[threadIdx.x], r2[threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[3 * blockIdx.x] = r0[0];
+        partials[3 * blockIdx.x + 1] = r1[0];
+        partials[3 * blockIdx.x + 2] = r2[0];
+    }
+}
+
+// ------------------------------------------------------------ launchers
+void rev_drift_keys(int R, uint32_t* Y, const double* P, const double* lY, double* lP, long np, double cd_fx,
+                    double cd, int lgN, const PosArgs& pa, uint32_t* keys, uint32_t* pids, uint32_t* lmax,
+                    hipStream_t st) {
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_rev_drift_keys<RR>), dim3(nblk(np, 256)), dim3(256), 0, st, Y, P, lY,
+                                         lP, np, cd_fx, cd, lgN, pa, keys, pids, lmax));
+    CPM_LAUNCH_CHECK();
+}
+
+void deposit_fx_vec(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* Y, const double* lP,
+                    long np, int lgN, const PosArgs& pa, const uint32_t* lmax, float* s, hipStream_t st) {
+    const unsigned nb = (unsigned)brick_count(pa);
+    const long M = 1L << (R * lgN);
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit_fx_vec<RR>), dim3(nb), dim3(FXB_NT), 0, st, bstart, perm, Y,
+                                         lP, np, lgN, pa, lmax, s, M));
+    CPM_LAUNCH_CHECK();
+}
+
+void neg_div_f32(int R, const float* s, int lgN, double res, float* psi, hipStream_t st) {
+    const long M = 1L << (R * lgN);
+    const float inv12res = (float)(1.0 / (12.0 * res));
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_neg_div_f32<RR>), dim3(nblk(M, 256)), dim3(256), 0, st, s, lgN, M,
+                                         inv12res, psi));
+    CPM_LAUNCH_CHECK();
+}
+
+void adj_gather_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* Y, const double* P,
+                         const double* lY, const double* lP, const uint32_t* idi, uint32_t* Yo, double* Po,
+                         double* lYo, double* lPo, uint32_t* ido, long np, const float* phi, const float* chi,
+                         int lgN, const PosArgs& pa, AdjCoef c, hipStream_t st) {
+    const unsigned nb = (unsigned)brick_count(pa);
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_adj_gather_fx_brick<RR>), dim3(nb), dim3(FXG_NT), 0, st, bstart, perm,
+                                         Y, P, lY, lP, idi, Yo, Po, lYo, lPo, ido, np, phi, chi, lgN, pa, c));
+    CPM_LAUNCH_CHECK();
+}
+
+template <typename T>
+void scatter_by_id(const T* src, const uint32_t* id, long np, int nc, T* dst, hipStream_t st) {
+    hipLaunchKernelGGL((k_scatter_by_id<T>), dim3(nblk(np, 256)), dim3(256), 0, st, src, id, np, nc, dst);
+    CPM_LAUNCH_CHECK();
+}
+template <typename T>
+void gather_by_id(const T* src, const uint32_t* id, long np, int nc, T* dst, hipStream_t st) {
+    hipLaunchKernelGGL((k_gather_by_id<T>), dim3(nblk(np, 256)), dim3(256), 0, st, src, id, np, nc, dst);
+    CPM_LAUNCH_CHECK();
+}
+template void scatter_by_id<double>(const double*, const uint32_t*, long, int, double*, hipStream_t);
+template void scatter_by_id<uint32_t>(const uint32_t*, const uint32_t*, long, int, uint32_t*, hipStream_t);
+template void gather_by_id<double>(const double*, const uint32_t*, long, int, double*, hipStream_t);
+template void gather_by_id<uint32_t>(const uint32_t*, const uint32_t*, long, int, uint32_t*, hipStream_t);
+
+void neg_div_f64(int R, const double* u, int lgN, double res, double* psi, hipStream_t st) {
+    const long n = 1L << (R * lgN);
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_neg_div_f64<RR>), dim3(nblk(n, 256)), dim3(256), 0, st, u, lgN, n, res,
+                                         psi));
+    CPM_LAUNCH_CHECK();
+}
+
+void axpby(long n, double a, const double* x, double b, const double* y, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_axpby, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, x, b, y, out);
+    CPM_LAUNCH_CHECK();
+}
+
+void dot(const double* a, const double* b, long n, double* partials, long max_partials, double* out,
+         hipStream_t st) {
+    long nb = max_partials < 1 ? 1 : max_partials;
+    long chunk = (n + nb - 1) / nb;
+    if (chunk < DOT_NT) chunk = DOT_NT;
+    nb = (n + chunk - 1) / chunk;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_dot_partial, dim3((unsigned)nb), dim3(DOT_NT), 0, st, a, b, n, chunk, partials);
+    CPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(DOT_NT), 0, st, (const double*)partials, nb, out);
+    CPM_LAUNCH_CHECK();
+}
+
+void f32_to_f64(const float* a, long n, double* b, hipStream_t st) {
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(nblk(n, 256)), dim3(256), 0, st, a, n, b);
+    CPM_LAUNCH_CHECK();
+}
+void f64_to_f32(const double* a, long n, float* b, hipStream_t st) {
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(nblk(n, 256)), dim3(256), 0, st, a, n, b);
+    CPM_LAUNCH_CHECK();
+}
+void iota_u32(uint32_t* p, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_iota_u32, dim3(nblk(n, 256)), dim3(256), 0, st, p, n);
+    CPM_LAUNCH_CHECK();
+}
+void fill_u32(uint32_t* p, long n, uint32_t v, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(nblk(n, 256)), dim3(256), 0, st, p, n, v);
+    CPM_LAUNCH_CHECK();
+}
+void state_diff(const uint32_t* Ya, const uint32_t* Yb, const double* Pa, const double* Pb, long n,
+                double* partials, int nb, hipStream_t st) {
+    hipLaunchKernelGGL(k_state_diff, dim3((unsigned)nb), dim3(256), 0, st, Ya, Yb, Pa, Pb, n, partials);
+    CPM_LAUNCH_CHECK();
+}
+
+}  // namespace cpm

@@ -1,0 +1,65 @@
+// pm_adjoint.hpp -- host interface of the adjoint kernels of the FX PM step
+// (pm_adjoint.hip) and of the small vector kernels of the differentiable
+// driver.
+#pragma once
+#include "pm_particles.hpp"
+
+namespace cpm {
+
+// Reverse of the drift that a fused kick appends, and the drift's adjoint:
+//   Y -= round(P * cd_fx)        (exact inverse of the forward's integer step)
+//   lP += cd * lY                (lY: dL/dY in 1/length, cd: drift factor)
+// then the FX sort keys of the new Y, and max |lP| (float bits) into *lmax
+// (which must hold 0 on entry).  cd_fx == 0 leaves Y as it is.
+void rev_drift_keys(int R, uint32_t* Y, const double* P, const double* lY, double* lP, long np, double cd_fx,
+                    double cd, int lgN, const PosArgs& pa, uint32_t* keys, uint32_t* pids, uint32_t* lmax,
+                    hipStream_t st);
+
+// s_k(c) = sum_p lP_k(p) * W(Y_p, c), k < R: the CIC deposit of the R
+// components of lP onto R meshes of M cells (s + k * M), accumulated in
+// 64-bit fixed point with a scale taken from *lmax (deterministic).
+void deposit_fx_vec(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* Y, const double* lP,
+                    long np, int lgN, const PosArgs& pa, const uint32_t* lmax, float* s, hipStream_t st);
+
+// psi = -sum_k D_k s_k with D_k the FP32 4-point derivative of the gather:
+// the adjoint of the FD gradient, applied to R meshes.
+void neg_div_f32(int R, const float* s, int lgN, double res, float* psi, hipStream_t st);
+
+// Coefficients of the fused adjoint gather (per particle, per component k):
+//   P_k  += cF * F_k                          F = interpolated FD gradient of phi
+//   lY_k += cT * T_k + cU * U_k              T_k = sum_i lP_i dF_i/dY_k (phi fixed)
+//                                             U_k = sum_c dW(Y, c)/dY_k * chi(c)
+struct AdjCoef {
+    double cF, cT, cU;
+};
+// One workgroup per brick; reads particle perm[s] and writes it, its id and
+// its adjoints to position s.  phi == nullptr: F = T = 0; chi == nullptr: U = 0.
+void adj_gather_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* Y, const double* P,
+                         const double* lY, const double* lP, const uint32_t* idi, uint32_t* Yo, double* Po,
+                         double* lYo, double* lPo, uint32_t* ido, long np, const float* phi, const float* chi,
+                         int lgN, const PosArgs& pa, AdjCoef c, hipStream_t st);
+
+// dst[k * np + id[s]] = src[k * np + s]  /  dst[k * np + s] = src[k * np + id[s]]
+template <typename T>
+void scatter_by_id(const T* src, const uint32_t* id, long np, int nc, T* dst, hipStream_t st);
+template <typename T>
+void gather_by_id(const T* src, const uint32_t* id, long np, int nc, T* dst, hipStream_t st);
+
+// psi = -sum_k D_k u_k on an N^R grid (u: R SoA components) with the FP64
+// 4-point derivative of the Zel'dovich start (src/run/numeric.hh:96-101).
+void neg_div_f64(int R, const double* u, int lgN, double res, double* psi, hipStream_t st);
+
+void axpby(long n, double a, const double* x, double b, const double* y, double* out, hipStream_t st);
+// Deterministic FP64 dot product (fixed block tree): *out = sum a[i] * b[i].
+void dot(const double* a, const double* b, long n, double* partials, long max_partials, double* out,
+         hipStream_t st);
+void f32_to_f64(const float* a, long n, double* b, hipStream_t st);
+void f64_to_f32(const double* a, long n, float* b, hipStream_t st);
+void iota_u32(uint32_t* p, long n, hipStream_t st);
+void fill_u32(uint32_t* p, long n, uint32_t v, hipStream_t st);
+// Per-block {count(Ya != Yb), max |Pa - Pb|, max |Pb|} over n elements (Y, P
+// in the same order); nb blocks, partials[3 * nb].
+void state_diff(const uint32_t* Ya, const uint32_t* Yb, const double* Pa, const double* Pb, long n,
+                double* partials, int nb, hipStream_t st);
+
+}  // namespace cpm

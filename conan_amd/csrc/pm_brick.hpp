@@ -1,0 +1,115 @@
+// pm_brick.hpp -- device-side helpers of the FX brick order, shared by the
+// forward kernels (pm_particles.hip) and their adjoints (pm_adjoint.hip).
+//
+// The FX form keeps its particle arrays grouped by brick (FXB_X x FXB_Y x
+// FXB_Z cells, bricks in x-fastest order) in the order of the previous step:
+// `perm` maps sorted position s -> storage index.  The sort key of a particle
+// is (brick << R) | face class (brick_key), and bstart[g] is the first sorted
+// position of key group g, so bstart[b << R] .. bstart[(b + 1) << R] is brick b.
+#pragma once
+#include "pm_particles.hpp"
+
+namespace cpm {
+
+static constexpr int FXB_NT = 256;  // deposit workgroup size
+#ifndef CPM_FXG_NT
+#define CPM_FXG_NT 512
+#endif
+static constexpr int FXG_NT = CPM_FXG_NT;  // gather workgroup size
+static constexpr int FXB_CELLS = FXB_X * FXB_Y * FXB_Z;
+// potential tile of a brick: 2 cells of halo below and 3 above along each axis
+static constexpr int FXB_TILE = (FXB_X + 5) * (FXB_Y + 5) * (FXB_Z + 5);
+static constexpr double FX_ONE = 1099511627776.0;  // 2^40: fixed-point unit of the deposit accumulators
+
+struct BrickCoord {
+    int x0[3], B[3], nb[3], bi[3];
+};
+
+__device__ __forceinline__ BrickCoord brick_coord(uint32_t b, const PosArgs& pa) {
+    BrickCoord c;
+    uint32_t t = b;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        c.nb[k] = 1 << pa.lnb[k];
+        c.bi[k] = (int)(t & (uint32_t)(c.nb[k] - 1));
+        t >>= pa.lnb[k];
+        c.B[k] = 1 << pa.lb[k];
+        c.x0[k] = c.bi[k] << pa.lb[k];
+    }
+    return c;
+}
+
+// The sorted-particle ranges whose CIC clouds reach a brick: the whole brick
+// itself, and of each lower neighbour only the face classes towards it.
+struct BrickRanges {
+    uint32_t rs0[64], rlen[64], roff[65];
+    int n;
+};
+
+// Fills `r` (in LDS) for brick bc; called by the whole workgroup.  The first
+// wave resolves every (neighbour, class) pair at once (lane = neighbour x
+// class) and compacts the non-empty ones with a ballot, so the workgroup
+// waits for one round of offset loads.  Ends with a barrier.
+template <int R>
+__device__ __forceinline__ void brick_ranges(BrickRanges& r, const uint32_t* __restrict__ bstart,
+                                             const BrickCoord& bc, const PosArgs& pa, int tid) {
+    if (tid < 64) {
+        const int nbr = tid >> R, c = tid & ((1 << R) - 1);
+        const int d[3] = {nbr & 1, (nbr >> 1) & 1, (nbr >> 2) & 1};
+        bool ok = nbr < (1 << R);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) ok &= !(d[k] && bc.nb[k] == 1);  // that brick would be this one
+        ok &= (nbr == 0) ? (c == 0) : ((c & nbr) == nbr);
+        uint32_t s0 = 0, s1 = 0;
+        if (ok) {
+            uint32_t nb = 0;
+            int sh = 0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                nb |= (uint32_t)((bc.bi[k] - d[k]) & (bc.nb[k] - 1)) << sh;
+                sh += pa.lnb[k];
+            }
+            const uint32_t g0 = (nb << R) + (nbr == 0 ? 0 : c);
+            s0 = bstart[g0];
+            s1 = bstart[nbr == 0 ? g0 + (1u << R) : g0 + 1];
+        }
+        const bool have = ok && s1 > s0;
+        const uint64_t m = __ballot(have);
+        if (have) {
+            const int slot = __popcll(m & ((tid == 0) ? 0ull : (~0ull >> (64 - tid))));
+            r.rs0[slot] = s0;
+            r.rlen[slot] = s1 - s0;
+        }
+        if (tid == 0) r.n = __popcll(m);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t a = 0;
+        for (int q = 0; q < r.n; ++q) {
+            r.roff[q] = a;
+            a += r.rlen[q];
+        }
+        r.roff[r.n] = a;
+    }
+    __syncthreads();
+}
+
+// Sorted position of the t-th particle of the concatenated ranges.
+__device__ __forceinline__ uint32_t brick_range_pos(const BrickRanges& r, uint32_t t) {
+    int q = 0;
+    while (r.roff[q + 1] <= t) ++q;
+    return r.rs0[q] + (t - r.roff[q]);
+}
+
+// Host side: launch-grid size, and a dispatch on the number of dimensions
+// that binds the compile-time RR for the kernel template.
+static inline unsigned nblk(long n, int nt) { return (unsigned)((n + nt - 1) / nt); }
+
+#define CPM_R_DISPATCH(R, ...)                                             \
+    do {                                                                   \
+        if ((R) == 3) { constexpr int RR = 3; __VA_ARGS__; }               \
+        else if ((R) == 2) { constexpr int RR = 2; __VA_ARGS__; }          \
+        else throw Error(-2, "only 2-D and 3-D runs are supported");      \
+    } while (0)
+
+}  // namespace cpm

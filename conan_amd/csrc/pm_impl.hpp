@@ -1,0 +1,585 @@
+// pm_impl.hpp -- the device state of one PM run (the pm_handle of the C ABI)
+// and the host-side step sequence over it; shared by the C ABI and the
+// differentiable driver.
+//
+// The handle is the MI355X counterpart of Gravity<R> (src/run/nbody.hh:53-219):
+// it keeps positions and momenta resident in HBM across steps, and one kick is
+//   keys -> stable (key, id) sort -> row bounds -> deposit (rho)
+//   -> rows r2c -> [y fwd] -> last axis fwd * Green * bwd -> [y bwd] -> rows c2r (phi)
+//   -> gather + kick (+ the next step's drift).
+// Scalars that the reference computes per call (adot, G, the drift and kick
+// factors, the Poisson scale) are computed here on the host with the
+// reference's expressions, so both sides see the same doubles.
+#pragma once
+#include "../../include/conanpm.h"
+#include "pm_fft_api.hpp"
+#include "pm_particles.hpp"
+#include "pm_sort.hpp"
+#include "pm_spectral.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace cpm {
+
+// src/run/cosmology.hh:6-28
+struct Cosmo {
+    double H0, Om, OL, Ok, G;
+    Cosmo(double H0_, double m_, double L_) : H0(H0_), Om(m_), OL(L_), Ok(1 - m_ - L_), G(3. / 2 * m_ * H0_ * H0_) {}
+    double adot(double a) const { return H0 * a * sqrt(OL + Om * pow(a, -3) + Ok * pow(a, -2)); }
+};
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n_) { alloc(n_); }
+    void alloc(size_t n_) {
+        release();
+        n = n_;
+        if (n) CPM_HIP(hipMalloc(&p, sizeof(T) * n));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+};
+
+inline void check_finite(const double* a, size_t n, const char* what) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(a[i])) throw Error(PM_ENONFINITE, std::string("non-finite value in ") + what);
+}
+
+inline int ilog2_exact(long n, const char* what) {
+    if (n < 1 || (n & (n - 1))) throw Error(PM_EUNSUP, std::string(what) + " must be a power of two");
+    return ilog2(n);
+}
+
+// Per-phase device time from HIP event pairs recorded on the handle's
+// stream.  Recording does not synchronise: pairs queue up in a pool and are
+// resolved when the totals are read, so profiling can stay on in a timed run.
+struct Prof {
+    bool on = false;
+    std::vector<hipEvent_t> evb, eve;
+    std::vector<int> phase;
+    size_t used = 0;
+    double ms[PM_NPHASES] = {};
+    long long cnt[PM_NPHASES] = {};
+    void enable(bool v) {
+        if (v) {
+            for (int i = 0; i < PM_NPHASES; ++i) {
+                ms[i] = 0;
+                cnt[i] = 0;
+            }
+            used = 0;
+        }
+        on = v;
+    }
+    void begin(int ph, hipStream_t st) {
+        if (!on) return;
+        if (used == evb.size()) {
+            hipEvent_t a, b;
+            CPM_HIP(hipEventCreate(&a));
+            CPM_HIP(hipEventCreate(&b));
+            evb.push_back(a);
+            eve.push_back(b);
+            phase.push_back(ph);
+        }
+        phase[used] = ph;
+        CPM_HIP(hipEventRecord(evb[used], st));
+    }
+    void end(int ph, hipStream_t st) {
+        if (!on) return;
+        CPM_HIP(hipEventRecord(eve[used], st));
+        ++used;
+        (void)ph;
+    }
+    // Folds the queued pairs into the totals (synchronises on the last event).
+    void collect() {
+        if (used == 0) return;
+        CPM_HIP(hipEventSynchronize(eve[used - 1]));
+        for (size_t i = 0; i < used; ++i) {
+            float t = 0;
+            CPM_HIP(hipEventElapsedTime(&t, evb[i], eve[i]));
+            ms[phase[i]] += t;
+            ++cnt[phase[i]];
+        }
+        used = 0;
+    }
+    ~Prof() {
+        for (size_t i = 0; i < evb.size(); ++i) {
+            (void)hipEventDestroy(evb[i]);
+            (void)hipEventDestroy(eve[i]);
+        }
+    }
+};
+
+struct ImplBase {
+    virtual ~ImplBase() = default;
+    virtual void init_zeldovich(const double* phi, double a_pos, double a_mom) = 0;
+    virtual void drift(double a, double da) = 0;
+    virtual void kick(double a, double da) = 0;
+    virtual double steps(double a, double da, int n) = 0;
+    virtual void get_state(double* X, double* P) = 0;
+    virtual void set_state(const double* X, const double* P) = 0;
+    virtual void set_positions_device(const double* Xsoa) = 0;
+    virtual void deposit(double* rho, int32_t* keys, double mass_override) = 0;
+    virtual void potential(double a, double* phi) = 0;
+    virtual void forces(double a, double* F) = 0;
+    virtual void poisson_mesh(const double* rho, double a, double* phi) = 0;
+    virtual void gather_mesh(const double* phi, double* F) = 0;
+    virtual void power_spectrum(int nbins, const double* edges, double* k, double* pk, double* cnt) = 0;
+    virtual void diagnostics(pm_diag* d) = 0;
+    double last_apos = 0, last_avel = 0;
+    bool have_state = false;
+    Prof prof;
+    hipStream_t st = nullptr;
+};
+
+template <typename PP>
+struct Impl : ImplBase {
+    using Pos = typename PP::Pos;
+    using Mesh = typename PP::Mesh;
+    static constexpr bool FX = sizeof(Mesh) == 4;
+
+    pm_config cfg;
+    int R, lgNf;
+    long Np, M, nrows;
+    Cosmo cos;
+    double mass;
+    PosArgs pa;
+    FFTPlan<Mesh> fft;
+    RadixSorter sorter;
+    DevBuf<Pos> X, X2;
+    DevBuf<double> P, P2;
+    // FX keeps particles in the cell order of the last deposit; id[s] is the
+    // mass-grid index of the particle stored at s (X2/P2/id2: the next order).
+    DevBuf<uint32_t> id, id2;
+    DevBuf<uint32_t> k0, v0, k1, v1, row_start;
+    DevBuf<Mesh> mesh;
+    DevBuf<cx<Mesh>> spec, nyq;
+    DevBuf<double> row_partial, red, scal;
+    const uint32_t* skeys = nullptr;
+    const uint32_t* spids = nullptr;
+    bool have_deposit = false;
+    long npartials = 0;     // FP64 mesh-sum partials written by the last deposit
+    long nsort_groups = 0;  // rows (FP64) or bricks (FX) delimited by row_start
+
+    static int fx_keybits(int dim, int lgN) {
+        PosArgs t{};
+        brick_layout(dim, lgN, t);
+        return t.lnb[0] + t.lnb[1] + t.lnb[2] + dim;  // brick, then face class
+    }
+
+    static long ipow(long n, int r) {
+        long s = 1;
+        for (int i = 0; i < r; ++i) s *= n;
+        return s;
+    }
+
+    explicit Impl(const pm_config& c)
+        : cfg(c), R(c.dim), lgNf(ilog2_exact(c.Nf, "Nf")), Np(ipow(c.Nm, c.dim)), M(ipow(c.Nf, c.dim)),
+          nrows(M / c.Nf), cos(c.H0, c.Omega_m, c.Omega_L), fft(c.dim, c.Nf),
+          // FP64: (cell, id) pairs sorted by the full cell index; FX: by brick
+          sorter(Np, FX ? fx_keybits(c.dim, lgNf) : c.dim * lgNf) {
+        mass = pow(double(c.Nf) / c.Nm, R);  // src/run/nbody.hh:85
+        pa.L = c.L;
+        pa.res = c.L / c.Nf;
+        pa.wrap = (c.flags & PM_FLAG_WRAP_GATHER) ? 1 : 0;
+        pa.fshift = 32 - lgNf;
+        pa.fmask = (uint32_t)((1ull << pa.fshift) - 1);
+        pa.fscale = (float)ldexp(1.0, -pa.fshift);
+        pa.brick = 0;
+        for (int k = 0; k < 3; ++k) pa.lb[k] = pa.lnb[k] = 0;
+        if (FX) brick_layout(R, lgNf, pa);
+        nsort_groups = FX ? (brick_count(pa) << R) : nrows;
+        CPM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        X.alloc((size_t)R * Np);
+        P.alloc((size_t)R * Np);
+        if (FX) {
+            X2.alloc((size_t)R * Np);
+            P2.alloc((size_t)R * Np);
+            id.alloc(Np);
+            id2.alloc(Np);
+        }
+        k0.alloc(Np);
+        v0.alloc(Np);
+        k1.alloc(Np);
+        v1.alloc(Np);
+        row_start.alloc((nsort_groups > nrows ? nsort_groups : nrows) + 1);
+        mesh.alloc(M);
+        spec.alloc(fft.spec_elems);
+        nyq.alloc(fft.nyq_elems);
+        row_partial.alloc(FX && brick_count(pa) > nrows ? brick_count(pa) : nrows);
+        red.alloc(4096);
+        scal.alloc(16);
+    }
+    ~Impl() override {
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
+    }
+
+    void need_state() const {
+        if (!have_state) throw Error(PM_ESTATE, "no particle state: call pm_init_zeldovich or pm_set_state first");
+    }
+
+    // Storage order = mass-grid order (set after ZA / set_state).
+    void reset_ids() {
+        if (!FX) return;
+        std::vector<uint32_t> h(Np);
+        for (long i = 0; i < Np; ++i) h[i] = (uint32_t)i;
+        CPM_HIP(hipMemcpyAsync(id.p, h.data(), sizeof(uint32_t) * Np, hipMemcpyHostToDevice, st));
+        CPM_HIP(hipStreamSynchronize(st));
+    }
+    // Mass-grid index of each stored particle.
+    std::vector<uint32_t> host_ids() {
+        std::vector<uint32_t> h(Np);
+        if (FX) {
+            CPM_HIP(hipMemcpyAsync(h.data(), id.p, sizeof(uint32_t) * Np, hipMemcpyDeviceToHost, st));
+            CPM_HIP(hipStreamSynchronize(st));
+        } else {
+            for (long i = 0; i < Np; ++i) h[i] = (uint32_t)i;
+        }
+        return h;
+    }
+    // SoA in storage order -> AoS in mass-grid order.
+    void soa_to_aos(const std::vector<double>& soa, const std::vector<uint32_t>& ids, double* out) const {
+        for (long s = 0; s < Np; ++s)
+            for (int k = 0; k < R; ++k) out[(size_t)ids[s] * R + k] = soa[(size_t)k * Np + s];
+    }
+    double drift_factor(double a, double da) const {
+        double adot = cos.adot(a);
+        double c = da / (a * a * adot);  // src/run/nbody.hh:124,129
+        return FX ? c * (4294967296.0 / cfg.L) : c;
+    }
+
+    void init_zeldovich(const double* phi, double a_pos, double a_mom) override {
+        const long npm = Np;
+        if (cfg.Nm < 4 || (cfg.Nm & (cfg.Nm - 1))) throw Error(PM_EUNSUP, "Zel'dovich start needs a power-of-two Nm >= 4");
+        check_finite(phi, npm, "potential");
+        DevBuf<double> d(npm);
+        CPM_HIP(hipMemcpyAsync(d.p, phi, sizeof(double) * npm, hipMemcpyHostToDevice, st));
+        ParticleOps<PP>::zeldovich(R, d.p, ilog2(cfg.Nm), cfg.L, a_pos, a_mom, X.p, P.p, st);
+        CPM_HIP(hipStreamSynchronize(st));
+        reset_ids();
+        last_apos = a_pos;
+        last_avel = a_mom;
+        have_state = true;
+        have_deposit = false;
+    }
+
+    void do_deposit(Mesh m) {
+        prof.begin(0, st);
+        ParticleOps<PP>::keys(R, X.p, Np, lgNf, pa, k0.p, v0.p, st);
+        const int where = sorter.sort(k0.p, v0.p, k1.p, v1.p, st);
+        skeys = where ? k1.p : k0.p;
+        spids = where ? v1.p : v0.p;
+        row_bounds(skeys, Np, FX ? 0 : lgNf, nsort_groups, row_start.p, st);
+        prof.end(0, st);
+        prof.begin(1, st);
+        if constexpr (FX) {
+            deposit_fx_brick(R, row_start.p, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p, st);
+            npartials = brick_count(pa);
+        } else {
+            ParticleOps<PP>::deposit(R, row_start.p, skeys, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p,
+                                     nrows, st);
+            npartials = nrows;
+        }
+        prof.end(1, st);
+        have_deposit = true;
+    }
+
+    // mesh (rho) -> mesh (phi) at scale factor a.
+    // FX: scal[0] must hold the FP64 mean of the mesh (mean_ready), or the
+    // row partials of the last deposit provide it.
+    void do_poisson(double a, bool mean_ready = false) {
+        const double s = fft.M / cos.G * a;  // real_part(size / G * a), src/run/nbody.hh:110
+        const double kf = 2 * M_PI / cfg.L;
+        const double* mean_ptr = nullptr;
+        Mesh shift = (Mesh)1.0;  // delta = rho - 1 (src/run/nbody.hh:103-104)
+        if (FX) {
+            // FP64 mean of the FP32 deposit, subtracted on the device
+            if (!mean_ready) sum_partials(row_partial.p, npartials, 1.0 / (double)M, red.p, 4096, scal.p, st);
+            mean_ptr = scal.p;
+        }
+        for (int stage = 0; stage < 5; ++stage) {
+            prof.begin(2 + stage, st);
+            fft.poisson_stage(stage, 0, mesh.p, mesh.p, spec.p, nyq.p, shift, mean_ptr, kf, s, st);
+            prof.end(2 + stage, st);
+            if (stage >= 1 && stage <= 3) {
+                prof.begin(9, st);
+                fft.poisson_stage(stage, 1, mesh.p, mesh.p, spec.p, nyq.p, shift, mean_ptr, kf, s, st);
+                prof.end(9, st);
+            }
+        }
+    }
+
+    void drift(double a, double da) override {
+        need_state();
+        prof.begin(8, st);
+        ParticleOps<PP>::drift(X.p, P.p, (long)R * Np, drift_factor(a, da), st);
+        prof.end(8, st);
+        CPM_HIP(hipStreamSynchronize(st));
+        last_apos = a;
+    }
+
+    void kick_impl(double a, double da, bool fuse, double a_next, double da_next) {
+        do_deposit((Mesh)mass);
+        do_poisson(a);
+        const double adot = cos.adot(a);
+        const double ck = da / adot;  // src/run/nbody.hh:112-116
+        const double cd = fuse ? drift_factor(a_next, da_next) : 0.0;
+        prof.begin(7, st);
+        if constexpr (FX) {
+            // kick (+ drift) and move the particles into the deposit's brick order
+            gather_kick_fx_brick(R, row_start.p, spids, reinterpret_cast<const uint32_t*>(X.p), P.p, id.p,
+                                 reinterpret_cast<uint32_t*>(X2.p), P2.p, id2.p, Np,
+                                 reinterpret_cast<const float*>(mesh.p), lgNf, pa, ck, cd, fuse ? 1 : 0, st);
+            std::swap(X.p, X2.p);
+            std::swap(P.p, P2.p);
+            std::swap(id.p, id2.p);
+        } else {
+            ParticleOps<PP>::gather_kick(R, X.p, P.p, Np, mesh.p, lgNf, pa, ck, cd, fuse ? 1 : 0, nullptr,
+                                         Reorder<PP>(), st);
+        }
+        prof.end(7, st);
+        last_avel = a;
+        if (fuse) last_apos = a_next;
+    }
+
+    void kick(double a, double da) override {
+        need_state();
+        kick_impl(a, da, false, 0, 0);
+        CPM_HIP(hipStreamSynchronize(st));
+    }
+
+    // LeapFrog::run's loop body n times (src/run/leapfrog.hh:38-44).
+    double steps(double a, double da, int n) override {
+        need_state();
+        if (n <= 0) return a;
+        prof.begin(8, st);
+        ParticleOps<PP>::drift(X.p, P.p, (long)R * Np, drift_factor(a, da), st);
+        prof.end(8, st);
+        last_apos = a;
+        for (int i = 0; i < n; ++i) {
+            const double ak = a + da / 2;
+            const bool more = i + 1 < n;
+            const double anext = a + da;
+            kick_impl(ak, da, more, anext, da);
+            a += da;
+        }
+        CPM_HIP(hipStreamSynchronize(st));
+        return a;
+    }
+
+    void get_state(double* Xh, double* Ph) override {
+        need_state();
+        std::vector<double> tmp((size_t)R * Np);
+        const std::vector<uint32_t> ids = host_ids();
+        if (Xh) {
+            if (FX) {
+                DevBuf<double> d((size_t)R * Np);
+                fx_to_f64(reinterpret_cast<const uint32_t*>(X.p), (long)R * Np, cfg.L, d.p, st);
+                CPM_HIP(hipMemcpyAsync(tmp.data(), d.p, sizeof(double) * R * Np, hipMemcpyDeviceToHost, st));
+                CPM_HIP(hipStreamSynchronize(st));
+            } else {
+                CPM_HIP(hipMemcpyAsync(tmp.data(), X.p, sizeof(double) * R * Np, hipMemcpyDeviceToHost, st));
+                CPM_HIP(hipStreamSynchronize(st));
+            }
+            soa_to_aos(tmp, ids, Xh);
+        }
+        if (Ph) {
+            CPM_HIP(hipMemcpyAsync(tmp.data(), P.p, sizeof(double) * R * Np, hipMemcpyDeviceToHost, st));
+            CPM_HIP(hipStreamSynchronize(st));
+            soa_to_aos(tmp, ids, Ph);
+        }
+    }
+
+    void upload_positions_soa(const double* soa_host_or_dev, bool device) {
+        if (FX) {
+            DevBuf<double> d;
+            const double* src = soa_host_or_dev;
+            if (!device) {
+                d.alloc((size_t)R * Np);
+                CPM_HIP(hipMemcpyAsync(d.p, soa_host_or_dev, sizeof(double) * R * Np, hipMemcpyHostToDevice, st));
+                src = d.p;
+            }
+            f64_to_fx(src, (long)R * Np, cfg.L, reinterpret_cast<uint32_t*>(X.p), st);
+            CPM_HIP(hipStreamSynchronize(st));
+        } else {
+            CPM_HIP(hipMemcpyAsync(X.p, soa_host_or_dev, sizeof(double) * R * Np,
+                                   device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+            CPM_HIP(hipStreamSynchronize(st));
+        }
+    }
+
+    void set_state(const double* Xh, const double* Ph) override {
+        std::vector<double> tmp((size_t)R * Np);
+        if (Xh) {
+            check_finite(Xh, (size_t)R * Np, "positions");
+            for (long i = 0; i < Np; ++i)
+                for (int k = 0; k < R; ++k) tmp[(size_t)k * Np + i] = Xh[i * R + k];
+            upload_positions_soa(tmp.data(), false);
+            reset_ids();
+        }
+        if (Ph) {
+            check_finite(Ph, (size_t)R * Np, "momenta");
+            const std::vector<uint32_t> ids = host_ids();  // momenta follow the stored order
+            for (long s = 0; s < Np; ++s)
+                for (int k = 0; k < R; ++k) tmp[(size_t)k * Np + s] = Ph[(size_t)ids[s] * R + k];
+            CPM_HIP(hipMemcpyAsync(P.p, tmp.data(), sizeof(double) * R * Np, hipMemcpyHostToDevice, st));
+            CPM_HIP(hipStreamSynchronize(st));
+        } else if (!have_state) {
+            CPM_HIP(hipMemsetAsync(P.p, 0, sizeof(double) * R * Np, st));
+            CPM_HIP(hipStreamSynchronize(st));
+        }
+        if (Xh) have_state = true;
+        have_deposit = false;
+    }
+
+    void set_positions_device(const double* Xsoa) override {
+        upload_positions_soa(Xsoa, true);
+        reset_ids();
+        CPM_HIP(hipMemsetAsync(P.p, 0, sizeof(double) * R * Np, st));
+        CPM_HIP(hipStreamSynchronize(st));
+        have_state = true;
+    }
+
+    void mesh_to_host(double* out) {
+        if (FX) {
+            std::vector<float> t(M);
+            CPM_HIP(hipMemcpyAsync(t.data(), mesh.p, sizeof(float) * M, hipMemcpyDeviceToHost, st));
+            CPM_HIP(hipStreamSynchronize(st));
+            for (long i = 0; i < M; ++i) out[i] = t[i];
+        } else {
+            CPM_HIP(hipMemcpyAsync(out, mesh.p, sizeof(double) * M, hipMemcpyDeviceToHost, st));
+            CPM_HIP(hipStreamSynchronize(st));
+        }
+    }
+    void mesh_from_host(const double* in) {
+        if (FX) {
+            std::vector<float> t(M);
+            for (long i = 0; i < M; ++i) t[i] = (float)in[i];
+            CPM_HIP(hipMemcpyAsync(mesh.p, t.data(), sizeof(float) * M, hipMemcpyHostToDevice, st));
+            CPM_HIP(hipStreamSynchronize(st));
+        } else {
+            CPM_HIP(hipMemcpyAsync(mesh.p, in, sizeof(double) * M, hipMemcpyHostToDevice, st));
+            CPM_HIP(hipStreamSynchronize(st));
+        }
+    }
+
+    void deposit(double* rho, int32_t* keys, double mass_override) override {
+        need_state();
+        do_deposit((Mesh)(mass_override > 0 ? mass_override : mass));
+        if (rho) mesh_to_host(rho);
+        if (keys) {
+            PosArgs pk = pa;
+            pk.brick = 0;  // the full cell index idx(origin)
+            ParticleOps<PP>::keys(R, X.p, Np, lgNf, pk, k1.p, v1.p, st);
+            std::vector<uint32_t> k(Np);
+            CPM_HIP(hipMemcpyAsync(k.data(), k1.p, sizeof(uint32_t) * Np, hipMemcpyDeviceToHost, st));
+            CPM_HIP(hipStreamSynchronize(st));
+            const std::vector<uint32_t> ids = host_ids();
+            for (long s = 0; s < Np; ++s) keys[ids[s]] = (int32_t)k[s];
+        }
+        CPM_HIP(hipStreamSynchronize(st));
+    }
+
+    void potential(double a, double* phi) override {
+        need_state();
+        do_deposit((Mesh)mass);
+        do_poisson(a);
+        mesh_to_host(phi);
+    }
+
+    void forces(double a, double* F) override {
+        need_state();
+        do_deposit((Mesh)mass);
+        do_poisson(a);
+        gather_to_host(F);
+    }
+
+    void gather_mesh(const double* phi, double* F) override {
+        need_state();
+        check_finite(phi, M, "potential");
+        mesh_from_host(phi);
+        have_deposit = false;
+        gather_to_host(F);
+    }
+
+    void gather_to_host(double* F) {
+        DevBuf<double> d((size_t)R * Np);
+        ParticleOps<PP>::gather_kick(R, X.p, P.p, Np, mesh.p, lgNf, pa, 0.0, 0.0, 0, d.p, Reorder<PP>(), st);
+        std::vector<double> tmp((size_t)R * Np);
+        CPM_HIP(hipMemcpyAsync(tmp.data(), d.p, sizeof(double) * R * Np, hipMemcpyDeviceToHost, st));
+        CPM_HIP(hipStreamSynchronize(st));
+        soa_to_aos(tmp, host_ids(), F);
+    }
+
+    void poisson_mesh(const double* rho, double a, double* phi) override {
+        check_finite(rho, M, "density");
+        mesh_from_host(rho);
+        if (FX) reduce_sum(mesh.p, M, 0, 1.0 / (double)M, red.p, 4096, scal.p, st);
+        do_poisson(a, true);
+        mesh_to_host(phi);
+    }
+
+    void power_spectrum(int nbins, const double* edges, double* kmean, double* pk, double* cnt) override {
+        need_state();
+        if (nbins < 1) throw Error(PM_EINVAL, "nbins must be positive");
+        do_deposit((Mesh)mass);
+        std::vector<double> e(nbins + 1);
+        const double kf = 2 * M_PI / cfg.L, kn = M_PI * cfg.Nf / cfg.L;
+        if (edges) {
+            for (int b = 0; b <= nbins; ++b) e[b] = edges[b];
+        } else {
+            for (int b = 0; b <= nbins; ++b) e[b] = kf * pow(kn / kf, (double)b / nbins);
+        }
+        const double* mean_ptr = nullptr;
+        if (FX) {
+            sum_partials(row_partial.p, npartials, 1.0 / (double)M, red.p, 4096, scal.p, st);
+            mean_ptr = scal.p;
+        }
+        fft.forward(mesh.p, spec.p, nyq.p, (Mesh)1.0, mean_ptr, st);
+        DevBuf<double> de(nbins + 1), part(pk_partial_elems(fft.spec_elems, fft.nyq_elems, nbins)), out(3 * nbins);
+        CPM_HIP(hipMemcpyAsync(de.p, e.data(), sizeof(double) * (nbins + 1), hipMemcpyHostToDevice, st));
+        const double norm = pow(cfg.L, R) / ((double)M * (double)M);
+        cpm::power_spectrum<Mesh>(spec.p, nyq.p, (long)fft.spec_elems, (long)fft.nyq_elems, R, lgNf, kf, de.p, nbins,
+                                  norm, part.p, out.p, out.p + nbins, out.p + 2 * nbins, st);
+        std::vector<double> h(3 * nbins);
+        CPM_HIP(hipMemcpyAsync(h.data(), out.p, sizeof(double) * 3 * nbins, hipMemcpyDeviceToHost, st));
+        CPM_HIP(hipStreamSynchronize(st));
+        for (int b = 0; b < nbins; ++b) {
+            if (kmean) kmean[b] = h[b];
+            if (pk) pk[b] = h[nbins + b];
+            if (cnt) cnt[b] = h[2 * nbins + b];
+        }
+        have_deposit = false;  // the mesh was consumed
+    }
+
+    void diagnostics(pm_diag* d) override {
+        need_state();
+        std::memset(d, 0, sizeof(*d));
+        double h[8] = {0};
+        if (have_deposit) sum_partials(row_partial.p, npartials, 1.0, red.p, 4096, scal.p + 2, st);
+        for (int k = 0; k < R; ++k) reduce_sum(P.p + (size_t)k * Np, Np, 0, 1.0, red.p, 4096, scal.p + 3 + k, st);
+        reduce_sum(P.p, (long)R * Np, 1, 1.0, red.p, 4096, scal.p + 6, st);
+        CPM_HIP(hipMemcpyAsync(h, scal.p + 2, sizeof(double) * 5, hipMemcpyDeviceToHost, st));
+        CPM_HIP(hipStreamSynchronize(st));
+        d->mass = have_deposit ? h[0] : 0.0;
+        for (int k = 0; k < R; ++k) d->momentum[k] = h[1 + k];
+        d->p2 = h[4];
+    }
+};
+
+}  // namespace cpm

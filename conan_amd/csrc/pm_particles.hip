@@ -13,7 +13,7 @@
 //        so the periodic wrap is exact integer overflow; mesh and gather in
 //        FP32, momenta and reductions in FP64.
 // Particle arrays are SoA: component k of particle i at [k * np + i].
-#include "pm_particles.hpp"
+#include "pm_brick.hpp"
 
 namespace cpm {
 
@@ -138,37 +138,8 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit(const uint32_t* __restrict__
 }
 
 // ------------------------------------------------------------ FX: persistent brick order
-// The FX form keeps its particle arrays grouped by brick (FXB_X x FXB_Y x
-// FXB_Z cells, bricks in x-fastest order) in the order of the previous step:
-// `perm` maps sorted position s -> storage index, and particles move less
-// than a cell per step, so perm is close to the identity and reads through
-// it are nearly contiguous.  bstart[b] is the first sorted position of brick b.
-static constexpr int FXB_NT = 256;
-#ifndef CPM_FXG_NT
-#define CPM_FXG_NT 512
-#endif
-static constexpr int FXG_NT = CPM_FXG_NT;  // gather workgroup size
-static constexpr int FXB_CELLS = FXB_X * FXB_Y * FXB_Z;
-static constexpr int FXB_TILE = (FXB_X + 5) * (FXB_Y + 5) * (FXB_Z + 5);
-static constexpr double FX_ONE = 1099511627776.0;  // 2^40
-
-struct BrickCoord {
-    int x0[3], B[3], nb[3], bi[3];
-};
-
-__device__ __forceinline__ BrickCoord brick_coord(uint32_t b, const PosArgs& pa) {
-    BrickCoord c;
-    uint32_t t = b;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        c.nb[k] = 1 << pa.lnb[k];
-        c.bi[k] = (int)(t & (uint32_t)(c.nb[k] - 1));
-        t >>= pa.lnb[k];
-        c.B[k] = 1 << pa.lb[k];
-        c.x0[k] = c.bi[k] << pa.lb[k];
-    }
-    return c;
-}
+// Particles move less than a cell per step, so `perm` (pm_brick.hpp) is close
+// to the identity and reads through it are nearly contiguous.
 
 // Deposit, one workgroup per brick.  The brick's cells receive the CIC
 // weights of the particles of the brick and of its lower neighbours (a
@@ -177,12 +148,13 @@ __device__ __forceinline__ BrickCoord brick_coord(uint32_t b, const PosArgs& pa)
 // brick.  The accumulators are 64-bit fixed point (2^-40 mass units):
 // integer addition is associative, so the sum does not depend on the order of
 // the LDS atomics and the deposit is deterministic; each cell is rounded once
-// to FP32 on the way out and written by exactly one workgroup.
-template <int R>
+// to the output type (FP32 for the step) on the way out and written by
+// exactly one workgroup.
+template <int R, typename MT>
 __global__ __launch_bounds__(FXB_NT) void k_deposit_fx_brick(const uint32_t* __restrict__ bstart,
                                                              const uint32_t* __restrict__ perm,
                                                              const uint32_t* __restrict__ X, long np, int lgN,
-                                                             PosArgs pa, float mass, float* __restrict__ mesh,
+                                                             PosArgs pa, float mass, MT* __restrict__ mesh,
                                                              double* __restrict__ partial) {
     __shared__ unsigned long long acc[FXB_CELLS];
     __shared__ double red[FXB_NT];
@@ -191,58 +163,12 @@ __global__ __launch_bounds__(FXB_NT) void k_deposit_fx_brick(const uint32_t* __r
     const int ncell = bc.B[0] * bc.B[1] * bc.B[2];
     const int tid = threadIdx.x;
     for (int e = tid; e < ncell; e += FXB_NT) acc[e] = 0ull;
-    // The particle ranges that reach this brick: the whole own brick, and of
-    // each lower neighbour only the face classes towards this brick.  The
-    // first wave resolves all of them at once (lane = neighbour x class), so
-    // the workgroup waits for one round of offset loads, then walks the
-    // concatenated ranges.
-    __shared__ uint32_t rs0[64], rlen[64], roff[65];
-    __shared__ int nrange;
-    if (tid < 64) {
-        const int nbr = tid >> R, c = tid & ((1 << R) - 1);
-        const int d[3] = {nbr & 1, (nbr >> 1) & 1, (nbr >> 2) & 1};
-        bool ok = nbr < (1 << R);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) ok &= !(d[k] && bc.nb[k] == 1);  // that brick would be this one
-        ok &= (nbr == 0) ? (c == 0) : ((c & nbr) == nbr);
-        uint32_t s0 = 0, s1 = 0;
-        if (ok) {
-            uint32_t nb = 0;
-            int sh = 0;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                nb |= (uint32_t)((bc.bi[k] - d[k]) & (bc.nb[k] - 1)) << sh;
-                sh += pa.lnb[k];
-            }
-            const uint32_t g0 = (nb << R) + (nbr == 0 ? 0 : c);
-            s0 = bstart[g0];
-            s1 = bstart[nbr == 0 ? g0 + (1u << R) : g0 + 1];
-        }
-        const bool have = ok && s1 > s0;
-        const uint64_t m = __ballot(have);
-        if (have) {
-            const int slot = __popcll(m & ((tid == 0) ? 0ull : (~0ull >> (64 - tid))));
-            rs0[slot] = s0;
-            rlen[slot] = s1 - s0;
-        }
-        if (tid == 0) nrange = __popcll(m);
-    }
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t a = 0;
-        for (int r = 0; r < nrange; ++r) {
-            roff[r] = a;
-            a += rlen[r];
-        }
-        roff[nrange] = a;
-    }
-    __syncthreads();
+    __shared__ BrickRanges rg;
+    brick_ranges<R>(rg, bstart, bc, pa, tid);
     {
-        const uint32_t total = roff[nrange];
+        const uint32_t total = rg.roff[rg.n];
         for (uint32_t t = tid; t < total; t += FXB_NT) {
-            int r = 0;
-            while (roff[r + 1] <= t) ++r;
-            const uint32_t s = rs0[r] + (t - roff[r]);
+            const uint32_t s = brick_range_pos(rg, t);
             const uint32_t i = perm[s];
             int o[3] = {0, 0, 0};
             float f[3] = {0.f, 0.f, 0.f};
@@ -270,7 +196,7 @@ __global__ __launch_bounds__(FXB_NT) void k_deposit_fx_brick(const uint32_t* __r
     double rsum = 0;
     for (int e = tid; e < ncell; e += FXB_NT) {
         const int lx = e & (bc.B[0] - 1), ly = (e >> pa.lb[0]) & (bc.B[1] - 1), lz = e >> (pa.lb[0] + pa.lb[1]);
-        const float v = (float)((double)(long long)acc[e] * (1.0 / FX_ONE));
+        const MT v = (MT)((double)(long long)acc[e] * (1.0 / FX_ONE));
         const long idx = (long)(bc.x0[0] + lx) + (long)N * ((bc.x0[1] + ly) + (long)N * (bc.x0[2] + lz));
         mesh[idx] = v;
         rsum += (double)v;
@@ -696,14 +622,6 @@ void subsample(int R, const double* X, int Nm, int s, double L, double* Y, hipSt
 }
 
 // ------------------------------------------------------------ launchers
-static inline unsigned nblk(long n, int nt) { return (unsigned)((n + nt - 1) / nt); }
-
-#define CPM_R_DISPATCH(R, ...)                          \
-    do {                                                \
-        if ((R) == 3) { constexpr int RR = 3; __VA_ARGS__; } \
-        else if ((R) == 2) { constexpr int RR = 2; __VA_ARGS__; } \
-        else throw Error(-2, "only 2-D and 3-D runs are supported"); \
-    } while (0)
 
 template <typename PP>
 void ParticleOps<PP>::keys(int R, const Pos* X, long np, int lgN, const PosArgs& pa, uint32_t* keys,
@@ -749,13 +667,18 @@ void brick_layout(int R, int lgN, PosArgs& pa) {
 
 long brick_count(const PosArgs& pa) { return 1L << (pa.lnb[0] + pa.lnb[1] + pa.lnb[2]); }
 
+template <typename MT>
 void deposit_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, long np, int lgN,
-                      const PosArgs& pa, float mass, float* mesh, double* partial, hipStream_t st) {
+                      const PosArgs& pa, float mass, MT* mesh, double* partial, hipStream_t st) {
     const unsigned nb = (unsigned)brick_count(pa);
-    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit_fx_brick<RR>), dim3(nb), dim3(FXB_NT), 0, st, bstart, perm, X,
-                                         np, lgN, pa, mass, mesh, partial));
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit_fx_brick<RR, MT>), dim3(nb), dim3(FXB_NT), 0, st, bstart, perm,
+                                         X, np, lgN, pa, mass, mesh, partial));
     CPM_LAUNCH_CHECK();
 }
+template void deposit_fx_brick<float>(int, const uint32_t*, const uint32_t*, const uint32_t*, long, int,
+                                      const PosArgs&, float, float*, double*, hipStream_t);
+template void deposit_fx_brick<double>(int, const uint32_t*, const uint32_t*, const uint32_t*, long, int,
+                                       const PosArgs&, float, double*, double*, hipStream_t);
 
 void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, const double* P,
                           const uint32_t* idi, uint32_t* Xo, double* Po, uint32_t* ido, long np, const float* phi,

@@ -122,9 +122,11 @@ void brick_layout(int R, int lgN, PosArgs& pa);
 long brick_count(const PosArgs& pa);
 // FX deposit / gather+kick(+drift) over particles grouped by brick (perm:
 // sorted position -> storage index; bstart: brick_count + 1 offsets).  The
-// deposit writes one FP64 partial sum per brick.
+// deposit writes one FP64 partial sum per brick; MT is float (the step) or
+// double (the exact value of the fixed-point accumulators).
+template <typename MT>
 void deposit_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, long np, int lgN,
-                      const PosArgs& pa, float mass, float* mesh, double* partial, hipStream_t st);
+                      const PosArgs& pa, float mass, MT* mesh, double* partial, hipStream_t st);
 void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, const double* P,
                           const uint32_t* idi, uint32_t* Xo, double* Po, uint32_t* ido, long np, const float* phi,
                           int lgN, const PosArgs& pa, double ck, double cd, int cd_on, hipStream_t st);

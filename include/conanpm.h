@@ -184,6 +184,59 @@ int pm_fft_c2r(int dim, int N, int precision, const double* spec, double* out);
  * grid s times per axis by interpolating displacements.  rho: Nf^dim doubles. */
 int pm_cic(int dim, int Nm, int Nf, double L, int s, const double* X, double* rho);
 
+/* ---------------------------------------------------------------- differentiable PM
+ * No counterpart in the reference (it has no gradients).  The map
+ *   white noise w (N^dim, the input of `conan ic --scale-free`)
+ *     -> density and potential (pm_ic_scale_free) -> Zel'dovich start at a0
+ *        (pm_init_zeldovich with a_mom = a0 + da/2, as LeapFrog::run,
+ *        src/run/leapfrog.hh:36-37) -> nsteps leapfrog steps of da (pm_steps)
+ *     -> final CIC density rho on the force mesh (pm_deposit)
+ * run in the FP32 form, and its adjoint: given dL/drho, dL/dw by reverse-mode
+ * differentiation with hand-written adjoint kernels.  The backward pass
+ * integrates the leapfrog backwards (the drift is undone exactly in the
+ * fixed-point positions, the kick with the recomputed force) and restores the
+ * exact state from the checkpoint stored every ckpt_every steps of the
+ * forward pass, instead of storing the trajectory.
+ * pm_diff_forward: noise (N^dim doubles) -> rho ((f N)^dim doubles).
+ * pm_diff_backward: grad_rho ((f N)^dim) -> grad_noise (N^dim), for the
+ *   state of the last pm_diff_forward on this handle.
+ * device != 0: the pointers are device memory of the current HIP device
+ * (the caller orders its own work before the call; the call returns after
+ * the handle's stream is synchronised); device == 0: host memory. */
+typedef struct pm_diff_params {
+    int dim;          /* 2 or 3 */
+    int mbits;        /* particles (and noise) per side N = 2^mbits */
+    int f;            /* force mesh per side = f * N */
+    double L;         /* box size */
+    double ns;        /* scale-free power-law slope */
+    int smooth;       /* ic --smooth */
+    double scale;     /* ic --scale: smoothing scale */
+    double sigma;     /* ic --sigma: normalisation */
+    double H0, Omega_m, Omega_L;
+    double a0, da;    /* Zel'dovich start and step in a */
+    int nsteps;       /* leapfrog steps */
+    int ckpt_every;   /* checkpoint interval in steps (>= 1) */
+    int precision;    /* PM_FP32 (the only form with an adjoint) */
+} pm_diff_params;
+
+typedef struct pm_diff pm_diff;
+
+typedef struct pm_diff_stats {
+    double forward_ms;      /* device time of the last pm_diff_forward (IC, steps, final deposit) */
+    double backward_ms;     /* device time of the last pm_diff_backward */
+    double device_bytes;    /* device memory held by the handle */
+    int ncheckpoints;       /* stored states */
+    long long recon_pos;    /* position components of the reversed states that differed from the checkpoints */
+    double recon_mom;       /* max |P_reversed - P_checkpoint| / max |P| over the checkpoints */
+} pm_diff_stats;
+
+int pm_diff_create(const pm_diff_params* p, pm_diff** out);
+void pm_diff_destroy(pm_diff* d);
+const char* pm_diff_last_error(const pm_diff* d);
+int pm_diff_forward(pm_diff* d, const double* noise, double* rho, int device);
+int pm_diff_backward(pm_diff* d, const double* grad_rho, double* grad_noise, int device);
+int pm_diff_get_stats(const pm_diff* d, pm_diff_stats* s);
+
 #ifdef __cplusplus
 }
 #endif

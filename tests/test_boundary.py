@@ -22,11 +22,9 @@ def declared_functions():
 
 
 def test_library_exports_every_declared_symbol():
-    import ctypes
-
     from conan_amd import _lib
 
-    lib = ctypes.CDLL(_lib.LIB_PATH)
+    lib = _lib.load()
     names = declared_functions()
     assert len(names) >= 20
     missing = [n for n in names if not hasattr(lib, n)]
@@ -81,6 +79,43 @@ def test_create_without_device_fails_cleanly():
     p = _lib.pm_ic_params(3, 4, 100.0, -2.0, 0, 8.0, 0.83)
     noise = np.zeros(16 ** 3)
     assert lib.pm_ic_scale_free(ctypes.byref(p), _lib.ptr(noise), _lib.ptr(noise), None) == -3
+
+
+def _dparams(**kw):
+    from conan_amd import _lib
+
+    c = dict(dim=3, mbits=4, f=2, L=100.0, ns=-2.0, smooth=0, scale=8.0, sigma=0.83, H0=68.5, Omega_m=1.0,
+             Omega_L=0.0, a0=0.05, da=0.05, nsteps=4, ckpt_every=2, precision=1)
+    c.update(kw)
+    return _lib.pm_diff_params(*[c[k] for k, _ in _lib.pm_diff_params._fields_])
+
+
+@pytest.mark.parametrize("kw,code", [({"dim": 4}, -2), ({"mbits": 11}, -2), ({"f": 3}, -2), ({"f": 128}, -2),
+                                     ({"precision": 0}, -2), ({"nsteps": 0}, -1), ({"ckpt_every": 0}, -1),
+                                     ({"da": 0.0}, -1), ({"sigma": -1.0}, -1)])
+def test_diff_create_rejects_bad_params(kw, code):
+    import ctypes
+
+    from conan_amd import _lib
+
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.pm_diff_create(ctypes.byref(_dparams(**kw)), ctypes.byref(h)) == code and not h.value
+    assert lib.pm_diff_last_error(None)
+    assert lib.pm_diff_forward(None, None, None, 0) == -1
+    assert lib.pm_diff_backward(None, None, None, 0) == -1
+
+
+@pytest.mark.skipif(HAS_GPU, reason="checks the no-device error path")
+def test_diff_create_without_device_fails_cleanly():
+    import ctypes
+
+    from conan_amd import _lib
+
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.pm_diff_create(ctypes.byref(_dparams()), ctypes.byref(h)) == -3
+    assert b"no HIP device" in lib.pm_diff_last_error(None)
 
 
 # ------------------------------------------------------------------ CLI
