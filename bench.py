@@ -9,10 +9,12 @@ a = 0.02, leapfrog in a with da = 0.0049 (200 steps reach a = 0.99).
 A step is one leapfrog iteration (drift + deposit + Poisson solve + gather +
 kick).  The timed region runs --steps steps of the FP32 form (32-bit
 fixed-point positions, FP32 mesh/FFT/gather, FP64 momenta) with particles and
-meshes resident in HBM; the same is measured for the FP64 plain form.  One JSON
-line goes to stdout; progress goes to stderr.
+meshes resident in HBM; the same is measured for the FP64 plain form.  The
+differentiable runs (noise -> final density and its adjoint, at 128^3 and
+512^3 particles) give the gradient-to-forward time ratio.  One JSON line goes
+to stdout; progress goes to stderr.
 
-Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--no-cpu] [--no-fp64] [--no-grad]
 """
 from __future__ import annotations
 
@@ -117,6 +119,50 @@ def cpu_baseline(phi, nsteps: int, threads: int):
                       f"{secs:.1f} s"}
 
 
+# Differentiable runs: (name, mbits, f, ns, a0, da, nsteps, ckpt_every)
+DIFF_CONFIGS = [
+    ("128^3 particles on 128^3, ns=-2, 20 steps a=0.05->1, checkpoints every 5", 7, 1, -2.0, 0.05, 0.0475, 20, 5),
+    ("512^3 particles on 512^3, ns=-1.5, 40 steps a=0.05->1, checkpoints every 8", 9, 1, -1.5, 0.05, 0.02375, 40, 8),
+]
+
+
+def diff_bench(name, mbits, f, ns, a0, da, nsteps, ckpt, repeats=2):
+    """Forward (noise -> final density) and backward (dL/drho -> dL/dnoise) of
+    the differentiable FP32 run, loss = sum (rho - rho_target)^2 with the
+    target from a second seed, all buffers on the device."""
+    import torch
+
+    from conan_amd.diff import DiffPM
+
+    Np = (1 << mbits) ** DIM
+    eng = DiffPM(DIM, mbits, f, L, ns, a0, da, nsteps, ckpt_every=ckpt)
+    try:
+        w = torch.tensor(np.random.default_rng(1).standard_normal(Np), device="cuda")
+        target = eng.forward_device(torch.tensor(np.random.default_rng(2).standard_normal(Np), device="cuda"))
+        best = None
+        for _ in range(repeats):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rho = eng.forward_device(w)
+            t1 = time.perf_counter()
+            g = eng.backward_device(2 * (rho - target))
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            st = eng.stats()
+            if best is None or st["forward_ms"] + st["backward_ms"] < best["forward_ms"] + best["backward_ms"]:
+                best = dict(st, wall_forward_s=t1 - t0, wall_backward_s=t2 - t1)
+        assert bool(torch.isfinite(g).all())
+        f_ms, b_ms = best["forward_ms"], best["backward_ms"]
+        return {"workload": name, "particles": Np, "mesh": f << mbits, "steps": nsteps, "ckpt_every": ckpt,
+                "forward_ms": round(f_ms, 3), "backward_ms": round(b_ms, 3),
+                "backward_over_forward": round(b_ms / f_ms, 3),
+                "grad_fwd_ratio": round((f_ms + b_ms) / f_ms, 3),
+                "device_GB": round(best["device_bytes"] / 1e9, 2), "checkpoints": best["ncheckpoints"],
+                "reversal_position_mismatches": best["recon_pos"], "reversal_momentum_rel": best["recon_mom"]}
+    finally:
+        eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +172,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-grad", action="store_true", help="skip the differentiable runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box here has one GPU); "
@@ -217,6 +264,15 @@ def main():
         out["pk_fp32_vs_fp64"] = {"a": rw["a"], "bins": int(m.sum()), "max_rel": float(rel.max()),
                                   "median_rel": float(np.median(rel))}
     out["grad_fwd_ratio"] = None
+    if not args.no_grad:
+        runs = []
+        for cfg in DIFF_CONFIGS:
+            log(f"differentiable run: {cfg[0]}")
+            runs.append(diff_bench(*cfg))
+            log(f"  {runs[-1]}")
+        # (forward + adjoint) / forward of the 128^3 differentiable run
+        out["grad_fwd_ratio"] = runs[0]["grad_fwd_ratio"]
+        out["differentiable"] = runs
     out["final_a"] = r32["a"]
     out["momentum_sum_fp32"] = r32["diag"]["momentum"]
 

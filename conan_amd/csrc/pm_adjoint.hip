@@ -295,7 +295,7 @@ __global__ __launch_bounds__(FXG_NT) void k_adj_gather_fx_brick(
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             Yo[k * np + s] = ys[k];
-            Po[k * np + s] = P[k * np + i] + (double)F[k] * cf.cF;
+            Po[k * np + s] = P[k * np + i] + fx_kick_increment((double)F[k] * cf.cF, cf.pq);
             lYo[k * np + s] = lY[k * np + i] + cf.cT * T[k] + cf.cU * U[k];
             lPo[k * np + s] = lam[k];
         }

@@ -26,11 +26,12 @@ void deposit_fx_vec(int R, const uint32_t* bstart, const uint32_t* perm, const u
 void neg_div_f32(int R, const float* s, int lgN, double res, float* psi, hipStream_t st);
 
 // Coefficients of the fused adjoint gather (per particle, per component k):
-//   P_k  += cF * F_k                          F = interpolated FD gradient of phi
+//   P_k  += cF * F_k (on the pq grid)         F = interpolated FD gradient of phi
 //   lY_k += cT * T_k + cU * U_k              T_k = sum_i lP_i dF_i/dY_k (phi fixed)
 //                                             U_k = sum_c dW(Y, c)/dY_k * chi(c)
 struct AdjCoef {
     double cF, cT, cU;
+    double pq;  // momentum quantum of the forward kick (fx_kick_increment)
 };
 // One workgroup per brick; reads particle perm[s] and writes it, its id and
 // its adjoints to position s.  phi == nullptr: F = T = 0; chi == nullptr: U = 0.

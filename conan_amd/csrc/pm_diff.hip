@@ -166,6 +166,7 @@ struct DiffPM {
         ic_potential();
         // Zel'dovich start and the leapfrog (pm_init_zeldovich + pm_steps)
         ParticleOps<PosFX>::zeldovich(R, icm.p, lgNm, p.L, p.a0, p.a0 + p.da / 2, sim.X.p, sim.P.p, st);
+        sim.fx_quantize_momenta();
         iota_u32(sim.id.p, Np, st);
         sim.have_state = true;
         ParticleOps<PosFX>::drift(sim.X.p, sim.P.p, (long)R * Np, sim.drift_factor(a_seq[0], p.da), st);
@@ -285,7 +286,7 @@ struct DiffPM {
         const double m = sim.mass;
         // final density: rho = m sum W(Y, c)  ->  lY = m dW/dY . grad_rho
         sort_particles();
-        adj_gather(nullptr, grho.p, AdjCoef{0.0, 0.0, m});
+        adj_gather(nullptr, grho.p, AdjCoef{0.0, 0.0, m, sim.pq});
         for (int i = p.nsteps - 1; i >= 0; --i) {
             const bool more = i + 1 < p.nsteps;
             const double cd_fx = more ? sim.drift_factor(a_seq[i + 1], p.da) : 0.0;
@@ -303,7 +304,7 @@ struct DiffPM {
             deposit_fx_vec(R, sim.row_start.p, sim.spids, sim.X.p, lP.p, Np, sim.lgNf, sim.pa, lmax.p, svec.p, st);
             neg_div_f32(R, svec.p, sim.lgNf, sim.pa.res, psi.p, st);
             poisson_into(psi.p, chi.p, ak);
-            adj_gather(sim.mesh.p, chi.p, AdjCoef{ck, -ck, -ck * m});
+            adj_gather(sim.mesh.p, chi.p, AdjCoef{ck, -ck, -ck * m, sim.pq});
             if (i % p.ckpt_every == 0) restore(i / p.ckpt_every);
         }
         sim.have_deposit = false;

@@ -172,6 +172,7 @@ struct Impl : ImplBase {
     bool have_deposit = false;
     long npartials = 0;     // FP64 mesh-sum partials written by the last deposit
     long nsort_groups = 0;  // rows (FP64) or bricks (FX) delimited by row_start
+    double pq = 0;          // FX: momentum quantum (a power of two), see fx_quantize_momenta
 
     static int fx_keybits(int dim, int lgN) {
         PosArgs t{};
@@ -263,6 +264,29 @@ struct Impl : ImplBase {
         return FX ? c * (4294967296.0 / cfg.L) : c;
     }
 
+    // FX momenta live on a grid of pq = 2^(e - 36), max |P| < 2^e (2^-40 L
+    // if all P are 0): kicks add multiples of pq (fx_kick_increment), which is
+    // exact while |P| < 2^53 pq = 2^17 2^e, so every kick can be undone bit
+    // for bit.  The grid is 2^-36 of the largest initial momentum: finer than
+    // the FP32 force by far.
+    void fx_quantize_momenta() {
+        if constexpr (FX) {
+            reduce_max_abs(P.p, (long)R * Np, red.p, 4096, scal.p + 8, st);
+            double mx = 0;
+            CPM_HIP(hipMemcpyAsync(&mx, scal.p + 8, sizeof(double), hipMemcpyDeviceToHost, st));
+            CPM_HIP(hipStreamSynchronize(st));
+            int e = 0;
+            if (mx > 0 && std::isfinite(mx)) {
+                (void)frexp(mx, &e);
+                pq = ldexp(1.0, e - 36);
+            } else {
+                pq = ldexp(1.0, ilogb(cfg.L) - 40);
+            }
+            quantize(P.p, (long)R * Np, pq, st);
+            CPM_HIP(hipStreamSynchronize(st));
+        }
+    }
+
     void init_zeldovich(const double* phi, double a_pos, double a_mom) override {
         const long npm = Np;
         if (cfg.Nm < 4 || (cfg.Nm & (cfg.Nm - 1))) throw Error(PM_EUNSUP, "Zel'dovich start needs a power-of-two Nm >= 4");
@@ -271,6 +295,7 @@ struct Impl : ImplBase {
         CPM_HIP(hipMemcpyAsync(d.p, phi, sizeof(double) * npm, hipMemcpyHostToDevice, st));
         ParticleOps<PP>::zeldovich(R, d.p, ilog2(cfg.Nm), cfg.L, a_pos, a_mom, X.p, P.p, st);
         CPM_HIP(hipStreamSynchronize(st));
+        fx_quantize_momenta();
         reset_ids();
         last_apos = a_pos;
         last_avel = a_mom;
@@ -344,7 +369,7 @@ struct Impl : ImplBase {
             // kick (+ drift) and move the particles into the deposit's brick order
             gather_kick_fx_brick(R, row_start.p, spids, reinterpret_cast<const uint32_t*>(X.p), P.p, id.p,
                                  reinterpret_cast<uint32_t*>(X2.p), P2.p, id2.p, Np,
-                                 reinterpret_cast<const float*>(mesh.p), lgNf, pa, ck, cd, fuse ? 1 : 0, st);
+                                 reinterpret_cast<const float*>(mesh.p), lgNf, pa, ck, pq, cd, fuse ? 1 : 0, st);
             std::swap(X.p, X2.p);
             std::swap(P.p, P2.p);
             std::swap(id.p, id2.p);
@@ -445,6 +470,7 @@ struct Impl : ImplBase {
         }
         if (Xh) have_state = true;
         have_deposit = false;
+        fx_quantize_momenta();
     }
 
     void set_positions_device(const double* Xsoa) override {
@@ -452,6 +478,7 @@ struct Impl : ImplBase {
         reset_ids();
         CPM_HIP(hipMemsetAsync(P.p, 0, sizeof(double) * R * Np, st));
         CPM_HIP(hipStreamSynchronize(st));
+        fx_quantize_momenta();
         have_state = true;
     }
 

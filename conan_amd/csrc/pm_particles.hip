@@ -220,7 +220,7 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
     const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ X,
     const double* __restrict__ P, const uint32_t* __restrict__ idi, uint32_t* __restrict__ Xo,
     double* __restrict__ Po, uint32_t* __restrict__ ido, long np, const float* __restrict__ phi, int lgN,
-    PosArgs pa, double ck, double cd, int cd_on) {
+    PosArgs pa, double ck, double pq, double cd, int cd_on) {
     __shared__ float tile[FXB_TILE];
     const int N = 1 << lgN, msk = N - 1;
     const BrickCoord bc = brick_coord(blockIdx.x, pa);
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
         }
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            const double p = P[k * np + i] - (double)F[k] * ck;
+            const double p = P[k * np + i] - fx_kick_increment((double)F[k] * ck, pq);
             Po[k * np + s] = p;
             Xo[k * np + s] = cd_on ? xs[k] + (uint32_t)__double2ll_rn(p * cd) : xs[k];
         }
@@ -537,6 +537,41 @@ __global__ __launch_bounds__(RED_NT) void k_final_sum(const double* __restrict__
     if (threadIdx.x == 0) *out = red[0] * scale;
 }
 
+__global__ __launch_bounds__(RED_NT) void k_partial_max_abs(const double* __restrict__ a, long n, long chunk,
+                                                            double* __restrict__ partials) {
+    __shared__ double red[RED_NT];
+    const long b0 = (long)blockIdx.x * chunk;
+    const long b1 = b0 + chunk < n ? b0 + chunk : n;
+    double m = 0;
+    for (long i = b0 + threadIdx.x; i < b1; i += RED_NT) m = fmax(m, fabs(a[i]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = RED_NT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(RED_NT) void k_final_max(const double* __restrict__ partials, long n,
+                                                      double* __restrict__ out) {
+    __shared__ double red[RED_NT];
+    double m = 0;
+    for (long i = threadIdx.x; i < n; i += RED_NT) m = fmax(m, partials[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = RED_NT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_quantize(double* __restrict__ a, long n, double q) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = rint(a[i] * (1.0 / q)) * q;
+}
+
 // FX positions -> doubles in [0, L)
 __global__ __launch_bounds__(256) void k_fx_to_f64(const uint32_t* __restrict__ q, long n, double L,
                                                    double* __restrict__ x) {
@@ -682,10 +717,10 @@ template void deposit_fx_brick<double>(int, const uint32_t*, const uint32_t*, co
 
 void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, const double* P,
                           const uint32_t* idi, uint32_t* Xo, double* Po, uint32_t* ido, long np, const float* phi,
-                          int lgN, const PosArgs& pa, double ck, double cd, int cd_on, hipStream_t st) {
+                          int lgN, const PosArgs& pa, double ck, double pq, double cd, int cd_on, hipStream_t st) {
     const unsigned nb = (unsigned)brick_count(pa);
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_kick_fx_brick<RR>), dim3(nb), dim3(FXG_NT), 0, st, bstart, perm,
-                                         X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck, cd, cd_on));
+                                         X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck, pq, cd, cd_on));
     CPM_LAUNCH_CHECK();
 }
 
@@ -722,6 +757,23 @@ void reduce_sum(const T* a, long n, int square, double scale, double* partials, 
 void sum_partials(const double* partials, long n, double scale, double* scratch, long scratch_n, double* out,
                   hipStream_t st) {
     reduce_sum<double>(partials, n, 0, scale, scratch, scratch_n, out, st);
+}
+
+void reduce_max_abs(const double* a, long n, double* partials, long max_partials, double* out, hipStream_t st) {
+    long nb = max_partials < 1 ? 1 : max_partials;
+    long chunk = (n + nb - 1) / nb;
+    if (chunk < RED_NT) chunk = RED_NT;
+    nb = (n + chunk - 1) / chunk;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_partial_max_abs, dim3((unsigned)nb), dim3(RED_NT), 0, st, a, n, chunk, partials);
+    CPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_final_max, dim3(1), dim3(RED_NT), 0, st, (const double*)partials, nb, out);
+    CPM_LAUNCH_CHECK();
+}
+
+void quantize(double* a, long n, double q, hipStream_t st) {
+    hipLaunchKernelGGL(k_quantize, dim3(nblk(n, 256)), dim3(256), 0, st, a, n, q);
+    CPM_LAUNCH_CHECK();
 }
 
 void fx_to_f64(const uint32_t* q, long n, double L, double* x, hipStream_t st) {

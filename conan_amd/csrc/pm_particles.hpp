@@ -49,6 +49,13 @@ __device__ __forceinline__ float fd4(float fm2, float fm1, float fp1, float fp2,
     return ((fm2 - fp2) + 8.f * (fp1 - fm1)) * inv12res;
 }
 
+// FX kick increment F * ck rounded to a multiple of the momentum quantum pq
+// (a power of two).  FX momenta are kept on that grid, so P - dp and P + dp
+// are exact and a kick can be undone bit for bit.
+__host__ __device__ __forceinline__ double fx_kick_increment(double fck, double pq) {
+    return rint(fck * (1.0 / pq)) * pq;
+}
+
 // ------------------------------------------------------------ position policies
 struct PosF64 {
     using Pos = double;
@@ -129,7 +136,10 @@ void deposit_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const
                       const PosArgs& pa, float mass, MT* mesh, double* partial, hipStream_t st);
 void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, const double* P,
                           const uint32_t* idi, uint32_t* Xo, double* Po, uint32_t* ido, long np, const float* phi,
-                          int lgN, const PosArgs& pa, double ck, double cd, int cd_on, hipStream_t st);
+                          int lgN, const PosArgs& pa, double ck, double pq, double cd, int cd_on, hipStream_t st);
+// max |a[i]| (deterministic), and a[i] := rint(a[i] / q) * q.
+void reduce_max_abs(const double* a, long n, double* partials, long max_partials, double* out, hipStream_t st);
+void quantize(double* a, long n, double q, hipStream_t st);
 template <typename T>
 void reduce_sum(const T* a, long n, int square, double scale, double* partials, long max_partials,
                 double* out, hipStream_t st);

@@ -48,10 +48,10 @@ def test_reverse_leapfrog_reconstructs_checkpoints(orc, k):
     _, gw = _loss_and_grad(eng, w, target)
     s = eng.stats()
     assert s["ncheckpoints"] == (n + k - 1) // k
-    assert s["recon_pos"] == 0  # every position bit recovered by the reversed drift
-    # undoing a kick rounds once per step, at the ulp of the later (larger)
-    # momenta: relative to the checkpoint's max |P| this stays ~1e-13
-    assert s["recon_mom"] < 1e-11
+    # every position and momentum bit recovered: the drift is an integer step
+    # and the kick adds a multiple of the momentum quantum
+    assert s["recon_pos"] == 0
+    assert s["recon_mom"] == 0.0
     assert np.all(np.isfinite(gw))
 
 
