@@ -163,6 +163,27 @@ def diff_bench(name, mbits, f, ns, a0, da, nsteps, ckpt, repeats=2):
         eng.close()
 
 
+def copy_probe(nbytes=1 << 30, reps=5):
+    """Device-to-device copy of 1 GiB (torch's copy kernel): the practical
+    streaming bandwidth of this GPU, and the reference kernel of known size
+    for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return {"bytes_per_copy": nbytes, "ms": round(ms, 4), "GBps": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -263,6 +284,7 @@ def main():
         rel = np.abs(p32[m] / pw[m] - 1)
         out["pk_fp32_vs_fp64"] = {"a": rw["a"], "bins": int(m.sum()), "max_rel": float(rel.max()),
                                   "median_rel": float(np.median(rel))}
+    out["copy_probe"] = copy_probe()
     out["grad_fwd_ratio"] = None
     if not args.no_grad:
         runs = []

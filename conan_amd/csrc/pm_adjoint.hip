@@ -29,9 +29,13 @@ template <int R>
 __global__ __launch_bounds__(256) void k_rev_drift_keys(uint32_t* __restrict__ Y, const double* __restrict__ P,
                                                          const double* __restrict__ lY, double* __restrict__ lP,
                                                          long np, double cd_fx, double cd, int lgN, PosArgs pa,
+                                                         const uint32_t* __restrict__ prevkeys,
                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ pids,
-                                                         uint32_t* __restrict__ lmax) {
+                                                         uint32_t* __restrict__ flags, uint32_t* __restrict__ in_count,
+                                                         uint32_t* __restrict__ bmax) {
+    __shared__ float wmax[4];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && prevkeys) flags[np] = 0u;
     float mx = 0.f;
     if (i < np) {
         const int mask = (1 << lgN) - 1;
@@ -48,20 +52,43 @@ __global__ __launch_bounds__(256) void k_rev_drift_keys(uint32_t* __restrict__ Y
             mx = fmaxf(mx, fabsf((float)lp));
             oc[k] = (int)(y >> pa.fshift) & mask;
         }
-        keys[i] = brick_key(oc[0], oc[1], oc[2], R, pa);
-        pids[i] = (uint32_t)i;
+        const uint32_t key = brick_key(oc[0], oc[1], oc[2], R, pa);
+        keys[i] = key;
+        fx_note_key(key, i, prevkeys, pids, flags, in_count);
     }
+    // block max -> bmax[block] (no atomics: one contended address would
+    // serialise millions of them)
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-    if ((threadIdx.x & 63) == 0 && mx > 0.f) atomicMax(lmax, __float_as_uint(mx));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        bmax[blockIdx.x] = __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3])));
+}
+
+// *lmax = max of n non-negative float bit patterns (one workgroup)
+__global__ __launch_bounds__(1024) void k_max_u32(const uint32_t* __restrict__ b, long n, uint32_t* __restrict__ lmax) {
+    __shared__ uint32_t red[1024];
+    uint32_t m = 0;
+    for (long i = threadIdx.x; i < n; i += 1024) m = max(m, b[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *lmax = red[0];
 }
 
 // ------------------------------------------------------------ vector deposit
 // The brick deposit of pm_particles.hip with R value channels instead of the
 // mass: 64-bit fixed-point LDS accumulators, scaled so that the largest
-// |value| maps below 2^40 (headroom for 2^23 contributions per cell).
+// |value| maps below 2^40 (headroom for 2^23 contributions per cell).  The
+// R accumulator tiles take ~96 KB of LDS, so one workgroup fills a CU: it
+// runs 1024 threads to keep 4 waves per SIMD waiting on the LDS atomics.
+static constexpr int VEC_NT = 1024;
 template <int R>
-__global__ __launch_bounds__(FXB_NT) void k_deposit_fx_vec(const uint32_t* __restrict__ bstart,
+__global__ __launch_bounds__(VEC_NT) void k_deposit_fx_vec(const uint32_t* __restrict__ bstart,
                                                            const uint32_t* __restrict__ perm,
                                                            const uint32_t* __restrict__ Y,
                                                            const double* __restrict__ lP, long np, int lgN,
@@ -73,7 +100,7 @@ __global__ __launch_bounds__(FXB_NT) void k_deposit_fx_vec(const uint32_t* __res
     const BrickCoord bc = brick_coord(blockIdx.x, pa);
     const int ncell = bc.B[0] * bc.B[1] * bc.B[2];
     const int tid = threadIdx.x;
-    for (int e = tid; e < ncell; e += FXB_NT)
+    for (int e = tid; e < ncell; e += VEC_NT)
 #pragma unroll
         for (int c = 0; c < R; ++c) acc[c][e] = 0ull;
     const float mx = __uint_as_float(*lmax);
@@ -85,7 +112,7 @@ __global__ __launch_bounds__(FXB_NT) void k_deposit_fx_vec(const uint32_t* __res
     }
     brick_ranges<R>(rg, bstart, bc, pa, tid);
     const uint32_t total = rg.roff[rg.n];
-    for (uint32_t t = tid; t < total; t += FXB_NT) {
+    for (uint32_t t = tid; t < total; t += VEC_NT) {
         const uint32_t sp = brick_range_pos(rg, t);
         const uint32_t i = perm[sp];
         int o[3] = {0, 0, 0};
@@ -117,7 +144,7 @@ __global__ __launch_bounds__(FXB_NT) void k_deposit_fx_vec(const uint32_t* __res
     }
     __syncthreads();
     const double inv = scale > 0 ? 1.0 / scale : 0.0;  // exact: scale is a power of two
-    for (int e = tid; e < ncell; e += FXB_NT) {
+    for (int e = tid; e < ncell; e += VEC_NT) {
         const int lx = e & (bc.B[0] - 1), ly = (e >> pa.lb[0]) & (bc.B[1] - 1), lz = e >> (pa.lb[0] + pa.lb[1]);
         const long idx = (long)(bc.x0[0] + lx) + (long)N * ((bc.x0[1] + ly) + (long)N * (bc.x0[2] + lz));
 #pragma unroll
@@ -295,7 +322,7 @@ __global__ __launch_bounds__(FXG_NT) void k_adj_gather_fx_brick(
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             Yo[k * np + s] = ys[k];
-            Po[k * np + s] = P[k * np + i] + fx_kick_increment((double)F[k] * cf.cF, cf.pq);
+            Po[k * np + s] = P[k * np + i] + fx_kick_increment((double)F[k] * cf.cF, cf.pq, cf.ipq);
             lYo[k * np + s] = lY[k * np + i] + cf.cT * T[k] + cf.cU * U[k];
             lPo[k * np + s] = lam[k];
         }
@@ -425,10 +452,13 @@ __global__ __launch_bounds__(256) void k_state_diff(const uint32_t* __restrict__
 
 // ------------------------------------------------------------ launchers
 void rev_drift_keys(int R, uint32_t* Y, const double* P, const double* lY, double* lP, long np, double cd_fx,
-                    double cd, int lgN, const PosArgs& pa, uint32_t* keys, uint32_t* pids, uint32_t* lmax,
-                    hipStream_t st) {
-    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_rev_drift_keys<RR>), dim3(nblk(np, 256)), dim3(256), 0, st, Y, P, lY,
-                                         lP, np, cd_fx, cd, lgN, pa, keys, pids, lmax));
+                    double cd, int lgN, const PosArgs& pa, const uint32_t* prevkeys, uint32_t* keys, uint32_t* pids,
+                    uint32_t* flags, uint32_t* in_count, uint32_t* bmax, uint32_t* lmax, hipStream_t st) {
+    const unsigned nb = nblk(np, 256);
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_rev_drift_keys<RR>), dim3(nb), dim3(256), 0, st, Y, P, lY, lP, np, cd_fx,
+                                         cd, lgN, pa, prevkeys, keys, pids, flags, in_count, bmax));
+    CPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_max_u32, dim3(1), dim3(1024), 0, st, (const uint32_t*)bmax, (long)nb, lmax);
     CPM_LAUNCH_CHECK();
 }
 
@@ -436,7 +466,7 @@ void deposit_fx_vec(int R, const uint32_t* bstart, const uint32_t* perm, const u
                     long np, int lgN, const PosArgs& pa, const uint32_t* lmax, float* s, hipStream_t st) {
     const unsigned nb = (unsigned)brick_count(pa);
     const long M = 1L << (R * lgN);
-    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit_fx_vec<RR>), dim3(nb), dim3(FXB_NT), 0, st, bstart, perm, Y,
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit_fx_vec<RR>), dim3(nb), dim3(VEC_NT), 0, st, bstart, perm, Y,
                                          lP, np, lgN, pa, lmax, s, M));
     CPM_LAUNCH_CHECK();
 }

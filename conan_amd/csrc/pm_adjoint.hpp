@@ -9,11 +9,12 @@ namespace cpm {
 // Reverse of the drift that a fused kick appends, and the drift's adjoint:
 //   Y -= round(P * cd_fx)        (exact inverse of the forward's integer step)
 //   lP += cd * lY                (lY: dL/dY in 1/length, cd: drift factor)
-// then the FX sort keys of the new Y, and max |lP| (float bits) into *lmax
-// (which must hold 0 on entry).  cd_fx == 0 leaves Y as it is.
+// then the FX sort keys of the new Y (as keys_fx: full sort when prevkeys is
+// null, else incremental), and max |lP| (float bits) into *lmax (bmax:
+// scratch of one word per 256 particles).  cd_fx == 0 leaves Y as it is.
 void rev_drift_keys(int R, uint32_t* Y, const double* P, const double* lY, double* lP, long np, double cd_fx,
-                    double cd, int lgN, const PosArgs& pa, uint32_t* keys, uint32_t* pids, uint32_t* lmax,
-                    hipStream_t st);
+                    double cd, int lgN, const PosArgs& pa, const uint32_t* prevkeys, uint32_t* keys, uint32_t* pids,
+                    uint32_t* flags, uint32_t* in_count, uint32_t* bmax, uint32_t* lmax, hipStream_t st);
 
 // s_k(c) = sum_p lP_k(p) * W(Y_p, c), k < R: the CIC deposit of the R
 // components of lP onto R meshes of M cells (s + k * M), accumulated in
@@ -31,7 +32,7 @@ void neg_div_f32(int R, const float* s, int lgN, double res, float* psi, hipStre
 //                                             U_k = sum_c dW(Y, c)/dY_k * chi(c)
 struct AdjCoef {
     double cF, cT, cU;
-    double pq;  // momentum quantum of the forward kick (fx_kick_increment)
+    double pq, ipq;  // momentum quantum of the forward kick and 1 / pq (fx_kick_increment)
 };
 // One workgroup per brick; reads particle perm[s] and writes it, its id and
 // its adjoints to position s.  phi == nullptr: F = T = 0; chi == nullptr: U = 0.

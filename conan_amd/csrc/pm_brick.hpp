@@ -94,6 +94,22 @@ __device__ __forceinline__ void brick_ranges(BrickRanges& r, const uint32_t* __r
     __syncthreads();
 }
 
+// Bookkeeping of a new FX sort key at storage position s.  Full sort
+// (prevkeys == nullptr): pids[s] = s for the radix sort.  Incremental re-sort:
+// flags[s] = the key differs from the key sorted at s last time, and movers
+// are counted per new group.
+__device__ __forceinline__ void fx_note_key(uint32_t key, long s, const uint32_t* __restrict__ prevkeys,
+                                            uint32_t* __restrict__ pids, uint32_t* __restrict__ flags,
+                                            uint32_t* __restrict__ in_count) {
+    if (prevkeys) {
+        const uint32_t mv = key != prevkeys[s] ? 1u : 0u;
+        flags[s] = mv;
+        if (mv) atomicAdd(&in_count[key], 1u);
+    } else {
+        pids[s] = (uint32_t)s;
+    }
+}
+
 // Sorted position of the t-th particle of the concatenated ranges.
 __device__ __forceinline__ uint32_t brick_range_pos(const BrickRanges& r, uint32_t t) {
     int q = 0;

@@ -42,7 +42,7 @@ struct DiffPM {
     std::vector<std::unique_ptr<DevBuf<uint32_t>>> ckX, ckid;
     std::vector<std::unique_ptr<DevBuf<double>>> ckP;
     DevBuf<double> lY, lP, lY2, lP2;
-    DevBuf<uint32_t> tmpY, lmax;
+    DevBuf<uint32_t> tmpY, bmax, lmax;
     DevBuf<float> svec, psi, chi, grho;
     DevBuf<double> dpart, dscal;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -87,6 +87,7 @@ struct DiffPM {
         lY2.alloc((size_t)R * Np);
         lP2.alloc((size_t)R * Np);
         tmpY.alloc((size_t)R * Np);
+        bmax.alloc((Np + 255) / 256);
         lmax.alloc(1);
         svec.alloc((size_t)R * sim.M);
         psi.alloc(sim.M);
@@ -168,6 +169,7 @@ struct DiffPM {
         ParticleOps<PosFX>::zeldovich(R, icm.p, lgNm, p.L, p.a0, p.a0 + p.da / 2, sim.X.p, sim.P.p, st);
         sim.fx_quantize_momenta();
         iota_u32(sim.id.p, Np, st);
+        sim.resort_ok = false;  // storage order is the mass-grid order now
         sim.have_state = true;
         ParticleOps<PosFX>::drift(sim.X.p, sim.P.p, (long)R * Np, sim.drift_factor(a_seq[0], p.da), st);
         for (int i = 0; i < p.nsteps; ++i) {
@@ -189,7 +191,7 @@ struct DiffPM {
             t.alloc(sim.M);
             out = t.p;
         }
-        deposit_fx_brick<double>(R, sim.row_start.p, sim.spids, sim.X.p, Np, sim.lgNf, sim.pa, (float)sim.mass, out,
+        deposit_fx_brick<double>(R, sim.bstart, sim.spids, sim.X.p, Np, sim.lgNf, sim.pa, (float)sim.mass, out,
                                  sim.row_partial.p, st);
         if (!dev_out) {
             CPM_HIP(hipMemcpyAsync(rho, t.p, sizeof(double) * sim.M, hipMemcpyDeviceToHost, st));
@@ -203,17 +205,12 @@ struct DiffPM {
         have_forward = true;
     }
 
-    // keys of the current positions -> stable sort -> brick offsets (the
-    // first half of Impl::do_deposit)
+    // keys of the current positions -> sort -> brick offsets (the first half
+    // of Impl::do_deposit)
     void sort_particles() {
-        ParticleOps<PosFX>::keys(R, sim.X.p, Np, sim.lgNf, sim.pa, sim.k0.p, sim.v0.p, st);
-        sort_keyed();
-    }
-    void sort_keyed() {
-        const int where = sim.sorter.sort(sim.k0.p, sim.v0.p, sim.k1.p, sim.v1.p, st);
-        sim.skeys = where ? sim.k1.p : sim.k0.p;
-        sim.spids = where ? sim.v1.p : sim.v0.p;
-        row_bounds(sim.skeys, Np, 0, sim.nsort_groups, sim.row_start.p, st);
+        const Impl<PosFX>::FxKeyArgs ka = sim.fx_key_args();
+        keys_fx(R, sim.X.p, Np, sim.lgNf, sim.pa, ka.prevkeys, ka.keys, ka.pids, ka.flags, ka.in_count, st);
+        sim.fx_sort();
     }
 
     void swap_all() {
@@ -225,9 +222,10 @@ struct DiffPM {
     }
 
     void adj_gather(const float* phi, const float* chiv, AdjCoef c) {
-        adj_gather_fx_brick(R, sim.row_start.p, sim.spids, sim.X.p, sim.P.p, lY.p, lP.p, sim.id.p, sim.X2.p, sim.P2.p,
+        adj_gather_fx_brick(R, sim.bstart, sim.spids, sim.X.p, sim.P.p, lY.p, lP.p, sim.id.p, sim.X2.p, sim.P2.p,
                             lY2.p, lP2.p, sim.id2.p, Np, phi, chiv, sim.lgNf, sim.pa, c, st);
         swap_all();
+        sim.resort_ok = true;  // the particles now sit in the order of the last sort
     }
 
     // Compare the reversed state with checkpoint c and continue from the
@@ -253,6 +251,7 @@ struct DiffPM {
         CPM_HIP(hipMemcpyAsync(sim.X.p, cX.p, sizeof(uint32_t) * R * Np, hipMemcpyDeviceToDevice, st));
         CPM_HIP(hipMemcpyAsync(sim.P.p, cP.p, sizeof(double) * R * Np, hipMemcpyDeviceToDevice, st));
         CPM_HIP(hipMemcpyAsync(sim.id.p, cid.p, sizeof(uint32_t) * Np, hipMemcpyDeviceToDevice, st));
+        sim.resort_ok = false;  // the checkpoint's order is not the last sort's
         CPM_HIP(hipStreamSynchronize(st));
         double cnt = 0, mdp = 0, mp = 0;
         for (int b = 0; b < nb; ++b) {
@@ -286,25 +285,25 @@ struct DiffPM {
         const double m = sim.mass;
         // final density: rho = m sum W(Y, c)  ->  lY = m dW/dY . grad_rho
         sort_particles();
-        adj_gather(nullptr, grho.p, AdjCoef{0.0, 0.0, m, sim.pq});
+        adj_gather(nullptr, grho.p, AdjCoef{0.0, 0.0, m, sim.pq, 1.0 / sim.pq});
         for (int i = p.nsteps - 1; i >= 0; --i) {
             const bool more = i + 1 < p.nsteps;
             const double cd_fx = more ? sim.drift_factor(a_seq[i + 1], p.da) : 0.0;
             const double cd = more ? p.da / (a_seq[i + 1] * a_seq[i + 1] * sim.cos.adot(a_seq[i + 1])) : 0.0;
             const double ak = a_seq[i] + p.da / 2;
             const double ck = p.da / sim.cos.adot(ak);  // src/run/nbody.hh:112-116
-            fill_u32(lmax.p, 1, 0u, st);
-            rev_drift_keys(R, sim.X.p, sim.P.p, lY.p, lP.p, Np, cd_fx, cd, sim.lgNf, sim.pa, sim.k0.p, sim.v0.p,
-                           lmax.p, st);
-            sort_keyed();
-            deposit_fx_brick<float>(R, sim.row_start.p, sim.spids, sim.X.p, Np, sim.lgNf, sim.pa, (float)m, sim.mesh.p,
+            const Impl<PosFX>::FxKeyArgs ka = sim.fx_key_args();
+            rev_drift_keys(R, sim.X.p, sim.P.p, lY.p, lP.p, Np, cd_fx, cd, sim.lgNf, sim.pa, ka.prevkeys, ka.keys,
+                           ka.pids, ka.flags, ka.in_count, bmax.p, lmax.p, st);
+            sim.fx_sort();
+            deposit_fx_brick<float>(R, sim.bstart, sim.spids, sim.X.p, Np, sim.lgNf, sim.pa, (float)m, sim.mesh.p,
                              sim.row_partial.p, st);
             sim.npartials = brick_count(sim.pa);
             sim.do_poisson(ak);  // phi_i, as in the forward
-            deposit_fx_vec(R, sim.row_start.p, sim.spids, sim.X.p, lP.p, Np, sim.lgNf, sim.pa, lmax.p, svec.p, st);
+            deposit_fx_vec(R, sim.bstart, sim.spids, sim.X.p, lP.p, Np, sim.lgNf, sim.pa, lmax.p, svec.p, st);
             neg_div_f32(R, svec.p, sim.lgNf, sim.pa.res, psi.p, st);
             poisson_into(psi.p, chi.p, ak);
-            adj_gather(sim.mesh.p, chi.p, AdjCoef{ck, -ck, -ck * m, sim.pq});
+            adj_gather(sim.mesh.p, chi.p, AdjCoef{ck, -ck, -ck * m, sim.pq, 1.0 / sim.pq});
             if (i % p.ckpt_every == 0) restore(i / p.ckpt_every);
         }
         sim.have_deposit = false;

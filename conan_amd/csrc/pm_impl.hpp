@@ -164,6 +164,12 @@ struct Impl : ImplBase {
     // mass-grid index of the particle stored at s (X2/P2/id2: the next order).
     DevBuf<uint32_t> id, id2;
     DevBuf<uint32_t> k0, v0, k1, v1, row_start;
+    // FX incremental re-sort (see fx_sort): new keys, moved flags and their
+    // scan, sorted keys / perm outputs, the other group-start buffer,
+    // per-group counters (movers in | fill), stayers, group totals.
+    DevBuf<uint32_t> knew, flags, mscan, skb0, skb1, permb, bs2, cnt2, stay, tot, scan_scr;
+    const uint32_t* bstart = nullptr;  // group starts of the last sort (row_start or bs2)
+    bool resort_ok = false;            // storage order == output order of the last FX sort
     DevBuf<Mesh> mesh;
     DevBuf<cx<Mesh>> spec, nyq;
     DevBuf<double> row_partial, red, scal;
@@ -216,6 +222,20 @@ struct Impl : ImplBase {
         k1.alloc(Np);
         v1.alloc(Np);
         row_start.alloc((nsort_groups > nrows ? nsort_groups : nrows) + 1);
+        if (FX) {
+            knew.alloc(Np);
+            flags.alloc(Np + 1);
+            mscan.alloc(Np + 1);
+            skb0.alloc(Np);
+            skb1.alloc(Np);
+            permb.alloc(Np);
+            bs2.alloc(nsort_groups + 1);
+            cnt2.alloc(2 * nsort_groups);
+            stay.alloc(nsort_groups);
+            tot.alloc(nsort_groups + 1);
+            const size_t a = scan_scratch_elems(Np + 1), b = scan_scratch_elems(nsort_groups + 1);
+            scan_scr.alloc(a > b ? a : b);
+        }
         mesh.alloc(M);
         spec.alloc(fft.spec_elems);
         nyq.alloc(fft.nyq_elems);
@@ -236,6 +256,7 @@ struct Impl : ImplBase {
 
     // Storage order = mass-grid order (set after ZA / set_state).
     void reset_ids() {
+        resort_ok = false;
         if (!FX) return;
         std::vector<uint32_t> h(Np);
         for (long i = 0; i < Np; ++i) h[i] = (uint32_t)i;
@@ -303,17 +324,63 @@ struct Impl : ImplBase {
         have_deposit = false;
     }
 
+    // FX sort, in two steps around a keys kernel (keys_fx, or the adjoint's
+    // reverse drift): fx_key_args() gives the kernel its outputs, fx_sort()
+    // turns them into bstart and the permutation spids.  While the storage
+    // order is the output order of the previous sort (resort_ok: a gather
+    // moved the particles there), the sort is incremental: only particles
+    // whose brick or face class changed are placed anew.  Otherwise a full
+    // stable radix sort of (key, index) runs.
+    struct FxKeyArgs {
+        const uint32_t* prevkeys;
+        uint32_t *keys, *pids, *flags, *in_count;
+    };
+    FxKeyArgs fx_key_args() {
+        if (resort_ok) {
+            zero_u32(cnt2.p, 2 * nsort_groups, st);
+            return FxKeyArgs{skeys, knew.p, nullptr, flags.p, cnt2.p};
+        }
+        return FxKeyArgs{nullptr, k0.p, v0.p, nullptr, nullptr};
+    }
+    void fx_sort() {
+        if (resort_ok) {
+            uint32_t* bnew = (bstart == row_start.p) ? bs2.p : row_start.p;
+            uint32_t* sk = (skeys == skb0.p) ? skb1.p : skb0.p;
+            exclusive_scan(flags.p, mscan.p, Np + 1, scan_scr.p, st);
+            resort_counts(bstart, mscan.p, cnt2.p, nsort_groups, stay.p, tot.p, st);
+            exclusive_scan(tot.p, bnew, nsort_groups + 1, scan_scr.p, st);
+            resort_place(knew.p, skeys, mscan.p, bstart, bnew, stay.p, cnt2.p + nsort_groups, Np, permb.p, sk, st);
+            skeys = sk;
+            spids = permb.p;
+            bstart = bnew;
+        } else {
+            const int where = sorter.sort(k0.p, v0.p, k1.p, v1.p, st);
+            skeys = where ? k1.p : k0.p;
+            spids = where ? v1.p : v0.p;
+            row_bounds(skeys, Np, 0, nsort_groups, row_start.p, st);
+            bstart = row_start.p;
+        }
+        resort_ok = false;  // until a gather moves the particles into this order
+    }
+
     void do_deposit(Mesh m) {
         prof.begin(0, st);
-        ParticleOps<PP>::keys(R, X.p, Np, lgNf, pa, k0.p, v0.p, st);
-        const int where = sorter.sort(k0.p, v0.p, k1.p, v1.p, st);
-        skeys = where ? k1.p : k0.p;
-        spids = where ? v1.p : v0.p;
-        row_bounds(skeys, Np, FX ? 0 : lgNf, nsort_groups, row_start.p, st);
+        if constexpr (FX) {
+            const FxKeyArgs ka = fx_key_args();
+            keys_fx(R, X.p, Np, lgNf, pa, ka.prevkeys, ka.keys, ka.pids, ka.flags, ka.in_count, st);
+            fx_sort();
+        } else {
+            ParticleOps<PP>::keys(R, X.p, Np, lgNf, pa, k0.p, v0.p, st);
+            const int where = sorter.sort(k0.p, v0.p, k1.p, v1.p, st);
+            skeys = where ? k1.p : k0.p;
+            spids = where ? v1.p : v0.p;
+            row_bounds(skeys, Np, lgNf, nsort_groups, row_start.p, st);
+            bstart = row_start.p;
+        }
         prof.end(0, st);
         prof.begin(1, st);
         if constexpr (FX) {
-            deposit_fx_brick(R, row_start.p, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p, st);
+            deposit_fx_brick(R, bstart, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p, st);
             npartials = brick_count(pa);
         } else {
             ParticleOps<PP>::deposit(R, row_start.p, skeys, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p,
@@ -367,12 +434,13 @@ struct Impl : ImplBase {
         prof.begin(7, st);
         if constexpr (FX) {
             // kick (+ drift) and move the particles into the deposit's brick order
-            gather_kick_fx_brick(R, row_start.p, spids, reinterpret_cast<const uint32_t*>(X.p), P.p, id.p,
+            gather_kick_fx_brick(R, bstart, spids, reinterpret_cast<const uint32_t*>(X.p), P.p, id.p,
                                  reinterpret_cast<uint32_t*>(X2.p), P2.p, id2.p, Np,
                                  reinterpret_cast<const float*>(mesh.p), lgNf, pa, ck, pq, cd, fuse ? 1 : 0, st);
             std::swap(X.p, X2.p);
             std::swap(P.p, P2.p);
             std::swap(id.p, id2.p);
+            resort_ok = true;
         } else {
             ParticleOps<PP>::gather_kick(R, X.p, P.p, Np, mesh.p, lgNf, pa, ck, cd, fuse ? 1 : 0, nullptr,
                                          Reorder<PP>(), st);

@@ -39,6 +39,25 @@ __global__ __launch_bounds__(256) void k_keys(const typename PP::Pos* __restrict
     pids[i] = (uint32_t)i;
 }
 
+// FX brick keys, for a full sort (prevkeys == nullptr: pids = iota) or an
+// incremental re-sort (moved flags over np + 1 entries, movers per group).
+template <int R>
+__global__ __launch_bounds__(256) void k_keys_fx(const uint32_t* __restrict__ X, long np, int lgN, PosArgs pa,
+                                                  const uint32_t* __restrict__ prevkeys, uint32_t* __restrict__ keys,
+                                                  uint32_t* __restrict__ pids, uint32_t* __restrict__ flags,
+                                                  uint32_t* __restrict__ in_count) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0 && prevkeys) flags[np] = 0u;
+    if (s >= np) return;
+    const int mask = (1 << lgN) - 1;
+    int oc[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < R; ++k) oc[k] = (int)(X[k * np + s] >> pa.fshift) & mask;
+    const uint32_t key = brick_key(oc[0], oc[1], oc[2], R, pa);
+    keys[s] = key;
+    fx_note_key(key, s, prevkeys, pids, flags, in_count);
+}
+
 // row_start[r] = first sorted position whose key lies in mesh row r or later.
 __global__ __launch_bounds__(256) void k_row_bounds(const uint32_t* __restrict__ keys, long np, int lgN,
                                                     long nrows, uint32_t* __restrict__ row_start) {
@@ -220,7 +239,7 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
     const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ X,
     const double* __restrict__ P, const uint32_t* __restrict__ idi, uint32_t* __restrict__ Xo,
     double* __restrict__ Po, uint32_t* __restrict__ ido, long np, const float* __restrict__ phi, int lgN,
-    PosArgs pa, double ck, double pq, double cd, int cd_on) {
+    PosArgs pa, double ck, double pq, double ipq, double cd, int cd_on) {
     __shared__ float tile[FXB_TILE];
     const int N = 1 << lgN, msk = N - 1;
     const BrickCoord bc = brick_coord(blockIdx.x, pa);
@@ -312,7 +331,7 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
         }
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            const double p = P[k * np + i] - fx_kick_increment((double)F[k] * ck, pq);
+            const double p = P[k * np + i] - fx_kick_increment((double)F[k] * ck, pq, ipq);
             Po[k * np + s] = p;
             Xo[k * np + s] = cd_on ? xs[k] + (uint32_t)__double2ll_rn(p * cd) : xs[k];
         }
@@ -666,6 +685,13 @@ void ParticleOps<PP>::keys(int R, const Pos* X, long np, int lgN, const PosArgs&
     CPM_LAUNCH_CHECK();
 }
 
+void keys_fx(int R, const uint32_t* X, long np, int lgN, const PosArgs& pa, const uint32_t* prevkeys, uint32_t* keys,
+             uint32_t* pids, uint32_t* flags, uint32_t* in_count, hipStream_t st) {
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_keys_fx<RR>), dim3(nblk(np, 256)), dim3(256), 0, st, X, np, lgN, pa,
+                                         prevkeys, keys, pids, flags, in_count));
+    CPM_LAUNCH_CHECK();
+}
+
 void row_bounds(const uint32_t* keys, long np, int lgN, long nrows, uint32_t* row_start, hipStream_t st) {
     hipLaunchKernelGGL(k_row_bounds, dim3(nblk(np + 1, 256)), dim3(256), 0, st, keys, np, lgN, nrows,
                        row_start);
@@ -720,7 +746,7 @@ void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, c
                           int lgN, const PosArgs& pa, double ck, double pq, double cd, int cd_on, hipStream_t st) {
     const unsigned nb = (unsigned)brick_count(pa);
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_kick_fx_brick<RR>), dim3(nb), dim3(FXG_NT), 0, st, bstart, perm,
-                                         X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck, pq, cd, cd_on));
+                                         X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck, pq, 1.0 / pq, cd, cd_on));
     CPM_LAUNCH_CHECK();
 }
 

@@ -52,8 +52,9 @@ __device__ __forceinline__ float fd4(float fm2, float fm1, float fp1, float fp2,
 // FX kick increment F * ck rounded to a multiple of the momentum quantum pq
 // (a power of two).  FX momenta are kept on that grid, so P - dp and P + dp
 // are exact and a kick can be undone bit for bit.
-__host__ __device__ __forceinline__ double fx_kick_increment(double fck, double pq) {
-    return rint(fck * (1.0 / pq)) * pq;
+// ipq = 1 / pq is passed in, exact for a power of two.
+__host__ __device__ __forceinline__ double fx_kick_increment(double fck, double pq, double ipq) {
+    return rint(fck * ipq) * pq;
 }
 
 // ------------------------------------------------------------ position policies
@@ -113,6 +114,11 @@ struct ParticleOps {
 };
 
 void row_bounds(const uint32_t* keys, long np, int lgN, long nrows, uint32_t* row_start, hipStream_t st);
+// FX brick keys in storage order: prevkeys == nullptr writes pids = iota for
+// a full radix sort; otherwise flags (np + 1 entries) and in_count for the
+// incremental re-sort against prevkeys (pm_sort.hpp).
+void keys_fx(int R, const uint32_t* X, long np, int lgN, const PosArgs& pa, const uint32_t* prevkeys, uint32_t* keys,
+             uint32_t* pids, uint32_t* flags, uint32_t* in_count, hipStream_t st);
 // FX brick order: bricks of FXB_X x FXB_Y x FXB_Z cells (2-D: FXB_X x FXB_X),
 // clamped to the mesh.  brick_layout() fills pa.brick / lb / lnb.
 #ifndef CPM_FXB_X

@@ -252,3 +252,76 @@ int RadixSorter::sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, hi
 }
 
 }  // namespace cpm
+
+// ------------------------------------------------------------ incremental re-sort
+// Keys that were sorted one step ago change for few elements (particles that
+// crossed into another brick or face class).  With prevkeys the sorted keys of
+// the last sort, in the current storage order, bprev their group starts and
+// M the exclusive scan of moved = (key != prevkey), the stayers of group g keep
+// their order at the front of the group and the movers follow them.
+namespace cpm {
+
+__global__ __launch_bounds__(256) void k_resort_counts(const uint32_t* __restrict__ bprev,
+                                                       const uint32_t* __restrict__ M,
+                                                       const uint32_t* __restrict__ in_count, long ngroups,
+                                                       uint32_t* __restrict__ stay, uint32_t* __restrict__ tot) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g > ngroups) return;
+    if (g == ngroups) {
+        tot[g] = 0;
+        return;
+    }
+    const uint32_t b0 = bprev[g], b1 = bprev[g + 1];
+    const uint32_t s = (b1 - b0) - (M[b1] - M[b0]);
+    stay[g] = s;
+    tot[g] = s + in_count[g];
+}
+
+__global__ __launch_bounds__(256) void k_resort_place(const uint32_t* __restrict__ keys,
+                                                      const uint32_t* __restrict__ prevkeys,
+                                                      const uint32_t* __restrict__ M,
+                                                      const uint32_t* __restrict__ bprev,
+                                                      const uint32_t* __restrict__ bnew,
+                                                      const uint32_t* __restrict__ stay, uint32_t* __restrict__ fill,
+                                                      long n, uint32_t* __restrict__ perm,
+                                                      uint32_t* __restrict__ skeys) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t g = keys[s];
+    uint32_t pos;
+    if (g == prevkeys[s]) {
+        const uint32_t b0 = bprev[g];
+        pos = bnew[g] + ((uint32_t)s - b0) - (M[s] - M[b0]);
+    } else {
+        pos = bnew[g] + stay[g] + atomicAdd(&fill[g], 1u);
+    }
+    perm[pos] = (uint32_t)s;
+    skeys[pos] = g;
+}
+
+__global__ __launch_bounds__(256) void k_zero_u32(uint32_t* __restrict__ p, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+void resort_counts(const uint32_t* bprev, const uint32_t* M, const uint32_t* in_count, long ngroups, uint32_t* stay,
+                   uint32_t* tot, hipStream_t st) {
+    hipLaunchKernelGGL(k_resort_counts, dim3((unsigned)((ngroups + 256) / 256)), dim3(256), 0, st, bprev, M, in_count,
+                       ngroups, stay, tot);
+    CPM_LAUNCH_CHECK();
+}
+
+void resort_place(const uint32_t* keys, const uint32_t* prevkeys, const uint32_t* M, const uint32_t* bprev,
+                  const uint32_t* bnew, const uint32_t* stay, uint32_t* fill, long n, uint32_t* perm, uint32_t* skeys,
+                  hipStream_t st) {
+    hipLaunchKernelGGL(k_resort_place, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, prevkeys, M, bprev,
+                       bnew, stay, fill, n, perm, skeys);
+    CPM_LAUNCH_CHECK();
+}
+
+void zero_u32(uint32_t* p, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+    CPM_LAUNCH_CHECK();
+}
+
+}  // namespace cpm
