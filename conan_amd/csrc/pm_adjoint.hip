@@ -193,32 +193,19 @@ __global__ __launch_bounds__(FXG_NT) void k_adj_gather_fx_brick(
     __shared__ float tile[FXB_TILE];
     __shared__ float ctile[FXB_CTILE];
     const int N = 1 << lgN, msk = N - 1;
-    const BrickCoord bc = brick_coord(blockIdx.x, pa);
+    const uint32_t bid = blockIdx.x;
+    const BrickCoord bc = brick_coord(bid, pa);
     const int TX = bc.B[0] + 5, TY = bc.B[1] + 5, TZ = (R == 3) ? bc.B[2] + 5 : 1;
     const int CX = bc.B[0] + 1, CY = bc.B[1] + 1, CZ = (R == 3) ? bc.B[2] + 1 : 1;
     const int tid = threadIdx.x;
     const bool hp = phi != nullptr, hc = chi != nullptr;
-    if (hp) {
-        for (int e = tid; e < TX * TY * TZ; e += FXG_NT) {
-            const int tx = e % TX, r = e / TX, ty = r % TY, tz = r / TY;
-            const int gx = (bc.x0[0] - 2 + tx) & msk, gy = (bc.x0[1] - 2 + ty) & msk;
-            const int gz = (R == 3) ? ((bc.x0[2] - 2 + tz) & msk) : 0;
-            tile[e] = phi[gx + (long)N * (gy + (long)N * gz)];
-        }
-    }
-    if (hc) {
-        for (int e = tid; e < CX * CY * CZ; e += FXG_NT) {
-            const int tx = e % CX, r = e / CX, ty = r % CY, tz = r / CY;
-            const int gx = (bc.x0[0] + tx) & msk, gy = (bc.x0[1] + ty) & msk;
-            const int gz = (R == 3) ? ((bc.x0[2] + tz) & msk) : 0;
-            ctile[e] = chi[gx + (long)N * (gy + (long)N * gz)];
-        }
-    }
+    if (hp) stage_box<FXG_NT, FXB_TILE, R>(tile, phi, N, bc.x0, -2, TX, TY, TZ, tid);
+    if (hc) stage_box<FXG_NT, FXB_CTILE, R>(ctile, chi, N, bc.x0, 0, CX, CY, CZ, tid);
     __syncthreads();
     const float inv12res = (float)(1.0 / (12.0 * pa.res));
     const double ires = 1.0 / pa.res;
     constexpr int NC = 1 << R;
-    const uint32_t s0 = bstart[blockIdx.x << R], s1 = bstart[(blockIdx.x + 1) << R];
+    const uint32_t s0 = bstart[bid << R], s1 = bstart[(bid + 1) << R];
     for (uint32_t s = s0 + tid; s < s1; s += FXG_NT) {
         const uint32_t i = perm[s];
         ido[s] = idi[i];

@@ -39,6 +39,46 @@ __device__ __forceinline__ BrickCoord brick_coord(uint32_t b, const PosArgs& pa)
     return c;
 }
 
+// Stages the box of cells x0[k] + lo + t, t < T[k] (wrapped), of an N^R mesh
+// into tile[tx + T0 (ty + T1 tz)], with NT threads and at most MAXE cells.
+// The tile is walked in steps of NT cells, carrying (tx, ty, tz): all
+// offsets first, then all loads in flight together, then the LDS stores (a
+// load-store pair per cell would wait out the memory latency once per cell).
+// The caller synchronises before reading the tile.
+template <int NT, int MAXE, int R>
+__device__ __forceinline__ void stage_box(float* tile, const float* __restrict__ m, int N, const int* x0, int lo,
+                                          int T0, int T1, int T2, int tid) {
+    constexpr int NL = (MAXE + NT - 1) / NT;
+    const int msk = N - 1, total = T0 * T1 * T2;
+    uint32_t off[NL];
+    int tx = tid % T0, r = tid / T0;
+    int ty = r % T1, tz = r / T1;
+    const int dtx = NT % T0, dr = NT / T0;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        const uint32_t gx = (uint32_t)((x0[0] + lo + tx) & msk), gy = (uint32_t)((x0[1] + lo + ty) & msk);
+        const uint32_t gz = (R == 3) ? (uint32_t)((x0[2] + lo + tz) & msk) : 0u;
+        off[l] = gx + (uint32_t)N * (gy + (uint32_t)N * gz);
+        tx += dtx;
+        int carry = dr;
+        if (tx >= T0) {
+            tx -= T0;
+            ++carry;
+        }
+        ty += carry;
+        while (ty >= T1) {
+            ty -= T1;
+            ++tz;
+        }
+    }
+    float v[NL];
+#pragma unroll
+    for (int l = 0; l < NL; ++l) v[l] = (tid + l * NT < total) ? m[off[l]] : 0.f;
+#pragma unroll
+    for (int l = 0; l < NL; ++l)
+        if (tid + l * NT < total) tile[tid + l * NT] = v[l];
+}
+
 // The sorted-particle ranges whose CIC clouds reach a brick: the whole brick
 // itself, and of each lower neighbour only the face classes towards it.
 struct BrickRanges {

@@ -202,10 +202,9 @@ struct ColCfg {
 };
 
 template <typename T, int N>
-__device__ __forceinline__ void col_load(cx<T>* sh, const cx<T>* __restrict__ data, const ColGeom& g,
+__device__ __forceinline__ void col_load(cx<T>* sh, const cx<T>* __restrict__ data, const ColGeom& g, long bbase,
                                          long c0, int tid) {
     using C = ColCfg<T, N>;
-    const long bbase = (long)blockIdx.y * g.bs;
     for (int e = tid; e < C::C * N; e += C::NT) {
         const int c = e % C::C, n = e / C::C;
         const long col = c0 + c;
@@ -214,10 +213,9 @@ __device__ __forceinline__ void col_load(cx<T>* sh, const cx<T>* __restrict__ da
 }
 
 template <typename T, int N>
-__device__ __forceinline__ void col_store(const cx<T>* sh, cx<T>* __restrict__ data, const ColGeom& g,
+__device__ __forceinline__ void col_store(const cx<T>* sh, cx<T>* __restrict__ data, const ColGeom& g, long bbase,
                                           long c0, int tid) {
     using C = ColCfg<T, N>;
-    const long bbase = (long)blockIdx.y * g.bs;
     for (int e = tid; e < C::C * N; e += C::NT) {
         const int c = e % C::C, n = e / C::C;
         const long col = c0 + c;
@@ -225,19 +223,38 @@ __device__ __forceinline__ void col_store(const cx<T>* sh, cx<T>* __restrict__ d
     }
 }
 
+// The tile of workgroup blockIdx.x of a launch over two parts (the main
+// block of the half spectrum and the kx = N/2 plane): part, batch base and
+// first pencil.
+template <typename T, int N>
+struct ColTile {
+    const ColPart<T>* part;
+    long bbase, c0;
+};
+template <typename T, int N>
+__device__ __forceinline__ ColTile<T, N> col_tile(const ColPart<T>& pa, const ColPart<T>& pb) {
+    uint32_t w = blockIdx.x;
+    const bool second = w >= (uint32_t)pa.nblk;
+    const ColPart<T>& p = second ? pb : pa;
+    if (second) w -= (uint32_t)pa.nblk;
+    const uint32_t gx = (uint32_t)p.gx;
+    return ColTile<T, N>{&p, (long)(w / gx) * p.g.bs, (long)(w % gx) * ColCfg<T, N>::C};
+}
+
 template <typename T, int N, int DIR>
-__global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols(cx<T>* __restrict__ data, ColGeom g,
+__global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols(ColPart<T> pa, ColPart<T> pb,
                                                            const cx<T>* __restrict__ tw) {
     using C = ColCfg<T, N>;
     __shared__ cx<T> sh[C::C * N];
     __shared__ cx<T> stw[N];
     const int tid = threadIdx.x;
-    const long c0 = (long)blockIdx.x * C::C;
+    const ColTile<T, N> t = col_tile<T, N>(pa, pb);
+    const ColPart<T>& p = *t.part;
     for (int m = tid; m < N; m += C::NT) stw[m] = tw[m];
-    col_load<T, N>(sh, data, g, c0, tid);
+    col_load<T, N>(sh, p.data, p.g, t.bbase, t.c0, tid);
     __syncthreads();
     lds_fft<T, N, C::E, C::NT, DIR>(sh, stw, 1, tid);
-    col_store<T, N>(sh, data, g, c0, tid);
+    col_store<T, N>(sh, p.data, p.g, t.bbase, t.c0, tid);
 }
 
 // Signed mode number of index n on an N mesh: n > N/2 maps to n - N, and
@@ -264,23 +281,24 @@ __device__ __forceinline__ T green_value(int ix, int iy, int iz, int dim, int N,
 // Last-axis forward transform, multiplication by the Green's function and
 // last-axis backward transform, in one read and one write of the spectrum.
 template <typename T, int N>
-__global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols_green(cx<T>* __restrict__ data, ColGeom g,
-                                                                 GreenGeom gg,
+__global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols_green(ColPart<T> pa, ColPart<T> pb,
                                                                  const cx<T>* __restrict__ tw) {
     using C = ColCfg<T, N>;
     __shared__ cx<T> sh[C::C * N];
     __shared__ cx<T> stw[N];
     const int tid = threadIdx.x;
-    const long c0 = (long)blockIdx.x * C::C;
+    const ColTile<T, N> t = col_tile<T, N>(pa, pb);
+    const ColPart<T>& p = *t.part;
+    const GreenGeom gg = p.gg;
     for (int m = tid; m < N; m += C::NT) stw[m] = tw[m];
-    col_load<T, N>(sh, data, g, c0, tid);
+    col_load<T, N>(sh, p.data, p.g, t.bbase, t.c0, tid);
     __syncthreads();
     lds_fft<T, N, C::E, C::NT, -1>(sh, stw, 1, tid);
     const int H = N / 2;
     const T kf = (T)gg.kf;
     for (int e = tid; e < C::C * N; e += C::NT) {
         const int c = e / N, n = e % N;
-        const long col = c0 + c;
+        const long col = t.c0 + c;
         int ix, iy, iz = 0;
         if (gg.dim == 3) {
             ix = gg.nyq ? H : (int)(col & (H - 1));
@@ -296,8 +314,10 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols_green(cx<T>* __rest
     }
     __syncthreads();
     lds_fft<T, N, C::E, C::NT, +1>(sh, stw, 1, tid);
-    col_store<T, N>(sh, data, g, c0, tid);
+    col_store<T, N>(sh, p.data, p.g, t.bbase, t.c0, tid);
 }
+
+
 
 // ------------------------------------------------------------ dispatch
 #define CPM_FOR_N(MACRO) \
@@ -344,17 +364,17 @@ void launch_rows_c2r(int N, const cx<T>* spec, const cx<T>* nyq, T* out, long nr
 }
 
 template <typename T>
-void launch_cols(int N, int dir, cx<T>* data, const ColGeom& g, const cx<T>* tw, hipStream_t st) {
+void launch_cols(int N, int dir, const ColPart<T>& a, const ColPart<T>& b, const cx<T>* tw, hipStream_t st) {
+    const dim3 grid((unsigned)(a.nblk + b.nblk));
     switch (N) {
-#define CASE(n)                                                                              \
-    case n: {                                                                                \
-        using C = ColCfg<T, n>;                                                              \
-        dim3 grid((unsigned)((g.nc + C::C - 1) / C::C), (unsigned)g.nb);                     \
-        if (dir < 0)                                                                         \
-            hipLaunchKernelGGL((k_cols<T, n, -1>), grid, dim3(C::NT), 0, st, data, g, tw);   \
-        else                                                                                 \
-            hipLaunchKernelGGL((k_cols<T, n, +1>), grid, dim3(C::NT), 0, st, data, g, tw);   \
-        break;                                                                               \
+#define CASE(n)                                                                             \
+    case n: {                                                                               \
+        using C = ColCfg<T, n>;                                                             \
+        if (dir < 0)                                                                        \
+            hipLaunchKernelGGL((k_cols<T, n, -1>), grid, dim3(C::NT), 0, st, a, b, tw);     \
+        else                                                                                \
+            hipLaunchKernelGGL((k_cols<T, n, +1>), grid, dim3(C::NT), 0, st, a, b, tw);     \
+        break;                                                                              \
     }
         CPM_FOR_N(CASE)
 #undef CASE
@@ -365,15 +385,14 @@ void launch_cols(int N, int dir, cx<T>* data, const ColGeom& g, const cx<T>* tw,
 }
 
 template <typename T>
-void launch_cols_green(int N, cx<T>* data, const ColGeom& g, const GreenGeom& gg, const cx<T>* tw,
-                       hipStream_t st) {
+void launch_cols_green(int N, const ColPart<T>& a, const ColPart<T>& b, const cx<T>* tw, hipStream_t st) {
+    const dim3 grid((unsigned)(a.nblk + b.nblk));
     switch (N) {
-#define CASE(n)                                                                                  \
-    case n: {                                                                                    \
-        using C = ColCfg<T, n>;                                                                  \
-        dim3 grid((unsigned)((g.nc + C::C - 1) / C::C), (unsigned)g.nb);                         \
-        hipLaunchKernelGGL((k_cols_green<T, n>), grid, dim3(C::NT), 0, st, data, g, gg, tw);     \
-        break;                                                                                   \
+#define CASE(n)                                                                                \
+    case n: {                                                                                  \
+        using C = ColCfg<T, n>;                                                                \
+        hipLaunchKernelGGL((k_cols_green<T, n>), grid, dim3(C::NT), 0, st, a, b, tw);          \
+        break;                                                                                 \
     }
         CPM_FOR_N(CASE)
 #undef CASE
@@ -381,6 +400,25 @@ void launch_cols_green(int N, cx<T>* data, const ColGeom& g, const GreenGeom& gg
             throw Error(-2, "unsupported FFT length " + std::to_string(N));
     }
     CPM_LAUNCH_CHECK();
+}
+
+// One part of a column launch: gx workgroups per batch, gx * nb in all.
+template <typename T, int N>
+ColPart<T> col_part_n(cx<T>* d, const ColGeom& g, const GreenGeom& gg) {
+    const long gx = (g.nc + ColCfg<T, N>::C - 1) / ColCfg<T, N>::C;
+    return ColPart<T>{d, g, gg, gx, gx * g.nb};
+}
+template <typename T>
+ColPart<T> col_part(int N, cx<T>* d, const ColGeom& g, const GreenGeom& gg = GreenGeom{}) {
+    switch (N) {
+#define CASE(n) \
+    case n:     \
+        return col_part_n<T, n>(d, g, gg);
+        CPM_FOR_N(CASE)
+#undef CASE
+        default:
+            throw Error(-2, "unsupported FFT length " + std::to_string(N));
+    }
 }
 
 // ------------------------------------------------------------ the plan
@@ -432,24 +470,16 @@ void FFTPlan<T>::forward(const T* in, cx<T>* spec, cx<T>* nyq, T shift, const do
     launch_rows_r2c<T>(N, in, spec, nyq, nrows, shift, mean_ptr, tw, st);
     ColGeom ym, yn, zm, zn;
     geoms(&ym, &yn, &zm, &zn);
-    launch_cols<T>(N, -1, spec, ym, tw, st);
-    launch_cols<T>(N, -1, nyq, yn, tw, st);
-    if (dim == 3) {
-        launch_cols<T>(N, -1, spec, zm, tw, st);
-        launch_cols<T>(N, -1, nyq, zn, tw, st);
-    }
+    launch_cols<T>(N, -1, col_part(N, spec, ym), col_part(N, nyq, yn), tw, st);
+    if (dim == 3) launch_cols<T>(N, -1, col_part(N, spec, zm), col_part(N, nyq, zn), tw, st);
 }
 
 template <typename T>
 void FFTPlan<T>::backward(cx<T>* spec, cx<T>* nyq, T* out, double s, hipStream_t st) const {
     ColGeom ym, yn, zm, zn;
     geoms(&ym, &yn, &zm, &zn);
-    if (dim == 3) {
-        launch_cols<T>(N, +1, spec, zm, tw, st);
-        launch_cols<T>(N, +1, nyq, zn, tw, st);
-    }
-    launch_cols<T>(N, +1, spec, ym, tw, st);
-    launch_cols<T>(N, +1, nyq, yn, tw, st);
+    if (dim == 3) launch_cols<T>(N, +1, col_part(N, spec, zm), col_part(N, nyq, zn), tw, st);
+    launch_cols<T>(N, +1, col_part(N, spec, ym), col_part(N, nyq, yn), tw, st);
     launch_rows_c2r<T>(N, spec, nyq, out, nrows, s, tw, st);
 }
 
@@ -464,20 +494,15 @@ void FFTPlan<T>::poisson_stage(int stage, int part, const T* rho, T* phi, cx<T>*
             if (part == 0) launch_rows_r2c<T>(N, rho, spec, nyq, nrows, shift, mean_ptr, tw, st);
             break;
         case 1:
-            if (dim == 3) {
-                if (part == 0) launch_cols<T>(N, -1, spec, ym, tw, st);
-                else launch_cols<T>(N, -1, nyq, yn, tw, st);
-            }
+            if (dim == 3 && part == 0) launch_cols<T>(N, -1, col_part(N, spec, ym), col_part(N, nyq, yn), tw, st);
             break;
         case 2:
-            if (part == 0) launch_cols_green<T>(N, spec, dim == 3 ? zm : ym, gmain, tw, st);
-            else launch_cols_green<T>(N, nyq, dim == 3 ? zn : yn, gnyq, tw, st);
+            if (part == 0)
+                launch_cols_green<T>(N, col_part(N, spec, dim == 3 ? zm : ym, gmain),
+                                     col_part(N, nyq, dim == 3 ? zn : yn, gnyq), tw, st);
             break;
         case 3:
-            if (dim == 3) {
-                if (part == 0) launch_cols<T>(N, +1, spec, ym, tw, st);
-                else launch_cols<T>(N, +1, nyq, yn, tw, st);
-            }
+            if (dim == 3 && part == 0) launch_cols<T>(N, +1, col_part(N, spec, ym), col_part(N, nyq, yn), tw, st);
             break;
         case 4:
             if (part == 0) launch_rows_c2r<T>(N, spec, nyq, phi, nrows, s, tw, st);

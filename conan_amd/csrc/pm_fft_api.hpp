@@ -21,6 +21,19 @@ struct GreenGeom {
     double kf;  // 2 pi / L
 };
 
+// One part of a column-pass launch (pm_fft.hip): the pencils of geometry g in
+// data, with the Green's-function geometry gg for the Poisson pass; gx
+// workgroups per batch, nblk in all.  A launch covers two parts, the main
+// block of the half spectrum and the kx = N/2 plane.
+template <typename T>
+struct ColPart {
+    cx<T>* data;
+    ColGeom g;
+    GreenGeom gg;
+    long gx;
+    long nblk;
+};
+
 template <typename T>
 struct FFTPlan {
     int dim, N, H;
@@ -45,7 +58,7 @@ struct FFTPlan {
     // phi = IFFT[ FFT(rho - shift) * (-1/|k|^2) ] / s, in 2*dim - 1 sweeps.
     // One sweep of poisson(): 0 rows r2c, 1 y forward (3-D), 2 last axis with
     // the Green's function, 3 y backward (3-D), 4 rows c2r; part 0 launches the
-    // main block (and the rows), part 1 the kx = N/2 plane.
+    // sweep (main block and kx = N/2 plane together), part 1 launches nothing.
     void poisson_stage(int stage, int part, const T* rho, T* phi, cx<T>* spec, cx<T>* nyq, T shift,
                        const double* mean_ptr, double kf, double s, hipStream_t st) const;
     void poisson(const T* rho, T* phi, cx<T>* spec, cx<T>* nyq, T shift, const double* mean_ptr,

@@ -408,11 +408,6 @@ struct Impl : ImplBase {
             prof.begin(2 + stage, st);
             fft.poisson_stage(stage, 0, mesh.p, mesh.p, spec.p, nyq.p, shift, mean_ptr, kf, s, st);
             prof.end(2 + stage, st);
-            if (stage >= 1 && stage <= 3) {
-                prof.begin(9, st);
-                fft.poisson_stage(stage, 1, mesh.p, mesh.p, spec.p, nyq.p, shift, mean_ptr, kf, s, st);
-                prof.end(9, st);
-            }
         }
     }
 

@@ -178,7 +178,8 @@ __global__ __launch_bounds__(FXB_NT) void k_deposit_fx_brick(const uint32_t* __r
     __shared__ unsigned long long acc[FXB_CELLS];
     __shared__ double red[FXB_NT];
     const int N = 1 << lgN, msk = N - 1;
-    const BrickCoord bc = brick_coord(blockIdx.x, pa);
+    const uint32_t bid = blockIdx.x;
+    const BrickCoord bc = brick_coord(bid, pa);
     const int ncell = bc.B[0] * bc.B[1] * bc.B[2];
     const int tid = threadIdx.x;
     for (int e = tid; e < ncell; e += FXB_NT) acc[e] = 0ull;
@@ -186,28 +187,44 @@ __global__ __launch_bounds__(FXB_NT) void k_deposit_fx_brick(const uint32_t* __r
     brick_ranges<R>(rg, bstart, bc, pa, tid);
     {
         const uint32_t total = rg.roff[rg.n];
-        for (uint32_t t = tid; t < total; t += FXB_NT) {
-            const uint32_t s = brick_range_pos(rg, t);
-            const uint32_t i = perm[s];
-            int o[3] = {0, 0, 0};
-            float f[3] = {0.f, 0.f, 0.f};
+        // DEP_BATCH particles per thread at a time: their perm and position
+        // loads are all issued before the first is used.
+        constexpr int DEP_BATCH = 4;
+        for (uint32_t t0 = tid; t0 < total; t0 += DEP_BATCH * FXB_NT) {
+            uint32_t pi[DEP_BATCH];
 #pragma unroll
-            for (int k = 0; k < R; ++k) PosFX::dep(X[k * np + i], pa, o[k], f[k]);
+            for (int q = 0; q < DEP_BATCH; ++q) {
+                const uint32_t t = t0 + q * FXB_NT;
+                pi[q] = t < total ? perm[brick_range_pos(rg, t)] : 0u;
+            }
+            uint32_t xq[DEP_BATCH][3];
 #pragma unroll
-            for (int j = 0; j < (1 << R); ++j) {
-                int l[3];
-                bool in = true;
+            for (int q = 0; q < DEP_BATCH; ++q)
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    l[k] = (k < R) ? (((o[k] + ((j >> k) & 1)) - bc.x0[k]) & msk) : 0;
-                    in &= l[k] < bc.B[k];
+                for (int k = 0; k < 3; ++k) xq[q][k] = (k < R && t0 + q * FXB_NT < total) ? X[k * np + pi[q]] : 0u;
+#pragma unroll
+            for (int q = 0; q < DEP_BATCH; ++q) {
+                if (t0 + q * FXB_NT >= total) break;
+                int o[3] = {0, 0, 0};
+                float f[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                for (int k = 0; k < R; ++k) PosFX::dep(xq[q][k], pa, o[k], f[k]);
+#pragma unroll
+                for (int j = 0; j < (1 << R); ++j) {
+                    int l[3];
+                    bool in = true;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        l[k] = (k < R) ? (((o[k] + ((j >> k) & 1)) - bc.x0[k]) & msk) : 0;
+                        in &= l[k] < bc.B[k];
+                    }
+                    if (!in) continue;
+                    float w = mass;
+#pragma unroll
+                    for (int k = 0; k < R; ++k) w *= ((j >> k) & 1) ? f[k] : 1.f - f[k];
+                    const long long qv = __double2ll_rn((double)w * FX_ONE);
+                    atomicAdd(&acc[l[0] + bc.B[0] * (l[1] + bc.B[1] * l[2])], (unsigned long long)qv);
                 }
-                if (!in) continue;
-                float w = mass;
-#pragma unroll
-                for (int k = 0; k < R; ++k) w *= ((j >> k) & 1) ? f[k] : 1.f - f[k];
-                const long long q = __double2ll_rn((double)w * FX_ONE);
-                atomicAdd(&acc[l[0] + bc.B[0] * (l[1] + bc.B[1] * l[2])], (unsigned long long)q);
             }
         }
     }
@@ -226,7 +243,7 @@ __global__ __launch_bounds__(FXB_NT) void k_deposit_fx_brick(const uint32_t* __r
         if (tid < off) red[tid] += red[tid + off];
         __syncthreads();
     }
-    if (tid == 0) partial[blockIdx.x] = red[0];
+    if (tid == 0) partial[bid] = red[0];
 }
 
 // Gather + kick (+ drift), one workgroup per brick: the potential of the
@@ -242,34 +259,14 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
     PosArgs pa, double ck, double pq, double ipq, double cd, int cd_on) {
     __shared__ float tile[FXB_TILE];
     const int N = 1 << lgN, msk = N - 1;
-    const BrickCoord bc = brick_coord(blockIdx.x, pa);
+    const uint32_t bid = blockIdx.x;
+    const BrickCoord bc = brick_coord(bid, pa);
     const int TX = bc.B[0] + 5, TY = bc.B[1] + 5, TZ = (R == 3) ? bc.B[2] + 5 : 1;
     const int tid = threadIdx.x;
-    {
-        // walk the tile in steps of FXG_NT elements, carrying (tx, ty, tz)
-        int tx = tid % TX, r = tid / TX;
-        int ty = r % TY, tz = r / TY;
-        const int dtx = FXG_NT % TX, dr = FXG_NT / TX;
-        for (int e = tid; e < TX * TY * TZ; e += FXG_NT) {
-            const int gx = (bc.x0[0] - 2 + tx) & msk, gy = (bc.x0[1] - 2 + ty) & msk;
-            const int gz = (R == 3) ? ((bc.x0[2] - 2 + tz) & msk) : 0;
-            tile[e] = phi[gx + (long)N * (gy + (long)N * gz)];
-            tx += dtx;
-            int carry = dr;
-            if (tx >= TX) {
-                tx -= TX;
-                ++carry;
-            }
-            ty += carry;
-            while (ty >= TY) {
-                ty -= TY;
-                ++tz;
-            }
-        }
-    }
+    stage_box<FXG_NT, FXB_TILE, R>(tile, phi, N, bc.x0, -2, TX, TY, TZ, tid);
     __syncthreads();
     const float inv12res = (float)(1.0 / (12.0 * pa.res));
-    const uint32_t s0 = bstart[blockIdx.x << R], s1 = bstart[(blockIdx.x + 1) << R];
+    const uint32_t s0 = bstart[bid << R], s1 = bstart[(bid + 1) << R];
     for (uint32_t s = s0 + tid; s < s1; s += FXG_NT) {
         const uint32_t i = perm[s];
         ido[s] = idi[i];

@@ -134,8 +134,8 @@ int pm_diagnostics(pm_handle* h, pm_diag* d);
 /* Per-phase device time (HIP events on the handle's stream), accumulated
  * while enabled.  Phases: 0 keys+sort, 1 deposit, 2 FFT rows forward,
  * 3 FFT y forward, 4 Green pass, 5 FFT y backward, 6 FFT rows backward,
- * 7 gather+kick(+drift), 8 drift, 9 the kx = N/2 plane's column passes.
- * Phases 1-7 are one kernel launch each. */
+ * 7 gather+kick(+drift), 8 drift, 9 reserved (the kx = N/2 plane's column
+ * passes now run inside phases 3-5).  Phases 1-7 are one kernel launch each. */
 #define PM_NPHASES 10
 int pm_profile(pm_handle* h, int enable, double* ms, long long* counts);
 
