@@ -97,15 +97,20 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_r2c(
     const long row0 = (long)blockIdx.x * P;
     const T sft = mean_ptr ? (T)(*mean_ptr) : shift;
     for (int m = tid; m < N; m += NT) stw[m] = tw[m];
-    for (int e = tid; e < P * H; e += NT) {
-        const int p = e / H, n = e % H;
-        const long row = row0 + p;
-        cx<T> z = mk<T>(0, 0);
-        if (row < nrows) {
-            cx<T> pr = *reinterpret_cast<const cx<T>*>(in + row * N + 2 * n);
-            z = mk<T>(pr.re - sft, pr.im - sft);
+    {
+        // all loads of the thread first, then the LDS stores
+        cx<T> z[C::E];
+#pragma unroll
+        for (int i = 0; i < C::E; ++i) {
+            const int e = tid + i * NT;
+            const long row = row0 + e / H;
+            z[i] = row < nrows ? *reinterpret_cast<const cx<T>*>(in + row * N + 2 * (e % H)) : mk<T>(sft, sft);
         }
-        sh[lidx<H>(p, n)] = z;
+#pragma unroll
+        for (int i = 0; i < C::E; ++i) {
+            const int e = tid + i * NT;
+            sh[lidx<H>(e / H, e % H)] = mk<T>(z[i].re - sft, z[i].im - sft);
+        }
     }
     __syncthreads();
     lds_fft<T, H, C::E, NT, -1>(sh, stw, 2, tid);
@@ -135,12 +140,22 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_c2r(
     const int tid = threadIdx.x;
     const long row0 = (long)blockIdx.x * P;
     for (int m = tid; m < N; m += NT) stw[m] = tw[m];
-    for (int e = tid; e < P * H; e += NT) {
-        const int p = e / H, n = e % H;
-        const long row = row0 + p;
-        sh[lidx<H>(p, n)] = row < nrows ? spec[row * H + n] : mk<T>(0, 0);
+    {
+        // all loads of the thread first, then the LDS stores
+        cx<T> z[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int e = tid + i * NT;
+            const long row = row0 + e / H;
+            z[i] = row < nrows ? spec[row * H + e % H] : mk<T>(0, 0);
+        }
+        for (int p = tid; p < P; p += NT) shn[p] = (row0 + p < nrows) ? nyq[row0 + p] : mk<T>(0, 0);
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int e = tid + i * NT;
+            sh[lidx<H>(e / H, e % H)] = z[i];
+        }
     }
-    for (int p = tid; p < P; p += NT) shn[p] = (row0 + p < nrows) ? nyq[row0 + p] : mk<T>(0, 0);
     __syncthreads();
     cx<T> z[E];
 #pragma unroll
@@ -201,14 +216,24 @@ struct ColCfg {
     static constexpr int E = C * N / NT;
 };
 
+// Global <-> LDS copies of a column tile: each thread issues all its E loads
+// before the first LDS store, so their latencies overlap.
 template <typename T, int N>
 __device__ __forceinline__ void col_load(cx<T>* sh, const cx<T>* __restrict__ data, const ColGeom& g, long bbase,
                                          long c0, int tid) {
     using C = ColCfg<T, N>;
-    for (int e = tid; e < C::C * N; e += C::NT) {
+    cx<T> v[C::E];
+#pragma unroll
+    for (int i = 0; i < C::E; ++i) {
+        const int e = tid + i * C::NT;
         const int c = e % C::C, n = e / C::C;
         const long col = c0 + c;
-        sh[lidx<N>(c, n)] = col < g.nc ? data[bbase + col * g.cs + (long)n * g.es] : mk<T>(0, 0);
+        v[i] = col < g.nc ? data[bbase + col * g.cs + (long)n * g.es] : mk<T>(0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < C::E; ++i) {
+        const int e = tid + i * C::NT;
+        sh[lidx<N>(e % C::C, e / C::C)] = v[i];
     }
 }
 
