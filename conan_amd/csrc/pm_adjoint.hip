@@ -31,11 +31,10 @@ __global__ __launch_bounds__(256) void k_rev_drift_keys(uint32_t* __restrict__ Y
                                                          long np, double cd_fx, double cd, int lgN, PosArgs pa,
                                                          const uint32_t* __restrict__ prevkeys,
                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ pids,
-                                                         uint32_t* __restrict__ flags, uint32_t* __restrict__ in_count,
+                                                         uint32_t* __restrict__ in_count, uint32_t* __restrict__ out_count,
                                                          uint32_t* __restrict__ bmax) {
     __shared__ float wmax[4];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0 && prevkeys) flags[np] = 0u;
     float mx = 0.f;
     if (i < np) {
         const int mask = (1 << lgN) - 1;
@@ -54,7 +53,7 @@ __global__ __launch_bounds__(256) void k_rev_drift_keys(uint32_t* __restrict__ Y
         }
         const uint32_t key = brick_key(oc[0], oc[1], oc[2], R, pa);
         keys[i] = key;
-        fx_note_key(key, i, prevkeys, pids, flags, in_count);
+        fx_note_key(key, i, prevkeys, pids, in_count, out_count);
     }
     // block max -> bmax[block] (no atomics: one contended address would
     // serialise millions of them)
@@ -440,10 +439,10 @@ __global__ __launch_bounds__(256) void k_state_diff(const uint32_t* __restrict__
 // ------------------------------------------------------------ launchers
 void rev_drift_keys(int R, uint32_t* Y, const double* P, const double* lY, double* lP, long np, double cd_fx,
                     double cd, int lgN, const PosArgs& pa, const uint32_t* prevkeys, uint32_t* keys, uint32_t* pids,
-                    uint32_t* flags, uint32_t* in_count, uint32_t* bmax, uint32_t* lmax, hipStream_t st) {
+                    uint32_t* in_count, uint32_t* out_count, uint32_t* bmax, uint32_t* lmax, hipStream_t st) {
     const unsigned nb = nblk(np, 256);
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_rev_drift_keys<RR>), dim3(nb), dim3(256), 0, st, Y, P, lY, lP, np, cd_fx,
-                                         cd, lgN, pa, prevkeys, keys, pids, flags, in_count, bmax));
+                                         cd, lgN, pa, prevkeys, keys, pids, in_count, out_count, bmax));
     CPM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_max_u32, dim3(1), dim3(1024), 0, st, (const uint32_t*)bmax, (long)nb, lmax);
     CPM_LAUNCH_CHECK();

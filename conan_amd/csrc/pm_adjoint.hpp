@@ -14,7 +14,7 @@ namespace cpm {
 // scratch of one word per 256 particles).  cd_fx == 0 leaves Y as it is.
 void rev_drift_keys(int R, uint32_t* Y, const double* P, const double* lY, double* lP, long np, double cd_fx,
                     double cd, int lgN, const PosArgs& pa, const uint32_t* prevkeys, uint32_t* keys, uint32_t* pids,
-                    uint32_t* flags, uint32_t* in_count, uint32_t* bmax, uint32_t* lmax, hipStream_t st);
+                    uint32_t* in_count, uint32_t* out_count, uint32_t* bmax, uint32_t* lmax, hipStream_t st);
 
 // s_k(c) = sum_p lP_k(p) * W(Y_p, c), k < R: the CIC deposit of the R
 // components of lP onto R meshes of M cells (s + k * M), accumulated in

@@ -136,15 +136,17 @@ __device__ __forceinline__ void brick_ranges(BrickRanges& r, const uint32_t* __r
 
 // Bookkeeping of a new FX sort key at storage position s.  Full sort
 // (prevkeys == nullptr): pids[s] = s for the radix sort.  Incremental re-sort:
-// flags[s] = the key differs from the key sorted at s last time, and movers
-// are counted per new group.
+// a particle whose key differs from the key sorted at s last time (a mover)
+// is counted into its new group (in_count) and out of its old one (out_count).
 __device__ __forceinline__ void fx_note_key(uint32_t key, long s, const uint32_t* __restrict__ prevkeys,
-                                            uint32_t* __restrict__ pids, uint32_t* __restrict__ flags,
-                                            uint32_t* __restrict__ in_count) {
+                                            uint32_t* __restrict__ pids, uint32_t* __restrict__ in_count,
+                                            uint32_t* __restrict__ out_count) {
     if (prevkeys) {
-        const uint32_t mv = key != prevkeys[s] ? 1u : 0u;
-        flags[s] = mv;
-        if (mv) atomicAdd(&in_count[key], 1u);
+        const uint32_t pk = prevkeys[s];
+        if (key != pk) {
+            atomicAdd(&in_count[key], 1u);
+            atomicAdd(&out_count[pk], 1u);
+        }
     } else {
         pids[s] = (uint32_t)s;
     }

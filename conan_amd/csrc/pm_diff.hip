@@ -209,7 +209,7 @@ struct DiffPM {
     // of Impl::do_deposit)
     void sort_particles() {
         const Impl<PosFX>::FxKeyArgs ka = sim.fx_key_args();
-        keys_fx(R, sim.X.p, Np, sim.lgNf, sim.pa, ka.prevkeys, ka.keys, ka.pids, ka.flags, ka.in_count, st);
+        keys_fx(R, sim.X.p, Np, sim.lgNf, sim.pa, ka.prevkeys, ka.keys, ka.pids, ka.in_count, ka.out_count, st);
         sim.fx_sort();
     }
 
@@ -294,7 +294,7 @@ struct DiffPM {
             const double ck = p.da / sim.cos.adot(ak);  // src/run/nbody.hh:112-116
             const Impl<PosFX>::FxKeyArgs ka = sim.fx_key_args();
             rev_drift_keys(R, sim.X.p, sim.P.p, lY.p, lP.p, Np, cd_fx, cd, sim.lgNf, sim.pa, ka.prevkeys, ka.keys,
-                           ka.pids, ka.flags, ka.in_count, bmax.p, lmax.p, st);
+                           ka.pids, ka.in_count, ka.out_count, bmax.p, lmax.p, st);
             sim.fx_sort();
             deposit_fx_brick<float>(R, sim.bstart, sim.spids, sim.X.p, Np, sim.lgNf, sim.pa, (float)m, sim.mesh.p,
                              sim.row_partial.p, st);

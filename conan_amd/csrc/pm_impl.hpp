@@ -164,10 +164,10 @@ struct Impl : ImplBase {
     // mass-grid index of the particle stored at s (X2/P2/id2: the next order).
     DevBuf<uint32_t> id, id2;
     DevBuf<uint32_t> k0, v0, k1, v1, row_start;
-    // FX incremental re-sort (see fx_sort): new keys, moved flags and their
-    // scan, sorted keys / perm outputs, the other group-start buffer,
-    // per-group counters (movers in | fill), stayers, group totals.
-    DevBuf<uint32_t> knew, flags, mscan, skb0, skb1, permb, bs2, cnt2, stay, tot, scan_scr;
+    // FX incremental re-sort (see fx_sort): new keys, sorted keys / perm
+    // outputs, the other group-start buffer, per-group counters (movers in |
+    // movers out | fill), stayers, group totals.
+    DevBuf<uint32_t> knew, skb0, skb1, permb, bs2, cnt3, stay, tot, scan_scr;
     const uint32_t* bstart = nullptr;  // group starts of the last sort (row_start or bs2)
     bool resort_ok = false;            // storage order == output order of the last FX sort
     DevBuf<Mesh> mesh;
@@ -224,17 +224,14 @@ struct Impl : ImplBase {
         row_start.alloc((nsort_groups > nrows ? nsort_groups : nrows) + 1);
         if (FX) {
             knew.alloc(Np);
-            flags.alloc(Np + 1);
-            mscan.alloc(Np + 1);
             skb0.alloc(Np);
             skb1.alloc(Np);
             permb.alloc(Np);
             bs2.alloc(nsort_groups + 1);
-            cnt2.alloc(2 * nsort_groups);
+            cnt3.alloc(3 * nsort_groups);
             stay.alloc(nsort_groups);
             tot.alloc(nsort_groups + 1);
-            const size_t a = scan_scratch_elems(Np + 1), b = scan_scratch_elems(nsort_groups + 1);
-            scan_scr.alloc(a > b ? a : b);
+            scan_scr.alloc(scan_scratch_elems(nsort_groups + 1));
         }
         mesh.alloc(M);
         spec.alloc(fft.spec_elems);
@@ -333,12 +330,12 @@ struct Impl : ImplBase {
     // stable radix sort of (key, index) runs.
     struct FxKeyArgs {
         const uint32_t* prevkeys;
-        uint32_t *keys, *pids, *flags, *in_count;
+        uint32_t *keys, *pids, *in_count, *out_count;
     };
     FxKeyArgs fx_key_args() {
         if (resort_ok) {
-            zero_u32(cnt2.p, 2 * nsort_groups, st);
-            return FxKeyArgs{skeys, knew.p, nullptr, flags.p, cnt2.p};
+            zero_u32(cnt3.p, 3 * nsort_groups, st);
+            return FxKeyArgs{skeys, knew.p, nullptr, cnt3.p, cnt3.p + nsort_groups};
         }
         return FxKeyArgs{nullptr, k0.p, v0.p, nullptr, nullptr};
     }
@@ -346,10 +343,10 @@ struct Impl : ImplBase {
         if (resort_ok) {
             uint32_t* bnew = (bstart == row_start.p) ? bs2.p : row_start.p;
             uint32_t* sk = (skeys == skb0.p) ? skb1.p : skb0.p;
-            exclusive_scan(flags.p, mscan.p, Np + 1, scan_scr.p, st);
-            resort_counts(bstart, mscan.p, cnt2.p, nsort_groups, stay.p, tot.p, st);
+            resort_counts(bstart, cnt3.p, cnt3.p + nsort_groups, nsort_groups, stay.p, tot.p, st);
             exclusive_scan(tot.p, bnew, nsort_groups + 1, scan_scr.p, st);
-            resort_place(knew.p, skeys, mscan.p, bstart, bnew, stay.p, cnt2.p + nsort_groups, Np, permb.p, sk, st);
+            resort_place(knew.p, skeys, bstart, bnew, stay.p, cnt3.p + 2 * nsort_groups, brick_count(pa), 1 << R,
+                         permb.p, sk, st);
             skeys = sk;
             spids = permb.p;
             bstart = bnew;
@@ -367,7 +364,7 @@ struct Impl : ImplBase {
         prof.begin(0, st);
         if constexpr (FX) {
             const FxKeyArgs ka = fx_key_args();
-            keys_fx(R, X.p, Np, lgNf, pa, ka.prevkeys, ka.keys, ka.pids, ka.flags, ka.in_count, st);
+            keys_fx(R, X.p, Np, lgNf, pa, ka.prevkeys, ka.keys, ka.pids, ka.in_count, ka.out_count, st);
             fx_sort();
         } else {
             ParticleOps<PP>::keys(R, X.p, Np, lgNf, pa, k0.p, v0.p, st);

@@ -40,14 +40,13 @@ __global__ __launch_bounds__(256) void k_keys(const typename PP::Pos* __restrict
 }
 
 // FX brick keys, for a full sort (prevkeys == nullptr: pids = iota) or an
-// incremental re-sort (moved flags over np + 1 entries, movers per group).
+// incremental re-sort (movers counted into their new group and out of their old one).
 template <int R>
 __global__ __launch_bounds__(256) void k_keys_fx(const uint32_t* __restrict__ X, long np, int lgN, PosArgs pa,
                                                   const uint32_t* __restrict__ prevkeys, uint32_t* __restrict__ keys,
-                                                  uint32_t* __restrict__ pids, uint32_t* __restrict__ flags,
-                                                  uint32_t* __restrict__ in_count) {
+                                                  uint32_t* __restrict__ pids, uint32_t* __restrict__ in_count,
+                                                  uint32_t* __restrict__ out_count) {
     const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s == 0 && prevkeys) flags[np] = 0u;
     if (s >= np) return;
     const int mask = (1 << lgN) - 1;
     int oc[3] = {0, 0, 0};
@@ -55,7 +54,7 @@ __global__ __launch_bounds__(256) void k_keys_fx(const uint32_t* __restrict__ X,
     for (int k = 0; k < R; ++k) oc[k] = (int)(X[k * np + s] >> pa.fshift) & mask;
     const uint32_t key = brick_key(oc[0], oc[1], oc[2], R, pa);
     keys[s] = key;
-    fx_note_key(key, s, prevkeys, pids, flags, in_count);
+    fx_note_key(key, s, prevkeys, pids, in_count, out_count);
 }
 
 // row_start[r] = first sorted position whose key lies in mesh row r or later.
@@ -683,9 +682,9 @@ void ParticleOps<PP>::keys(int R, const Pos* X, long np, int lgN, const PosArgs&
 }
 
 void keys_fx(int R, const uint32_t* X, long np, int lgN, const PosArgs& pa, const uint32_t* prevkeys, uint32_t* keys,
-             uint32_t* pids, uint32_t* flags, uint32_t* in_count, hipStream_t st) {
+             uint32_t* pids, uint32_t* in_count, uint32_t* out_count, hipStream_t st) {
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_keys_fx<RR>), dim3(nblk(np, 256)), dim3(256), 0, st, X, np, lgN, pa,
-                                         prevkeys, keys, pids, flags, in_count));
+                                         prevkeys, keys, pids, in_count, out_count));
     CPM_LAUNCH_CHECK();
 }
 

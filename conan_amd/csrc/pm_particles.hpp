@@ -115,10 +115,10 @@ struct ParticleOps {
 
 void row_bounds(const uint32_t* keys, long np, int lgN, long nrows, uint32_t* row_start, hipStream_t st);
 // FX brick keys in storage order: prevkeys == nullptr writes pids = iota for
-// a full radix sort; otherwise flags (np + 1 entries) and in_count for the
+// a full radix sort; otherwise movers per new (in_count) and old (out_count) group for the
 // incremental re-sort against prevkeys (pm_sort.hpp).
 void keys_fx(int R, const uint32_t* X, long np, int lgN, const PosArgs& pa, const uint32_t* prevkeys, uint32_t* keys,
-             uint32_t* pids, uint32_t* flags, uint32_t* in_count, hipStream_t st);
+             uint32_t* pids, uint32_t* in_count, uint32_t* out_count, hipStream_t st);
 // FX brick order: bricks of FXB_X x FXB_Y x FXB_Z cells (2-D: FXB_X x FXB_X),
 // clamped to the mesh.  brick_layout() fills pa.brick / lb / lnb.
 #ifndef CPM_FXB_X

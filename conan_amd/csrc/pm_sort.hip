@@ -256,14 +256,14 @@ int RadixSorter::sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, hi
 // ------------------------------------------------------------ incremental re-sort
 // Keys that were sorted one step ago change for few elements (particles that
 // crossed into another brick or face class).  With prevkeys the sorted keys of
-// the last sort, in the current storage order, bprev their group starts and
-// M the exclusive scan of moved = (key != prevkey), the stayers of group g keep
-// their order at the front of the group and the movers follow them.
+// the last sort, in the current storage order, and bprev their group starts,
+// the stayers of group g keep their order at the front of the group and the
+// movers follow them.
 namespace cpm {
 
 __global__ __launch_bounds__(256) void k_resort_counts(const uint32_t* __restrict__ bprev,
-                                                       const uint32_t* __restrict__ M,
-                                                       const uint32_t* __restrict__ in_count, long ngroups,
+                                                       const uint32_t* __restrict__ in_count,
+                                                       const uint32_t* __restrict__ out_count, long ngroups,
                                                        uint32_t* __restrict__ stay, uint32_t* __restrict__ tot) {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g > ngroups) return;
@@ -271,32 +271,67 @@ __global__ __launch_bounds__(256) void k_resort_counts(const uint32_t* __restric
         tot[g] = 0;
         return;
     }
-    const uint32_t b0 = bprev[g], b1 = bprev[g + 1];
-    const uint32_t s = (b1 - b0) - (M[b1] - M[b0]);
+    const uint32_t s = (bprev[g + 1] - bprev[g]) - out_count[g];
     stay[g] = s;
     tot[g] = s + in_count[g];
 }
 
-__global__ __launch_bounds__(256) void k_resort_place(const uint32_t* __restrict__ keys,
-                                                      const uint32_t* __restrict__ prevkeys,
-                                                      const uint32_t* __restrict__ M,
-                                                      const uint32_t* __restrict__ bprev,
-                                                      const uint32_t* __restrict__ bnew,
-                                                      const uint32_t* __restrict__ stay, uint32_t* __restrict__ fill,
-                                                      long n, uint32_t* __restrict__ perm,
-                                                      uint32_t* __restrict__ skeys) {
-    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t g = keys[s];
-    uint32_t pos;
-    if (g == prevkeys[s]) {
-        const uint32_t b0 = bprev[g];
-        pos = bnew[g] + ((uint32_t)s - b0) - (M[s] - M[b0]);
-    } else {
-        pos = bnew[g] + stay[g] + atomicAdd(&fill[g], 1u);
+// One workgroup per brick: the brick's previous range (its G groups, one
+// after the other) is walked in chunks of RS_NT; a block scan of the stayer
+// flags (wave ballots) gives each stayer its rank among the stayers of its
+// group; a mover takes the next free slot after the stayers of its new group.
+static constexpr int RS_NT = 256;
+__global__ __launch_bounds__(RS_NT) void k_resort_place(const uint32_t* __restrict__ keys,
+                                                        const uint32_t* __restrict__ prevkeys,
+                                                        const uint32_t* __restrict__ bprev,
+                                                        const uint32_t* __restrict__ bnew,
+                                                        const uint32_t* __restrict__ stay, uint32_t* __restrict__ fill,
+                                                        int G, uint32_t* __restrict__ perm,
+                                                        uint32_t* __restrict__ skeys) {
+    __shared__ uint32_t before[8];  // stayers of the brick's earlier groups
+    __shared__ uint32_t wsum[RS_NT / 64];
+    const uint32_t g0 = blockIdx.x * (uint32_t)G;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) {
+        uint32_t a = 0;
+        for (int c = 0; c < G; ++c) {
+            before[c] = a;
+            a += stay[g0 + c];
+        }
     }
-    perm[pos] = (uint32_t)s;
-    skeys[pos] = g;
+    const uint32_t r0 = bprev[g0], r1 = bprev[g0 + G];
+    __syncthreads();
+    uint32_t carry = 0;
+    for (uint32_t base = r0; base < r1; base += RS_NT) {
+        const uint32_t s = base + threadIdx.x;
+        const bool valid = s < r1;
+        uint32_t key = 0, pk = 0;
+        if (valid) {
+            key = keys[s];
+            pk = prevkeys[s];
+        }
+        const bool st = valid && key == pk;
+        const uint64_t m = __ballot(st);
+        const uint32_t lanepre = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t wpre = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < RS_NT / 64; ++w) {
+            const uint32_t c = wsum[w];
+            wpre += (w < wave) ? c : 0u;
+            total += c;
+        }
+        if (valid) {
+            uint32_t pos;
+            if (st) pos = bnew[pk] + (carry + wpre + lanepre - before[pk - g0]);
+            else pos = bnew[key] + stay[key] + atomicAdd(&fill[key], 1u);
+            perm[pos] = s;
+            skeys[pos] = key;
+        }
+        carry += total;
+        __syncthreads();  // wsum is rewritten by the next chunk
+    }
 }
 
 __global__ __launch_bounds__(256) void k_zero_u32(uint32_t* __restrict__ p, long n) {
@@ -304,18 +339,18 @@ __global__ __launch_bounds__(256) void k_zero_u32(uint32_t* __restrict__ p, long
     if (i < n) p[i] = 0u;
 }
 
-void resort_counts(const uint32_t* bprev, const uint32_t* M, const uint32_t* in_count, long ngroups, uint32_t* stay,
-                   uint32_t* tot, hipStream_t st) {
-    hipLaunchKernelGGL(k_resort_counts, dim3((unsigned)((ngroups + 256) / 256)), dim3(256), 0, st, bprev, M, in_count,
-                       ngroups, stay, tot);
+void resort_counts(const uint32_t* bprev, const uint32_t* in_count, const uint32_t* out_count, long ngroups,
+                   uint32_t* stay, uint32_t* tot, hipStream_t st) {
+    hipLaunchKernelGGL(k_resort_counts, dim3((unsigned)((ngroups + 256) / 256)), dim3(256), 0, st, bprev, in_count,
+                       out_count, ngroups, stay, tot);
     CPM_LAUNCH_CHECK();
 }
 
-void resort_place(const uint32_t* keys, const uint32_t* prevkeys, const uint32_t* M, const uint32_t* bprev,
-                  const uint32_t* bnew, const uint32_t* stay, uint32_t* fill, long n, uint32_t* perm, uint32_t* skeys,
+void resort_place(const uint32_t* keys, const uint32_t* prevkeys, const uint32_t* bprev, const uint32_t* bnew,
+                  const uint32_t* stay, uint32_t* fill, long nbricks, int G, uint32_t* perm, uint32_t* skeys,
                   hipStream_t st) {
-    hipLaunchKernelGGL(k_resort_place, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, prevkeys, M, bprev,
-                       bnew, stay, fill, n, perm, skeys);
+    hipLaunchKernelGGL(k_resort_place, dim3((unsigned)nbricks), dim3(RS_NT), 0, st, keys, prevkeys, bprev, bnew, stay,
+                       fill, G, perm, skeys);
     CPM_LAUNCH_CHECK();
 }
 
