@@ -262,21 +262,31 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
     const BrickCoord bc = brick_coord(bid, pa);
     const int TX = bc.B[0] + 5, TY = bc.B[1] + 5, TZ = (R == 3) ? bc.B[2] + 5 : 1;
     const int tid = threadIdx.x;
-    stage_box<FXG_NT, FXB_TILE, R>(tile, phi, N, bc.x0, -2, TX, TY, TZ, tid);
-    __syncthreads();
-    const float inv12res = (float)(1.0 / (12.0 * pa.res));
+    // A thread's first particle is loaded before the tile is staged, so the
+    // two latencies overlap; each later particle is loaded at the end of the
+    // previous iteration.
     const uint32_t s0 = bstart[bid << R], s1 = bstart[(bid + 1) << R];
-    for (uint32_t s = s0 + tid; s < s1; s += FXG_NT) {
-        const uint32_t i = perm[s];
-        ido[s] = idi[i];
-        int o[3] = {0, 0, 0};
-        float a1[3] = {0.f, 0.f, 0.f};
-        uint32_t xs[3];
+    uint32_t xs[3] = {0u, 0u, 0u}, idv = 0u;
+    double pv[3] = {0, 0, 0};
+    auto fetch = [&](uint32_t sp) {
+        const uint32_t i = perm[sp];
+        idv = idi[i];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             xs[k] = X[k * np + i];
-            PosFX::gat(xs[k], pa, o[k], a1[k]);
+            pv[k] = P[k * np + i];
         }
+    };
+    if (s0 + tid < s1) fetch(s0 + tid);
+    stage_box<FXG_NT, FXB_TILE, R>(tile, phi, N, bc.x0, -2, TX, TY, TZ, tid);
+    __syncthreads();
+    const float inv12res = (float)(1.0 / (12.0 * pa.res));
+    for (uint32_t s = s0 + tid; s < s1; s += FXG_NT) {
+        ido[s] = idv;
+        int o[3] = {0, 0, 0};
+        float a1[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < R; ++k) PosFX::gat(xs[k], pa, o[k], a1[k]);
         // tile coordinates of the stencil origin (cell o - 2 along each axis)
         const int lx = (o[0] - bc.x0[0]) & msk, ly = (o[1] - bc.x0[1]) & msk;
         const int lz = (R == 3) ? ((o[2] - bc.x0[2]) & msk) : 0;
@@ -327,10 +337,11 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
         }
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            const double p = P[k * np + i] - fx_kick_increment((double)F[k] * ck, pq, ipq);
+            const double p = pv[k] - fx_kick_increment((double)F[k] * ck, pq, ipq);
             Po[k * np + s] = p;
             Xo[k * np + s] = cd_on ? xs[k] + (uint32_t)__double2ll_rn(p * cd) : xs[k];
         }
+        if (s + FXG_NT < s1) fetch(s + FXG_NT);
     }
 }
 
