@@ -266,6 +266,79 @@ __device__ __forceinline__ ColTile<T, N> col_tile(const ColPart<T>& pa, const Co
     return ColTile<T, N>{&p, (long)(w / gx) * p.g.bs, (long)(w % gx) * ColCfg<T, N>::C};
 }
 
+// Register-fused first and last Stockham stages (N >= 16, where every
+// thread owns one radix-8 butterfly of the last stage: NT = C N / 8).
+// Thread (pencil p = tid mod C, q = tid / C) loads the inputs of its
+// first-stage butterflies straight from HBM, transforms them and writes them
+// to LDS; the last stage writes the positions it reads (j + t N/8), so it
+// runs on values read from LDS and stores them straight to HBM.  Only the
+// middle stages exchange through LDS.
+template <int N>
+struct Fused {
+    using S = Sched<N>;
+    static constexpr bool ok = N >= 16;
+    static constexpr int R0 = S::lead ? S::lead : 8;  // radix of the first stage
+    static constexpr int mid = S::n8 - (S::lead ? 1 : 2);  // radix-8 stages between first and last
+};
+
+template <typename T, int N, int DIR>
+__device__ __forceinline__ void col_first(cx<T>* sh, const cx<T>* __restrict__ data, const ColGeom& g, long bbase,
+                                          long c0, int tid) {
+    using C = ColCfg<T, N>;
+    static_assert(C::NT * 8 == C::C * N, "one radix-8 butterfly per thread and stage");
+    constexpr int R0 = Fused<N>::R0, NB = 8 / R0;
+    const int p = tid % C::C, q = tid / C::C;
+    const long col = c0 + p;
+    const bool ok = col < g.nc;
+    const cx<T>* __restrict__ src = data + bbase + col * g.cs;
+    cx<T> v[8];
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+#pragma unroll
+        for (int t = 0; t < R0; ++t)
+            v[u * R0 + t] = ok ? src[(long)(q + u * (N / 8) + t * (N / R0)) * g.es] : mk<T>(0, 0);
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        dft<T, R0, DIR>(&v[u * R0]);  // first stage: no twiddles
+        const int j = q + u * (N / 8);
+#pragma unroll
+        for (int t = 0; t < R0; ++t) sh[lidx<N>(p, j * R0 + t)] = v[u * R0 + t];
+    }
+}
+
+template <typename T, int N, int DIR>
+__device__ __forceinline__ void col_middle(cx<T>* sh, const cx<T>* __restrict__ tw, int tid) {
+    using C = ColCfg<T, N>;
+    int Ns = Fused<N>::R0;
+#pragma unroll
+    for (int s = 0; s < Fused<N>::mid; ++s) {
+        lds_stage<T, N, C::E, C::NT, DIR, 8>(sh, Ns, tw, 1, tid);
+        Ns *= 8;
+    }
+}
+
+// The last stage of thread (p, j) in registers: v[t] = element j + t N/8.
+template <typename T, int N, int DIR>
+__device__ __forceinline__ void col_last_regs(const cx<T>* sh, const cx<T>* __restrict__ tw, int tid, cx<T>* v) {
+    using C = ColCfg<T, N>;
+    const int p = tid % C::C, j = tid / C::C;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = sh[lidx<N>(p, j + t * (N / 8))];
+    butterfly<T, N, 8, DIR>(v, j, N / 8, tw, 1);
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void col_store_regs(const cx<T>* v, cx<T>* __restrict__ data, const ColGeom& g, long bbase,
+                                               long c0, int tid) {
+    using C = ColCfg<T, N>;
+    const int p = tid % C::C, j = tid / C::C;
+    const long col = c0 + p;
+    if (col >= g.nc) return;
+    cx<T>* __restrict__ dst = data + bbase + col * g.cs;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) dst[(long)(j + t * (N / 8)) * g.es] = v[t];
+}
+
 template <typename T, int N, int DIR>
 __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols(ColPart<T> pa, ColPart<T> pb,
                                                            const cx<T>* __restrict__ tw) {
@@ -276,10 +349,19 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols(ColPart<T> pa, ColP
     const ColTile<T, N> t = col_tile<T, N>(pa, pb);
     const ColPart<T>& p = *t.part;
     for (int m = tid; m < N; m += C::NT) stw[m] = tw[m];
-    col_load<T, N>(sh, p.data, p.g, t.bbase, t.c0, tid);
-    __syncthreads();
-    lds_fft<T, N, C::E, C::NT, DIR>(sh, stw, 1, tid);
-    col_store<T, N>(sh, p.data, p.g, t.bbase, t.c0, tid);
+    if constexpr (Fused<N>::ok) {
+        col_first<T, N, DIR>(sh, p.data, p.g, t.bbase, t.c0, tid);
+        __syncthreads();
+        col_middle<T, N, DIR>(sh, stw, tid);
+        cx<T> v[8];
+        col_last_regs<T, N, DIR>(sh, stw, tid, v);
+        col_store_regs<T, N>(v, p.data, p.g, t.bbase, t.c0, tid);
+    } else {
+        col_load<T, N>(sh, p.data, p.g, t.bbase, t.c0, tid);
+        __syncthreads();
+        lds_fft<T, N, C::E, C::NT, DIR>(sh, stw, 1, tid);
+        col_store<T, N>(sh, p.data, p.g, t.bbase, t.c0, tid);
+    }
 }
 
 // Signed mode number of index n on an N mesh: n > N/2 maps to n - N, and
@@ -300,7 +382,11 @@ __device__ __forceinline__ T green_value(int ix, int iy, int iz, int dim, int N,
         T kz = (T)signed_mode(iz, N) * kf;
         v += kz * kz;
     }
-    return (T)-1. / v;
+    if constexpr (sizeof(T) == 4) {
+        return -__builtin_amdgcn_rcpf(v);  // 1 ulp reciprocal: below the FP32 FFT's own rounding
+    } else {
+        return (T)-1. / v;
+    }
 }
 
 // Last-axis forward transform, multiplication by the Green's function and
@@ -315,12 +401,56 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols_green(ColPart<T> pa
     const ColTile<T, N> t = col_tile<T, N>(pa, pb);
     const ColPart<T>& p = *t.part;
     const GreenGeom gg = p.gg;
+    const int H = N / 2;
+    const T kf = (T)gg.kf;
     for (int m = tid; m < N; m += C::NT) stw[m] = tw[m];
+    if constexpr (Fused<N>::ok) {
+        // forward: first stage on load, middle stages, last stage in
+        // registers; the Green's function on the registers; backward: first
+        // stage in registers (when it is radix 8 it reads exactly these
+        // positions), middle stages, last stage on store.
+        col_first<T, N, -1>(sh, p.data, p.g, t.bbase, t.c0, tid);
+        __syncthreads();
+        col_middle<T, N, -1>(sh, stw, tid);
+        cx<T> v[8];
+        col_last_regs<T, N, -1>(sh, stw, tid, v);
+        const int pc = tid % C::C, j = tid / C::C;
+        const long col = t.c0 + pc;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int n = j + q * (N / 8);
+            int ix, iy, iz = 0;
+            if (gg.dim == 3) {
+                ix = gg.nyq ? H : (int)(col & (H - 1));
+                iy = gg.nyq ? (int)col : (int)(col / H);
+                iz = n;
+            } else {
+                ix = gg.nyq ? H : (int)col;
+                iy = n;
+            }
+            const T gv = green_value<T>(ix, iy, iz, gg.dim, N, kf);
+            v[q] = (ix == 0 && iy == 0 && iz == 0) ? mk<T>(0, 0) : scale(v[q], gv);
+        }
+        __syncthreads();  // every thread has read its last-stage inputs
+        if constexpr (Fused<N>::R0 == 8) {
+            dft<T, 8, +1>(v);  // backward first stage, butterfly j, no twiddles
+#pragma unroll
+            for (int q = 0; q < 8; ++q) sh[lidx<N>(pc, j * 8 + q)] = v[q];
+            __syncthreads();
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) sh[lidx<N>(pc, j + q * (N / 8))] = v[q];
+            __syncthreads();
+            lds_stage<T, N, C::E, C::NT, +1, Fused<N>::R0>(sh, 1, stw, 1, tid);
+        }
+        col_middle<T, N, +1>(sh, stw, tid);
+        col_last_regs<T, N, +1>(sh, stw, tid, v);
+        col_store_regs<T, N>(v, p.data, p.g, t.bbase, t.c0, tid);
+        return;
+    }
     col_load<T, N>(sh, p.data, p.g, t.bbase, t.c0, tid);
     __syncthreads();
     lds_fft<T, N, C::E, C::NT, -1>(sh, stw, 1, tid);
-    const int H = N / 2;
-    const T kf = (T)gg.kf;
     for (int e = tid; e < C::C * N; e += C::NT) {
         const int c = e / N, n = e % N;
         const long col = t.c0 + c;
