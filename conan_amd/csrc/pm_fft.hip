@@ -257,13 +257,16 @@ struct ColTile {
     long bbase, c0;
 };
 template <typename T, int N>
-__device__ __forceinline__ ColTile<T, N> col_tile(const ColPart<T>& pa, const ColPart<T>& pb) {
-    uint32_t w = blockIdx.x;
+__device__ __forceinline__ ColTile<T, N> col_tile_at(const ColPart<T>& pa, const ColPart<T>& pb, uint32_t w) {
     const bool second = w >= (uint32_t)pa.nblk;
     const ColPart<T>& p = second ? pb : pa;
     if (second) w -= (uint32_t)pa.nblk;
     const uint32_t gx = (uint32_t)p.gx;
     return ColTile<T, N>{&p, (long)(w / gx) * p.g.bs, (long)(w % gx) * ColCfg<T, N>::C};
+}
+template <typename T, int N>
+__device__ __forceinline__ ColTile<T, N> col_tile(const ColPart<T>& pa, const ColPart<T>& pb) {
+    return col_tile_at<T, N>(pa, pb, blockIdx.x);
 }
 
 // Register-fused first and last Stockham stages (N >= 16, where every
@@ -281,28 +284,56 @@ struct Fused {
     static constexpr int mid = S::n8 - (S::lead ? 1 : 2);  // radix-8 stages between first and last
 };
 
-template <typename T, int N, int DIR>
-__device__ __forceinline__ void col_first(cx<T>* sh, const cx<T>* __restrict__ data, const ColGeom& g, long bbase,
-                                          long c0, int tid) {
+// HBM loads of the first-stage inputs of thread (p, q) of tile t.
+template <typename T, int N>
+__device__ __forceinline__ void col_first_load(const ColTile<T, N>& t, int tid, cx<T>* v) {
     using C = ColCfg<T, N>;
     static_assert(C::NT * 8 == C::C * N, "one radix-8 butterfly per thread and stage");
     constexpr int R0 = Fused<N>::R0, NB = 8 / R0;
+    const ColPart<T>& P = *t.part;
     const int p = tid % C::C, q = tid / C::C;
-    const long col = c0 + p;
-    const bool ok = col < g.nc;
-    const cx<T>* __restrict__ src = data + bbase + col * g.cs;
-    cx<T> v[8];
+    const long col = t.c0 + p;
+    const bool ok = col < P.g.nc;
+    const cx<T>* __restrict__ src = P.data + t.bbase + col * P.g.cs;
 #pragma unroll
     for (int u = 0; u < NB; ++u)
 #pragma unroll
-        for (int t = 0; t < R0; ++t)
-            v[u * R0 + t] = ok ? src[(long)(q + u * (N / 8) + t * (N / R0)) * g.es] : mk<T>(0, 0);
+        for (int k = 0; k < R0; ++k)
+            v[u * R0 + k] = ok ? src[(long)(q + u * (N / 8) + k * (N / R0)) * P.g.es] : mk<T>(0, 0);
+}
+
+// The first stage (no twiddles) on those values, written to LDS.
+template <typename T, int N, int DIR>
+__device__ __forceinline__ void col_first_store(cx<T>* sh, cx<T>* v, int tid) {
+    using C = ColCfg<T, N>;
+    constexpr int R0 = Fused<N>::R0, NB = 8 / R0;
+    const int p = tid % C::C, q = tid / C::C;
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
-        dft<T, R0, DIR>(&v[u * R0]);  // first stage: no twiddles
+        dft<T, R0, DIR>(&v[u * R0]);
         const int j = q + u * (N / 8);
 #pragma unroll
-        for (int t = 0; t < R0; ++t) sh[lidx<N>(p, j * R0 + t)] = v[u * R0 + t];
+        for (int k = 0; k < R0; ++k) sh[lidx<N>(p, j * R0 + k)] = v[u * R0 + k];
+    }
+}
+
+// Persistent loop over the tiles of a column launch (fused path): while a
+// tile is transformed, the HBM loads of the workgroup's next tile are in
+// flight.  body(tile, v) gets the first-stage inputs in v.
+template <typename T, int N, typename Body>
+__device__ __forceinline__ void col_pipeline(const ColPart<T>& pa, const ColPart<T>& pb, int tid, Body body) {
+    const uint32_t ntiles = (uint32_t)(pa.nblk + pb.nblk);
+    uint32_t w = blockIdx.x;
+    cx<T> nv[8];
+    if (w < ntiles) col_first_load<T, N>(col_tile_at<T, N>(pa, pb, w), tid, nv);
+    for (; w < ntiles; w += gridDim.x) {
+        const ColTile<T, N> t = col_tile_at<T, N>(pa, pb, w);
+        cx<T> v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = nv[i];
+        if (w + gridDim.x < ntiles) col_first_load<T, N>(col_tile_at<T, N>(pa, pb, w + gridDim.x), tid, nv);
+        body(t, v);
+        __syncthreads();  // LDS is refilled by the next tile
     }
 }
 
@@ -346,17 +377,21 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols(ColPart<T> pa, ColP
     __shared__ cx<T> sh[C::C * N];
     __shared__ cx<T> stw[N];
     const int tid = threadIdx.x;
-    const ColTile<T, N> t = col_tile<T, N>(pa, pb);
-    const ColPart<T>& p = *t.part;
     for (int m = tid; m < N; m += C::NT) stw[m] = tw[m];
     if constexpr (Fused<N>::ok) {
-        col_first<T, N, DIR>(sh, p.data, p.g, t.bbase, t.c0, tid);
+        // one tile per workgroup: with a single FFT per tile, prefetching
+        // the next tile costs more in registers than it hides
+        const ColTile<T, N> t = col_tile<T, N>(pa, pb);
+        cx<T> v[8];
+        col_first_load<T, N>(t, tid, v);
+        col_first_store<T, N, DIR>(sh, v, tid);
         __syncthreads();
         col_middle<T, N, DIR>(sh, stw, tid);
-        cx<T> v[8];
         col_last_regs<T, N, DIR>(sh, stw, tid, v);
-        col_store_regs<T, N>(v, p.data, p.g, t.bbase, t.c0, tid);
+        col_store_regs<T, N>(v, t.part->data, t.part->g, t.bbase, t.c0, tid);
     } else {
+        const ColTile<T, N> t = col_tile<T, N>(pa, pb);
+        const ColPart<T>& p = *t.part;
         col_load<T, N>(sh, p.data, p.g, t.bbase, t.c0, tid);
         __syncthreads();
         lds_fft<T, N, C::E, C::NT, DIR>(sh, stw, 1, tid);
@@ -398,21 +433,19 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols_green(ColPart<T> pa
     __shared__ cx<T> sh[C::C * N];
     __shared__ cx<T> stw[N];
     const int tid = threadIdx.x;
-    const ColTile<T, N> t = col_tile<T, N>(pa, pb);
-    const ColPart<T>& p = *t.part;
-    const GreenGeom gg = p.gg;
     const int H = N / 2;
-    const T kf = (T)gg.kf;
     for (int m = tid; m < N; m += C::NT) stw[m] = tw[m];
     if constexpr (Fused<N>::ok) {
         // forward: first stage on load, middle stages, last stage in
         // registers; the Green's function on the registers; backward: first
         // stage in registers (when it is radix 8 it reads exactly these
         // positions), middle stages, last stage on store.
-        col_first<T, N, -1>(sh, p.data, p.g, t.bbase, t.c0, tid);
+        col_pipeline<T, N>(pa, pb, tid, [&](const ColTile<T, N>& t, cx<T>* v) {
+        const GreenGeom gg = t.part->gg;
+        const T kf = (T)gg.kf;
+        col_first_store<T, N, -1>(sh, v, tid);
         __syncthreads();
         col_middle<T, N, -1>(sh, stw, tid);
-        cx<T> v[8];
         col_last_regs<T, N, -1>(sh, stw, tid, v);
         const int pc = tid % C::C, j = tid / C::C;
         const long col = t.c0 + pc;
@@ -445,9 +478,14 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols_green(ColPart<T> pa
         }
         col_middle<T, N, +1>(sh, stw, tid);
         col_last_regs<T, N, +1>(sh, stw, tid, v);
-        col_store_regs<T, N>(v, p.data, p.g, t.bbase, t.c0, tid);
+        col_store_regs<T, N>(v, t.part->data, t.part->g, t.bbase, t.c0, tid);
+        });
         return;
     }
+    const ColTile<T, N> t = col_tile<T, N>(pa, pb);
+    const ColPart<T>& p = *t.part;
+    const GreenGeom gg = p.gg;
+    const T kf = (T)gg.kf;
     col_load<T, N>(sh, p.data, p.g, t.bbase, t.c0, tid);
     __syncthreads();
     lds_fft<T, N, C::E, C::NT, -1>(sh, stw, 1, tid);
@@ -518,13 +556,37 @@ void launch_rows_c2r(int N, const cx<T>* spec, const cx<T>* nyq, T* out, long nr
     CPM_LAUNCH_CHECK();
 }
 
+// Workgroups of a Green's-function launch: one per tile, or for the fused
+// (persistent) kernel as many as the GPU holds at once.
+static long resident_cus() {
+    static int cus[64] = {0};
+    int dev = 0;
+    CPM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) dev = 0;
+    if (!cus[dev]) CPM_HIP(hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev));
+    return cus[dev];
+}
+template <typename T, int N>
+dim3 col_grid(long ntiles) {
+    if (!Fused<N>::ok) return dim3((unsigned)ntiles);
+    using C = ColCfg<T, N>;
+    const long lds = (long)(C::C + 1) * N * (long)sizeof(cx<T>);
+    long per_cu = 163840 / lds;
+    const long by_threads = 2048 / C::NT;
+    if (per_cu > by_threads) per_cu = by_threads;
+    if (per_cu < 1) per_cu = 1;
+    const long g = resident_cus() * per_cu;
+    return dim3((unsigned)(ntiles < g ? ntiles : g));
+}
+
 template <typename T>
 void launch_cols(int N, int dir, const ColPart<T>& a, const ColPart<T>& b, const cx<T>* tw, hipStream_t st) {
-    const dim3 grid((unsigned)(a.nblk + b.nblk));
+    const long ntiles = a.nblk + b.nblk;
     switch (N) {
 #define CASE(n)                                                                             \
     case n: {                                                                               \
         using C = ColCfg<T, n>;                                                             \
+        const dim3 grid((unsigned)ntiles);                                                  \
         if (dir < 0)                                                                        \
             hipLaunchKernelGGL((k_cols<T, n, -1>), grid, dim3(C::NT), 0, st, a, b, tw);     \
         else                                                                                \
@@ -541,11 +603,12 @@ void launch_cols(int N, int dir, const ColPart<T>& a, const ColPart<T>& b, const
 
 template <typename T>
 void launch_cols_green(int N, const ColPart<T>& a, const ColPart<T>& b, const cx<T>* tw, hipStream_t st) {
-    const dim3 grid((unsigned)(a.nblk + b.nblk));
+    const long ntiles = a.nblk + b.nblk;
     switch (N) {
 #define CASE(n)                                                                                \
     case n: {                                                                                  \
         using C = ColCfg<T, n>;                                                                \
+        const dim3 grid = col_grid<T, n>(ntiles);                                              \
         hipLaunchKernelGGL((k_cols_green<T, n>), grid, dim3(C::NT), 0, st, a, b, tw);          \
         break;                                                                                 \
     }
