@@ -43,7 +43,7 @@ __global__ __launch_bounds__(256) void k_rev_drift_keys(uint32_t* __restrict__ Y
         for (int k = 0; k < R; ++k) {
             uint32_t y = Y[k * np + i];
             if (cd_fx != 0.0) {
-                y -= (uint32_t)__double2ll_rn(P[k * np + i] * cd_fx);
+                y -= fx_round_u32(P[k * np + i] * cd_fx);
                 Y[k * np + i] = y;
             }
             const double lp = lP[k * np + i] + cd * lY[k * np + i];
@@ -191,6 +191,7 @@ __global__ __launch_bounds__(FXG_NT) void k_adj_gather_fx_brick(
     const float* __restrict__ phi, const float* __restrict__ chi, int lgN, PosArgs pa, AdjCoef cf) {
     __shared__ float tile[FXB_TILE];
     __shared__ float ctile[FXB_CTILE];
+    const double cFq = cf.cF * cf.ipq;  // the forward's ck / pq for cF = ck (exact: ipq is a power of two)
     const int N = 1 << lgN, msk = N - 1;
     const uint32_t bid = blockIdx.x;
     const BrickCoord bc = brick_coord(bid, pa);
@@ -308,7 +309,7 @@ __global__ __launch_bounds__(FXG_NT) void k_adj_gather_fx_brick(
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             Yo[k * np + s] = ys[k];
-            Po[k * np + s] = P[k * np + i] + fx_kick_increment((double)F[k] * cf.cF, cf.pq, cf.ipq);
+            Po[k * np + s] = P[k * np + i] + fx_kick_increment((double)F[k], cFq, cf.pq);
             lYo[k * np + s] = lY[k * np + i] + cf.cT * T[k] + cf.cU * U[k];
             lPo[k * np + s] = lam[k];
         }

@@ -16,6 +16,9 @@ static constexpr int FXB_NT = 256;  // deposit workgroup size
 #define CPM_FXG_NT 512
 #endif
 static constexpr int FXG_NT = CPM_FXG_NT;  // gather workgroup size
+#ifndef CPM_GATHER_LSEG
+#define CPM_GATHER_LSEG 3  // log2 of the bricks one gather workgroup slides through
+#endif
 static constexpr int FXB_CELLS = FXB_X * FXB_Y * FXB_Z;
 // potential tile of a brick: 2 cells of halo below and 3 above along each axis
 static constexpr int FXB_TILE = (FXB_X + 5) * (FXB_Y + 5) * (FXB_Z + 5);
@@ -78,6 +81,22 @@ __device__ __forceinline__ void stage_box(float* tile, const float* __restrict__
     for (int l = 0; l < NL; ++l)
         if (tid + l * NT < total) tile[tid + l * NT] = v[l];
 }
+
+// One dword per active lane from src straight into LDS at byte address
+// lds_base + 4 * lane (global_load_lds_dword; lds_base must be the same for
+// the whole wave, lanes that are switched off write nothing).  The load writes no register.  It is issued from inline assembly so
+// that the compiler does not make later LDS reads wait for it (it cannot tell
+// which LDS it writes); the caller waits with __builtin_amdgcn_s_waitcnt
+// (WAIT_VMCNT0) and then synchronises before the data is read.  The LDS
+// address comes from M0, which needs one wait state after the SALU write
+// before an LDS-DMA load reads it (the s_nop).
+__device__ __forceinline__ void lds_dma_dword(const float* src, uint32_t lds_base) {
+    const uint32_t a = __builtin_amdgcn_readfirstlane(lds_base);
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off" : : "v"(src), "s"(a) : "memory");
+}
+// s_waitcnt vmcnt(0) with the other counters left alone (gfx9 encoding:
+// vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+static constexpr int WAIT_VMCNT0 = 0x0F70;
 
 // The sorted-particle ranges whose CIC clouds reach a brick: the whole brick
 // itself, and of each lower neighbour only the face classes towards it.

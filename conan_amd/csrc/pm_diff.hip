@@ -169,7 +169,7 @@ struct DiffPM {
         ParticleOps<PosFX>::zeldovich(R, icm.p, lgNm, p.L, p.a0, p.a0 + p.da / 2, sim.X.p, sim.P.p, st);
         sim.fx_quantize_momenta();
         iota_u32(sim.id.p, Np, st);
-        sim.resort_ok = false;  // storage order is the mass-grid order now
+        sim.invalidate_order();  // storage order is the mass-grid order now
         sim.have_state = true;
         ParticleOps<PosFX>::drift(sim.X.p, sim.P.p, (long)R * Np, sim.drift_factor(a_seq[0], p.da), st);
         for (int i = 0; i < p.nsteps; ++i) {
@@ -208,9 +208,7 @@ struct DiffPM {
     // keys of the current positions -> sort -> brick offsets (the first half
     // of Impl::do_deposit)
     void sort_particles() {
-        const Impl<PosFX>::FxKeyArgs ka = sim.fx_key_args();
-        keys_fx(R, sim.X.p, Np, sim.lgNf, sim.pa, ka.prevkeys, ka.keys, ka.pids, ka.in_count, ka.out_count, st);
-        sim.fx_sort();
+        sim.fx_sort_current();
     }
 
     void swap_all() {
@@ -251,7 +249,7 @@ struct DiffPM {
         CPM_HIP(hipMemcpyAsync(sim.X.p, cX.p, sizeof(uint32_t) * R * Np, hipMemcpyDeviceToDevice, st));
         CPM_HIP(hipMemcpyAsync(sim.P.p, cP.p, sizeof(double) * R * Np, hipMemcpyDeviceToDevice, st));
         CPM_HIP(hipMemcpyAsync(sim.id.p, cid.p, sizeof(uint32_t) * Np, hipMemcpyDeviceToDevice, st));
-        sim.resort_ok = false;  // the checkpoint's order is not the last sort's
+        sim.invalidate_order();  // the checkpoint's order is not the last sort's
         CPM_HIP(hipStreamSynchronize(st));
         double cnt = 0, mdp = 0, mp = 0;
         for (int b = 0; b < nb; ++b) {

@@ -170,6 +170,7 @@ struct Impl : ImplBase {
     DevBuf<uint32_t> knew, skb0, skb1, permb, bs2, cnt3, stay, tot, scan_scr;
     const uint32_t* bstart = nullptr;  // group starts of the last sort (row_start or bs2)
     bool resort_ok = false;            // storage order == output order of the last FX sort
+    bool keys_ready = false;           // knew / cnt3 hold the keys of the current positions (fused gather)
     DevBuf<Mesh> mesh;
     DevBuf<cx<Mesh>> spec, nyq;
     DevBuf<double> row_partial, red, scal;
@@ -253,7 +254,7 @@ struct Impl : ImplBase {
 
     // Storage order = mass-grid order (set after ZA / set_state).
     void reset_ids() {
-        resort_ok = false;
+        invalidate_order();
         if (!FX) return;
         std::vector<uint32_t> h(Np);
         for (long i = 0; i < Np; ++i) h[i] = (uint32_t)i;
@@ -332,7 +333,14 @@ struct Impl : ImplBase {
         const uint32_t* prevkeys;
         uint32_t *keys, *pids, *in_count, *out_count;
     };
+    // Positions or storage order changed outside a gather: the next FX sort
+    // is a full one.
+    void invalidate_order() {
+        resort_ok = false;
+        keys_ready = false;
+    }
     FxKeyArgs fx_key_args() {
+        keys_ready = false;
         if (resort_ok) {
             zero_u32(cnt3.p, 3 * nsort_groups, st);
             return FxKeyArgs{skeys, knew.p, nullptr, cnt3.p, cnt3.p + nsort_groups};
@@ -340,6 +348,7 @@ struct Impl : ImplBase {
         return FxKeyArgs{nullptr, k0.p, v0.p, nullptr, nullptr};
     }
     void fx_sort() {
+        keys_ready = false;
         if (resort_ok) {
             uint32_t* bnew = (bstart == row_start.p) ? bs2.p : row_start.p;
             uint32_t* sk = (skeys == skb0.p) ? skb1.p : skb0.p;
@@ -359,13 +368,20 @@ struct Impl : ImplBase {
         }
         resort_ok = false;  // until a gather moves the particles into this order
     }
+    // Keys of the current positions (unless a fused gather produced them), then fx_sort.
+    void fx_sort_current() {
+        if (!keys_ready) {
+            const FxKeyArgs ka = fx_key_args();
+            keys_fx(R, reinterpret_cast<const uint32_t*>(X.p), Np, lgNf, pa, ka.prevkeys, ka.keys, ka.pids,
+                    ka.in_count, ka.out_count, st);
+        }
+        fx_sort();
+    }
 
     void do_deposit(Mesh m) {
         prof.begin(0, st);
         if constexpr (FX) {
-            const FxKeyArgs ka = fx_key_args();
-            keys_fx(R, X.p, Np, lgNf, pa, ka.prevkeys, ka.keys, ka.pids, ka.in_count, ka.out_count, st);
-            fx_sort();
+            fx_sort_current();
         } else {
             ParticleOps<PP>::keys(R, X.p, Np, lgNf, pa, k0.p, v0.p, st);
             const int where = sorter.sort(k0.p, v0.p, k1.p, v1.p, st);
@@ -413,6 +429,7 @@ struct Impl : ImplBase {
         prof.begin(8, st);
         ParticleOps<PP>::drift(X.p, P.p, (long)R * Np, drift_factor(a, da), st);
         prof.end(8, st);
+        keys_ready = false;
         CPM_HIP(hipStreamSynchronize(st));
         last_apos = a;
     }
@@ -425,14 +442,18 @@ struct Impl : ImplBase {
         const double cd = fuse ? drift_factor(a_next, da_next) : 0.0;
         prof.begin(7, st);
         if constexpr (FX) {
-            // kick (+ drift) and move the particles into the deposit's brick order
+            // kick (+ drift) and move the particles into the deposit's brick
+            // order; with the drift fused, also the keys of the next re-sort
+            if (fuse) zero_u32(cnt3.p, 3 * nsort_groups, st);
             gather_kick_fx_brick(R, bstart, spids, reinterpret_cast<const uint32_t*>(X.p), P.p, id.p,
                                  reinterpret_cast<uint32_t*>(X2.p), P2.p, id2.p, Np,
-                                 reinterpret_cast<const float*>(mesh.p), lgNf, pa, ck, pq, cd, fuse ? 1 : 0, st);
+                                 reinterpret_cast<const float*>(mesh.p), lgNf, pa, ck, pq, cd, fuse ? 1 : 0,
+                                 fuse ? knew.p : nullptr, cnt3.p, cnt3.p + nsort_groups, st);
             std::swap(X.p, X2.p);
             std::swap(P.p, P2.p);
             std::swap(id.p, id2.p);
             resort_ok = true;
+            keys_ready = fuse;
         } else {
             ParticleOps<PP>::gather_kick(R, X.p, P.p, Np, mesh.p, lgNf, pa, ck, cd, fuse ? 1 : 0, nullptr,
                                          Reorder<PP>(), st);
@@ -455,6 +476,7 @@ struct Impl : ImplBase {
         prof.begin(8, st);
         ParticleOps<PP>::drift(X.p, P.p, (long)R * Np, drift_factor(a, da), st);
         prof.end(8, st);
+        keys_ready = false;
         last_apos = a;
         for (int i = 0; i < n; ++i) {
             const double ak = a + da / 2;

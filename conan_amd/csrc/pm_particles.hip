@@ -245,31 +245,158 @@ __global__ __launch_bounds__(FXB_NT) void k_deposit_fx_brick(const uint32_t* __r
     if (tid == 0) partial[bid] = red[0];
 }
 
-// Gather + kick (+ drift), one workgroup per brick: the potential of the
-// brick and a halo of 2 cells below / 3 above along each axis (the 4-point
-// stencil of the 2^R corners) is staged in LDS once; every particle of the
-// brick then reads its 56 stencil values from LDS.  Particle s (storage
-// perm[s]) is written to position s of the output arrays.
+// Gather + kick (+ drift).  A workgroup takes a segment of 2^lseg bricks
+// that follow each other along the last axis (same x0 and, in 3-D, y0).  A
+// brick reads the potential of its cells and a halo of 2 cells below / 3
+// above along each axis (the 4-point stencil at the 2^R corners): a tile of
+// TP = B + 5 planes across the last axis (B: the brick's extent along it).
+// Consecutive bricks share 5 of those planes, so the planes live in an LDS
+// ring of TR = TP + B slots: plane p of the current brick's tile (global
+// plane x0 - 2 + p) sits in slot (p + rot) mod TR, and rot advances by B per
+// brick.  The B slots the current tile leaves free receive the next brick's
+// new planes by direct global->LDS loads issued while the current brick's
+// particles are processed, so the loads overlap that work and occupy no
+// registers.  A slot holds PS = (cells of a plane) floats, x fastest; planes
+// in consecutive slots are contiguous, which the direct load (64 lanes to
+// 64 consecutive dwords) needs; a wave whose 64 cells straddle the ring's
+// wrap issues its two parts separately.  Ring and guard stay below 64 KiB.
+// Particle s (storage perm[s]) is written to position s of the output arrays.
+// With keys != nullptr (drift fused), keys[s] is the sort key of the
+// particle's new position, and a particle whose key changed is counted into
+// its new group (in_count) and out of its old one (out_count): what keys_fx
+// computes for an incremental re-sort, whose previous keys are those of the
+// positions read here.
 template <int R>
 __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
     const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ X,
     const double* __restrict__ P, const uint32_t* __restrict__ idi, uint32_t* __restrict__ Xo,
     double* __restrict__ Po, uint32_t* __restrict__ ido, long np, const float* __restrict__ phi, int lgN,
-    PosArgs pa, double ck, double pq, double ipq, double cd, int cd_on) {
-    __shared__ float tile[FXB_TILE];
+    PosArgs pa, double ckq, double pq, double cd, int cd_on, int lseg, uint32_t* __restrict__ keys,
+    uint32_t* __restrict__ in_count, uint32_t* __restrict__ out_count) {
+    constexpr int LA = R - 1;                     // the axis the tile slides along
+    constexpr int BL = (R == 3) ? FXB_Z : FXB_X;  // brick extent along it, at most
+    constexpr int PSMAX = (R == 3) ? (FXB_X + 5) * (FXB_Y + 5) : FXB_X + 5;
+    // The guard keeps the ring at least 256 bytes above LDS address 0, so the
+    // base of the second part of a straddling wave (up to 63 cells before the
+    // ring) is a valid address.
+    __shared__ struct {
+        float guard[64];
+        float ring[(2 * BL + 5) * PSMAX];
+    } sm;
+    static_assert(sizeof(sm) <= 65536, "the ring must stay addressable below 64 KiB");
+    float* const ring = sm.ring;
     const int N = 1 << lgN, msk = N - 1;
-    const uint32_t bid = blockIdx.x;
-    const BrickCoord bc = brick_coord(bid, pa);
-    const int TX = bc.B[0] + 5, TY = bc.B[1] + 5, TZ = (R == 3) ? bc.B[2] + 5 : 1;
     const int tid = threadIdx.x;
-    // A thread's first particle is loaded before the tile is staged, so the
-    // two latencies overlap; each later particle is loaded at the end of the
-    // previous iteration.
-    const uint32_t s0 = bstart[bid << R], s1 = bstart[(bid + 1) << R];
+    const int lcol = pa.lnb[0] + ((R == 3) ? pa.lnb[1] : 0);
+    const uint32_t cstride = 1u << lcol;  // brick-index step along the last axis
+    // Workgroup w runs on XCD w mod 8; XCD k takes logical workgroups
+    // k G/8 .. (k + 1) G/8 - 1 (columns fastest), so the columns running on
+    // one XCD at a time are neighbours and share their halo lines in its L2.
+    const uint32_t G = gridDim.x;
+    const uint32_t lw = (G % 8 == 0) ? (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+    uint32_t bid = (lw & (cstride - 1)) | ((uint32_t)(lw >> lcol) << (lcol + lseg));
+    BrickCoord bc = brick_coord(bid, pa);
+    const int B = bc.B[LA];
+    const int TX = bc.B[0] + 5, TP = B + 5, TR = TP + B;
+    const int PS = (R == 3) ? TX * (bc.B[1] + 5) : TX;  // cells of a plane
+    // e / PS and r / TX by multiply-high: exact for the e, r < 2^14 used here
+    const uint32_t psinv = (uint32_t)((0x100000000ull + PS - 1) / PS);
+    const uint32_t txinv = (uint32_t)((0x100000000ull + TX - 1) / TX);
+    const uint32_t ring_lds = (uint32_t)(uintptr_t)ring;  // LDS byte address of slot 0
+    int rot = 0;
+    auto slot = [&](int p) {
+        const int q = p + rot;
+        return q >= TR ? q - TR : q;
+    };
+    // (index products in 24-bit multiplies: every factor is below 2^12)
+    auto T = [&](int tx, int ty, int tz) -> float {
+        return (R == 3) ? ring[__umul24(PS, slot(tz)) + tx + __umul24(TX, ty)] : ring[__umul24(PS, slot(ty)) + tx];
+    };
+    // Planes p0 .. p0 + npl - 1 of the tile of the brick at last-axis origin
+    // zb, into slots (p + rp) mod TR.  Cell e of the run (plane e / PS) goes
+    // to LDS slot-start * PS + e before the wrap and e - E after it.
+    auto load_planes = [&](int p0, int npl, int zb, int rp) {
+        constexpr int NLMAX = ((BL + 5) * PSMAX + FXG_NT - 1) / FXG_NT;
+        const int ne = npl * PS;
+        int start = p0 + rp;
+        start = start >= TR ? start - TR : start;
+        const int E = (TR - start) * PS;  // first cell whose plane wraps to slot 0
+        const int wbase = tid & ~63, lane = tid & 63;
+#pragma unroll 1
+        for (int l = 0; l < NLMAX; ++l) {
+            const int e0 = wbase + l * FXG_NT;  // the wave's first cell
+            if (e0 >= ne) break;
+            const int e = e0 + lane;
+            if (e < ne) {
+                const int q = (int)__umulhi((uint32_t)e, psinv), r = e - q * PS;
+                const int ty = (R == 3) ? (int)__umulhi((uint32_t)r, txinv) : 0;
+                const int tx = r - ty * TX;
+                const uint32_t gl = (uint32_t)((zb - 2 + p0 + q) & msk);
+                uint32_t off = (uint32_t)((bc.x0[0] - 2 + tx) & msk);
+                off += (R == 3) ? (uint32_t)N * ((uint32_t)((bc.x0[1] - 2 + ty) & msk) + (uint32_t)N * gl)
+                                : (uint32_t)N * gl;
+                if (e < E)
+                    lds_dma_dword(phi + off, ring_lds + 4u * (uint32_t)(start * PS + e0));
+                else
+                    lds_dma_dword(phi + off, ring_lds + 4u * (uint32_t)(e0 - E));
+            }
+        }
+    };
+    // The next brick's B new planes: cell e = wbase + l FXG_NT + lane of the
+    // run (plane e / PS) has the same in-plane mesh offset for every brick of
+    // the segment, so it is computed once, packed as (plane << 27) | offset
+    // (~0u: no cell).
+    constexpr int NLN = (BL * PSMAX + FXG_NT - 1) / FXG_NT;
+    const int wbase = __builtin_amdgcn_readfirstlane(tid & ~63), lane = tid & 63;
+    const int lgP = (R == 3) ? 2 * lgN : lgN;  // log2 of the mesh stride between planes
+    uint32_t noff[NLN];
+#pragma unroll
+    for (int l = 0; l < NLN; ++l) {
+        const int e = wbase + l * FXG_NT + lane;
+        const int q = (int)__umulhi((uint32_t)e, psinv), r = e - q * PS;
+        const int ty = (R == 3) ? (int)__umulhi((uint32_t)r, txinv) : 0;
+        const int tx = r - ty * TX;
+        uint32_t xy = (uint32_t)((bc.x0[0] - 2 + tx) & msk);
+        if (R == 3) xy += (uint32_t)N * (uint32_t)((bc.x0[1] - 2 + ty) & msk);
+        noff[l] = (e < B * PS) ? ((uint32_t)q << 27) | xy : ~0u;
+    }
+    // the next brick's new planes (last-axis origin zb) into slots
+    // (p + rp) mod TR, p = 5 .. TP - 1
+    auto load_new_planes = [&](int zb, int rp) {
+        int start = 5 + rp;
+        start = start >= TR ? start - TR : start;
+        const int E = (TR - start) * PS;  // first cell whose plane wraps to slot 0
+#pragma unroll
+        for (int l = 0; l < NLN; ++l) {
+            const uint32_t v = noff[l];
+            if (v != ~0u) {
+                const uint32_t gl = (uint32_t)((zb + 3 + (int)(v >> 27)) & msk);
+                const uint32_t off = (v & 0x7FFFFFFu) + (gl << lgP);
+                const int e0 = wbase + l * FXG_NT;
+                if (e0 + lane < E)
+                    lds_dma_dword(phi + off, ring_lds + 4u * (uint32_t)(start * PS + e0));
+                else
+                    lds_dma_dword(phi + off, ring_lds + 4u * (uint32_t)(e0 - E));
+            }
+        }
+    };
+    // Particles are processed in rounds of FXG_NT (round r: sorted position
+    // s0 + r FXG_NT + tid); a thread's particle of round r + 1 is loaded at
+    // the end of round r, its first one of the next brick before the barrier
+    // that ends the brick.  The next brick's planes are issued at the start of
+    // the last round, after a wait for everything loaded so far: no load the
+    // compiler waits for is then younger than them, so they stay in flight
+    // through the last round instead of being drained by the first wait.
+    uint32_t s0 = bstart[bid << R], s1 = bstart[(bid + 1) << R];
+    const int nseg = 1 << lseg;
+    uint32_t nn0 = 0, nn1 = 0;
+    if (nseg > 1) {
+        nn0 = bstart[(bid + cstride) << R];
+        nn1 = bstart[(bid + cstride + 1) << R];
+    }
     uint32_t xs[3] = {0u, 0u, 0u}, idv = 0u;
     double pv[3] = {0, 0, 0};
-    auto fetch = [&](uint32_t sp) {
-        const uint32_t i = perm[sp];
+    auto fetch = [&](uint32_t i) {
         idv = idi[i];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
@@ -277,11 +404,12 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
             pv[k] = P[k * np + i];
         }
     };
-    if (s0 + tid < s1) fetch(s0 + tid);
-    stage_box<FXG_NT, FXB_TILE, R>(tile, phi, N, bc.x0, -2, TX, TY, TZ, tid);
+    if (s0 + tid < s1) fetch(perm[s0 + tid]);
+    load_planes(0, TP, bc.x0[LA], 0);
+    __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
     __syncthreads();
     const float inv12res = (float)(1.0 / (12.0 * pa.res));
-    for (uint32_t s = s0 + tid; s < s1; s += FXG_NT) {
+    auto process = [&](uint32_t s) {
         ido[s] = idv;
         int o[3] = {0, 0, 0};
         float a1[3] = {0.f, 0.f, 0.f};
@@ -293,14 +421,13 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
         constexpr int NC = 1 << R;
         float z[NC];
 #pragma unroll
-        for (int j = 0; j < NC; ++j) {
+        for (int c = 0; c < NC; ++c) {
             float w = 1.f;
 #pragma unroll
-            for (int k = 0; k < R; ++k) w *= ((j >> k) & 1) ? a1[k] : 1.f - a1[k];
-            z[j] = w;
+            for (int k = 0; k < R; ++k) w *= ((c >> k) & 1) ? a1[k] : 1.f - a1[k];
+            z[c] = w;
         }
         const int zc = (R == 3) ? 2 : 0;  // tile z of the origin's own plane offset
-        auto T = [&](int tx, int ty, int tz) -> float { return tile[tx + TX * (ty + TY * tz)]; };
         float F[3] = {0.f, 0.f, 0.f};
         // axis 0: lines along x through the (y, z) corners
 #pragma unroll
@@ -335,13 +462,73 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
                     F[2] += fd4(f[bz], f[bz + 1], f[bz + 3], f[bz + 4], inv12res) * z[bx | (by << 1) | (bz << 2)];
             }
         }
+        uint32_t xn[3] = {0u, 0u, 0u};
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            const double p = pv[k] - fx_kick_increment((double)F[k] * ck, pq, ipq);
+            const double p = pv[k] - fx_kick_increment((double)F[k], ckq, pq);
             Po[k * np + s] = p;
-            Xo[k * np + s] = cd_on ? xs[k] + (uint32_t)__double2ll_rn(p * cd) : xs[k];
+            xn[k] = cd_on ? xs[k] + fx_round_u32(p * cd) : xs[k];
+            Xo[k * np + s] = xn[k];
         }
-        if (s + FXG_NT < s1) fetch(s + FXG_NT);
+        if (keys) {
+            int co[3] = {0, 0, 0}, cn[3] = {0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                co[k] = (int)(xs[k] >> pa.fshift) & msk;
+                cn[k] = (int)(xn[k] >> pa.fshift) & msk;
+            }
+            const uint32_t ko = brick_key(co[0], co[1], co[2], R, pa), kn = brick_key(cn[0], cn[1], cn[2], R, pa);
+            keys[s] = kn;
+            if (kn != ko) {
+                atomicAdd(&in_count[kn], 1u);
+                atomicAdd(&out_count[ko], 1u);
+            }
+        }
+    };
+    const uint32_t plast = (uint32_t)(np - 1);
+    for (int j = 0;; ++j) {
+        const bool more = j + 1 < nseg;
+        const int nr = (int)((s1 - s0 + FXG_NT - 1) / FXG_NT);
+        uint32_t npi;
+        // (npi is loaded unconditionally: a value merged from two paths would
+        // be copied, and the copy would wait for the load at once)
+        auto issue_next = [&]() {
+            __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+            if (more) {
+                int rn = rot + B;
+                rn = rn >= TR ? rn - TR : rn;
+                load_new_planes(bc.x0[LA] + B, rn);
+            }
+            const uint32_t t = nn0 + tid;
+            npi = perm[t < plast ? t : plast];
+        };
+        // all rounds but the last: every thread has a particle
+        for (int r = 0; r + 1 < nr; ++r) {
+            const uint32_t s = s0 + (uint32_t)r * FXG_NT + tid;
+            process(s);
+            if (s + FXG_NT < s1) fetch(perm[s + FXG_NT]);
+        }
+        // the last round, a separate copy of the body so that the compiler
+        // sees no outstanding load of its own in it
+        issue_next();
+        if (nr > 0) {
+            const uint32_t s = s0 + (uint32_t)(nr - 1) * FXG_NT + tid;
+            if (s < s1) process(s);
+        }
+        if (!more) break;
+        if (nn0 + tid < nn1) fetch(npi);
+        s0 = nn0;
+        s1 = nn1;
+        if (j + 2 < nseg) {
+            nn0 = bstart[(bid + 2 * cstride) << R];
+            nn1 = bstart[(bid + 2 * cstride + 1) << R];
+        }
+        __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);  // this wave's direct loads have landed
+        __syncthreads();
+        rot += B;
+        rot = rot >= TR ? rot - TR : rot;
+        bid += cstride;
+        bc.x0[LA] += B;
     }
 }
 
@@ -466,7 +653,7 @@ __global__ __launch_bounds__(256) void k_gather_kick(typename PP::Pos* __restric
             if constexpr (sizeof(M) == 8) {
                 xn = xs[k] + p * cd;
             } else {
-                xn = xs[k] + (uint32_t)__double2ll_rn(p * cd);
+                xn = xs[k] + fx_round_u32(p * cd);
             }
         }
         if (cd_on || ro.perm) Xo[k * np + w] = xn;
@@ -482,7 +669,7 @@ __global__ __launch_bounds__(256) void k_drift(typename PP::Pos* __restrict__ X,
     if constexpr (sizeof(typename PP::Mesh) == 8) {
         X[i] = X[i] + P[i] * cd;
     } else {
-        X[i] = X[i] + (uint32_t)__double2ll_rn(P[i] * cd);
+        X[i] = X[i] + fx_round_u32(P[i] * cd);
     }
 }
 
@@ -750,10 +937,15 @@ template void deposit_fx_brick<double>(int, const uint32_t*, const uint32_t*, co
 
 void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, const double* P,
                           const uint32_t* idi, uint32_t* Xo, double* Po, uint32_t* ido, long np, const float* phi,
-                          int lgN, const PosArgs& pa, double ck, double pq, double cd, int cd_on, hipStream_t st) {
-    const unsigned nb = (unsigned)brick_count(pa);
+                          int lgN, const PosArgs& pa, double ck, double pq, double cd, int cd_on, uint32_t* keys,
+                          uint32_t* in_count, uint32_t* out_count, hipStream_t st) {
+    // segments of up to 2^CPM_GATHER_LSEG bricks along the last axis
+    const int la = pa.lnb[R - 1];
+    const int lseg = CPM_GATHER_LSEG < la ? CPM_GATHER_LSEG : la;
+    const unsigned nb = (unsigned)(brick_count(pa) >> lseg);
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_kick_fx_brick<RR>), dim3(nb), dim3(FXG_NT), 0, st, bstart, perm,
-                                         X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck, pq, 1.0 / pq, cd, cd_on));
+                                         X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck * (1.0 / pq), pq, cd, cd_on, lseg,
+                                         keys, in_count, out_count));
     CPM_LAUNCH_CHECK();
 }
 

@@ -51,10 +51,16 @@ __device__ __forceinline__ float fd4(float fm2, float fm1, float fp1, float fp2,
 
 // FX kick increment F * ck rounded to a multiple of the momentum quantum pq
 // (a power of two).  FX momenta are kept on that grid, so P - dp and P + dp
-// are exact and a kick can be undone bit for bit.
-// ipq = 1 / pq is passed in, exact for a power of two.
-__host__ __device__ __forceinline__ double fx_kick_increment(double fck, double pq, double ipq) {
-    return rint(fck * ipq) * pq;
+// are exact and a kick can be undone bit for bit.  ckq = ck / pq: scaling
+// by a power of two is exact, so F * ckq is the rounded F * ck over pq.
+__host__ __device__ __forceinline__ double fx_kick_increment(double f, double ckq, double pq) {
+    return rint(f * ckq) * pq;
+}
+// Low 32 bits of rint(y) (round to nearest even), for |y| < 2^51: adding
+// 1.5 * 2^52 leaves rint(y) + 2^51 in the low mantissa bits.  The FX drift
+// adds this to a 32-bit position (y: the step in units of L / 2^32).
+__device__ __forceinline__ uint32_t fx_round_u32(double y) {
+    return (uint32_t)__double_as_longlong(y + 6755399441055744.0);
 }
 
 // ------------------------------------------------------------ position policies
@@ -136,13 +142,17 @@ long brick_count(const PosArgs& pa);
 // FX deposit / gather+kick(+drift) over particles grouped by brick (perm:
 // sorted position -> storage index; bstart: brick_count + 1 offsets).  The
 // deposit writes one FP64 partial sum per brick; MT is float (the step) or
-// double (the exact value of the fixed-point accumulators).
+// double (the exact value of the fixed-point accumulators).  The gather
+// writes particle s to position s; with keys != nullptr it also produces
+// what keys_fx would for the new positions against the keys of the old ones
+// (incremental re-sort: keys, in_count, out_count, all counters zeroed).
 template <typename MT>
 void deposit_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, long np, int lgN,
                       const PosArgs& pa, float mass, MT* mesh, double* partial, hipStream_t st);
 void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, const double* P,
                           const uint32_t* idi, uint32_t* Xo, double* Po, uint32_t* ido, long np, const float* phi,
-                          int lgN, const PosArgs& pa, double ck, double pq, double cd, int cd_on, hipStream_t st);
+                          int lgN, const PosArgs& pa, double ck, double pq, double cd, int cd_on, uint32_t* keys,
+                          uint32_t* in_count, uint32_t* out_count, hipStream_t st);
 // max |a[i]| (deterministic), and a[i] := rint(a[i] / q) * q.
 void reduce_max_abs(const double* a, long n, double* partials, long max_partials, double* out, hipStream_t st);
 void quantize(double* a, long n, double q, hipStream_t st);
