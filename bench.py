@@ -154,6 +154,7 @@ def diff_bench(name, mbits, f, ns, a0, da, nsteps, ckpt, repeats=2):
         assert bool(torch.isfinite(g).all())
         f_ms, b_ms = best["forward_ms"], best["backward_ms"]
         return {"workload": name, "particles": Np, "mesh": f << mbits, "steps": nsteps, "ckpt_every": ckpt,
+                "potentials_stored": True, "checkpoint_check": "checksum",
                 "forward_ms": round(f_ms, 3), "backward_ms": round(b_ms, 3),
                 "backward_over_forward": round(b_ms / f_ms, 3),
                 "grad_fwd_ratio": round((f_ms + b_ms) / f_ms, 3),

@@ -17,6 +17,8 @@ LIB_PATH = os.path.join(HERE, "libconanpm.so")
 PM_FP64 = 0
 PM_FP32 = 1
 PM_FLAG_WRAP_GATHER = 1
+PM_DIFF_STORE_POTENTIALS = 1
+PM_DIFF_VERIFY_CHECKPOINTS = 2
 PM_NPHASES = 10
 PHASE_NAMES = ["keys+sort", "deposit", "fft rows fwd", "fft y fwd", "green (z fwd*G*bwd)",
                "fft y bwd", "fft rows bwd", "gather+kick(+drift)", "drift", "nyquist-plane passes"]
@@ -57,7 +59,7 @@ class pm_diff_params(C.Structure):
     _fields_ = [("dim", C.c_int), ("mbits", C.c_int), ("f", C.c_int), ("L", C.c_double), ("ns", C.c_double),
                 ("smooth", C.c_int), ("scale", C.c_double), ("sigma", C.c_double), ("H0", C.c_double),
                 ("Omega_m", C.c_double), ("Omega_L", C.c_double), ("a0", C.c_double), ("da", C.c_double),
-                ("nsteps", C.c_int), ("ckpt_every", C.c_int), ("precision", C.c_int)]
+                ("nsteps", C.c_int), ("ckpt_every", C.c_int), ("precision", C.c_int), ("flags", C.c_int)]
 
 
 class pm_diff_stats(C.Structure):

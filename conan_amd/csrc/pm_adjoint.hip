@@ -21,6 +21,7 @@
 // is order independent, so it reproduces the forward's mesh bit for bit).
 #include "pm_adjoint.hpp"
 #include "pm_brick.hpp"
+#include "pm_sort.hpp"
 
 namespace cpm {
 
@@ -183,14 +184,15 @@ __global__ __launch_bounds__(256) void k_neg_div_f32(const float* __restrict__ s
 static constexpr int FXB_CTILE = (FXB_X + 1) * (FXB_Y + 1) * (FXB_Z + 1);
 
 template <int R>
-__global__ __launch_bounds__(FXG_NT) void k_adj_gather_fx_brick(
+__global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_gather_fx_brick(
     const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ Y,
     const double* __restrict__ P, const double* __restrict__ lY, const double* __restrict__ lP,
     const uint32_t* __restrict__ idi, uint32_t* __restrict__ Yo, double* __restrict__ Po,
     double* __restrict__ lYo, double* __restrict__ lPo, uint32_t* __restrict__ ido, long np,
-    const float* __restrict__ phi, const float* __restrict__ chi, int lgN, PosArgs pa, AdjCoef cf) {
+    const float* __restrict__ phi, const float* __restrict__ chi, int lgN, PosArgs pa, AdjCoef cf, AdjDrift dr) {
     __shared__ float tile[FXB_TILE];
     __shared__ float ctile[FXB_CTILE];
+    __shared__ float wmax[FXG_NT / 64];
     const double cFq = cf.cF * cf.ipq;  // the forward's ck / pq for cF = ck (exact: ipq is a power of two)
     const int N = 1 << lgN, msk = N - 1;
     const uint32_t bid = blockIdx.x;
@@ -199,26 +201,34 @@ __global__ __launch_bounds__(FXG_NT) void k_adj_gather_fx_brick(
     const int CX = bc.B[0] + 1, CY = bc.B[1] + 1, CZ = (R == 3) ? bc.B[2] + 1 : 1;
     const int tid = threadIdx.x;
     const bool hp = phi != nullptr, hc = chi != nullptr;
+    // A thread's first particle is loaded before the tiles are staged, so the
+    // latencies overlap; each later one at the end of the previous iteration.
+    const uint32_t s0 = bstart[bid << R], s1 = bstart[(bid + 1) << R];
+    uint32_t ys[3] = {0u, 0u, 0u}, idv = 0u, iv = 0u;
+    double lam[3] = {0, 0, 0};
+    auto fetch = [&](uint32_t sp) {
+        iv = perm[sp];
+        idv = idi[iv];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            ys[k] = Y[k * np + iv];
+            lam[k] = lP[k * np + iv];
+        }
+    };
+    if (s0 + tid < s1) fetch(s0 + tid);
     if (hp) stage_box<FXG_NT, FXB_TILE, R>(tile, phi, N, bc.x0, -2, TX, TY, TZ, tid);
     if (hc) stage_box<FXG_NT, FXB_CTILE, R>(ctile, chi, N, bc.x0, 0, CX, CY, CZ, tid);
     __syncthreads();
     const float inv12res = (float)(1.0 / (12.0 * pa.res));
     const double ires = 1.0 / pa.res;
     constexpr int NC = 1 << R;
-    const uint32_t s0 = bstart[bid << R], s1 = bstart[(bid + 1) << R];
+    float mx = 0.f;
     for (uint32_t s = s0 + tid; s < s1; s += FXG_NT) {
-        const uint32_t i = perm[s];
-        ido[s] = idi[i];
+        ido[s] = idv;
         int o[3] = {0, 0, 0};
         float a1[3] = {0.f, 0.f, 0.f};
-        uint32_t ys[3];
-        double lam[3] = {0, 0, 0};
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-            ys[k] = Y[k * np + i];
-            PosFX::gat(ys[k], pa, o[k], a1[k]);
-            lam[k] = lP[k * np + i];
-        }
+        for (int k = 0; k < R; ++k) PosFX::gat(ys[k], pa, o[k], a1[k]);
         const int lx = (o[0] - bc.x0[0]) & msk, ly = (o[1] - bc.x0[1]) & msk;
         const int lz = (R == 3) ? ((o[2] - bc.x0[2]) & msk) : 0;
         float z[NC];
@@ -306,14 +316,95 @@ __global__ __launch_bounds__(FXG_NT) void k_adj_gather_fx_brick(
                 for (int k = 0; k < R; ++k) U[k] += dz[k][j] * cv;
             }
         }
+        uint32_t yn[3] = {0u, 0u, 0u};
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            Yo[k * np + s] = ys[k];
-            Po[k * np + s] = P[k * np + i] + fx_kick_increment((double)F[k], cFq, cf.pq);
-            lYo[k * np + s] = lY[k * np + i] + cf.cT * T[k] + cf.cU * U[k];
-            lPo[k * np + s] = lam[k];
+            const double p = P[k * np + iv] + fx_kick_increment((double)F[k], cFq, cf.pq);
+            const double lyk = lY[k * np + iv] + cf.cT * T[k] + cf.cU * U[k];
+            double lpk = lam[k];
+            yn[k] = ys[k];
+            if (dr.on) {  // the previous step's drift, undone, and its adjoint
+                yn[k] -= fx_round_u32(p * dr.cd_fx);
+                lpk += dr.cd * lyk;
+            }
+            Yo[k * np + s] = yn[k];
+            Po[k * np + s] = p;
+            lYo[k * np + s] = lyk;
+            lPo[k * np + s] = lpk;
+            mx = fmaxf(mx, fabsf((float)lpk));
+        }
+        if (dr.keys) {
+            int co[3] = {0, 0, 0}, cn[3] = {0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                co[k] = (int)(ys[k] >> pa.fshift) & msk;
+                cn[k] = (int)(yn[k] >> pa.fshift) & msk;
+            }
+            const uint32_t ko = brick_key(co[0], co[1], co[2], R, pa), kn = brick_key(cn[0], cn[1], cn[2], R, pa);
+            dr.keys[s] = kn;
+            if (kn != ko) {
+                atomicAdd(&dr.in_count[kn], 1u);
+                atomicAdd(&dr.out_count[ko], 1u);
+            }
+        }
+        if (s + FXG_NT < s1) fetch(s + FXG_NT);
+    }
+    if (dr.bmax) {  // block max of |lP| out -> bmax[block]
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+        if ((tid & 63) == 0) wmax[tid >> 6] = mx;
+        __syncthreads();
+        if (tid == 0) {
+            float m = 0.f;
+#pragma unroll
+            for (int w = 0; w < FXG_NT / 64; ++w) m = fmaxf(m, wmax[w]);
+            dr.bmax[bid] = __float_as_uint(m);
         }
     }
+}
+
+// ------------------------------------------------------------ state checksum
+// splitmix64's output function (Steele, Lea, Flood 2014; constants of
+// Stafford's "Mix13"): a bijective 64-bit mixer.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ull;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// *sum += sum over particles of a hash of (id, positions, momentum bits):
+// integer addition, so the value does not depend on the storage order.  With
+// Xc != nullptr the state is also copied to (Xc, Pc, idc) in the same pass.
+template <int R>
+__global__ __launch_bounds__(256) void k_state_hash(const uint32_t* __restrict__ X, const double* __restrict__ P,
+                                                    const uint32_t* __restrict__ id, long np,
+                                                    uint32_t* __restrict__ Xc, double* __restrict__ Pc,
+                                                    uint32_t* __restrict__ idc, unsigned long long* __restrict__ sum) {
+    __shared__ unsigned long long red[4];
+    unsigned long long acc = 0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < np; i += (long)gridDim.x * 256) {
+        const uint32_t d = id[i];
+        unsigned long long h = mix64((unsigned long long)d + 0x9E3779B97F4A7C15ull);
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const uint32_t x = X[k * np + i];
+            const double pk = P[k * np + i];
+            h = mix64(h ^ x);
+            h = mix64(h ^ (unsigned long long)__double_as_longlong(pk));
+            if (Xc) {
+                Xc[k * np + i] = x;
+                Pc[k * np + i] = pk;
+            }
+        }
+        if (idc) idc[i] = d;
+        acc += h;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(sum, red[0] + red[1] + red[2] + red[3]);
 }
 
 // ------------------------------------------------------------ permutations by particle id
@@ -469,10 +560,25 @@ void neg_div_f32(int R, const float* s, int lgN, double res, float* psi, hipStre
 void adj_gather_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* Y, const double* P,
                          const double* lY, const double* lP, const uint32_t* idi, uint32_t* Yo, double* Po,
                          double* lYo, double* lPo, uint32_t* ido, long np, const float* phi, const float* chi,
-                         int lgN, const PosArgs& pa, AdjCoef c, hipStream_t st) {
+                         int lgN, const PosArgs& pa, AdjCoef c, AdjDrift dr, uint32_t* lmax, hipStream_t st) {
     const unsigned nb = (unsigned)brick_count(pa);
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_adj_gather_fx_brick<RR>), dim3(nb), dim3(FXG_NT), 0, st, bstart, perm,
-                                         Y, P, lY, lP, idi, Yo, Po, lYo, lPo, ido, np, phi, chi, lgN, pa, c));
+                                         Y, P, lY, lP, idi, Yo, Po, lYo, lPo, ido, np, phi, chi, lgN, pa, c, dr));
+    CPM_LAUNCH_CHECK();
+    if (dr.bmax) {
+        hipLaunchKernelGGL(k_max_u32, dim3(1), dim3(1024), 0, st, (const uint32_t*)dr.bmax, (long)nb, lmax);
+        CPM_LAUNCH_CHECK();
+    }
+}
+
+void state_hash(int R, const uint32_t* X, const double* P, const uint32_t* id, long np, uint32_t* Xc, double* Pc,
+                uint32_t* idc, unsigned long long* sum, hipStream_t st) {
+    zero_u32(reinterpret_cast<uint32_t*>(sum), 2, st);
+    long nb = (np + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    if (nb < 1) nb = 1;
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_state_hash<RR>), dim3((unsigned)nb), dim3(256), 0, st, X, P, id, np, Xc,
+                                         Pc, idc, sum));
     CPM_LAUNCH_CHECK();
 }
 

@@ -34,12 +34,29 @@ struct AdjCoef {
     double cF, cT, cU;
     double pq, ipq;  // momentum quantum of the forward kick and 1 / pq (fx_kick_increment)
 };
+// The previous step's reverse drift, fused into the adjoint gather's output
+// (what rev_drift_keys does, on the reconstructed momentum and the updated
+// lY): on != 0 undoes Y -= round(P cd_fx) and adds lP += cd lY.  keys !=
+// nullptr writes the incremental re-sort's keys and mover counts (in_count,
+// out_count zeroed) of the output positions against the input ones; bmax !=
+// nullptr takes max |lP| per workgroup, reduced into *lmax.
+struct AdjDrift {
+    double cd_fx, cd;
+    int on;
+    uint32_t *keys, *in_count, *out_count, *bmax;
+};
 // One workgroup per brick; reads particle perm[s] and writes it, its id and
 // its adjoints to position s.  phi == nullptr: F = T = 0; chi == nullptr: U = 0.
 void adj_gather_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* Y, const double* P,
                          const double* lY, const double* lP, const uint32_t* idi, uint32_t* Yo, double* Po,
                          double* lYo, double* lPo, uint32_t* ido, long np, const float* phi, const float* chi,
-                         int lgN, const PosArgs& pa, AdjCoef c, hipStream_t st);
+                         int lgN, const PosArgs& pa, AdjCoef c, AdjDrift dr, uint32_t* lmax, hipStream_t st);
+
+// *sum = sum over particles of a 64-bit hash of (id, X, P bits), independent
+// of the storage order; with Xc != nullptr the state is copied to (Xc, Pc,
+// idc) in the same pass.
+void state_hash(int R, const uint32_t* X, const double* P, const uint32_t* id, long np, uint32_t* Xc, double* Pc,
+                uint32_t* idc, unsigned long long* sum, hipStream_t st);
 
 // dst[k * np + id[s]] = src[k * np + s]  /  dst[k * np + s] = src[k * np + id[s]]
 template <typename T>

@@ -65,4 +65,14 @@ struct Error : std::runtime_error {
 
 #define CPM_LAUNCH_CHECK() CPM_HIP(hipGetLastError())
 
+// Compute units of the current device (cached per device), for grid sizing.
+inline long device_cus() {
+    static int cus[64] = {0};
+    int dev = 0;
+    CPM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) dev = 0;
+    if (!cus[dev]) CPM_HIP(hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev));
+    return cus[dev];
+}
+
 }  // namespace cpm

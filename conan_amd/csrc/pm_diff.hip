@@ -4,15 +4,17 @@
 //
 // Forward: the same device operations as pm_ic_scale_free + pm_init_zeldovich
 // + pm_steps + pm_deposit (so the density is bit-identical to that sequence),
-// storing the particle state (positions, momenta, ids) before every
-// ckpt_every-th kick.
+// storing the particle state (positions, momenta, ids) and its checksum
+// before every ckpt_every-th kick, and with PM_DIFF_STORE_POTENTIALS the
+// potential of every kick.
 //
 // Backward, from lY = m dW/dY . dL/drho at the final positions:
-//   for i = n-1 .. 0:  undo the drift (exact), adjoint of the drift, sort,
-//     deposit + Poisson (phi_i, bit-identical to the forward), deposit of the
-//     momentum adjoint, -div, Poisson (chi), fused adjoint gather that also
-//     undoes the kick; at a checkpoint, compare with the stored state and
-//     continue from it.
+//   for i = n-1 .. 0:  sort, phi_i (stored, or deposit + Poisson:
+//     bit-identical to the forward), deposit of the momentum adjoint, -div,
+//     Poisson (chi), fused adjoint gather that undoes kick i and, between
+//     checkpoints, the drift before it (exact) with the drift's adjoint; at a
+//     checkpoint, compare with the stored state (checksum, or value by value
+//     with PM_DIFF_VERIFY_CHECKPOINTS) and continue from it.
 //   then the Zel'dovich start and the IC field in FP64 on the mass grid:
 //     X = q - a0 u, P = -a_mom u, u = D phi_IC / res  (src/run/nbody.hh:40-50)
 //     phi_IC = Pi delta, Pi = IFFT[-1/k^2 FFT] / n       (src/ic/ic.cc:100-117)
@@ -41,6 +43,9 @@ struct DiffPM {
     int nck;
     std::vector<std::unique_ptr<DevBuf<uint32_t>>> ckX, ckid;
     std::vector<std::unique_ptr<DevBuf<double>>> ckP;
+    DevBuf<unsigned long long> ckhash, hbuf;        // checkpoint checksums, scratch
+    std::vector<std::unique_ptr<DevBuf<float>>> phis;  // PM_DIFF_STORE_POTENTIALS: phi of every kick
+    bool store_phi, verify;
     DevBuf<double> lY, lP, lY2, lP2;
     DevBuf<uint32_t> tmpY, bmax, lmax;
     DevBuf<float> svec, psi, chi, grho;
@@ -65,7 +70,8 @@ struct DiffPM {
 
     explicit DiffPM(const pm_diff_params& q)
         : p(q), R(q.dim), lgNm(q.mbits), Nm(1 << q.mbits), Np(1L << (q.dim * q.mbits)), sim(sim_config(q)),
-          st(sim.st), icfft(q.dim, 1 << q.mbits), nck((q.nsteps + q.ckpt_every - 1) / q.ckpt_every) {
+          st(sim.st), icfft(q.dim, 1 << q.mbits), nck((q.nsteps + q.ckpt_every - 1) / q.ckpt_every),
+          store_phi(q.flags & PM_DIFF_STORE_POTENTIALS), verify(q.flags & PM_DIFF_VERIFY_CHECKPOINTS) {
         size_t f0 = 0, t0 = 0, f1 = 0;
         CPM_HIP(hipMemGetInfo(&f0, &t0));
         noise.alloc(Np);
@@ -82,12 +88,16 @@ struct DiffPM {
             ckP.emplace_back(new DevBuf<double>((size_t)R * Np));
             ckid.emplace_back(new DevBuf<uint32_t>(Np));
         }
+        ckhash.alloc(nck);
+        hbuf.alloc(1);
+        if (store_phi)
+            for (int i = 0; i < q.nsteps; ++i) phis.emplace_back(new DevBuf<float>(sim.M));
         lY.alloc((size_t)R * Np);
         lP.alloc((size_t)R * Np);
         lY2.alloc((size_t)R * Np);
         lP2.alloc((size_t)R * Np);
         tmpY.alloc((size_t)R * Np);
-        bmax.alloc((Np + 255) / 256);
+        bmax.alloc(std::max((Np + 255) / 256, brick_count(sim.pa)));
         lmax.alloc(1);
         svec.alloc((size_t)R * sim.M);
         psi.alloc(sim.M);
@@ -173,15 +183,15 @@ struct DiffPM {
         sim.have_state = true;
         ParticleOps<PosFX>::drift(sim.X.p, sim.P.p, (long)R * Np, sim.drift_factor(a_seq[0], p.da), st);
         for (int i = 0; i < p.nsteps; ++i) {
-            if (i % p.ckpt_every == 0) {
+            if (i % p.ckpt_every == 0) {  // copy and checksum in one pass
                 const int c = i / p.ckpt_every;
-                CPM_HIP(hipMemcpyAsync(ckX[c]->p, sim.X.p, sizeof(uint32_t) * R * Np, hipMemcpyDeviceToDevice, st));
-                CPM_HIP(hipMemcpyAsync(ckP[c]->p, sim.P.p, sizeof(double) * R * Np, hipMemcpyDeviceToDevice, st));
-                CPM_HIP(hipMemcpyAsync(ckid[c]->p, sim.id.p, sizeof(uint32_t) * Np, hipMemcpyDeviceToDevice, st));
+                state_hash(R, sim.X.p, sim.P.p, sim.id.p, Np, ckX[c]->p, ckP[c]->p, ckid[c]->p, ckhash.p + c, st);
             }
             const bool more = i + 1 < p.nsteps;
+            sim.phi_out = store_phi ? phis[i]->p : nullptr;
             sim.kick_impl(a_seq[i] + p.da / 2, p.da, more, a_seq[i + 1], p.da);
         }
+        sim.phi_out = nullptr;
         // final density: the deposit of pm_deposit, with the fixed-point
         // accumulators converted to FP64 instead of rounded to FP32
         sort_particles();
@@ -219,16 +229,41 @@ struct DiffPM {
         std::swap(lP.p, lP2.p);
     }
 
-    void adj_gather(const float* phi, const float* chiv, AdjCoef c) {
+    // Drift factor of the adjoint, lP += cd lY, for the drift at a (lY in
+    // 1/length): da / (a^2 adot), src/run/nbody.hh:124,129.
+    double drift_adj(double a) const { return p.da / (a * a * sim.cos.adot(a)); }
+
+    // Adjoint gather (undoes the kick).  fuse: also undo the drift before it
+    // (cd_fx, cd; both 0 when there is none) and produce the next sort's keys
+    // and max |lP| (into lmax).
+    void adj_gather(const float* phi, const float* chiv, AdjCoef c, bool fuse, double cd_fx, double cd) {
+        AdjDrift dr{0.0, 0.0, 0, nullptr, nullptr, nullptr, nullptr};
+        if (fuse) {
+            zero_u32(sim.cnt3.p, 3 * sim.nsort_groups, st);
+            dr = AdjDrift{cd_fx, cd, (cd_fx != 0.0 || cd != 0.0) ? 1 : 0, sim.knew.p, sim.cnt3.p,
+                          sim.cnt3.p + sim.nsort_groups, bmax.p};
+        }
         adj_gather_fx_brick(R, sim.bstart, sim.spids, sim.X.p, sim.P.p, lY.p, lP.p, sim.id.p, sim.X2.p, sim.P2.p,
-                            lY2.p, lP2.p, sim.id2.p, Np, phi, chiv, sim.lgNf, sim.pa, c, st);
+                            lY2.p, lP2.p, sim.id2.p, Np, phi, chiv, sim.lgNf, sim.pa, c, dr, lmax.p, st);
         swap_all();
         sim.resort_ok = true;  // the particles now sit in the order of the last sort
+        sim.keys_ready = fuse;
     }
 
-    // Compare the reversed state with checkpoint c and continue from the
-    // checkpoint: the adjoints are moved into its particle order via the ids.
+    // Compare the reversed state with checkpoint c.  Without verify, equal
+    // checksums end it: the state is the checkpoint's up to the storage
+    // order, which nothing downstream depends on.  Otherwise (or on a
+    // mismatch) compare value by value and continue from the checkpoint: the
+    // adjoints are moved into its particle order via the ids.
     void restore(int c) {
+        if (!verify) {
+            state_hash(R, sim.X.p, sim.P.p, sim.id.p, Np, nullptr, nullptr, nullptr, hbuf.p, st);
+            unsigned long long hv[2] = {0, 1};
+            CPM_HIP(hipMemcpyAsync(&hv[0], hbuf.p, sizeof(hv[0]), hipMemcpyDeviceToHost, st));
+            CPM_HIP(hipMemcpyAsync(&hv[1], ckhash.p + c, sizeof(hv[1]), hipMemcpyDeviceToHost, st));
+            CPM_HIP(hipStreamSynchronize(st));
+            if (hv[0] == hv[1]) return;
+        }
         const DevBuf<uint32_t>& cX = *ckX[c];
         const DevBuf<double>& cP = *ckP[c];
         const DevBuf<uint32_t>& cid = *ckid[c];
@@ -281,28 +316,45 @@ struct DiffPM {
         CPM_HIP(hipMemsetAsync(lY.p, 0, sizeof(double) * R * Np, st));
         CPM_HIP(hipMemsetAsync(lP.p, 0, sizeof(double) * R * Np, st));
         const double m = sim.mass;
-        // final density: rho = m sum W(Y, c)  ->  lY = m dW/dY . grad_rho
-        sort_particles();
-        adj_gather(nullptr, grho.p, AdjCoef{0.0, 0.0, m, sim.pq, 1.0 / sim.pq});
+        // final density: rho = m sum W(Y, c)  ->  lY = m dW/dY . grad_rho;
+        // the last kick has no drift after it: only the keys and max |lP|.
+        // The forward's final sort still describes the particles.
+        adj_gather(nullptr, grho.p, AdjCoef{0.0, 0.0, m, sim.pq, 1.0 / sim.pq}, true, 0.0, 0.0);
+        bool drifted = true;  // the drift after kick i is undone (keys, lmax ready)
         for (int i = p.nsteps - 1; i >= 0; --i) {
-            const bool more = i + 1 < p.nsteps;
-            const double cd_fx = more ? sim.drift_factor(a_seq[i + 1], p.da) : 0.0;
-            const double cd = more ? p.da / (a_seq[i + 1] * a_seq[i + 1] * sim.cos.adot(a_seq[i + 1])) : 0.0;
             const double ak = a_seq[i] + p.da / 2;
             const double ck = p.da / sim.cos.adot(ak);  // src/run/nbody.hh:112-116
-            const Impl<PosFX>::FxKeyArgs ka = sim.fx_key_args();
-            rev_drift_keys(R, sim.X.p, sim.P.p, lY.p, lP.p, Np, cd_fx, cd, sim.lgNf, sim.pa, ka.prevkeys, ka.keys,
-                           ka.pids, ka.in_count, ka.out_count, bmax.p, lmax.p, st);
-            sim.fx_sort();
-            deposit_fx_brick<float>(R, sim.bstart, sim.spids, sim.X.p, Np, sim.lgNf, sim.pa, (float)m, sim.mesh.p,
-                             sim.row_partial.p, st);
-            sim.npartials = brick_count(sim.pa);
-            sim.do_poisson(ak);  // phi_i, as in the forward
+            if (!drifted) {
+                // the drift after kick i (at a_{i+1}, fused into kick i forward)
+                const Impl<PosFX>::FxKeyArgs ka = sim.fx_key_args();
+                rev_drift_keys(R, sim.X.p, sim.P.p, lY.p, lP.p, Np, sim.drift_factor(a_seq[i + 1], p.da),
+                               drift_adj(a_seq[i + 1]), sim.lgNf, sim.pa, ka.prevkeys, ka.keys, ka.pids,
+                               ka.in_count, ka.out_count, bmax.p, lmax.p, st);
+            }
+            sim.fx_sort_current();
+            const float* phi = nullptr;
+            if (store_phi) {
+                phi = phis[i]->p;
+            } else {
+                deposit_fx_brick<float>(R, sim.bstart, sim.spids, sim.X.p, Np, sim.lgNf, sim.pa, (float)m,
+                                        sim.mesh.p, sim.row_partial.p, st);
+                sim.npartials = brick_count(sim.pa);
+                sim.do_poisson(ak);  // phi_i, as in the forward
+                phi = sim.mesh.p;
+            }
             deposit_fx_vec(R, sim.bstart, sim.spids, sim.X.p, lP.p, Np, sim.lgNf, sim.pa, lmax.p, svec.p, st);
             neg_div_f32(R, svec.p, sim.lgNf, sim.pa.res, psi.p, st);
             poisson_into(psi.p, chi.p, ak);
-            adj_gather(sim.mesh.p, chi.p, AdjCoef{ck, -ck, -ck * m, sim.pq, 1.0 / sim.pq});
-            if (i % p.ckpt_every == 0) restore(i / p.ckpt_every);
+            const AdjCoef coef{ck, -ck, -ck * m, sim.pq, 1.0 / sim.pq};
+            if (i % p.ckpt_every == 0) {
+                // the checkpoint holds the state before kick i: compare first
+                adj_gather(phi, chi.p, coef, false, 0.0, 0.0);
+                restore(i / p.ckpt_every);
+                drifted = false;
+            } else {
+                adj_gather(phi, chi.p, coef, true, sim.drift_factor(a_seq[i], p.da), drift_adj(a_seq[i]));
+                drifted = true;
+            }
         }
         sim.have_deposit = false;
         // Zel'dovich start: Y0 = X0 + cd0 P0, X0 = q - a0 u, P0 = -a_mom u
@@ -376,6 +428,8 @@ int pm_diff_create(const pm_diff_params* p, pm_diff** out) {
         throw cpm::Error(PM_EUNSUP, "f must be a power of two with f * 2^mbits <= 1024");
     if (p->nsteps < 1 || p->ckpt_every < 1) throw cpm::Error(PM_EINVAL, "nsteps and ckpt_every must be >= 1");
     if (p->precision != PM_FP32) throw cpm::Error(PM_EUNSUP, "the adjoint is implemented for the FP32 form");
+    if (p->flags & ~(PM_DIFF_STORE_POTENTIALS | PM_DIFF_VERIFY_CHECKPOINTS))
+        throw cpm::Error(PM_EINVAL, "unknown pm_diff flags");
     if (!(p->L > 0) || !(p->a0 > 0) || !(p->da > 0) || !(p->sigma > 0) || !(p->H0 > 0))
         throw cpm::Error(PM_EINVAL, "L, a0, da, sigma and H0 must be positive");
     int ndev = 0;

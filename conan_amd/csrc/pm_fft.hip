@@ -558,14 +558,6 @@ void launch_rows_c2r(int N, const cx<T>* spec, const cx<T>* nyq, T* out, long nr
 
 // Workgroups of a Green's-function launch: one per tile, or for the fused
 // (persistent) kernel as many as the GPU holds at once.
-static long resident_cus() {
-    static int cus[64] = {0};
-    int dev = 0;
-    CPM_HIP(hipGetDevice(&dev));
-    if (dev < 0 || dev >= 64) dev = 0;
-    if (!cus[dev]) CPM_HIP(hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev));
-    return cus[dev];
-}
 template <typename T, int N>
 dim3 col_grid(long ntiles) {
     if (!Fused<N>::ok) return dim3((unsigned)ntiles);
@@ -575,7 +567,7 @@ dim3 col_grid(long ntiles) {
     const long by_threads = 2048 / C::NT;
     if (per_cu > by_threads) per_cu = by_threads;
     if (per_cu < 1) per_cu = 1;
-    const long g = resident_cus() * per_cu;
+    const long g = device_cus() * per_cu;
     return dim3((unsigned)(ntiles < g ? ntiles : g));
 }
 

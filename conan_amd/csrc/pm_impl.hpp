@@ -180,6 +180,7 @@ struct Impl : ImplBase {
     long npartials = 0;     // FP64 mesh-sum partials written by the last deposit
     long nsort_groups = 0;  // rows (FP64) or bricks (FX) delimited by row_start
     double pq = 0;          // FX: momentum quantum (a power of two), see fx_quantize_momenta
+    Mesh* phi_out = nullptr;  // when set, the potential of a kick goes here instead of into mesh
 
     static int fx_keybits(int dim, int lgN) {
         PosArgs t{};
@@ -404,7 +405,8 @@ struct Impl : ImplBase {
         have_deposit = true;
     }
 
-    // mesh (rho) -> mesh (phi) at scale factor a.
+    Mesh* phi_dst() { return phi_out ? phi_out : mesh.p; }
+    // mesh (rho) -> phi_dst() (phi) at scale factor a.
     // FX: scal[0] must hold the FP64 mean of the mesh (mean_ready), or the
     // row partials of the last deposit provide it.
     void do_poisson(double a, bool mean_ready = false) {
@@ -419,7 +421,7 @@ struct Impl : ImplBase {
         }
         for (int stage = 0; stage < 5; ++stage) {
             prof.begin(2 + stage, st);
-            fft.poisson_stage(stage, 0, mesh.p, mesh.p, spec.p, nyq.p, shift, mean_ptr, kf, s, st);
+            fft.poisson_stage(stage, 0, mesh.p, phi_dst(), spec.p, nyq.p, shift, mean_ptr, kf, s, st);
             prof.end(2 + stage, st);
         }
     }
@@ -447,7 +449,7 @@ struct Impl : ImplBase {
             if (fuse) zero_u32(cnt3.p, 3 * nsort_groups, st);
             gather_kick_fx_brick(R, bstart, spids, reinterpret_cast<const uint32_t*>(X.p), P.p, id.p,
                                  reinterpret_cast<uint32_t*>(X2.p), P2.p, id2.p, Np,
-                                 reinterpret_cast<const float*>(mesh.p), lgNf, pa, ck, pq, cd, fuse ? 1 : 0,
+                                 reinterpret_cast<const float*>(phi_dst()), lgNf, pa, ck, pq, cd, fuse ? 1 : 0,
                                  fuse ? knew.p : nullptr, cnt3.p, cnt3.p + nsort_groups, st);
             std::swap(X.p, X2.p);
             std::swap(P.p, P2.p);
@@ -455,7 +457,7 @@ struct Impl : ImplBase {
             resort_ok = true;
             keys_ready = fuse;
         } else {
-            ParticleOps<PP>::gather_kick(R, X.p, P.p, Np, mesh.p, lgNf, pa, ck, cd, fuse ? 1 : 0, nullptr,
+            ParticleOps<PP>::gather_kick(R, X.p, P.p, Np, phi_dst(), lgNf, pa, ck, cd, fuse ? 1 : 0, nullptr,
                                          Reorder<PP>(), st);
         }
         prof.end(7, st);

@@ -939,10 +939,12 @@ void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, c
                           const uint32_t* idi, uint32_t* Xo, double* Po, uint32_t* ido, long np, const float* phi,
                           int lgN, const PosArgs& pa, double ck, double pq, double cd, int cd_on, uint32_t* keys,
                           uint32_t* in_count, uint32_t* out_count, hipStream_t st) {
-    // segments of up to 2^CPM_GATHER_LSEG bricks along the last axis
-    const int la = pa.lnb[R - 1];
-    const int lseg = CPM_GATHER_LSEG < la ? CPM_GATHER_LSEG : la;
-    const unsigned nb = (unsigned)(brick_count(pa) >> lseg);
+    // segments of up to 2^CPM_GATHER_LSEG bricks along the last axis, as long
+    // as that leaves at least 4 workgroups per compute unit
+    const long nbr = brick_count(pa), minwg = 4 * device_cus();
+    int lseg = 0;
+    while (lseg < CPM_GATHER_LSEG && lseg < pa.lnb[R - 1] && (nbr >> (lseg + 1)) >= minwg) ++lseg;
+    const unsigned nb = (unsigned)(nbr >> lseg);
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_kick_fx_brick<RR>), dim3(nb), dim3(FXG_NT), 0, st, bstart, perm,
                                          X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck * (1.0 / pq), pq, cd, cd_on, lseg,
                                          keys, in_count, out_count));

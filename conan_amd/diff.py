@@ -33,15 +33,18 @@ class DiffPM:
 
     def __init__(self, dim: int, mbits: int, f: int, L: float, ns: float, a0: float, da: float, nsteps: int,
                  ckpt_every: int = 5, sigma: float = 0.83, scale: float = 8.0, smooth: bool = False,
-                 H0: float = 68.5, Omega_m: float = 1.0, Omega_L: float = 0.0):
+                 H0: float = 68.5, Omega_m: float = 1.0, Omega_L: float = 0.0, store_potentials: bool = True,
+                 verify_checkpoints: bool = False):
         lib = _lib.load()
         self.dim, self.mbits, self.f, self.L = dim, mbits, f, L
         self.N = 1 << mbits
         self.Nf = f * self.N
         self.Np = self.N ** dim
         self.M = self.Nf ** dim
+        flags = (_lib.PM_DIFF_STORE_POTENTIALS if store_potentials else 0) | \
+            (_lib.PM_DIFF_VERIFY_CHECKPOINTS if verify_checkpoints else 0)
         p = _lib.pm_diff_params(dim, mbits, f, L, ns, 1 if smooth else 0, scale, sigma, H0, Omega_m, Omega_L,
-                                a0, da, nsteps, ckpt_every, PM_FP32)
+                                a0, da, nsteps, ckpt_every, PM_FP32, flags)
         h = C.c_void_p()
         _check(lib.pm_diff_create(C.byref(p), C.byref(h)), None)
         self._h = h

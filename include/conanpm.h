@@ -217,7 +217,18 @@ typedef struct pm_diff_params {
     int nsteps;       /* leapfrog steps */
     int ckpt_every;   /* checkpoint interval in steps (>= 1) */
     int precision;    /* PM_FP32 (the only form with an adjoint) */
+    int flags;        /* PM_DIFF_* below, or 0 */
 } pm_diff_params;
+
+/* pm_diff_params.flags */
+/* Keep the potential of every step from the forward pass (nsteps * Nf^dim
+   floats of device memory) so that the backward pass does not recompute it. */
+#define PM_DIFF_STORE_POTENTIALS 1
+/* At every checkpoint, compare the reversed state with the stored one value
+   by value (pm_diff_stats.recon_*) and continue from the stored state.
+   Without it the states are compared by an order-independent 64-bit
+   checksum, and only a mismatch triggers the full comparison and restore. */
+#define PM_DIFF_VERIFY_CHECKPOINTS 2
 
 typedef struct pm_diff pm_diff;
 

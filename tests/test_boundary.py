@@ -85,14 +85,14 @@ def _dparams(**kw):
     from conan_amd import _lib
 
     c = dict(dim=3, mbits=4, f=2, L=100.0, ns=-2.0, smooth=0, scale=8.0, sigma=0.83, H0=68.5, Omega_m=1.0,
-             Omega_L=0.0, a0=0.05, da=0.05, nsteps=4, ckpt_every=2, precision=1)
+             Omega_L=0.0, a0=0.05, da=0.05, nsteps=4, ckpt_every=2, precision=1, flags=0)
     c.update(kw)
     return _lib.pm_diff_params(*[c[k] for k, _ in _lib.pm_diff_params._fields_])
 
 
 @pytest.mark.parametrize("kw,code", [({"dim": 4}, -2), ({"mbits": 11}, -2), ({"f": 3}, -2), ({"f": 128}, -2),
                                      ({"precision": 0}, -2), ({"nsteps": 0}, -1), ({"ckpt_every": 0}, -1),
-                                     ({"da": 0.0}, -1), ({"sigma": -1.0}, -1)])
+                                     ({"da": 0.0}, -1), ({"sigma": -1.0}, -1), ({"flags": 4}, -1)])
 def test_diff_create_rejects_bad_params(kw, code):
     import ctypes
 

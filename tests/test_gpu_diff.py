@@ -42,7 +42,8 @@ def _loss_and_grad(eng, w, target):
 @pytest.mark.parametrize("k", [1, 3, 10])
 def test_reverse_leapfrog_reconstructs_checkpoints(orc, k):
     dim, mbits, f, n = 3, 5, 1, 10
-    eng = _engine(dim, mbits, f, -2.0, 0.05, 0.095, n, k)
+    # value-by-value comparison with every checkpoint
+    eng = _engine(dim, mbits, f, -2.0, 0.05, 0.095, n, k, verify_checkpoints=True)
     w = orc.white_noise(3, (1 << mbits) ** dim)
     target = np.ones((f << mbits) ** dim)
     _, gw = _loss_and_grad(eng, w, target)
@@ -53,6 +54,26 @@ def test_reverse_leapfrog_reconstructs_checkpoints(orc, k):
     assert s["recon_pos"] == 0
     assert s["recon_mom"] == 0.0
     assert np.all(np.isfinite(gw))
+
+
+@pytest.mark.parametrize("dim,mbits,f,n,k", [(3, 5, 1, 9, 4), (2, 6, 2, 7, 3)])
+def test_backward_modes_give_the_same_gradient(orc, dim, mbits, f, n, k):
+    """Stored or recomputed potentials, checksum or value-by-value checkpoint
+    checks: the same numbers (the latter restores the checkpoint's storage
+    order, which no result depends on)."""
+    w = orc.white_noise(5, (1 << mbits) ** dim)
+    target = np.ones((f << mbits) ** dim)
+    out = []
+    for store in (True, False):
+        for verify in (False, True):
+            eng = _engine(dim, mbits, f, -2.0, 0.05, 0.1, n, k, store_potentials=store, verify_checkpoints=verify)
+            lv, g = _loss_and_grad(eng, w, target)
+            s = eng.stats()
+            assert s["recon_pos"] == 0 and s["recon_mom"] == 0.0
+            out.append((lv, g))
+    for lv, g in out[1:]:
+        assert lv == out[0][0]
+        assert np.array_equal(g, out[0][1])
 
 
 def test_checkpoint_interval_does_not_change_the_gradient(orc):

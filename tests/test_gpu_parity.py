@@ -228,6 +228,28 @@ def test_fp32_form_tracks_fp64(pm, orc):
     assert np.sqrt(np.mean((r32 - r64) ** 2)) < 1e-3
 
 
+@pytest.mark.parametrize("dim,Nm,Nf", [(3, 128, 256), (3, 64, 256)])
+def test_fp32_kick_matches_fp64_on_large_meshes(pm, orc, dim, Nm, Nf):
+    """On meshes with enough bricks, a gather workgroup walks several bricks
+    along the last axis and keeps their potential planes in an LDS ring; the
+    FP32 kicks must then still match the FP64 form's (folded gather) to FP32
+    accuracy: a misplaced plane would show as an O(1) error."""
+    L, a0, da, n = 100.0, 0.02, 0.0098, 3
+    mbits = int(np.log2(Nm))
+    noise = orc.white_noise(41, Nm ** dim)
+    phi = orc.compute_potential(dim, mbits, L, orc.ic_scale_free(dim, mbits, L, -2.0, noise))
+    say("fp32 vs fp64 kicks", dim, Nm, Nf)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        with pm.Gravity(dim, Nm, Nf, L, precision=prec, wrap_gather=True) as g:
+            g.init(phi, a0, a0 + da / 2)
+            _, P0 = g.get_state()
+            g.steps(a0, da, n)
+            _, P1 = g.get_state()
+            out[prec] = P1 - P0
+    assert rel(out["fp32"], out["fp64"]) < 1e-4
+
+
 def test_cic_tool(pm, orc):
     dim, Nm, Nf, L = 3, 16, 32, 100.0
     X = _positions(dim, Nm, L, 1.0, special=False)
