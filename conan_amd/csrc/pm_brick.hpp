@@ -11,7 +11,7 @@
 
 namespace cpm {
 
-static constexpr int FXB_NT = 256;  // deposit workgroup size
+static constexpr int FXB_NT = 512;  // deposit workgroup size
 #ifndef CPM_FXG_NT
 #define CPM_FXG_NT 512
 #endif

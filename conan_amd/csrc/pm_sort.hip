@@ -277,10 +277,12 @@ __global__ __launch_bounds__(256) void k_resort_counts(const uint32_t* __restric
 }
 
 // One workgroup per brick: the brick's previous range (its G groups, one
-// after the other) is walked in chunks of RS_NT; a block scan of the stayer
-// flags (wave ballots) gives each stayer its rank among the stayers of its
-// group; a mover takes the next free slot after the stayers of its new group.
+// after the other) is walked in passes of RS_E x RS_NT elements, all loads of
+// a pass in flight together; a block scan of the stayer flags (wave ballots)
+// gives each stayer its rank among the stayers of its group (they keep their
+// order); a mover takes the next free slot after the stayers of its new group.
 static constexpr int RS_NT = 256;
+static constexpr int RS_E = 4;
 __global__ __launch_bounds__(RS_NT) void k_resort_place(const uint32_t* __restrict__ keys,
                                                         const uint32_t* __restrict__ prevkeys,
                                                         const uint32_t* __restrict__ bprev,
@@ -289,9 +291,11 @@ __global__ __launch_bounds__(RS_NT) void k_resort_place(const uint32_t* __restri
                                                         int G, uint32_t* __restrict__ perm,
                                                         uint32_t* __restrict__ skeys) {
     __shared__ uint32_t before[8];  // stayers of the brick's earlier groups
-    __shared__ uint32_t wsum[RS_NT / 64];
+    __shared__ uint32_t nstart[8];  // new starts of the brick's groups
+    __shared__ uint32_t wsum[RS_E][RS_NT / 64];
     const uint32_t g0 = blockIdx.x * (uint32_t)G;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x < (unsigned)G) nstart[threadIdx.x] = bnew[g0 + threadIdx.x];
     if (threadIdx.x == 0) {
         uint32_t a = 0;
         for (int c = 0; c < G; ++c) {
@@ -302,35 +306,47 @@ __global__ __launch_bounds__(RS_NT) void k_resort_place(const uint32_t* __restri
     const uint32_t r0 = bprev[g0], r1 = bprev[g0 + G];
     __syncthreads();
     uint32_t carry = 0;
-    for (uint32_t base = r0; base < r1; base += RS_NT) {
-        const uint32_t s = base + threadIdx.x;
-        const bool valid = s < r1;
-        uint32_t key = 0, pk = 0;
-        if (valid) {
-            key = keys[s];
-            pk = prevkeys[s];
-        }
-        const bool st = valid && key == pk;
-        const uint64_t m = __ballot(st);
-        const uint32_t lanepre = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        if (lane == 0) wsum[wave] = (uint32_t)__popcll(m);
-        __syncthreads();
-        uint32_t wpre = 0, total = 0;
+    for (uint32_t base = r0; base < r1; base += RS_E * RS_NT) {
+        uint32_t key[RS_E], pk[RS_E];
 #pragma unroll
-        for (int w = 0; w < RS_NT / 64; ++w) {
-            const uint32_t c = wsum[w];
-            wpre += (w < wave) ? c : 0u;
-            total += c;
+        for (int e = 0; e < RS_E; ++e) {
+            const uint32_t s = base + e * RS_NT + threadIdx.x;
+            key[e] = s < r1 ? keys[s] : 0u;
+            pk[e] = s < r1 ? prevkeys[s] : 1u;
         }
-        if (valid) {
-            uint32_t pos;
-            if (st) pos = bnew[pk] + (carry + wpre + lanepre - before[pk - g0]);
-            else pos = bnew[key] + stay[key] + atomicAdd(&fill[key], 1u);
-            perm[pos] = s;
-            skeys[pos] = key;
+        uint32_t lanepre[RS_E];
+#pragma unroll
+        for (int e = 0; e < RS_E; ++e) {
+            const bool st = (base + e * RS_NT + threadIdx.x < r1) && key[e] == pk[e];
+            const uint64_t m = __ballot(st);
+            lanepre[e] = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (lane == 0) wsum[e][wave] = (uint32_t)__popcll(m);
         }
-        carry += total;
-        __syncthreads();  // wsum is rewritten by the next chunk
+        __syncthreads();
+        uint32_t pre = carry;  // stayers of the brick before sub-chunk e, wave `wave`
+#pragma unroll
+        for (int e = 0; e < RS_E; ++e) {
+            const uint32_t s = base + e * RS_NT + threadIdx.x;
+            uint32_t wpre = 0, total = 0;
+#pragma unroll
+            for (int w = 0; w < RS_NT / 64; ++w) {
+                const uint32_t c = wsum[e][w];
+                wpre += (w < wave) ? c : 0u;
+                total += c;
+            }
+            if (s < r1) {
+                uint32_t pos;
+                if (key[e] == pk[e])
+                    pos = nstart[pk[e] - g0] + (pre + wpre + lanepre[e] - before[pk[e] - g0]);
+                else
+                    pos = bnew[key[e]] + stay[key[e]] + atomicAdd(&fill[key[e]], 1u);
+                perm[pos] = s;
+                skeys[pos] = key[e];
+            }
+            pre += total;
+        }
+        carry = pre;
+        __syncthreads();  // wsum is rewritten by the next pass
     }
 }
 
