@@ -207,10 +207,14 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_c2r(
 #ifndef CPM_COL_LDS
 #define CPM_COL_LDS 65536
 #endif
-template <typename T, int N>
+#ifndef CPM_GREEN_C
+#define CPM_GREEN_C 16
+#endif
+// CP: pencils per tile (0: the default for T)
+template <typename T, int N, int CP = 0>
 struct ColCfg {
     static constexpr int CS = (int)sizeof(cx<T>);
-    static constexpr int Ct = sizeof(T) == 4 ? CPM_COL_F32 : CPM_COL_F64;
+    static constexpr int Ct = CP ? CP : (sizeof(T) == 4 ? CPM_COL_F32 : CPM_COL_F64);
     static constexpr int C = (Ct * N * CS > CPM_COL_LDS) ? (CPM_COL_LDS / (N * CS)) : Ct;
     static constexpr int NT = N >= 8 ? ((C * N / 8 < CPM_COL_NT) ? C * N / 8 : CPM_COL_NT) : C;
     static constexpr int E = C * N / NT;
@@ -251,18 +255,18 @@ __device__ __forceinline__ void col_store(const cx<T>* sh, cx<T>* __restrict__ d
 // The tile of workgroup blockIdx.x of a launch over two parts (the main
 // block of the half spectrum and the kx = N/2 plane): part, batch base and
 // first pencil.
-template <typename T, int N>
+template <typename T, int N, int CP = 0>
 struct ColTile {
     const ColPart<T>* part;
     long bbase, c0;
 };
-template <typename T, int N>
-__device__ __forceinline__ ColTile<T, N> col_tile_at(const ColPart<T>& pa, const ColPart<T>& pb, uint32_t w) {
+template <typename T, int N, int CP = 0>
+__device__ __forceinline__ ColTile<T, N, CP> col_tile_at(const ColPart<T>& pa, const ColPart<T>& pb, uint32_t w) {
     const bool second = w >= (uint32_t)pa.nblk;
     const ColPart<T>& p = second ? pb : pa;
     if (second) w -= (uint32_t)pa.nblk;
     const uint32_t gx = (uint32_t)p.gx;
-    return ColTile<T, N>{&p, (long)(w / gx) * p.g.bs, (long)(w % gx) * ColCfg<T, N>::C};
+    return ColTile<T, N, CP>{&p, (long)(w / gx) * p.g.bs, (long)(w % gx) * ColCfg<T, N, CP>::C};
 }
 template <typename T, int N>
 __device__ __forceinline__ ColTile<T, N> col_tile(const ColPart<T>& pa, const ColPart<T>& pb) {
@@ -285,9 +289,9 @@ struct Fused {
 };
 
 // HBM loads of the first-stage inputs of thread (p, q) of tile t.
-template <typename T, int N>
-__device__ __forceinline__ void col_first_load(const ColTile<T, N>& t, int tid, cx<T>* v) {
-    using C = ColCfg<T, N>;
+template <typename T, int N, int CP = 0>
+__device__ __forceinline__ void col_first_load(const ColTile<T, N, CP>& t, int tid, cx<T>* v) {
+    using C = ColCfg<T, N, CP>;
     static_assert(C::NT * 8 == C::C * N, "one radix-8 butterfly per thread and stage");
     constexpr int R0 = Fused<N>::R0, NB = 8 / R0;
     const ColPart<T>& P = *t.part;
@@ -303,9 +307,9 @@ __device__ __forceinline__ void col_first_load(const ColTile<T, N>& t, int tid, 
 }
 
 // The first stage (no twiddles) on those values, written to LDS.
-template <typename T, int N, int DIR>
+template <typename T, int N, int DIR, int CP = 0>
 __device__ __forceinline__ void col_first_store(cx<T>* sh, cx<T>* v, int tid) {
-    using C = ColCfg<T, N>;
+    using C = ColCfg<T, N, CP>;
     constexpr int R0 = Fused<N>::R0, NB = 8 / R0;
     const int p = tid % C::C, q = tid / C::C;
 #pragma unroll
@@ -320,26 +324,27 @@ __device__ __forceinline__ void col_first_store(cx<T>* sh, cx<T>* v, int tid) {
 // Persistent loop over the tiles of a column launch (fused path): while a
 // tile is transformed, the HBM loads of the workgroup's next tile are in
 // flight.  body(tile, v) gets the first-stage inputs in v.
-template <typename T, int N, typename Body>
+template <typename T, int N, int CP, typename Body>
 __device__ __forceinline__ void col_pipeline(const ColPart<T>& pa, const ColPart<T>& pb, int tid, Body body) {
     const uint32_t ntiles = (uint32_t)(pa.nblk + pb.nblk);
     uint32_t w = blockIdx.x;
     cx<T> nv[8];
-    if (w < ntiles) col_first_load<T, N>(col_tile_at<T, N>(pa, pb, w), tid, nv);
+    if (w < ntiles) col_first_load<T, N, CP>(col_tile_at<T, N, CP>(pa, pb, w), tid, nv);
     for (; w < ntiles; w += gridDim.x) {
-        const ColTile<T, N> t = col_tile_at<T, N>(pa, pb, w);
+        const ColTile<T, N, CP> t = col_tile_at<T, N, CP>(pa, pb, w);
         cx<T> v[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) v[i] = nv[i];
-        if (w + gridDim.x < ntiles) col_first_load<T, N>(col_tile_at<T, N>(pa, pb, w + gridDim.x), tid, nv);
+        if (w + gridDim.x < ntiles)
+            col_first_load<T, N, CP>(col_tile_at<T, N, CP>(pa, pb, w + gridDim.x), tid, nv);
         body(t, v);
         __syncthreads();  // LDS is refilled by the next tile
     }
 }
 
-template <typename T, int N, int DIR>
+template <typename T, int N, int DIR, int CP = 0>
 __device__ __forceinline__ void col_middle(cx<T>* sh, const cx<T>* __restrict__ tw, int tid) {
-    using C = ColCfg<T, N>;
+    using C = ColCfg<T, N, CP>;
     int Ns = Fused<N>::R0;
 #pragma unroll
     for (int s = 0; s < Fused<N>::mid; ++s) {
@@ -349,19 +354,19 @@ __device__ __forceinline__ void col_middle(cx<T>* sh, const cx<T>* __restrict__ 
 }
 
 // The last stage of thread (p, j) in registers: v[t] = element j + t N/8.
-template <typename T, int N, int DIR>
+template <typename T, int N, int DIR, int CP = 0>
 __device__ __forceinline__ void col_last_regs(const cx<T>* sh, const cx<T>* __restrict__ tw, int tid, cx<T>* v) {
-    using C = ColCfg<T, N>;
+    using C = ColCfg<T, N, CP>;
     const int p = tid % C::C, j = tid / C::C;
 #pragma unroll
     for (int t = 0; t < 8; ++t) v[t] = sh[lidx<N>(p, j + t * (N / 8))];
     butterfly<T, N, 8, DIR>(v, j, N / 8, tw, 1);
 }
 
-template <typename T, int N>
+template <typename T, int N, int CP = 0>
 __device__ __forceinline__ void col_store_regs(const cx<T>* v, cx<T>* __restrict__ data, const ColGeom& g, long bbase,
                                                long c0, int tid) {
-    using C = ColCfg<T, N>;
+    using C = ColCfg<T, N, CP>;
     const int p = tid % C::C, j = tid / C::C;
     const long col = c0 + p;
     if (col >= g.nc) return;
@@ -427,9 +432,10 @@ __device__ __forceinline__ T green_value(int ix, int iy, int iz, int dim, int N,
 // Last-axis forward transform, multiplication by the Green's function and
 // last-axis backward transform, in one read and one write of the spectrum.
 template <typename T, int N>
-__global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols_green(ColPart<T> pa, ColPart<T> pb,
+__global__ __launch_bounds__((ColCfg<T, N, CPM_GREEN_C>::NT)) void k_cols_green(ColPart<T> pa, ColPart<T> pb,
                                                                  const cx<T>* __restrict__ tw) {
-    using C = ColCfg<T, N>;
+    constexpr int CP = CPM_GREEN_C;
+    using C = ColCfg<T, N, CP>;
     __shared__ cx<T> sh[C::C * N];
     __shared__ cx<T> stw[N];
     const int tid = threadIdx.x;
@@ -440,13 +446,13 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols_green(ColPart<T> pa
         // registers; the Green's function on the registers; backward: first
         // stage in registers (when it is radix 8 it reads exactly these
         // positions), middle stages, last stage on store.
-        col_pipeline<T, N>(pa, pb, tid, [&](const ColTile<T, N>& t, cx<T>* v) {
+        col_pipeline<T, N, CP>(pa, pb, tid, [&](const ColTile<T, N, CP>& t, cx<T>* v) {
         const GreenGeom gg = t.part->gg;
         const T kf = (T)gg.kf;
-        col_first_store<T, N, -1>(sh, v, tid);
+        col_first_store<T, N, -1, CP>(sh, v, tid);
         __syncthreads();
-        col_middle<T, N, -1>(sh, stw, tid);
-        col_last_regs<T, N, -1>(sh, stw, tid, v);
+        col_middle<T, N, -1, CP>(sh, stw, tid);
+        col_last_regs<T, N, -1, CP>(sh, stw, tid, v);
         const int pc = tid % C::C, j = tid / C::C;
         const long col = t.c0 + pc;
 #pragma unroll
@@ -476,13 +482,13 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols_green(ColPart<T> pa
             __syncthreads();
             lds_stage<T, N, C::E, C::NT, +1, Fused<N>::R0>(sh, 1, stw, 1, tid);
         }
-        col_middle<T, N, +1>(sh, stw, tid);
-        col_last_regs<T, N, +1>(sh, stw, tid, v);
-        col_store_regs<T, N>(v, t.part->data, t.part->g, t.bbase, t.c0, tid);
+        col_middle<T, N, +1, CP>(sh, stw, tid);
+        col_last_regs<T, N, +1, CP>(sh, stw, tid, v);
+        col_store_regs<T, N, CP>(v, t.part->data, t.part->g, t.bbase, t.c0, tid);
         });
         return;
     }
-    const ColTile<T, N> t = col_tile<T, N>(pa, pb);
+    const ColTile<T, N, CP> t = col_tile_at<T, N, CP>(pa, pb, blockIdx.x);
     const ColPart<T>& p = *t.part;
     const GreenGeom gg = p.gg;
     const T kf = (T)gg.kf;
@@ -558,15 +564,15 @@ void launch_rows_c2r(int N, const cx<T>* spec, const cx<T>* nyq, T* out, long nr
 
 // Workgroups of a Green's-function launch: one per tile, or for the fused
 // (persistent) kernel as many as the GPU holds at once.
-template <typename T, int N>
+template <typename T, int N, int CP>
 dim3 col_grid(long ntiles) {
     if (!Fused<N>::ok) return dim3((unsigned)ntiles);
-    using C = ColCfg<T, N>;
-    const long lds = (long)(C::C + 1) * N * (long)sizeof(cx<T>);
-    long per_cu = 163840 / lds;
-    const long by_threads = 2048 / C::NT;
-    if (per_cu > by_threads) per_cu = by_threads;
-    if (per_cu < 1) per_cu = 1;
+    static int per_cu = 0;  // workgroups of this kernel one CU holds (registers, LDS, threads)
+    if (!per_cu) {
+        CPM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)&k_cols_green<T, N>,
+                                                             ColCfg<T, N, CP>::NT, 0));
+        if (per_cu < 1) per_cu = 1;
+    }
     const long g = device_cus() * per_cu;
     return dim3((unsigned)(ntiles < g ? ntiles : g));
 }
@@ -599,8 +605,8 @@ void launch_cols_green(int N, const ColPart<T>& a, const ColPart<T>& b, const cx
     switch (N) {
 #define CASE(n)                                                                                \
     case n: {                                                                                  \
-        using C = ColCfg<T, n>;                                                                \
-        const dim3 grid = col_grid<T, n>(ntiles);                                              \
+        using C = ColCfg<T, n, CPM_GREEN_C>;                                                   \
+        const dim3 grid = col_grid<T, n, CPM_GREEN_C>(ntiles);                                 \
         hipLaunchKernelGGL((k_cols_green<T, n>), grid, dim3(C::NT), 0, st, a, b, tw);          \
         break;                                                                                 \
     }
@@ -613,17 +619,18 @@ void launch_cols_green(int N, const ColPart<T>& a, const ColPart<T>& b, const cx
 }
 
 // One part of a column launch: gx workgroups per batch, gx * nb in all.
-template <typename T, int N>
+template <typename T, int N, int CP>
 ColPart<T> col_part_n(cx<T>* d, const ColGeom& g, const GreenGeom& gg) {
-    const long gx = (g.nc + ColCfg<T, N>::C - 1) / ColCfg<T, N>::C;
+    const long gx = (g.nc + ColCfg<T, N, CP>::C - 1) / ColCfg<T, N, CP>::C;
     return ColPart<T>{d, g, gg, gx, gx * g.nb};
 }
+// green: a part of the Green's-function pass (its own tile width)
 template <typename T>
-ColPart<T> col_part(int N, cx<T>* d, const ColGeom& g, const GreenGeom& gg = GreenGeom{}) {
+ColPart<T> col_part(int N, cx<T>* d, const ColGeom& g, const GreenGeom& gg = GreenGeom{}, bool green = false) {
     switch (N) {
 #define CASE(n) \
     case n:     \
-        return col_part_n<T, n>(d, g, gg);
+        return green ? col_part_n<T, n, CPM_GREEN_C>(d, g, gg) : col_part_n<T, n, 0>(d, g, gg);
         CPM_FOR_N(CASE)
 #undef CASE
         default:
@@ -708,8 +715,8 @@ void FFTPlan<T>::poisson_stage(int stage, int part, const T* rho, T* phi, cx<T>*
             break;
         case 2:
             if (part == 0)
-                launch_cols_green<T>(N, col_part(N, spec, dim == 3 ? zm : ym, gmain),
-                                     col_part(N, nyq, dim == 3 ? zn : yn, gnyq), tw, st);
+                launch_cols_green<T>(N, col_part(N, spec, dim == 3 ? zm : ym, gmain, true),
+                                     col_part(N, nyq, dim == 3 ? zn : yn, gnyq, true), tw, st);
             break;
         case 3:
             if (dim == 3 && part == 0) launch_cols<T>(N, +1, col_part(N, spec, ym), col_part(N, nyq, yn), tw, st);
