@@ -105,50 +105,67 @@ struct BrickRanges {
     int n;
 };
 
-// Fills `r` (in LDS) for brick bc; called by the whole workgroup.  The first
-// wave resolves every (neighbour, class) pair at once (lane = neighbour x
-// class) and compacts the non-empty ones with a ballot, so the workgroup
-// waits for one round of offset loads.  Ends with a barrier.
+// Lane `lane` of one wave resolves one (neighbour, class) pair of brick bc:
+// whether it contributes (ok) and its sorted range [s0, s1).
+template <int R>
+__device__ __forceinline__ void brick_range_lane(const uint32_t* __restrict__ bstart, const BrickCoord& bc,
+                                                 const PosArgs& pa, int lane, bool& ok, uint32_t& s0,
+                                                 uint32_t& s1) {
+    const int nbr = lane >> R, c = lane & ((1 << R) - 1);
+    const int d[3] = {nbr & 1, (nbr >> 1) & 1, (nbr >> 2) & 1};
+    ok = nbr < (1 << R);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ok &= !(d[k] && bc.nb[k] == 1);  // that brick would be this one
+    ok &= (nbr == 0) ? (c == 0) : ((c & nbr) == nbr);
+    s0 = s1 = 0;
+    if (ok) {
+        uint32_t nb = 0;
+        int sh = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            nb |= (uint32_t)((bc.bi[k] - d[k]) & (bc.nb[k] - 1)) << sh;
+            sh += pa.lnb[k];
+        }
+        const uint32_t g0 = (nb << R) + (nbr == 0 ? 0 : c);
+        s0 = bstart[g0];
+        s1 = bstart[nbr == 0 ? g0 + (1u << R) : g0 + 1];
+    }
+}
+// One whole wave compacts its lanes' non-empty ranges into r (a ballot for
+// the slots, a shuffle scan for the offsets).  No barrier: other waves read
+// r after the caller's next one.
+__device__ __forceinline__ void brick_ranges_store(BrickRanges& r, bool ok, uint32_t s0, uint32_t s1, int lane) {
+    const bool have = ok && s1 > s0;
+    const uint64_t m = __ballot(have);
+    const int slot = __popcll(m & ((1ull << lane) - 1ull));
+    const uint32_t len = have ? s1 - s0 : 0u;
+    uint32_t incl = len;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off);
+        if (lane >= off) incl += t;
+    }
+    if (have) {
+        r.rs0[slot] = s0;
+        r.rlen[slot] = len;
+        r.roff[slot] = incl - len;
+    }
+    if (lane == 63) {
+        r.n = (int)__popcll(m);
+        r.roff[__popcll(m)] = incl;
+    }
+}
+
+// Fills `r` (in LDS) for brick bc; called by the whole workgroup (the first
+// wave resolves every (neighbour, class) pair at once).  Ends with a barrier.
 template <int R>
 __device__ __forceinline__ void brick_ranges(BrickRanges& r, const uint32_t* __restrict__ bstart,
                                              const BrickCoord& bc, const PosArgs& pa, int tid) {
     if (tid < 64) {
-        const int nbr = tid >> R, c = tid & ((1 << R) - 1);
-        const int d[3] = {nbr & 1, (nbr >> 1) & 1, (nbr >> 2) & 1};
-        bool ok = nbr < (1 << R);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) ok &= !(d[k] && bc.nb[k] == 1);  // that brick would be this one
-        ok &= (nbr == 0) ? (c == 0) : ((c & nbr) == nbr);
-        uint32_t s0 = 0, s1 = 0;
-        if (ok) {
-            uint32_t nb = 0;
-            int sh = 0;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                nb |= (uint32_t)((bc.bi[k] - d[k]) & (bc.nb[k] - 1)) << sh;
-                sh += pa.lnb[k];
-            }
-            const uint32_t g0 = (nb << R) + (nbr == 0 ? 0 : c);
-            s0 = bstart[g0];
-            s1 = bstart[nbr == 0 ? g0 + (1u << R) : g0 + 1];
-        }
-        const bool have = ok && s1 > s0;
-        const uint64_t m = __ballot(have);
-        if (have) {
-            const int slot = __popcll(m & ((tid == 0) ? 0ull : (~0ull >> (64 - tid))));
-            r.rs0[slot] = s0;
-            r.rlen[slot] = s1 - s0;
-        }
-        if (tid == 0) r.n = __popcll(m);
-    }
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t a = 0;
-        for (int q = 0; q < r.n; ++q) {
-            r.roff[q] = a;
-            a += r.rlen[q];
-        }
-        r.roff[r.n] = a;
+        bool ok;
+        uint32_t s0, s1;
+        brick_range_lane<R>(bstart, bc, pa, tid, ok, s0, s1);
+        brick_ranges_store(r, ok, s0, s1, tid);
     }
     __syncthreads();
 }

@@ -157,38 +157,86 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_c2r(
         }
     }
     __syncthreads();
-    cx<T> z[E];
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-        const int e = tid + i * NT;
-        const int p = e / H, k = e % H;
+    // Merge of the half spectrum into the H-point complex input of the
+    // inverse transform: z_k = (X_k + X*_{H-k}) + i conj(W_N^k) (X_k - X*_{H-k}).
+    auto merged = [&](int p, int k) -> cx<T> {
         const cx<T> Xk = sh[lidx<H>(p, k)];
         const cx<T> Xm = conj(k == 0 ? shn[p] : sh[lidx<H>(p, H - k)]);
         const cx<T> Ep = Xk + Xm;
         const cx<T> wO = conj(stw[k]) * (Xk - Xm);
-        z[i] = mk<T>(Ep.re - wO.im, Ep.im + wO.re);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-        const int e = tid + i * NT;
-        sh[lidx<H>(e / H, e % H)] = z[i];
-    }
-    __syncthreads();
-    lds_fft<T, H, E, NT, +1>(sh, stw, 2, tid);
+        return mk<T>(Ep.re - wO.im, Ep.im + wO.re);
+    };
     const T inv = (T)(1.0 / s);
-    for (int e = tid; e < P * H; e += NT) {
-        const int p = e / H, n = e % H;
-        const long row = row0 + p;
-        if (row >= nrows) continue;
-        const cx<T> v = sh[lidx<H>(p, n)];
-        cx<T> o;
+    auto scaled = [&](const cx<T>& v) -> cx<T> {
         if constexpr (sizeof(T) == 8) {
-            o = mk<T>(v.re / (T)s, v.im / (T)s);  // real_part(s): Re(z) / s (fourier.hh:31-37)
+            return mk<T>(v.re / (T)s, v.im / (T)s);  // real_part(s): Re(z) / s (fourier.hh:31-37)
         } else {
-            o = mk<T>(v.re * inv, v.im * inv);
+            return mk<T>(v.re * inv, v.im * inv);
         }
-        *reinterpret_cast<cx<T>*>(out + row * N + 2 * n) = o;
+    };
+    using S = Sched<H>;
+    constexpr int R0 = S::lead ? S::lead : 8;                // radix of the first stage
+    constexpr int NMID = S::n8 - (S::lead ? 1 : 2);          // radix-8 stages between first and last
+    if constexpr (NMID >= 0 && E % R0 == 0 && E % 8 == 0) {
+        // the merge fused into the first stage (its inputs read from the
+        // loaded spectrum), the last stage stored straight to HBM
+        cx<T> v[E];
+#pragma unroll
+        for (int i = 0; i < E / R0; ++i) {
+            const int b = tid + i * NT, p = b / (H / R0), j = b % (H / R0);
+#pragma unroll
+            for (int t = 0; t < R0; ++t) v[i * R0 + t] = merged(p, j + t * (H / R0));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < E / R0; ++i) {
+            const int b = tid + i * NT, p = b / (H / R0), j = b % (H / R0);
+            butterfly<T, H, R0, +1>(&v[i * R0], j, 1, stw, 2);
+#pragma unroll
+            for (int t = 0; t < R0; ++t) sh[lidx<H>(p, stock_out<H, R0>(j, 1, t))] = v[i * R0 + t];
+        }
+        __syncthreads();
+        int Ns = R0;
+#pragma unroll
+        for (int m = 0; m < NMID; ++m) {
+            lds_stage<T, H, E, NT, +1, 8>(sh, Ns, stw, 2, tid);
+            Ns *= 8;
+        }
+#pragma unroll
+        for (int i = 0; i < E / 8; ++i) {
+            const int b = tid + i * NT, p = b / (H / 8), j = b % (H / 8);
+            cx<T>* w = &v[i * 8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) w[t] = sh[lidx<H>(p, j + t * (H / 8))];
+            butterfly<T, H, 8, +1>(w, j, H / 8, stw, 2);
+            const long row = row0 + p;
+            if (row < nrows) {
+#pragma unroll
+                for (int t = 0; t < 8; ++t)
+                    *reinterpret_cast<cx<T>*>(out + row * N + 2 * stock_out<H, 8>(j, H / 8, t)) = scaled(w[t]);
+            }
+        }
+    } else {
+        cx<T> z[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int e = tid + i * NT;
+            z[i] = merged(e / H, e % H);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int e = tid + i * NT;
+            sh[lidx<H>(e / H, e % H)] = z[i];
+        }
+        __syncthreads();
+        lds_fft<T, H, E, NT, +1>(sh, stw, 2, tid);
+        for (int e = tid; e < P * H; e += NT) {
+            const int p = e / H, n = e % H;
+            const long row = row0 + p;
+            if (row >= nrows) continue;
+            *reinterpret_cast<cx<T>*>(out + row * N + 2 * n) = scaled(sh[lidx<H>(p, n)]);
+        }
     }
 }
 

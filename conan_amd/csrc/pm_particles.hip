@@ -67,6 +67,21 @@ __global__ __launch_bounds__(256) void k_row_bounds(const uint32_t* __restrict__
     for (long r = rp + 1; r <= rc; ++r) row_start[r] = (uint32_t)s;
 }
 
+// Sum of v over the NT threads of a workgroup, returned to thread 0, in a
+// fixed order (wave trees, then the waves in order); red: NT / 64 LDS words.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red, int tid) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    double t = 0;
+    if (tid == 0)
+#pragma unroll
+        for (int w = 0; w < NT / 64; ++w) t += red[w];
+    return t;
+}
+
 // ------------------------------------------------------------ deposit
 // One workgroup per mesh row.  Cell c receives, from each origin cell
 // c - block[j], the weight m * prod_k A[bit_k(j)][k] of every particle there
@@ -87,7 +102,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit(const uint32_t* __restrict__
     constexpr int NL = 1 << (R - 1);
     constexpr int NC = 1 << R;
     __shared__ uint32_t cs[NL][1025];
-    __shared__ double red[DEP_NT];
+    __shared__ double red[DEP_NT / 64];
     const int N = 1 << lgN, msk = N - 1;
     const long row = blockIdx.x;
     const int cy = (int)(row & msk);
@@ -146,13 +161,8 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit(const uint32_t* __restrict__
         mesh[row * N + cx] = acc;
         rsum += (double)acc;
     }
-    red[tid] = rsum;
-    __syncthreads();
-    for (int off = DEP_NT / 2; off > 0; off >>= 1) {
-        if (tid < off) red[tid] += red[tid + off];
-        __syncthreads();
-    }
-    if (tid == 0) row_partial[row] = red[0];
+    const double tot = block_sum<DEP_NT>(rsum, red, tid);
+    if (tid == 0) row_partial[row] = tot;
 }
 
 // ------------------------------------------------------------ FX: persistent brick order
@@ -175,7 +185,7 @@ __global__ __launch_bounds__(FXB_NT) void k_deposit_fx_brick(const uint32_t* __r
                                                              PosArgs pa, float mass, MT* __restrict__ mesh,
                                                              double* __restrict__ partial) {
     __shared__ unsigned long long acc[FXB_CELLS];
-    __shared__ double red[FXB_NT];
+    __shared__ double red[FXB_NT / 64];
     const int N = 1 << lgN, msk = N - 1;
     const uint32_t bid = blockIdx.x;
     const BrickCoord bc = brick_coord(bid, pa);
@@ -236,13 +246,8 @@ __global__ __launch_bounds__(FXB_NT) void k_deposit_fx_brick(const uint32_t* __r
         mesh[idx] = v;
         rsum += (double)v;
     }
-    red[tid] = rsum;
-    __syncthreads();
-    for (int off = FXB_NT / 2; off > 0; off >>= 1) {
-        if (tid < off) red[tid] += red[tid + off];
-        __syncthreads();
-    }
-    if (tid == 0) partial[bid] = red[0];
+    const double tot = block_sum<FXB_NT>(rsum, red, tid);
+    if (tid == 0) partial[bid] = tot;
 }
 
 // Gather + kick (+ drift).  A workgroup takes a segment of 2^lseg bricks
