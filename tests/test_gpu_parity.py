@@ -228,7 +228,7 @@ def test_fp32_form_tracks_fp64(pm, orc):
     assert np.sqrt(np.mean((r32 - r64) ** 2)) < 1e-3
 
 
-@pytest.mark.parametrize("dim,Nm,Nf", [(3, 128, 256), (3, 64, 256)])
+@pytest.mark.parametrize("dim,Nm,Nf", [(3, 128, 256), (3, 64, 256), (3, 128, 512)])
 def test_fp32_kick_matches_fp64_on_large_meshes(pm, orc, dim, Nm, Nf):
     """On meshes with enough bricks, a gather workgroup walks several bricks
     along the last axis and keeps their potential planes in an LDS ring; the
