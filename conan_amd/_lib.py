@@ -26,7 +26,8 @@ PHASE_NAMES = ["keys+sort", "deposit", "fft rows fwd", "fft y fwd", "green (z fw
 # Every exported entry point of include/conanpm.h.
 EXPORTS = [
     "pm_last_error", "pm_create", "pm_destroy", "pm_init_zeldovich", "pm_drift", "pm_kick",
-    "pm_steps", "pm_get_state", "pm_set_state", "pm_get_times", "pm_deposit", "pm_potential",
+    "pm_steps", "pm_get_state", "pm_set_state", "pm_get_times", "pm_get_momentum_quantum",
+    "pm_set_momentum_quantum", "pm_deposit", "pm_potential",
     "pm_forces", "pm_gather_mesh", "pm_poisson_mesh", "pm_power_spectrum", "pm_diagnostics", "pm_profile", "pm_sync",
     "pm_ic_scale_free", "pm_ic_cdm", "pm_ic_potential", "pm_ic_displacement", "pm_fft_r2c", "pm_fft_c2r",
     "pm_cic", "pm_diff_create", "pm_diff_destroy", "pm_diff_last_error", "pm_diff_forward", "pm_diff_backward",
@@ -99,6 +100,8 @@ def load() -> C.CDLL:
         "pm_get_state": (C.c_int, [V, _dp, _dp]),
         "pm_set_state": (C.c_int, [V, _dp, _dp]),
         "pm_get_times": (C.c_int, [V, _dp, _dp]),
+        "pm_get_momentum_quantum": (C.c_int, [V, _dp]),
+        "pm_set_momentum_quantum": (C.c_int, [V, C.c_double]),
         "pm_deposit": (C.c_int, [V, _dp, C.POINTER(C.c_int32)]),
         "pm_potential": (C.c_int, [V, C.c_double, _dp]),
         "pm_forces": (C.c_int, [V, C.c_double, _dp]),

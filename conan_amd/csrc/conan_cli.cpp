@@ -7,6 +7,7 @@
 //   ic  --noise FILE     read the white noise from record "noise" of FILE
 //   run --precision P    fp64 (reference arithmetic, default) or fp32
 //   run --wrap-gather    fold positions into the box before the gather
+//   run --resume FILE    continue from snapshot FILE (<id>.pos.<k>.conan)
 #include <cstring>
 
 #include "conan_format.hpp"
@@ -175,6 +176,13 @@ struct Snapshotter {
             S["dim"] = fmt(dim);
             S["a-mom"] = fmt(amom);
             S["a-pos"] = fmt(apos);
+            double pq = 0;
+            pm_ok(pm_get_momentum_quantum(h, &pq), h);
+            if (pq > 0) {  // FP32 form: the momentum grid, exactly (a power of two)
+                int e = 0;
+                (void)std::frexp(pq, &e);
+                S["momentum-quantum-log2"] = fmt(e - 1);
+            }
             History Hs;
             Hs.update("ran nbody");
             S.to_file(fo);
@@ -211,7 +219,10 @@ void cmd_run(int argc, char** argv) {
           "positions, FP32 mesh, FFT and gather, FP64 momenta)."},
          {0, "", "wrap-gather", "false",
           "fold positions into the box before the force interpolation (the reference truncates the "
-          "unwrapped coordinate)."}});
+          "unwrapped coordinate)."},
+         {O::VALUED | O::CHECK, "", "resume", "",
+          "continue from snapshot FILE (<id>.pos.<k>.conan of a run with the same parameters): its "
+          "positions, momenta and momentum grid are loaded and the run goes on after step 10 k."}});
     if (C.get<bool>("help")) help_and_exit(C);
 
     std::ifstream fi(C["id"] + ".density.init.conan", std::ios::binary);
@@ -252,9 +263,41 @@ void cmd_run(int argc, char** argv) {
 
     // LeapFrog::run (src/run/leapfrog.hh:33-46); kicks between snapshots run
     // as one fused sequence.
-    pm_ok(pm_init_zeldovich(h, phi.data(), a0, a0 + da / 2), h);
     double a = a0;
     unsigned i = 0;
+    const std::string resume = H["resume"];
+    if (resume.empty()) {
+        pm_ok(pm_init_zeldovich(h, phi.data(), a0, a0 + da / 2), h);
+    } else {
+        // snapshot k holds the state after step 10 k (save_snapshot, src/run/nbody.hh:186-208)
+        const size_t at = resume.rfind(".pos."), dot = resume.rfind(".conan");
+        if (at == std::string::npos || dot == std::string::npos || dot <= at + 5)
+            throw Failure{"--resume needs a snapshot named <id>.pos.<k>.conan"};
+        const unsigned k = parse<unsigned>(resume.substr(at + 5, dot - at - 5));
+        std::ifstream rf(resume, std::ios::binary);
+        if (!rf) throw Failure{"Could not read from file"};
+        Header RH(rf);
+        History RI(rf);
+        if (RH.get<unsigned>("N") != N || RH.get<int>("dim") != dim || RH.get<double>("L") != parse<double>(fmt(L)))
+            throw Failure{"the snapshot does not match this run's N, dim and L"};
+        std::vector<double> X = load_array<double>(rf, "pos"), P = load_array<double>(rf, "mom");
+        if (X.size() != np * dim || P.size() != np * dim) throw Failure{"snapshot records have the wrong size"};
+        if (k == 0 || 10 * k > n) throw Failure{"the snapshot index is outside this run"};
+        double alast = a0;  // a of the last step in the snapshot, accumulated as LeapFrog::run does
+        for (i = 0; i < 10 * k; ++i) {
+            alast = a;
+            a += da;
+        }
+        if (std::fabs(RH.get<double>("a-pos") - alast) > 1e-5 * std::fabs(alast))
+            throw Failure{"the snapshot's a-pos does not belong to this run's a0 and da"};
+        if (RH.has("momentum-quantum-log2")) {  // before the state: pm_set_state keeps it
+            if (prec != "fp32") throw Failure{"the snapshot was written by the fp32 form"};
+            pm_ok(pm_set_momentum_quantum(h, std::ldexp(1.0, RH.get<int>("momentum-quantum-log2"))), h);
+        } else if (prec == "fp32") {
+            throw Failure{"the snapshot has no momentum grid (not written by the fp32 form)"};
+        }
+        pm_ok(pm_set_state(h, X.data(), P.data()), h);
+    }
     while (i < n) {
         unsigned chunk = std::min<unsigned>(n - i, 10 - (i % 10));
         pm_ok(pm_steps(h, a, da, (int)chunk, &a), h);

@@ -156,6 +156,19 @@ int pm_get_times(const pm_handle* h, double* a_pos, double* a_mom) {
     return PM_OK;
 }
 
+int pm_get_momentum_quantum(const pm_handle* h, double* pq) {
+    if (!h || !pq) return PM_EINVAL;
+    *pq = h->impl->momentum_quantum();
+    return PM_OK;
+}
+
+int pm_set_momentum_quantum(pm_handle* h, double pq) {
+    PM_NEED(h)
+    PM_TRY
+    h->impl->set_momentum_quantum(pq);
+    PM_CATCH(h)
+}
+
 int pm_deposit(pm_handle* h, double* rho, int32_t* keys) {
     PM_NEED(h)
     PM_TRY

@@ -138,6 +138,8 @@ struct ImplBase {
     virtual void gather_mesh(const double* phi, double* F) = 0;
     virtual void power_spectrum(int nbins, const double* edges, double* k, double* pk, double* cnt) = 0;
     virtual void diagnostics(pm_diag* d) = 0;
+    virtual double momentum_quantum() const = 0;
+    virtual void set_momentum_quantum(double pq) = 0;
     double last_apos = 0, last_avel = 0;
     bool have_state = false;
     Prof prof;
@@ -180,6 +182,7 @@ struct Impl : ImplBase {
     long npartials = 0;     // FP64 mesh-sum partials written by the last deposit
     long nsort_groups = 0;  // rows (FP64) or bricks (FX) delimited by row_start
     double pq = 0;          // FX: momentum quantum (a power of two), see fx_quantize_momenta
+    bool pq_pinned = false; // FX: pq was set by the caller (kept by set_state)
     Mesh* phi_out = nullptr;  // when set, the potential of a kick goes here instead of into mesh
 
     static int fx_keybits(int dim, int lgN) {
@@ -291,6 +294,11 @@ struct Impl : ImplBase {
     // the FP32 force by far.
     void fx_quantize_momenta() {
         if constexpr (FX) {
+            if (pq_pinned) {
+                quantize(P.p, (long)R * Np, pq, st);
+                CPM_HIP(hipStreamSynchronize(st));
+                return;
+            }
             reduce_max_abs(P.p, (long)R * Np, red.p, 4096, scal.p + 8, st);
             double mx = 0;
             CPM_HIP(hipMemcpyAsync(&mx, scal.p + 8, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -307,6 +315,26 @@ struct Impl : ImplBase {
         }
     }
 
+    double momentum_quantum() const override { return FX ? pq : 0.0; }
+    // Pin the FX momentum grid (a power of two): set_state keeps it instead
+    // of choosing one from the momenta, until init_zeldovich.  A run that
+    // resumes from a snapshot sets it first and continues on its grid.
+    void set_momentum_quantum(double q) override {
+        if constexpr (FX) {
+            int e = 0;
+            if (!(q > 0) || !std::isfinite(q) || frexp(q, &e) != 0.5)
+                throw Error(PM_EINVAL, "the momentum quantum must be a positive power of two");
+            pq = q;
+            pq_pinned = true;
+            if (have_state) {
+                quantize(P.p, (long)R * Np, pq, st);
+                CPM_HIP(hipStreamSynchronize(st));
+            }
+        } else {
+            if (q != 0.0) throw Error(PM_EUNSUP, "the FP64 form keeps plain double momenta (quantum 0)");
+        }
+    }
+
     void init_zeldovich(const double* phi, double a_pos, double a_mom) override {
         const long npm = Np;
         if (cfg.Nm < 4 || (cfg.Nm & (cfg.Nm - 1))) throw Error(PM_EUNSUP, "Zel'dovich start needs a power-of-two Nm >= 4");
@@ -315,6 +343,7 @@ struct Impl : ImplBase {
         CPM_HIP(hipMemcpyAsync(d.p, phi, sizeof(double) * npm, hipMemcpyHostToDevice, st));
         ParticleOps<PP>::zeldovich(R, d.p, ilog2(cfg.Nm), cfg.L, a_pos, a_mom, X.p, P.p, st);
         CPM_HIP(hipStreamSynchronize(st));
+        pq_pinned = false;  // a new run: its grid comes from its initial momenta
         fx_quantize_momenta();
         reset_ids();
         last_apos = a_pos;

@@ -802,7 +802,12 @@ __global__ __launch_bounds__(256) void k_f64_to_fx(const double* __restrict__ x,
     if (i >= n) return;
     double t = x[i] / L;
     t = t - floor(t);
-    q[i] = (uint32_t)(unsigned long long)(t * 4294967296.0);
+    // floor(t 2^32), except within 2^-19 of an integer, which is taken: a
+    // fixed-point position written as a double (q L / 2^32, off by at most
+    // 2^-20 units after the division) comes back exactly
+    const double u = t * 4294967296.0, r = rint(u);
+    const double v = fabs(u - r) <= 0x1p-19 ? r : floor(u);
+    q[i] = (uint32_t)(unsigned long long)v;  // 2^32 wraps to 0
 }
 
 // ------------------------------------------------------------ conan cic -s

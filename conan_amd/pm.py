@@ -85,6 +85,15 @@ class Gravity:
         self._ck(self._lib.pm_get_times(self._h, _lib.C.byref(a), _lib.C.byref(b)))
         return a.value, b.value
 
+    def momentum_quantum(self) -> float:
+        """FP32 form: the power of two that momenta and kicks are rounded to (0 for FP64)."""
+        q = _lib.C.c_double()
+        self._ck(self._lib.pm_get_momentum_quantum(self._h, _lib.C.byref(q)))
+        return q.value
+
+    def set_momentum_quantum(self, pq: float) -> None:
+        self._ck(self._lib.pm_set_momentum_quantum(self._h, float(pq)))
+
     # operators
     def deposit(self, keys: bool = False):
         rho = np.empty(self.M)

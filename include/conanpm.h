@@ -94,6 +94,18 @@ int pm_set_state(pm_handle* h, const double* X, const double* P);
  * src/run/nbody.hh:174-175). */
 int pm_get_times(const pm_handle* h, double* a_pos, double* a_mom);
 
+/* PM_FP32: the grid (a power of two) that momenta and kicks are rounded to,
+ * chosen from the momenta at pm_init_zeldovich / pm_set_state; it makes
+ * kicks exactly reversible.  Setting it pins it: the momenta are rounded to
+ * it (no change when they already lie on it), and pm_set_state keeps it
+ * instead of choosing one, until pm_init_zeldovich.  A run resumed from a
+ * snapshot (conan run --resume) sets it, then the state, and continues bit
+ * for bit.  PM_FP64: the quantum is 0 and only 0 is accepted.  No
+ * counterpart in the reference (its momenta are plain doubles); for callers
+ * that resume runs. */
+int pm_get_momentum_quantum(const pm_handle* h, double* pq);
+int pm_set_momentum_quantum(pm_handle* h, double pq);
+
 /* CIC deposit of the current positions with particle mass (Nf/Nm)^dim
  * (md_cic, src/run/numeric.hh:106-139): rho (Nf^dim doubles, nullable),
  * keys (Nm^dim cell indices idx(origin), nullable). */

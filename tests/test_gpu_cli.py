@@ -115,3 +115,39 @@ def test_fp32_precision_flag(sim_dir):
     h, _, rec = conanfile.read(str(d / "t.pos.1.conan"))
     X = rec["pos"][1]
     assert X.min() >= 0 and X.max() < 100.0  # the fixed-point form keeps positions folded
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_resume_continues_bit_for_bit(sim_dir, prec):
+    """run --resume <id>.pos.1.conan repeats no step and ends in the same
+    snapshot as the uninterrupted run (FP32: the momentum grid travels in the
+    snapshot header)."""
+    import shutil
+
+    from conan_amd import conanfile
+
+    d, _ = sim_dir
+    args = ["run", "-i", "t", "-f", "2", "--a0", "0.02", "--da", "0.0098", "-n", "30", "--precision", prec]
+    run(args, d)
+    full = conanfile.read(str(d / "t.pos.3.conan"))
+    shutil.copy(d / "t.pos.1.conan", d / "r.pos.1.conan")
+    (d / "t.pos.2.conan").unlink()
+    (d / "t.pos.3.conan").unlink()
+    out = run(args + ["--resume", "r.pos.1.conan"], d)
+    assert out.stdout.count(".") == 20  # steps 11 .. 30 only
+    res = conanfile.read(str(d / "t.pos.3.conan"))
+    if prec == "fp32":
+        assert "momentum-quantum-log2" in full[0]
+        assert res[0]["momentum-quantum-log2"] == full[0]["momentum-quantum-log2"]
+    for name in ("pos", "mom"):
+        assert np.array_equal(res[2][name][1], full[2][name][1])
+    assert res[0]["a-pos"] == full[0]["a-pos"] and res[0]["a-mom"] == full[0]["a-mom"]
+
+
+def test_resume_rejects_a_foreign_snapshot(sim_dir):
+    d, _ = sim_dir
+    run(["run", "-i", "t", "-f", "2", "--a0", "0.02", "--da", "0.0098", "-n", "10"], d)
+    r = subprocess.run([CONAN, "run", "-i", "t", "-f", "2", "--a0", "0.05", "--da", "0.0098", "-n", "20",
+                        "--resume", "t.pos.1.conan"], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=120)
+    assert r.returncode != 0 and "a-pos" in (r.stdout + r.stderr)
