@@ -31,7 +31,8 @@ EXPORTS = [
     "pm_forces", "pm_gather_mesh", "pm_poisson_mesh", "pm_power_spectrum", "pm_diagnostics", "pm_profile", "pm_sync",
     "pm_ic_scale_free", "pm_ic_cdm", "pm_ic_potential", "pm_ic_displacement", "pm_fft_r2c", "pm_fft_c2r",
     "pm_cic", "pm_diff_create", "pm_diff_destroy", "pm_diff_last_error", "pm_diff_forward", "pm_diff_backward",
-    "pm_diff_get_stats",
+    "pm_diff_get_stats", "pm_fof_create", "pm_fof_destroy", "pm_fof_last_error", "pm_fof_run", "pm_fof_get_stats",
+    "pm_fof",
 ]
 
 
@@ -66,6 +67,12 @@ class pm_diff_params(C.Structure):
 class pm_diff_stats(C.Structure):
     _fields_ = [("forward_ms", C.c_double), ("backward_ms", C.c_double), ("device_bytes", C.c_double),
                 ("ncheckpoints", C.c_int), ("recon_pos", C.c_longlong), ("recon_mom", C.c_double)]
+
+
+class pm_fof_stats(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("particles", C.c_longlong), ("ngroups", C.c_longlong),
+                ("candidates", C.c_longlong), ("links", C.c_longlong), ("cells", C.c_int * 3),
+                ("table_bits", C.c_int)]
 
 
 _lib = None
@@ -124,6 +131,13 @@ def load() -> C.CDLL:
         "pm_diff_forward": (C.c_int, [V, V, V, C.c_int]),
         "pm_diff_backward": (C.c_int, [V, V, V, C.c_int]),
         "pm_diff_get_stats": (C.c_int, [V, C.POINTER(pm_diff_stats)]),
+        "pm_fof_create": (C.c_int, [C.c_int, C.c_double, C.c_double, C.POINTER(V)]),
+        "pm_fof_destroy": (None, [V]),
+        "pm_fof_last_error": (C.c_char_p, [V]),
+        "pm_fof_run": (C.c_int, [V, C.c_longlong, V, V, C.POINTER(C.c_longlong), C.c_int]),
+        "pm_fof_get_stats": (C.c_int, [V, C.POINTER(pm_fof_stats)]),
+        "pm_fof": (C.c_int, [C.c_int, C.c_longlong, C.c_double, C.c_double, _dp, C.POINTER(C.c_longlong),
+                             C.POINTER(C.c_longlong)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -22,7 +22,7 @@ LIB = os.path.join(HERE, "libconanpm.so")
 BIN = os.path.join(HERE, "bin", "conan")
 
 HIP_SOURCES = ["pm_fft.hip", "pm_sort.hip", "pm_particles.hip", "pm_spectral.hip", "pm_adjoint.hip", "pm_api.hip",
-               "pm_diff.hip"]
+               "pm_diff.hip", "pm_fof.hip"]
 HOST_SOURCES = ["conan_cli.cpp", "conan_format.cpp"]
 
 # -ffp-contract=off: no FMA contraction, so FP64 per-particle arithmetic

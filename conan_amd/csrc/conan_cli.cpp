@@ -1,7 +1,8 @@
-// conan_cli.cpp -- the `conan` command-line tool of this package: the ic, run
-// and cic commands of conan (src/ic/main.cc, src/run/main.cc,
-// src/cic/main.cc) with the same options, defaults, parameter headers and
-// output files, computing on the GPU through the C ABI of include/conanpm.h.
+// conan_cli.cpp -- the `conan` command-line tool of this package: the ic, run,
+// cic and fof commands of conan (src/ic/main.cc, src/run/main.cc,
+// src/cic/main.cc, src/fof/fof.cc) with the same options, defaults, parameter
+// headers and output files, computing on the GPU through the C ABI of
+// include/conanpm.h.
 //
 // Options added to the reference's (additive only):
 //   ic  --noise FILE     read the white noise from record "noise" of FILE
@@ -14,7 +15,10 @@
 #include "conanpm.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <fstream>
+#include <iomanip>
+#include <sstream>
 #include <functional>
 #include <iostream>
 #include <map>
@@ -347,9 +351,105 @@ void cmd_cic(int argc, char** argv) {
     save_record(fo, "density", "float64", rho.data(), rho.size() * sizeof(double));
 }
 
+// ------------------------------------------------------------------ fof
+// timed_filename (src/base/date.cc:34-50): id.stage.NNNNN.ext, NNNNN =
+// round(b * 10000) of the single-precision time, zero-padded to 5 digits.
+std::string timed_filename(const std::string& id, const std::string& stage, float b, const std::string& ext) {
+    std::ostringstream s;
+    if (b < 0.0)
+        s << id << "." << stage << ".init" << ext;
+    else
+        s << id << "." << stage << "." << std::setfill('0') << std::setw(5) << static_cast<int>(std::round(b * 10000))
+          << ext;
+    return s.str();
+}
+
+// System::modulus for doubles (src/base/misc.hh:144-148)
+double modulus(double a, double b) {
+    if (a < 0) return a - static_cast<int>(a / b - 1) * b;
+    return a - static_cast<int>(a / b) * b;
+}
+
+// random_colour (src/fof/fof.cc:150-154)
+int random_colour(size_t seed) {
+    srand((unsigned)seed);
+    return rand() % 1022;
+}
+
+// cmd_fof (src/fof/fof.cc:244-330) with fof_run's output (fof.cc:216-241):
+// the groups come from pm_fof on the GPU.
+void cmd_fof(int argc, char** argv) {
+    using O = Option;
+    Argv C = read_arguments(
+        argc, argv,
+        {{0, "h", "help", "false", "print help on the use of this program."},
+         {O::VALUED | O::CHECK, "i", "id", date_string(), "identifier for filenames."},
+         {O::VALUED | O::CHECK, "t", "time", "1.0", "time of snapshot."},
+         {O::VALUED | O::CHECK, "l", "linking-length", "0.1", "linking length in units of Mpc h^-1."},
+         {O::VALUED | O::CHECK, "f", "filename", "", "if given, supersedes internal naming system."},
+         {0, "", "ascii", "true", "give output in ASCII."}});
+    if (C.get<bool>("help")) help_and_exit(C);
+
+    const double t = C.get<double>("time");
+    const std::string fi_fn = C["filename"] == "" ? timed_filename(C["id"], "pos", (float)t, ".conan") : C["filename"];
+    std::ifstream fi(fi_fn, std::ios::binary);
+    if (!fi) throw Failure{"could not open " + fi_fn};
+    Header H(fi);
+    H << C;
+    History I(fi);
+    I << C;
+    const unsigned R = H.has("dim") ? H.get<unsigned>("dim") : 3;
+    (void)H.get<unsigned>("N");
+    const double L = H.get<double>("L"), ll = H.get<double>("linking-length");
+    if (R != 2 && R != 3) throw Failure{"only 2-D and 3-D snapshots are supported"};
+
+    std::ofstream fo(timed_filename(C["id"], "fof", (float)t, ".txt"));
+    H.to_txt_file(fo);
+    I.to_txt_file(fo);
+
+    std::vector<double> X = load_array<double>(fi, "pos");
+    const size_t n = X.size() / R;
+    std::vector<long long> group(n);
+    long long ng = 0;
+    std::cerr << "[friends-of-friends on the GPU] ";
+    if (pm_fof((int)R, (long long)n, L, ll, X.data(), group.data(), &ng) != 0)
+        throw Failure{std::string("GPU: ") + pm_fof_last_error(nullptr)};
+    std::cerr << "[found " << ng << " groups] [post-processing] ";
+
+    // particles of each group in index order (fof.cc:218-230)
+    std::vector<size_t> start(ng + 1, 0), order(n);
+    for (size_t i = 0; i < n; ++i) ++start[group[i] + 1];
+    for (long long g = 0; g < ng; ++g) start[g + 1] += start[g];
+    {
+        std::vector<size_t> fill(start.begin(), start.end() - 1);
+        for (size_t i = 0; i < n; ++i) order[fill[group[i]]++] = i;
+    }
+    auto put = [&](const double* p) {
+        for (unsigned k = 0; k < R; ++k) fo << p[k] << " ";
+    };
+    for (long long g = 0; g < ng; ++g) {
+        const size_t mass = start[g + 1] - start[g];
+        if (mass < 10) continue;
+        const double* p = &X[order[start[g]] * R];
+        double w[3];
+        for (unsigned k = 0; k < R; ++k) w[k] = modulus(p[k], L);
+        put(w);
+        fo << " " << mass << " " << g << std::endl;
+    }
+    fo << "\n\n";
+    for (long long g = 0; g < ng; ++g) {
+        const int colour = random_colour((size_t)g);
+        for (size_t q = start[g]; q < start[g + 1]; ++q) {
+            put(&X[order[q] * R]);
+            fo << " " << colour << std::endl;
+        }
+    }
+    std::cerr << "[done]\n";
+}
+
 const std::map<std::string, std::function<void(int, char**)>>& commands() {
     static const std::map<std::string, std::function<void(int, char**)>> m = {
-        {"cic", cmd_cic}, {"ic", cmd_ic}, {"run", cmd_run}};
+        {"cic", cmd_cic}, {"fof", cmd_fof}, {"ic", cmd_ic}, {"run", cmd_run}};
     return m;
 }
 

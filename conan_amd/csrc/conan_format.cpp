@@ -71,6 +71,12 @@ void Header::to_file(std::ostream& f) const {
     write_block(f, out.data(), out.size());
 }
 
+void Header::to_txt_file(std::ostream& f) const {
+    f << "# header\n";
+    for (const auto& kv : *this) f << "# " << kv.first << "=" << first_line(kv.second) << std::endl;
+    f << std::endl;
+}
+
 Header& Header::operator<<(const Argv& C) {
     for (const auto& o : C.opts) (*this)[o.long_name] = o.value;
     return *this;
@@ -96,6 +102,12 @@ void History::to_file(std::ostream& f) const {
     for (const auto& kv : *this) s << kv.first << ":" << first_line(kv.second) << "\n";
     const std::string out = s.str();
     write_block(f, out.data(), out.size());
+}
+
+void History::to_txt_file(std::ostream& f) const {
+    f << "# history\n";
+    for (const auto& kv : *this) f << "# " << kv.first << ":" << first_line(kv.second) << std::endl;
+    f << std::endl;
 }
 
 // ------------------------------------------------------------ records

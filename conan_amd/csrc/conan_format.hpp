@@ -80,6 +80,8 @@ class Header : public std::map<std::string, std::string> {
     explicit Header(std::istream& f) { from_file(f); }
     void from_file(std::istream& f);
     void to_file(std::ostream& f) const;
+    // "# header", "# key=value" per entry, blank line (header.cc:44-52)
+    void to_txt_file(std::ostream& f) const;
     bool has(const std::string& k) const { return find(k) != end(); }
     template <typename T>
     T get(const std::string& name) const {
@@ -97,6 +99,8 @@ class History : public std::multimap<time_t, std::string> {
     explicit History(std::istream& f) { from_file(f); }
     void from_file(std::istream& f);
     void to_file(std::ostream& f) const;
+    // "# history", "# time:command" per entry, blank line (history.cc:40-48)
+    void to_txt_file(std::ostream& f) const;
     void update(const std::string& cmd) { insert({time(nullptr), cmd}); }
     History& operator<<(const Argv& C) {
         update(C.command_line);

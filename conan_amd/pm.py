@@ -2,11 +2,14 @@
 
 ``Gravity`` follows Gravity<R> (src/run/nbody.hh:53-219) behind the Solver
 seam of src/run/leapfrog.hh:6-21 (init / kick / drift), and ``LeapFrog``
-follows LeapFrog::run (src/run/leapfrog.hh:24-46).  Arrays use the reference
+follows LeapFrog::run (src/run/leapfrog.hh:24-46); ``fof`` / ``FofPlan`` give
+the friends-of-friends groups of conan fof (src/fof/fof.cc:157-214).  Arrays use the reference
 layout: particles AoS (Np, dim) float64 in mass-grid order, meshes N^dim
 float64 with axis 0 fastest (returned as flat arrays).
 """
 from __future__ import annotations
+
+import ctypes as C
 
 import numpy as np
 
@@ -235,3 +238,68 @@ def cic(dim: int, Nm: int, Nf: int, L: float, X, s: int = 1):
     rho = np.empty(Nf ** dim)
     check(lib.pm_cic(dim, Nm, Nf, L, s, ptr(_c(X).ravel()), ptr(rho)))
     return rho
+
+
+def _fof_check(rc: int, h=None) -> None:
+    if rc != 0:
+        msg = _lib.load().pm_fof_last_error(h)
+        raise _lib.PMError(rc, msg.decode() if msg else "unknown error")
+
+
+def fof(dim: int, L: float, ll: float, X):
+    """Friends-of-friends groups of positions X (n x dim), the grouping of
+    conan fof (src/fof/fof.cc:157-214): returns (group per particle, numbered
+    in the order of each group's lowest index; number of groups)."""
+    lib = _lib.load()
+    X = _c(X).reshape(-1, dim)
+    g = np.empty(len(X), dtype=np.int64)
+    ng = C.c_longlong(0)
+    _fof_check(lib.pm_fof(dim, len(X), L, ll, ptr(X), g.ctypes.data_as(C.POINTER(C.c_longlong)), C.byref(ng)))
+    return g, int(ng.value)
+
+
+class FofPlan:
+    """Friends-of-friends on the GPU with device buffers kept between runs.
+    run() takes a float64 array (host) or a contiguous float64 torch tensor on
+    the GPU (device memory, complete before the call); the groups come back
+    in the same kind of memory, int64."""
+
+    def __init__(self, dim: int, L: float, ll: float):
+        self._lib = _lib.load()
+        self.dim = dim
+        h = C.c_void_p()
+        _fof_check(self._lib.pm_fof_create(dim, L, ll, C.byref(h)))
+        self._h = h
+
+    def run(self, X):
+        ng = C.c_longlong(0)
+        if isinstance(X, np.ndarray):
+            X = _c(X).reshape(-1, self.dim)
+            g = np.empty(len(X), dtype=np.int64)
+            _fof_check(self._lib.pm_fof_run(self._h, len(X), X.ctypes.data, g.ctypes.data, C.byref(ng), 0), self._h)
+            return g, int(ng.value)
+        import torch
+        if X.dtype != torch.float64 or not X.is_cuda or not X.is_contiguous():
+            raise TypeError("device positions must be a contiguous float64 tensor on the GPU")
+        n = X.numel() // self.dim
+        g = torch.empty(n, dtype=torch.int64, device=X.device)
+        torch.cuda.synchronize(X.device)
+        _fof_check(self._lib.pm_fof_run(self._h, n, X.data_ptr(), g.data_ptr(), C.byref(ng), 1), self._h)
+        return g, int(ng.value)
+
+    def stats(self) -> dict:
+        s = _lib.pm_fof_stats()
+        _fof_check(self._lib.pm_fof_get_stats(self._h, C.byref(s)), self._h)
+        return {"device_ms": s.device_ms, "particles": s.particles, "ngroups": s.ngroups,
+                "candidates": s.candidates, "links": s.links, "cells": list(s.cells), "table_bits": s.table_bits}
+
+    def close(self):
+        if self._h:
+            self._lib.pm_fof_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

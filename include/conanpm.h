@@ -196,6 +196,38 @@ int pm_fft_c2r(int dim, int N, int precision, const double* spec, double* out);
  * grid s times per axis by interpolating displacements.  rho: Nf^dim doubles. */
 int pm_cic(int dim, int Nm, int Nf, double L, int s, const double* X, double* rho);
 
+/* ---------------------------------------------------------------- conan fof
+ * The grouping of fof_run<R> (src/fof/fof.cc:157-214): friends-of-friends
+ * groups of n positions pos (n x dim doubles, as a snapshot's "pos" record
+ * holds them, unwrapped), linked when the Proximity predicate holds
+ * (fof.cc:93-118: per-axis difference wrapped once by L, squared length <=
+ * ll^2).  group[i] (n long longs) = the group of particle i, numbered like
+ * the reference's tags minus one: in the order of each group's lowest
+ * particle index.  *ngroups = the number of groups.  n < 2^31.
+ * pm_fof is the one-shot form of pm_fof_create + pm_fof_run (host memory);
+ * its errors are read with pm_fof_last_error(NULL).
+ * A pm_fof_plan handle keeps its device buffers between runs (grown on demand) and
+ * owns one HIP stream.  device != 0: pos and group are device memory of the
+ * current HIP device, complete before the call (the run is ordered on the
+ * handle's stream and synchronised before it returns); device == 0: host
+ * memory. */
+typedef struct pm_fof_plan pm_fof_plan;
+typedef struct {
+    double device_ms;      /* device time of the last run: from its first kernel to its last */
+    long long particles;
+    long long ngroups;
+    long long candidates;  /* pairs the predicate was evaluated on */
+    long long links;       /* pairs it accepted */
+    int cells[3];          /* cells per axis of the neighbour grid */
+    int table_bits;        /* log2 of the cell-key table size */
+} pm_fof_stats;
+int pm_fof_create(int dim, double L, double ll, pm_fof_plan** out);
+void pm_fof_destroy(pm_fof_plan* f);
+const char* pm_fof_last_error(const pm_fof_plan* f);
+int pm_fof_run(pm_fof_plan* f, long long n, const double* pos, long long* group, long long* ngroups, int device);
+int pm_fof_get_stats(const pm_fof_plan* f, pm_fof_stats* s);
+int pm_fof(int dim, long long n, double L, double ll, const double* pos, long long* group, long long* ngroups);
+
 /* ---------------------------------------------------------------- differentiable PM
  * No counterpart in the reference (it has no gradients).  The map
  *   white noise w (N^dim, the input of `conan ic --scale-free`)

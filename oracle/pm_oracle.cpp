@@ -24,7 +24,8 @@
 // closed forms for the Poisson solve and the FD gradient; exactness of the
 // gather on linear data and deposit/gather adjointness; the constant-momentum
 // drift, the zero-force lattice and single-plane-wave Zel'dovich growth for
-// the leapfrog; the mt19937 known answer for the noise.
+// the leapfrog; the mt19937 known answer for the noise; brute-force
+// connected components for the friends-of-friends groups.
 
 #include <algorithm>
 #include <chrono>
@@ -348,6 +349,103 @@ struct Sim {
 
 }  // namespace
 
+// ---------------------------------------------------------------- fof
+// Friends-of-friends of src/fof/fof.cc:157-214.  Particles are visited in
+// index order; an untagged one starts a flood fill (fof.cc:17-40) that tags
+// every particle reachable through the Proximity predicate with the next
+// group number.  Proximity (fof.cc:93-118): per axis b = pos[i] - a, wrapped
+// once (b > L/2: b - L; b < -L/2: b + L), accepted when the sum of squares,
+// accumulated from 0 in axis order (mvector.hh:128-133), is <= r*r.  The
+// reference finds the candidates with a kd-tree (fof.cc:171-173,206-208);
+// here a periodic grid of cells no smaller than r does, which yields the same
+// neighbour set, since a pair within r lies in the same or adjacent cells.
+inline bool fof_near(const double* a, const double* b, int R, double L, double rsqr) {
+    double v = 0;
+    for (int k = 0; k < R; ++k) {
+        double t = b[k] - a[k], d;
+        if (t > L / 2) d = t - L;
+        else if (t < -L / 2) d = t + L;
+        else d = t;
+        v += d * d;
+    }
+    return v <= rsqr;
+}
+
+static void fof_tags(int R, long n, double L, double r, const double* pos, std::vector<long>& tags, long& ngroups) {
+    long nc[3], total = 1;
+    double cs[3];
+    // cells a little wider than r (so that rounding in the cell index cannot
+    // separate a linked pair by two cells), at most about 8 per particle
+    const long cap = std::max(1L, (long)std::pow(8.0 * (double)n, 1.0 / R));
+    for (int k = 0; k < 3; ++k) {
+        long m = k < R ? (long)std::floor(L / (r * (1 + 1e-9))) : 1;
+        m = std::min(m, cap);
+        if (m < 3) m = 1;  // every cell along this axis is a neighbour anyway
+        nc[k] = m;
+        cs[k] = L / (double)m;
+        total *= m;
+    }
+    // particles by cell (counting sort): members[start[c] .. start[c + 1])
+    std::vector<long> cell_of(n), start(total + 1, 0), members(n);
+    for (long i = 0; i < n; ++i) {
+        long c = 0, stride = 1;
+        for (int k = 0; k < R; ++k) {
+            double w = pos[i * R + k] - std::floor(pos[i * R + k] / L) * L;
+            long ck = (long)std::floor(w / cs[k]);
+            ck = ((ck % nc[k]) + nc[k]) % nc[k];
+            c += ck * stride;
+            stride *= nc[k];
+        }
+        cell_of[i] = c;
+        ++start[c + 1];
+    }
+    for (long c = 0; c < total; ++c) start[c + 1] += start[c];
+    {
+        std::vector<long> fill(start.begin(), start.end() - 1);
+        for (long i = 0; i < n; ++i) members[fill[cell_of[i]]++] = i;
+    }
+    const double rsqr = r * r;
+    tags.assign(n, 0);
+    long current = 1;
+    std::vector<long> stack;
+    for (long idx = 0; idx < n; ++idx) {
+        if (tags[idx] != 0) continue;
+        tags[idx] = current;
+        stack.push_back(idx);
+        while (!stack.empty()) {
+            const long a = stack.back();
+            stack.pop_back();
+            long ca[3], c = cell_of[a];
+            for (int k = 0; k < 3; ++k) {
+                ca[k] = c % nc[k];
+                c /= nc[k];
+            }
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int d[3] = {dx, dy, dz};
+                        bool skip = false;
+                        long cc = 0, stride = 1;
+                        for (int k = 0; k < 3; ++k) {
+                            if (nc[k] == 1 && d[k] != 0) skip = true;
+                            cc += ((ca[k] + d[k] + nc[k]) % nc[k]) * stride;
+                            stride *= nc[k];
+                        }
+                        if (skip) continue;
+                        for (long q = start[cc]; q < start[cc + 1]; ++q) {
+                            const long j = members[q];
+                            if (tags[j] != current && fof_near(pos + a * R, pos + j * R, R, L, rsqr)) {
+                                tags[j] = current;
+                                stack.push_back(j);
+                            }
+                        }
+                    }
+        }
+        ++current;
+    }
+    ngroups = current - 1;
+}
+
 // ====================================================================== C ABI
 extern "C" {
 
@@ -621,6 +719,16 @@ void orc_power_spectrum(int R, int N, double L, const double* delta, int nbins,
         kmean[bb] = cnt[bb] ? ks[bb] / cnt[bb] : 0.0;
         pk[bb] = cnt[bb] ? ps[bb] / cnt[bb] * norm : 0.0;
     }
+}
+
+// fof_run's grouping (src/fof/fof.cc:175-214): group[i] = tag - 1, numbered
+// in the order of each group's lowest particle index; returns the count.
+long orc_fof(int R, long n, double L, double r, const double* pos, long* group) {
+    std::vector<long> tags;
+    long ng = 0;
+    fof_tags(R, n, L, r, pos, tags, ng);
+    for (long i = 0; i < n; ++i) group[i] = tags[i] - 1;
+    return ng;
 }
 
 }  // extern "C"

@@ -53,6 +53,7 @@ def load():
         "orc_compute_displacement": (None, [i, i, d, _dp, _dp, i]),
         "orc_cic_tool": (None, [i, i, i, d, i, _dp, _dp]),
         "orc_power_spectrum": (None, [i, i, d, _dp, i, _dp, _dp, _dp, C.POINTER(C.c_int64), i]),
+        "orc_fof": (C.c_long, [i, C.c_long, d, d, _dp, C.POINTER(C.c_long)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -227,3 +228,41 @@ def log_edges(N, L, nbins=64):
     kf = 2 * np.pi / L
     kn = np.pi * N / L
     return kf * (kn / kf) ** (np.arange(nbins + 1) / nbins)
+
+
+def fof(dim, L, ll, X):
+    """Friends-of-friends groups of positions X (n x dim): (group per
+    particle, numbered by each group's lowest index; group count)."""
+    X = _c(X).reshape(-1, dim)
+    g = np.empty(len(X), dtype=np.int64)
+    ng = load().orc_fof(dim, len(X), L, ll, _p(X), g.ctypes.data_as(C.POINTER(C.c_long)))
+    return g, int(ng)
+
+
+def fof_brute(dim, L, ll, X):
+    """Connected components of the all-pairs Proximity graph (the same
+    predicate, evaluated for every pair with numpy), labelled like fof():
+    an independent check of the cell-grid flood fill on small inputs."""
+    X = np.asarray(X, dtype=np.float64).reshape(-1, dim)
+    n = len(X)
+    parent = np.arange(n)
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+
+    for i in range(n):
+        b = X - X[i]
+        d = np.where(b > L / 2, b - L, np.where(b < -L / 2, b + L, b))
+        v = np.zeros(n)
+        for k in range(dim):
+            v = v + d[:, k] * d[:, k]
+        for j in np.nonzero(v <= ll * ll)[0]:
+            ri, rj = find(i), find(int(j))
+            if ri != rj:
+                parent[max(ri, rj)] = min(ri, rj)
+    roots = np.array([find(i) for i in range(n)])
+    uniq = np.unique(roots)  # the roots are the component minima, so sorted = discovery order
+    return np.searchsorted(uniq, roots).astype(np.int64), len(uniq)

@@ -118,6 +118,39 @@ def test_diff_create_without_device_fails_cleanly():
     assert b"no HIP device" in lib.pm_diff_last_error(None)
 
 
+@pytest.mark.parametrize("args,code", [((4, 10.0, 0.1), -2), ((1, 10.0, 0.1), -2), ((3, 0.0, 0.1), -1),
+                                       ((3, float("inf"), 0.1), -1), ((3, 10.0, 0.0), -1),
+                                       ((3, 10.0, float("nan")), -1)])
+def test_fof_create_rejects_bad_arguments(args, code):
+    import ctypes
+
+    from conan_amd import _lib
+
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.pm_fof_create(*args, ctypes.byref(h)) == code and not h.value
+    assert lib.pm_fof_last_error(None)
+    assert lib.pm_fof_run(None, 0, None, None, None, 0) == -1
+    assert b"null" in lib.pm_fof_last_error(None)
+    X = np.zeros((4, 3))
+    g = np.zeros(4, dtype=np.int64)
+    ng = ctypes.c_longlong(0)
+    assert lib.pm_fof(*args[:1], 4, *args[1:], _lib.ptr(X), g.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                      ctypes.byref(ng)) == code
+
+
+@pytest.mark.skipif(HAS_GPU, reason="checks the no-device error path")
+def test_fof_without_device_fails_cleanly():
+    import ctypes
+
+    from conan_amd import _lib
+
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.pm_fof_create(3, 10.0, 0.1, ctypes.byref(h)) == -3 and not h.value
+    assert b"no HIP device" in lib.pm_fof_last_error(None)
+
+
 # ------------------------------------------------------------------ CLI
 def run_cli(*args, cwd=None):
     return subprocess.run([CONAN, *args], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, cwd=cwd,
@@ -126,7 +159,7 @@ def run_cli(*args, cwd=None):
 
 def test_cli_usage_and_unknown_command():
     r = run_cli()
-    assert r.returncode == 0 and "where [command] is one of: cic, ic, run" in r.stdout
+    assert r.returncode == 0 and "where [command] is one of: cic, fof, ic, run" in r.stdout
     r = run_cli("nv")
     assert r.returncode == 255
     assert "[error] command [nv] not understood." in r.stderr
@@ -214,3 +247,18 @@ def test_conanfile_reads_thulsa_style_records(tmp_path):
     h, _, rec = conanfile.read(str(path))
     assert h["cosmology"] == "Planck-nbody"
     assert np.array_equal(rec["potential"][1], np.arange(16.0))
+
+
+def test_cli_fof_options_and_input_names(tmp_path):
+    r = run_cli("fof", "--help")
+    assert r.returncode == 0
+    for opt in ("--id | -i", "--time | -t (default: 1.0)", "--linking-length | -l (default: 0.1)",
+                "--filename | -f", "--ascii"):
+        assert opt in r.stdout
+    # timed_filename (src/base/date.cc:34-50): round(t * 10000), 5 digits
+    r = run_cli("fof", "-i", "x", "-t", "0.5", cwd=tmp_path)
+    assert r.returncode == 255 and "could not open x.pos.05000.conan" in r.stderr
+    r = run_cli("fof", "-i", "x", cwd=tmp_path)
+    assert "could not open x.pos.10000.conan" in r.stderr
+    r = run_cli("fof", "-f", "given.conan", cwd=tmp_path)
+    assert "could not open given.conan" in r.stderr

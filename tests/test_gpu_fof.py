@@ -1,0 +1,195 @@
+"""Friends-of-friends on the GPU (pm_fof) against the oracle's flood fill
+(src/fof/fof.cc:157-214): the group of every particle, bit for bit, and the
+conan fof command's text output."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from test_oracle import fof_cases
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CONAN = os.path.join(ROOT, "conan_amd", "bin", "conan")
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def blobs(n, L, nblob, sigma, uniform_frac, seed):
+    """Gaussian blobs (some across the box boundary) over a uniform background."""
+    r = np.random.default_rng(seed)
+    c = r.uniform(0, L, (nblob, 3))
+    X = c[r.integers(0, nblob, n)] + r.normal(0, sigma, (n, 3))
+    nu = int(n * uniform_frac)
+    X[:nu] = r.uniform(0, L, (nu, 3))
+    return r.permutation(X % L)
+
+
+@pytest.mark.parametrize("case", fof_cases(), ids=lambda c: c[0])
+def test_fof_matches_oracle(pm, orc, case):
+    name, dim, L, ll, X = case
+    say("fof", name)
+    g, ng = pm.fof(dim, L, ll, X)
+    go, ngo = orc.fof(dim, L, ll, X)
+    assert ng == ngo and np.array_equal(g, go)
+
+
+def test_fof_special_cases(pm, orc):
+    # periodic ring closing across the boundary, spacing just below / above ll
+    L, n = 8.0, 64
+    X = np.zeros((n, 3))
+    X[:, 0] = np.arange(n) * (L / n)
+    X[:, 1:] = 1.0
+    s = L / n
+    say("fof ring")
+    g, ng = pm.fof(3, L, s * 1.01, X)
+    assert ng == 1 and not g.any()
+    g, ng = pm.fof(3, L, s * 0.99, X)
+    assert ng == n and np.array_equal(g, np.arange(n))
+    # fewer than 3 cells per axis, and tiny inputs
+    r = np.random.default_rng(3)
+    X = r.uniform(0, 1.0, (300, 3))
+    for ll in (0.2, 0.34, 0.9):
+        say("fof large ll", ll)
+        g, ng = pm.fof(3, 1.0, ll, X)
+        go, ngo = orc.fof(3, 1.0, ll, X)
+        assert ng == ngo and np.array_equal(g, go)
+    g, ng = pm.fof(3, 1.0, 0.1, X[:1])
+    assert ng == 1 and g[0] == 0
+    g, ng = pm.fof(3, 1.0, 0.1, X[:0])
+    assert ng == 0 and len(g) == 0
+    # coincident particles and a linking length that spans the box
+    Y = np.repeat(X[:20], 5, axis=0)
+    g, ng = pm.fof(3, 1.0, 1e-3, Y)
+    go, ngo = orc.fof(3, 1.0, 1e-3, Y)
+    assert ng == ngo == 20 and np.array_equal(g, go)
+    g, ng = pm.fof(2, 1.0, 5.0, X[:, :2])
+    assert ng == 1
+
+
+def test_fof_on_an_evolved_snapshot(pm, orc):
+    """A clustered snapshot as conan run writes it (unwrapped FP64 positions):
+    64^3 particles evolved from scale-free initial conditions to a = 1,
+    grouped with b = 0.2 of the mean particle spacing."""
+    dim, Nm, Nf, L, a0, da, nsteps = 3, 64, 128, 100.0, 0.02, 0.0196, 50
+    noise = orc.white_noise(17, Nm ** dim)
+    phi = orc.compute_potential(dim, 6, L, orc.ic_scale_free(dim, 6, L, -2.0, noise))
+    say("evolve 64^3 for fof")
+    with pm.Gravity(dim, Nm, Nf, L) as g:
+        g.init(phi, a0, a0 + da / 2)
+        g.steps(a0, da, nsteps)
+        X, _ = g.get_state()
+    assert X.min() < 0 or X.max() >= L  # some positions left the box: unwrapped
+    ll = 0.2 * L / Nm
+    say("fof evolved snapshot")
+    gr, ng = pm.fof(dim, L, ll, X)
+    go, ngo = orc.fof(dim, L, ll, X)
+    assert ng == ngo and np.array_equal(gr, go)
+    mass = np.bincount(gr)
+    assert mass.max() > 100 and (mass >= 10).sum() > 20  # it is clustered
+
+
+def test_fof_two_million_particles(pm, orc):
+    n, L = 1 << 21, 100.0
+    X = blobs(n, L, 4000, 0.6, 0.25, seed=1)
+    ll = 0.4 * L / n ** (1 / 3)
+    say("fof 2M blobs")
+    plan = pm.FofPlan(3, L, ll)
+    try:
+        g, ng = plan.run(X)
+        st = plan.stats()
+        g2, ng2 = plan.run(X)  # same buffers again: the same groups
+    finally:
+        plan.close()
+    go, ngo = orc.fof(3, L, ll, X)
+    assert ng == ngo and np.array_equal(g, go)
+    assert ng2 == ng and np.array_equal(g2, g)
+    assert st["ngroups"] == ng and st["particles"] == n
+    assert st["candidates"] >= st["links"] > 0
+    assert st["device_ms"] > 0
+
+
+def test_fof_device_memory_and_resizing(pm, orc):
+    import torch
+
+    L = 50.0
+    plan = pm.FofPlan(3, L, 0.3)
+    try:
+        for n, seed in ((40000, 5), (9000, 6), (120000, 7)):  # grow, shrink, grow past the first size
+            X = blobs(n, L, 300, 0.5, 0.3, seed)
+            say("fof device", n)
+            go, ngo = orc.fof(3, L, 0.3, X)
+            gh, ngh = plan.run(X)
+            assert ngh == ngo and np.array_equal(gh, go)
+            Xd = torch.from_numpy(X).to("cuda")
+            gd, ngd = plan.run(Xd)
+            assert ngd == ngo and np.array_equal(gd.cpu().numpy(), go)
+    finally:
+        plan.close()
+
+
+# ------------------------------------------------------------------ conan fof
+def _fmt(x):
+    return "%g" % x
+
+
+def _modulus(a, b):
+    # System::modulus for doubles (src/base/misc.hh:144-148)
+    return a - int(a / b - 1) * b if a < 0 else a - int(a / b) * b
+
+
+def test_cli_fof_output(pm, orc, tmp_path):
+    from conan_amd import conanfile
+
+    dim, Nm, Nf, L, a0, da, nsteps = 3, 32, 64, 100.0, 0.02, 0.0392, 25
+    noise = orc.white_noise(23, Nm ** dim)
+    phi = orc.compute_potential(dim, 5, L, orc.ic_scale_free(dim, 5, L, -2.0, noise))
+    with pm.Gravity(dim, Nm, Nf, L) as g:
+        g.init(phi, a0, a0 + da / 2)
+        g.steps(a0, da, nsteps)
+        X, P = g.get_state()
+    hdr = {"id": "s", "N": Nm, "L": "100", "dim": dim, "a-mom": "1", "a-pos": "1"}
+    conanfile.write(str(tmp_path / "s.pos.10000.conan"), hdr, [(1700000000, "ran nbody")],
+                    {"pos": ("3f8", X), "mom": ("3f8", P)})
+    ll = 0.2 * L / Nm
+    say("conan fof")
+    r = subprocess.run([CONAN, "fof", "-i", "s", "-t", "1.0", "-l", repr(ll)], cwd=tmp_path, stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    text = (tmp_path / "s.fof.10000.txt").read_text()
+
+    go, ng = orc.fof(dim, L, ll, X)
+    libc = ctypes.CDLL(None)
+    lines = []
+    head, rest = text.split("# history\n", 1)
+    hl = head.splitlines()
+    assert hl[0] == "# header" and hl[-1] == ""
+    keys = dict(l[2:].split("=", 1) for l in hl[1:-1])
+    assert keys["linking-length"] == repr(ll) and keys["id"] == "s" and keys["N"] == "32"
+    assert keys["time"] == "1.0" and keys["ascii"] == "true"
+    hist, body = rest.split("\n\n", 1)
+    assert hist.splitlines()[0] == "# 1700000000:ran nbody" and "fof" in hist.splitlines()[1]
+    first = {}
+    members = [[] for _ in range(ng)]
+    for i, t in enumerate(go):
+        members[t].append(i)
+    for t in range(ng):
+        if len(members[t]) < 10:
+            continue
+        p = X[members[t][0]]
+        lines.append("".join(_fmt(_modulus(v, L)) + " " for v in p) + " %d %d" % (len(members[t]), t))
+    lines.append("")
+    lines.append("")
+    for t in range(ng):
+        libc.srand(t)
+        col = libc.rand() % 1022
+        for i in members[t]:
+            lines.append("".join(_fmt(v) + " " for v in X[i]) + " %d" % col)
+    expect = "\n".join(lines) + "\n"
+    assert sum(len(m) >= 10 for m in members) > 5
+    assert body == expect

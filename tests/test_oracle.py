@@ -397,3 +397,74 @@ def test_power_spectrum_plane_wave(orc):
     i = hit[0]
     # the two modes +-k0 carry L^3 A^2 / 4 each; the bin averages over c[i] modes
     assert abs(p[i] * c[i] - 2 * L ** 3 * A ** 2 / 4) < 1e-9 * L ** 3 * A ** 2
+
+
+# ------------------------------------------------------------------ fof
+def fof_cases():
+    """Seeded positions for the friends-of-friends checks: uniform, clustered
+    blobs straddling the periodic boundary, and unwrapped copies (snapshots
+    keep positions unwrapped, src/run/nbody.hh:181)."""
+    r = np.random.default_rng(7)
+    L = 10.0
+    out = []
+    for dim in (2, 3):
+        n = 1500
+        u = r.uniform(0, L, (n, dim))
+        sp = L / n ** (1.0 / dim)
+        out.append((f"uniform{dim}", dim, L, 0.7 * sp, u))
+        centres = r.uniform(0, L, (6, dim))
+        centres[0] = 0.05  # a blob across the corner of the box
+        blobs = centres[r.integers(0, 6, n)] + r.normal(0, 0.4, (n, dim))
+        out.append((f"blobs{dim}", dim, L, 0.5 * sp, blobs % L))
+        unw = blobs.copy()
+        unw[::3] += L * r.integers(-1, 2, (len(unw[::3]), dim))
+        out.append((f"unwrapped{dim}", dim, L, 0.5 * sp, unw))
+    return out
+
+
+@pytest.mark.parametrize("case", fof_cases(), ids=lambda c: c[0])
+def test_fof_matches_brute_force_components(orc, case):
+    _, dim, L, ll, X = case
+    g, ng = orc.fof(dim, L, ll, X)
+    gb, ngb = orc.fof_brute(dim, L, ll, X)
+    assert ng == ngb
+    assert np.array_equal(g, gb)
+    assert 1 < ng < len(X)  # the cases link some particles and not all
+
+
+def test_fof_groups_are_numbered_by_lowest_index(orc):
+    _, dim, L, ll, X = fof_cases()[1]
+    g, ng = orc.fof(dim, L, ll, X)
+    first = np.array([np.nonzero(g == t)[0][0] for t in range(ng)])
+    assert g[0] == 0 and np.all(np.diff(first) > 0)
+
+
+def test_fof_chains_link_below_the_length_and_not_above(orc):
+    # a periodic ring of particles spaced s along x: one group when s <= ll,
+    # singletons when s > ll; the ring closes across the boundary
+    L, n = 8.0, 64
+    X = np.zeros((n, 3))
+    X[:, 0] = np.arange(n) * (L / n)
+    X[:, 1:] = 1.0
+    s = L / n
+    g, ng = orc.fof(3, L, s * 1.01, X)
+    assert ng == 1 and not g.any()
+    g, ng = orc.fof(3, L, s * 0.99, X)
+    assert ng == n and np.array_equal(g, np.arange(n))
+    # cut the ring: the two ends join only through the periodic wrap
+    Y = X[np.r_[0:20, 30:64]]
+    g, ng = orc.fof(3, L, s * 1.01, Y)
+    assert ng == 1
+
+
+def test_fof_large_linking_length_and_tiny_inputs(orc):
+    r = np.random.default_rng(3)
+    X = r.uniform(0, 1.0, (300, 3))
+    for ll in (0.2, 0.4, 0.9):  # fewer than 3 cells per axis: the grid folds the axis
+        g, ng = orc.fof(3, 1.0, ll, X)
+        gb, ngb = orc.fof_brute(3, 1.0, ll, X)
+        assert ng == ngb and np.array_equal(g, gb)
+    g, ng = orc.fof(3, 1.0, 0.1, X[:1])
+    assert ng == 1 and g[0] == 0
+    g, ng = orc.fof(3, 1.0, 0.1, X[:0])
+    assert ng == 0 and len(g) == 0
