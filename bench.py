@@ -14,7 +14,9 @@ differentiable runs (noise -> final density and its adjoint, at 128^3 and
 512^3 particles) give the gradient-to-forward time ratio.  One JSON line goes
 to stdout; progress goes to stderr.
 
-Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--no-cpu] [--no-fp64] [--no-grad]
+conan fof's grouping runs on the final snapshot (b = 0.2): particles/s.
+
+Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--no-cpu] [--no-fp64] [--no-grad] [--no-fof]
 """
 from __future__ import annotations
 
@@ -83,7 +85,7 @@ def read_traffic(kernel_tag: str):
     return None
 
 
-def run_form(pm, phi, prec, steps, warmup, wrap=False, profile=True, pk_bins=64):
+def run_form(pm, phi, prec, steps, warmup, wrap=False, profile=True, pk_bins=64, keep_state=False):
     g = pm.Gravity(DIM, NM, NF, L, precision=prec, wrap_gather=wrap)
     try:
         if warmup > 0:
@@ -99,7 +101,8 @@ def run_form(pm, phi, prec, steps, warmup, wrap=False, profile=True, pk_bins=64)
         ms, cnt = g.profile(0) if profile else (None, None)
         k, p, c = g.power_spectrum(pk_bins)
         diag = g.diagnostics()
-        return {"seconds": t1 - t0, "a": a, "phase_ms": ms, "phase_cnt": cnt, "pk": (k, p, c), "diag": diag}
+        X = g.get_state()[0] if keep_state else None
+        return {"seconds": t1 - t0, "a": a, "phase_ms": ms, "phase_cnt": cnt, "pk": (k, p, c), "diag": diag, "X": X}
     finally:
         g.close()
 
@@ -117,6 +120,55 @@ def cpu_baseline(phi, nsteps: int, threads: int):
             "sample": f"{nsteps} leapfrog step(s) (drift+deposit+FFT Poisson+gather+kick) of the same "
                       f"256^3/512^3 FP64 run from the same potential, {threads} OpenMP threads, "
                       f"{secs:.1f} s"}
+
+
+FOF_B = 0.2  # linking length in units of the mean particle spacing
+FOF_BYTES = 32  # per particle: read its position (3 doubles), write its group (int64)
+
+
+def fof_bench(pm, X, runs=5, cpu=True):
+    """conan fof's grouping (pm_fof) on the final snapshot of the timed FP32
+    run (256^3 particles, a ~ 1), linking length b = 0.2 of the mean spacing,
+    positions resident in HBM; the oracle's flood fill timed on a slab of the
+    same snapshot as the CPU baseline."""
+    import torch
+
+    ll = FOF_B * L / NM
+    plan = pm.FofPlan(DIM, L, ll)
+    try:
+        Xd = torch.from_numpy(np.ascontiguousarray(X)).to("cuda")
+        g, ng = plan.run(Xd)  # warm-up: buffers allocated here
+        dev_ms, wall = [], []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            g, ng = plan.run(Xd)
+            wall.append(time.perf_counter() - t0)
+            dev_ms.append(plan.stats()["device_ms"])
+        st = plan.stats()
+        mass = torch.bincount(g).cpu().numpy()
+    finally:
+        plan.close()
+    t = float(np.median(dev_ms)) * 1e-3
+    out = {"workload": f"friends-of-friends, b = {FOF_B} (ll = {ll:.6g}), {NP} particles of the final "
+                       "FP32 snapshot", "particles": NP, "ngroups": int(ng), "groups_ge_10": int((mass >= 10).sum()),
+           "largest": int(mass.max()), "ms": round(t * 1e3, 4), "wall_ms": round(float(np.median(wall)) * 1e3, 4),
+           "value": round(NP / t, 1), "unit": "particles/s",
+           "candidates_per_particle": round(st["candidates"] / NP, 3), "links_per_particle": round(st["links"] / NP, 3),
+           "cells": st["cells"], "table_bits": st["table_bits"],
+           "hbm_frac_min_bytes": round(FOF_BYTES * NP / t / PEAK_HBM, 5)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle as orc
+
+        sl = np.mod(X[:, 0], L) < L / 8
+        Xs = np.ascontiguousarray(X[sl])
+        t0 = time.perf_counter()
+        _, ngs = orc.fof(DIM, L, ll, Xs)
+        secs = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(len(Xs) / secs, 1), "unit": "particles/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle flood fill (cell grid) over the {len(Xs)} particles of the slab "
+                                         f"x < L/8 of the same snapshot, {ngs} groups, {secs:.1f} s"}
+    return out
 
 
 # Differentiable runs: (name, mbits, f, ns, a0, da, nsteps, ckpt_every)
@@ -195,6 +247,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-grad", action="store_true", help="skip the differentiable runs")
+    ap.add_argument("--no-fof", action="store_true", help="skip the friends-of-friends run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box here has one GPU); "
@@ -209,7 +262,7 @@ def main():
     del noise
 
     log(f"FP32 form: {args.warmup} warmup + {args.steps} timed steps")
-    r32 = run_form(pm, phi, "fp32", args.steps, args.warmup)
+    r32 = run_form(pm, phi, "fp32", args.steps, args.warmup, keep_state=not args.no_fof)
     log(f"  {r32['seconds']:.3f} s -> {r32['seconds'] / args.steps * 1e3:.3f} ms/step")
     r64 = None
     if not args.no_fp64:
@@ -296,6 +349,10 @@ def main():
         # (forward + adjoint) / forward of the 128^3 differentiable run
         out["grad_fwd_ratio"] = runs[0]["grad_fwd_ratio"]
         out["differentiable"] = runs
+    if not args.no_fof:
+        log(f"friends-of-friends on the final FP32 snapshot (b = {FOF_B})")
+        out["fof"] = fof_bench(pm, r32["X"], cpu=not args.no_cpu)
+        log(f"  {out['fof']}")
     out["final_a"] = r32["a"]
     out["momentum_sum_fp32"] = r32["diag"]["momentum"]
 

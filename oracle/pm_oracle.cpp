@@ -34,6 +34,8 @@
 #include <cstdint>
 #include <cstring>
 #include <random>
+#include <unordered_map>
+#include <utility>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -357,8 +359,9 @@ struct Sim {
 // once (b > L/2: b - L; b < -L/2: b + L), accepted when the sum of squares,
 // accumulated from 0 in axis order (mvector.hh:128-133), is <= r*r.  The
 // reference finds the candidates with a kd-tree (fof.cc:171-173,206-208);
-// here a periodic grid of cells no smaller than r does, which yields the same
-// neighbour set, since a pair within r lies in the same or adjacent cells.
+// here a periodic grid of cells no smaller than r (the occupied ones in a hash
+// map) does, which yields the same neighbour set, since a pair within r lies
+// in the same or adjacent cells.
 inline bool fof_near(const double* a, const double* b, int R, double L, double rsqr) {
     double v = 0;
     for (int k = 0; k < R; ++k) {
@@ -372,37 +375,38 @@ inline bool fof_near(const double* a, const double* b, int R, double L, double r
 }
 
 static void fof_tags(int R, long n, double L, double r, const double* pos, std::vector<long>& tags, long& ngroups) {
-    long nc[3], total = 1;
+    // cells a little wider than r, so that rounding in a cell index cannot
+    // separate a linked pair by two cells; an axis with fewer than 3 cells is
+    // one cell (every cell along it is a neighbour anyway)
+    uint64_t nc[3];
     double cs[3];
-    // cells a little wider than r (so that rounding in the cell index cannot
-    // separate a linked pair by two cells), at most about 8 per particle
-    const long cap = std::max(1L, (long)std::pow(8.0 * (double)n, 1.0 / R));
     for (int k = 0; k < 3; ++k) {
-        long m = k < R ? (long)std::floor(L / (r * (1 + 1e-9))) : 1;
-        m = std::min(m, cap);
-        if (m < 3) m = 1;  // every cell along this axis is a neighbour anyway
-        nc[k] = m;
-        cs[k] = L / (double)m;
-        total *= m;
+        double m = k < R ? std::floor(L / (r * (1 + 1e-9))) : 1.0;
+        if (!(m >= 3.0)) m = 1.0;
+        if (m > 1048576.0) m = 1048576.0;
+        nc[k] = (uint64_t)m;
+        cs[k] = L / m;
     }
-    // particles by cell (counting sort): members[start[c] .. start[c + 1])
-    std::vector<long> cell_of(n), start(total + 1, 0), members(n);
-    for (long i = 0; i < n; ++i) {
-        long c = 0, stride = 1;
-        for (int k = 0; k < R; ++k) {
-            double w = pos[i * R + k] - std::floor(pos[i * R + k] / L) * L;
-            long ck = (long)std::floor(w / cs[k]);
-            ck = ((ck % nc[k]) + nc[k]) % nc[k];
-            c += ck * stride;
-            stride *= nc[k];
-        }
-        cell_of[i] = c;
-        ++start[c + 1];
-    }
-    for (long c = 0; c < total; ++c) start[c + 1] += start[c];
-    {
-        std::vector<long> fill(start.begin(), start.end() - 1);
-        for (long i = 0; i < n; ++i) members[fill[cell_of[i]]++] = i;
+    auto cell_coord = [&](long i, int k) -> uint64_t {
+        if (k >= R) return 0;
+        const double x = pos[i * R + k];
+        const double w = x - std::floor(x / L) * L;
+        long ck = (long)std::floor(w / cs[k]);
+        const long m = (long)nc[k];
+        return (uint64_t)(((ck % m) + m) % m);
+    };
+    // particles sorted by cell id; the occupied cells in a hash map
+    std::vector<std::pair<uint64_t, long>> byc(n);
+    for (long i = 0; i < n; ++i)
+        byc[i] = {cell_coord(i, 0) + nc[0] * (cell_coord(i, 1) + nc[1] * cell_coord(i, 2)), i};
+    std::sort(byc.begin(), byc.end());
+    std::unordered_map<uint64_t, std::pair<long, long>> cells;
+    cells.reserve(2 * n + 1);
+    for (long q = 0; q < n;) {
+        long e = q + 1;
+        while (e < n && byc[e].first == byc[q].first) ++e;
+        cells[byc[q].first] = {q, e};
+        q = e;
     }
     const double rsqr = r * r;
     tags.assign(n, 0);
@@ -415,25 +419,23 @@ static void fof_tags(int R, long n, double L, double r, const double* pos, std::
         while (!stack.empty()) {
             const long a = stack.back();
             stack.pop_back();
-            long ca[3], c = cell_of[a];
-            for (int k = 0; k < 3; ++k) {
-                ca[k] = c % nc[k];
-                c /= nc[k];
-            }
+            const uint64_t ca[3] = {cell_coord(a, 0), cell_coord(a, 1), cell_coord(a, 2)};
             for (int dz = -1; dz <= 1; ++dz)
                 for (int dy = -1; dy <= 1; ++dy)
                     for (int dx = -1; dx <= 1; ++dx) {
                         const int d[3] = {dx, dy, dz};
                         bool skip = false;
-                        long cc = 0, stride = 1;
+                        uint64_t cc = 0, stride = 1;
                         for (int k = 0; k < 3; ++k) {
                             if (nc[k] == 1 && d[k] != 0) skip = true;
-                            cc += ((ca[k] + d[k] + nc[k]) % nc[k]) * stride;
+                            cc += ((ca[k] + nc[k] + d[k]) % nc[k]) * stride;
                             stride *= nc[k];
                         }
                         if (skip) continue;
-                        for (long q = start[cc]; q < start[cc + 1]; ++q) {
-                            const long j = members[q];
+                        auto it = cells.find(cc);
+                        if (it == cells.end()) continue;
+                        for (long q = it->second.first; q < it->second.second; ++q) {
+                            const long j = byc[q].second;
                             if (tags[j] != current && fof_near(pos + a * R, pos + j * R, R, L, rsqr)) {
                                 tags[j] = current;
                                 stack.push_back(j);
