@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libconanpm.so")
 PM_FP64 = 0
 PM_FP32 = 1
 PM_FLAG_WRAP_GATHER = 1
+PM_FLAG_MIDPOINT = 2
 PM_DIFF_STORE_POTENTIALS = 1
 PM_DIFF_VERIFY_CHECKPOINTS = 2
 PM_NPHASES = 10

@@ -8,6 +8,7 @@
 //   ic  --noise FILE     read the white noise from record "noise" of FILE
 //   run --precision P    fp64 (reference arithmetic, default) or fp32
 //   run --wrap-gather    fold positions into the box before the gather
+//   run --midpoint       drift and kick factors at the middle of each step
 //   run --resume FILE    continue from snapshot FILE (<id>.pos.<k>.conan)
 #include <cstring>
 
@@ -224,6 +225,9 @@ void cmd_run(int argc, char** argv) {
          {0, "", "wrap-gather", "false",
           "fold positions into the box before the force interpolation (the reference truncates the "
           "unwrapped coordinate)."},
+         {0, "", "midpoint", "false",
+          "evaluate the drift and kick factors at the middle of each step, a second-order leapfrog (the "
+          "reference evaluates them at the start of the step)."},
          {O::VALUED | O::CHECK, "", "resume", "",
           "continue from snapshot FILE (<id>.pos.<k>.conan of a run with the same parameters): its "
           "positions, momenta and momentum grid are loaded and the run goes on after step 10 k."}});
@@ -259,7 +263,8 @@ void cmd_run(int argc, char** argv) {
     if (phi.size() != np) throw Failure{"potential record does not match N^dim"};
 
     pm_config cfg{dim, (int)N, (int)(f * N), L, H0, Om, OL, prec == "fp64" ? PM_FP64 : PM_FP32,
-                  H.get<bool>("wrap-gather") ? PM_FLAG_WRAP_GATHER : 0u};
+                  (H.get<bool>("wrap-gather") ? PM_FLAG_WRAP_GATHER : 0u) |
+                      (H.get<bool>("midpoint") ? PM_FLAG_MIDPOINT : 0u)};
     pm_handle* h = nullptr;
     pm_ok(pm_create(&cfg, &h));
     std::unique_ptr<pm_handle, void (*)(pm_handle*)> guard(h, pm_destroy);

@@ -79,6 +79,7 @@ static void validate(const pm_config* c) {
         throw cpm::Error(PM_EUNSUP, "Nf must be a power of two in [4, 1024]");
     if (!(c->L > 0) || !std::isfinite(c->L)) throw cpm::Error(PM_EINVAL, "L must be positive");
     if (c->precision != PM_FP64 && c->precision != PM_FP32) throw cpm::Error(PM_EINVAL, "unknown precision");
+    if (c->flags & ~(PM_FLAG_WRAP_GATHER | PM_FLAG_MIDPOINT)) throw cpm::Error(PM_EINVAL, "unknown flags");
     double np = std::pow((double)c->Nm, c->dim);
     if (np > 2147483647.0) throw cpm::Error(PM_EUNSUP, "too many particles");
     int dev = 0;

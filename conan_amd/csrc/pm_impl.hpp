@@ -281,7 +281,11 @@ struct Impl : ImplBase {
         for (long s = 0; s < Np; ++s)
             for (int k = 0; k < R; ++k) out[(size_t)ids[s] * R + k] = soa[(size_t)k * Np + s];
     }
-    double drift_factor(double a, double da) const {
+    // Scale factor at which the factors of a drift or kick over [a, a + da]
+    // are evaluated: its start (the reference) or its middle (PM_FLAG_MIDPOINT).
+    double a_eval(double a, double da) const { return (cfg.flags & PM_FLAG_MIDPOINT) ? a + da / 2 : a; }
+    double drift_factor(double a_in, double da) const {
+        const double a = a_eval(a_in, da);
         double adot = cos.adot(a);
         double c = da / (a * a * adot);  // src/run/nbody.hh:124,129
         return FX ? c * (4294967296.0 / cfg.L) : c;
@@ -467,8 +471,8 @@ struct Impl : ImplBase {
 
     void kick_impl(double a, double da, bool fuse, double a_next, double da_next) {
         do_deposit((Mesh)mass);
-        do_poisson(a);
-        const double adot = cos.adot(a);
+        do_poisson(a_eval(a, da));
+        const double adot = cos.adot(a_eval(a, da));
         const double ck = da / adot;  // src/run/nbody.hh:112-116
         const double cd = fuse ? drift_factor(a_next, da_next) : 0.0;
         prof.begin(7, st);

@@ -42,6 +42,13 @@ extern "C" {
 #define PM_FLAG_WRAP_GATHER 1u /* FP64: fold positions into [0, L) before the gather, as the
                                   deposit does; the reference truncates the unwrapped
                                   coordinate (src/run/numeric.hh:19,26-28) */
+#define PM_FLAG_MIDPOINT 2u    /* drift and kick factors (and the 1/a of the potential) at the
+                                  middle a + da/2 of the interval [a, a + da] each integrates
+                                  over: second order in da.  Without it they are taken at the
+                                  interval's start, as the reference does (src/run/nbody.hh:
+                                  110-129, leapfrog.hh:40-41): first order.  No counterpart in
+                                  the reference; for runs that want the accuracy of a proper
+                                  leapfrog at the same step count. */
 
 /* Configuration of a run: Gravity<R>(mbox, fbox, cosmos) (src/run/nbody.hh:75-86),
  * Cosmology(H0, Omega_m, Omega_L) (src/run/cosmology.hh:11-14). */

@@ -302,6 +302,9 @@ struct Sim {
     double a, da;
     long step;
     double last_apos, last_avel;
+    // factors at the middle of each drift / kick interval instead of its start
+    // (not in the reference, which takes the start: nbody.hh:110-129)
+    int midpoint = 0;
     Sim(int R, int Nm, int Nf, double L, double H0, double Om, double OL, int nthreads_, int wrap_)
         : mbox(R, Nm, L), fbox(R, Nf, L), cos(H0, Om, OL), nthreads(nthreads_), wrap(wrap_),
           X(mbox.size * R), P(mbox.size * R), delta(fbox.size), in(fbox.size), out(fbox.size),
@@ -328,8 +331,9 @@ struct Sim {
     void kick(double ak, double dak) {
         const int R = mbox.R;
         md_cic(fbox, delta.data(), X.data(), mbox.size, mass, nthreads, nullptr);
-        poisson(fbox, delta.data(), cos, ak, nthreads, in, out);
-        double adot = cos.adot(ak);
+        const double ae = midpoint ? ak + dak / 2 : ak;
+        poisson(fbox, delta.data(), cos, ae, nthreads, in, out);
+        double adot = cos.adot(ae);
         double c = dak / adot;
 #pragma omp parallel for num_threads(nthreads) schedule(static)
         for (long long ii = 0; ii < (long long)mbox.size; ++ii) {
@@ -341,8 +345,9 @@ struct Sim {
     }
     void drift(double ad, double dad) {
         const int R = mbox.R;
-        double adot = cos.adot(ad);
-        double c = dad / (ad * ad * adot);
+        const double ae = midpoint ? ad + dad / 2 : ad;
+        double adot = cos.adot(ae);
+        double c = dad / (ae * ae * adot);
 #pragma omp parallel for num_threads(nthreads) schedule(static)
         for (long long ii = 0; ii < (long long)(mbox.size * R); ++ii) X[ii] = X[ii] + P[ii] * c;
         last_apos = ad;
@@ -523,6 +528,7 @@ void* orc_sim_create(int R, int Nm, int Nf, double L, double H0, double Om, doub
     return new Sim(R, Nm, Nf, L, H0, Om, OL, nthreads < 1 ? 1 : nthreads, wrap);
 }
 void orc_sim_destroy(void* s) { delete static_cast<Sim*>(s); }
+void orc_sim_set_midpoint(void* s, int on) { static_cast<Sim*>(s)->midpoint = on; }
 void orc_sim_init(void* sp, const double* phi_m, double a0, double da) {
     Sim* s = static_cast<Sim*>(sp);
     s->init(phi_m, a0, a0 + da / 2);

@@ -41,6 +41,7 @@ def load():
         "orc_adot": (d, [d, d, d, d]),
         "orc_sim_create": (V, [i, i, i, d, d, d, d, i, i]),
         "orc_sim_destroy": (None, [V]),
+        "orc_sim_set_midpoint": (None, [V, i]),
         "orc_sim_init": (None, [V, _dp, d, d]),
         "orc_sim_steps": (d, [V, i]),
         "orc_sim_drift": (None, [V, d, d]),
@@ -149,10 +150,12 @@ def adot(H0, Om, OL, a):
 class Sim:
     """Gravity<R> + LeapFrog on the CPU (src/run/nbody.hh, leapfrog.hh)."""
 
-    def __init__(self, dim, Nm, Nf, L, H0=68.5, Om=1.0, OL=0.0, nthreads=1, wrap=0):
+    def __init__(self, dim, Nm, Nf, L, H0=68.5, Om=1.0, OL=0.0, nthreads=1, wrap=0, midpoint=False):
         self.lib = load()
         self.dim, self.Nm, self.Nf = dim, Nm, Nf
         self.h = self.lib.orc_sim_create(dim, Nm, Nf, L, H0, Om, OL, nthreads, wrap)
+        if midpoint:
+            self.lib.orc_sim_set_midpoint(self.h, 1)
 
     def __del__(self):
         if getattr(self, "h", None):

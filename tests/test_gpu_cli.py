@@ -151,3 +151,18 @@ def test_resume_rejects_a_foreign_snapshot(sim_dir):
                         "--resume", "t.pos.1.conan"], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
                        text=True, timeout=120)
     assert r.returncode != 0 and "a-pos" in (r.stdout + r.stderr)
+
+
+def test_run_midpoint_matches_oracle(sim_dir, orc):
+    from conan_amd import conanfile
+
+    d, _ = sim_dir
+    run(["run", "-i", "t", "-f", "2", "--a0", "0.02", "--da", "0.0098", "-n", "10", "--midpoint"], d)
+    h, hist, rec = conanfile.read(str(d / "t.pos.1.conan"))
+    _, _, irec = conanfile.read(str(d / "t.density.init.conan"))
+    sim = orc.Sim(3, 16, 32, 100.0, midpoint=True)
+    sim.init(irec["potential"][1], 0.02, 0.0098)
+    sim.steps(10)
+    Xo, Po, t = sim.state()
+    assert np.max(np.abs(rec["pos"][1] - Xo)) < 1e-9 * 100
+    assert np.max(np.abs(rec["mom"][1] - Po)) < 1e-9 * np.max(np.abs(Po))

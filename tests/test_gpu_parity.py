@@ -188,6 +188,58 @@ def test_leapfrog_fp64_vs_oracle(pm, orc, dim, Nm, Nf, nsteps):
     assert rel(P, Po) < 1e-9
 
 
+@pytest.mark.parametrize("dim,Nm,Nf,nsteps", [(2, 32, 32, 10), (3, 16, 32, 5)])
+def test_leapfrog_midpoint_fp64_vs_oracle(pm, orc, dim, Nm, Nf, nsteps):
+    """PM_FLAG_MIDPOINT: the same steps with the factors at the middle of each
+    interval; against the oracle run the same way."""
+    L, a0, da = 100.0, 0.02, 0.0098
+    mbits = int(np.log2(Nm))
+    noise = orc.white_noise(3, Nm ** dim)
+    phi = orc.compute_potential(dim, mbits, L, orc.ic_scale_free(dim, mbits, L, -2.0, noise))
+    say("leapfrog midpoint", dim, Nm, Nf, nsteps)
+    sim = orc.Sim(dim, Nm, Nf, L, midpoint=True)
+    sim.init(phi, a0, da)
+    sim.steps(nsteps)
+    Xo, Po, t = sim.state()
+    out = {}
+    for mid in (True, False):
+        with pm.Gravity(dim, Nm, Nf, L, midpoint=mid) as g:
+            g.init(phi, a0, a0 + da / 2)
+            g.steps(a0, da, nsteps)
+            out[mid] = g.get_state()
+    X, P = out[True]
+    assert np.max(np.abs(X - Xo)) < 1e-9 * L
+    assert rel(P, Po) < 1e-9
+    assert np.max(np.abs(X - out[False][0])) > 1e-6 * L  # the flag changes the run
+
+
+def _gpu_plane_wave_amplitude(pm, da, midpoint, prec="fp64"):
+    dim, N, L = 2, 32, 100.0
+    q = np.stack(np.meshgrid(*([np.arange(N) * (L / N)] * dim), indexing="ij"), -1).reshape(-1, dim)[:, ::-1].copy()
+    k = 2 * np.pi / L
+    phi = 1e-4 * np.cos(k * q[:, 0])
+    a0 = 0.02
+    n = int(round((0.5 - a0) / da))
+    with pm.Gravity(dim, N, N, L, precision=prec, midpoint=midpoint) as g:
+        g.init(phi, a0, a0)  # synchronised start: P at a0
+        g.kick(a0, da / 2)   # to the staggered start
+        g.steps(a0, da, n)
+        X, _ = g.get_state()
+    sn = np.sin(k * q[:, 0])
+    dx = (X[:, 0] - q[:, 0] + L / 2) % L - L / 2
+    return float(np.sum(dx * sn) / np.sum(sn * sn))
+
+
+@pytest.mark.parametrize("midpoint,order", [(False, 1), (True, 2)])
+def test_gpu_leapfrog_convergence_order(pm, midpoint, order):
+    """The Richardson ratio of three step sizes on a linear plane wave: first
+    order with the reference's factors, second order with PM_FLAG_MIDPOINT."""
+    say("convergence order, midpoint =", midpoint)
+    s = [_gpu_plane_wave_amplitude(pm, da, midpoint) for da in (0.004, 0.002, 0.001)]
+    p = np.log2(abs(s[0] - s[1]) / abs(s[1] - s[2]))
+    assert abs(p - order) < 0.25
+
+
 def test_steps_fused_equals_unfused(pm, orc):
     dim, Nm, Nf, L, a0, da = 3, 16, 16, 100.0, 0.02, 0.0098
     noise = orc.white_noise(5, Nm ** dim)

@@ -468,3 +468,41 @@ def test_fof_large_linking_length_and_tiny_inputs(orc):
     assert ng == 1 and g[0] == 0
     g, ng = orc.fof(3, 1.0, 0.1, X[:0])
     assert ng == 0 and len(g) == 0
+
+
+# ------------------------------------------------------------------ integrator order
+def _plane_wave_amplitude(orc, da, midpoint):
+    """Displacement amplitude at a = 0.5 of a small plane wave (linear regime)
+    started at a = 0.02 from the synchronised Zel'dovich state (P at a0 too),
+    brought to the staggered start by a half kick, then stepped with da."""
+    dim, N, L = 2, 32, 100.0
+    q = grid(dim, N, L)
+    k = 2 * np.pi / L
+    phi = 1e-4 * np.cos(k * q[:, 0])
+    a0 = 0.02
+    n = int(round((0.5 - a0) / da))
+    sim = orc.Sim(dim, N, N, L, midpoint=midpoint)
+    sim.init(phi, a0, 0.0)
+    X0, P0, _ = sim.state()
+    sim.init(phi, a0, da)
+    sim.set_state(X0, P0)
+    sim.kick(a0, da / 2)
+    sim.steps(n)
+    X, _, t = sim.state()
+    assert t[2] == pytest.approx(0.5, abs=1e-12)
+    sn = np.sin(k * q[:, 0])
+    return float(np.sum((X[:, 0] - q[:, 0]) * sn) / np.sum(sn * sn))
+
+
+@pytest.mark.parametrize("midpoint,order", [(False, 1), (True, 2)])
+def test_leapfrog_convergence_order(orc, midpoint, order):
+    """Halving da shrinks the step error by 2^order: the reference's factors at
+    the start of each interval (nbody.hh:110-129) make the scheme first order;
+    factors at the middle make it a second-order leapfrog.  Richardson ratio
+    of three step sizes, so the spatial discretisation (the same in all three)
+    cancels.  (The reference's own start, P at a0 + da/2 set to -(a0 + da/2) u,
+    moves with da at first order; the synchronised start with a half kick
+    keeps that out of the comparison.)"""
+    s = [_plane_wave_amplitude(orc, da, midpoint) for da in (0.004, 0.002, 0.001)]
+    p = math.log2(abs(s[0] - s[1]) / abs(s[1] - s[2]))
+    assert abs(p - order) < 0.25
