@@ -174,6 +174,7 @@ struct Impl : ImplBase {
     bool resort_ok = false;            // storage order == output order of the last FX sort
     bool keys_ready = false;           // knew / cnt3 hold the keys of the current positions (fused gather)
     DevBuf<Mesh> mesh;
+    DevBuf<Mesh> frac;  // FP64 deposit: fractional cell coordinates in sorted order
     DevBuf<cx<Mesh>> spec, nyq;
     DevBuf<double> row_partial, red, scal;
     const uint32_t* skeys = nullptr;
@@ -430,8 +431,9 @@ struct Impl : ImplBase {
             deposit_fx_brick(R, bstart, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p, st);
             npartials = brick_count(pa);
         } else {
+            if (frac.n < (size_t)R * Np) frac.alloc((size_t)R * Np);
             ParticleOps<PP>::deposit(R, row_start.p, skeys, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p,
-                                     nrows, st);
+                                     nrows, frac.p, st);
             npartials = nrows;
         }
         prof.end(1, st);
@@ -490,8 +492,11 @@ struct Impl : ImplBase {
             resort_ok = true;
             keys_ready = fuse;
         } else {
-            ParticleOps<PP>::gather_kick(R, X.p, P.p, Np, phi_dst(), lgNf, pa, ck, cd, fuse ? 1 : 0, nullptr,
-                                         Reorder<PP>(), st);
+            // threads in the deposit's cell order, particles updated in place
+            Reorder<PP> ro;
+            ro.perm = spids;
+            ro.in_place = true;
+            ParticleOps<PP>::gather_kick(R, X.p, P.p, Np, phi_dst(), lgNf, pa, ck, cd, fuse ? 1 : 0, nullptr, ro, st);
         }
         prof.end(7, st);
         last_avel = a;

@@ -90,11 +90,32 @@ __device__ __forceinline__ double block_sum(double v, double* red, int tid) {
 // in ascending particle order: the order of the reference's serial loop.
 static constexpr int DEP_NT = 256;
 
+// The fractional cell coordinates of every particle, in sorted order
+// (frac[k * np + s] for particle pids[s]): computed once per particle by the
+// same expression the merge would evaluate (PP::dep), so that the merge reads
+// them in sorted order instead of loading a particle's position 2^R times at
+// its storage index.
+template <typename PP, int R>
+__global__ __launch_bounds__(256) void k_dep_frac(const uint32_t* __restrict__ pids,
+                                                   const typename PP::Pos* __restrict__ X, long np, PosArgs pa,
+                                                   typename PP::Mesh* __restrict__ frac) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np) return;
+    const long m = pids[s];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        int o;
+        typename PP::Mesh f;
+        PP::dep(X[k * np + m], pa, o, f);
+        frac[k * np + s] = f;
+    }
+}
+
 template <typename PP, int R>
 __global__ __launch_bounds__(DEP_NT) void k_deposit(const uint32_t* __restrict__ row_start,
                                                      const uint32_t* __restrict__ keys,
                                                      const uint32_t* __restrict__ pids,
-                                                     const typename PP::Pos* __restrict__ X, long np,
+                                                     const typename PP::Mesh* __restrict__ frac, long np,
                                                      int lgN, PosArgs pa, typename PP::Mesh mass,
                                                      typename PP::Mesh* __restrict__ mesh,
                                                      double* __restrict__ row_partial) {
@@ -142,12 +163,14 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit(const uint32_t* __restrict__
                     jm = j;
                 }
             if (jm < 0) break;
+            uint32_t sp = 0;  // sorted position of the particle taken (m = pids[sp])
+#pragma unroll
+            for (int j = 0; j < NC; ++j)
+                if (j == jm) sp = cur[j];
             M z = mass;
 #pragma unroll
             for (int k = 0; k < R; ++k) {
-                int o;
-                M f;
-                PP::dep(X[k * np + m], pa, o, f);
+                const M f = frac[k * np + sp];
                 z *= ((jm >> k) & 1) ? f : (M)1 - f;
             }
             acc += z;
@@ -549,7 +572,10 @@ __device__ __forceinline__ T load_mesh(const T* __restrict__ phi, long idx) {
 
 // With ro.perm set, thread s reads particle perm[s] and writes the updated
 // particle (and its id) to position s of the output arrays: the kick also
-// moves the particles into the cell order of the deposit.
+// moves the particles into the cell order of the deposit.  With ro.in_place
+// as well, thread s updates particle perm[s] where it is: the threads of a
+// wave then handle particles of neighbouring cells, whose potential loads
+// coincide, while the particles stay in their order.
 template <typename PP, int R>
 __global__ __launch_bounds__(256) void k_gather_kick(typename PP::Pos* __restrict__ X, double* __restrict__ P,
                                                       long np, const typename PP::Mesh* __restrict__ phi,
@@ -558,11 +584,12 @@ __global__ __launch_bounds__(256) void k_gather_kick(typename PP::Pos* __restric
     using M = typename PP::Mesh;
     const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= np) return;
+    const bool move = ro.perm && !ro.in_place;
     const long i = ro.perm ? (long)ro.perm[s] : s;  // input index
-    const long w = ro.perm ? s : i;                   // output index
-    typename PP::Pos* __restrict__ Xo = ro.perm ? ro.Xo : X;
-    double* __restrict__ Po = ro.perm ? ro.Po : P;
-    if (ro.perm) ro.ido[s] = ro.idi[i];
+    const long w = move ? s : i;                      // output index
+    typename PP::Pos* __restrict__ Xo = move ? ro.Xo : X;
+    double* __restrict__ Po = move ? ro.Po : P;
+    if (move) ro.ido[s] = ro.idi[i];
     const int N = 1 << lgN, msk = N - 1;
     int o[R];
     M a1[R];
@@ -661,7 +688,7 @@ __global__ __launch_bounds__(256) void k_gather_kick(typename PP::Pos* __restric
                 xn = xs[k] + fx_round_u32(p * cd);
             }
         }
-        if (cd_on || ro.perm) Xo[k * np + w] = xn;
+        if (cd_on || move) Xo[k * np + w] = xn;
     }
 }
 
@@ -905,9 +932,13 @@ void row_bounds(const uint32_t* keys, long np, int lgN, long nrows, uint32_t* ro
 template <typename PP>
 void ParticleOps<PP>::deposit(int R, const uint32_t* row_start, const uint32_t* keys, const uint32_t* pids,
                               const Pos* X, long np, int lgN, const PosArgs& pa, Mesh mass, Mesh* mesh,
-                              double* row_partial, long nrows, hipStream_t st) {
+                              double* row_partial, long nrows, Mesh* frac, hipStream_t st) {
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_dep_frac<PP, RR>), dim3(nblk(np, 256)), dim3(256), 0, st, pids, X, np,
+                                         pa, frac));
+    CPM_LAUNCH_CHECK();
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit<PP, RR>), dim3((unsigned)nrows), dim3(DEP_NT), 0, st,
-                                         row_start, keys, pids, X, np, lgN, pa, mass, mesh, row_partial));
+                                         row_start, keys, pids, (const Mesh*)frac, np, lgN, pa, mass, mesh,
+                                         row_partial));
     CPM_LAUNCH_CHECK();
 }
 

@@ -101,6 +101,7 @@ struct Reorder {
     double* Po = nullptr;
     const uint32_t* idi = nullptr;
     uint32_t* ido = nullptr;
+    bool in_place = false;  // perm only orders the threads; particles are updated where they are
 };
 
 template <typename PP>
@@ -111,7 +112,7 @@ struct ParticleOps {
                      hipStream_t st);
     static void deposit(int R, const uint32_t* row_start, const uint32_t* keys, const uint32_t* pids,
                         const Pos* X, long np, int lgN, const PosArgs& pa, Mesh mass, Mesh* mesh,
-                        double* row_partial, long nrows, hipStream_t st);
+                        double* row_partial, long nrows, Mesh* frac, hipStream_t st);
     static void gather_kick(int R, Pos* X, double* P, long np, const Mesh* phi, int lgN, const PosArgs& pa,
                             double ck, double cd, int cd_on, double* Fout, const Reorder<PP>& ro, hipStream_t st);
     static void drift(Pos* X, const double* P, long n, double cd, hipStream_t st);
