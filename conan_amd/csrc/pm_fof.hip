@@ -434,7 +434,9 @@ struct FofPlan {
         if (st) (void)hipStreamDestroy(st);
     }
 
-    void reserve(long n) {
+    // Device buffers for n particles; the staging copies of host positions
+    // and groups only when the caller passes host memory.
+    void reserve(long n, bool device) {
         // table: about two keys per particle, no more than there are cells
         const double cells = (double)g.nc[0] * g.nc[1] * g.nc[2];
         int tb = ilog2(2 * std::max(n, 1L));
@@ -442,16 +444,18 @@ struct FofPlan {
         tbits = std::max(tb, 1);
         g.kmask = (uint32_t)((1ull << tbits) - 1);
         const long T = 1L << tbits;
+        if (!device && pos.n < (size_t)R * n) {
+            pos.alloc((size_t)R * n);
+            group.alloc(n);
+        }
         if (n > cap) {
             cap = n;
-            pos.alloc((size_t)R * n);
             spos.alloc((size_t)R * n);
             key.alloc(n);
             slot.alloc(n);
             sidx.alloc(n);
             parent.alloc(n);
             flag.alloc(n + 1);
-            group.alloc(n);
             qsub = (uint32_t)((n + FOF_NQ - 1) / FOF_NQ);
             queue.alloc((size_t)FOF_NQ * qsub);
         }
@@ -467,7 +471,7 @@ struct FofPlan {
             fill_stats(0, 0.0);
             return 0;
         }
-        reserve(n);
+        reserve(n, device);
         const uint32_t un = (uint32_t)n;
         const double* dpos = hpos;
         if (!device) {
