@@ -175,6 +175,9 @@ struct Impl : ImplBase {
     bool keys_ready = false;           // knew / cnt3 hold the keys of the current positions (fused gather)
     DevBuf<Mesh> mesh;
     DevBuf<Mesh> frac;  // FP64 deposit: fractional cell coordinates in sorted order
+    DevBuf<Pos> xsort;     // FP64: positions in the deposit's sorted order
+    DevBuf<double> fsort;  // FP64: forces in that order
+    DevBuf<uint32_t> inv;  // FP64: sorted position of each particle
     DevBuf<cx<Mesh>> spec, nyq;
     DevBuf<double> row_partial, red, scal;
     const uint32_t* skeys = nullptr;
@@ -431,9 +434,14 @@ struct Impl : ImplBase {
             deposit_fx_brick(R, bstart, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p, st);
             npartials = brick_count(pa);
         } else {
-            if (frac.n < (size_t)R * Np) frac.alloc((size_t)R * Np);
+            if (frac.n < (size_t)R * Np) {
+                frac.alloc((size_t)R * Np);
+                xsort.alloc((size_t)R * Np);
+                fsort.alloc((size_t)R * Np);
+                inv.alloc(Np);
+            }
             ParticleOps<PP>::deposit(R, row_start.p, skeys, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p,
-                                     nrows, frac.p, st);
+                                     nrows, frac.p, xsort.p, inv.p, st);
             npartials = nrows;
         }
         prof.end(1, st);
@@ -492,11 +500,9 @@ struct Impl : ImplBase {
             resort_ok = true;
             keys_ready = fuse;
         } else {
-            // threads in the deposit's cell order, particles updated in place
-            Reorder<PP> ro;
-            ro.perm = spids;
-            ro.in_place = true;
-            ParticleOps<PP>::gather_kick(R, X.p, P.p, Np, phi_dst(), lgNf, pa, ck, cd, fuse ? 1 : 0, nullptr, ro, st);
+            // forces in the deposit's cell order, then the update in storage order
+            ParticleOps<PP>::gather_kick_sorted(R, X.p, P.p, Np, phi_dst(), lgNf, pa, ck, cd, fuse ? 1 : 0, xsort.p,
+                                                fsort.p, inv.p, st);
         }
         prof.end(7, st);
         last_avel = a;

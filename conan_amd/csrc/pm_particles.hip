@@ -95,10 +95,13 @@ static constexpr int DEP_NT = 256;
 // same expression the merge would evaluate (PP::dep), so that the merge reads
 // them in sorted order instead of loading a particle's position 2^R times at
 // its storage index.
+// Also, when Xs and inv are given, the positions in sorted order and the
+// inverse order (inv[pids[s]] = s), for the FP64 kick's two passes.
 template <typename PP, int R>
 __global__ __launch_bounds__(256) void k_dep_frac(const uint32_t* __restrict__ pids,
                                                    const typename PP::Pos* __restrict__ X, long np, PosArgs pa,
-                                                   typename PP::Mesh* __restrict__ frac) {
+                                                   typename PP::Mesh* __restrict__ frac,
+                                                   typename PP::Pos* __restrict__ Xs, uint32_t* __restrict__ inv) {
     const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= np) return;
     const long m = pids[s];
@@ -106,9 +109,12 @@ __global__ __launch_bounds__(256) void k_dep_frac(const uint32_t* __restrict__ p
     for (int k = 0; k < R; ++k) {
         int o;
         typename PP::Mesh f;
-        PP::dep(X[k * np + m], pa, o, f);
+        const typename PP::Pos x = X[k * np + m];
+        PP::dep(x, pa, o, f);
         frac[k * np + s] = f;
+        if (Xs) Xs[k * np + s] = x;
     }
+    if (inv) inv[m] = (uint32_t)s;
 }
 
 template <typename PP, int R>
@@ -570,35 +576,18 @@ __device__ __forceinline__ T load_mesh(const T* __restrict__ phi, long idx) {
     return phi[idx];
 }
 
-// With ro.perm set, thread s reads particle perm[s] and writes the updated
-// particle (and its id) to position s of the output arrays: the kick also
-// moves the particles into the cell order of the deposit.  With ro.in_place
-// as well, thread s updates particle perm[s] where it is: the threads of a
-// wave then handle particles of neighbouring cells, whose potential loads
-// coincide, while the particles stay in their order.
+// The force at one particle: the trilinear interpolation of the 4-point FD
+// gradient at the 2^R surrounding nodes (src/run/numeric.hh:17-44,86-102),
+// in the reference's operation order.
 template <typename PP, int R>
-__global__ __launch_bounds__(256) void k_gather_kick(typename PP::Pos* __restrict__ X, double* __restrict__ P,
-                                                      long np, const typename PP::Mesh* __restrict__ phi,
-                                                      int lgN, PosArgs pa, double ck, double cd, int cd_on,
-                                                      double* __restrict__ Fout, Reorder<PP> ro) {
+__device__ __forceinline__ void gather_force(const typename PP::Pos* xs, const typename PP::Mesh* __restrict__ phi,
+                                             int lgN, const PosArgs& pa, typename PP::Mesh* F) {
     using M = typename PP::Mesh;
-    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= np) return;
-    const bool move = ro.perm && !ro.in_place;
-    const long i = ro.perm ? (long)ro.perm[s] : s;  // input index
-    const long w = move ? s : i;                      // output index
-    typename PP::Pos* __restrict__ Xo = move ? ro.Xo : X;
-    double* __restrict__ Po = move ? ro.Po : P;
-    if (move) ro.ido[s] = ro.idi[i];
     const int N = 1 << lgN, msk = N - 1;
     int o[R];
     M a1[R];
-    typename PP::Pos xs[R];
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-        xs[k] = X[k * np + i];
-        PP::gat(xs[k], pa, o[k], a1[k]);
-    }
+    for (int k = 0; k < R; ++k) PP::gat(xs[k], pa, o[k], a1[k]);
     // corner weights z_j = prod_k A[bit_k(j)][k] in the reference's order
     constexpr int NC = 1 << R;
     M z[NC];
@@ -621,7 +610,6 @@ __global__ __launch_bounds__(256) void k_gather_kick(typename PP::Pos* __restric
     M resv;
     if constexpr (sizeof(M) == 8) resv = (M)pa.res;
     else resv = (M)(1.0 / (12.0 * pa.res));
-    M F[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) F[k] = 0;
     // axis 0: lines along x through the (y, z) corners
@@ -671,6 +659,33 @@ __global__ __launch_bounds__(256) void k_gather_kick(typename PP::Pos* __restric
 #pragma unroll
         for (int j = 0; j < NC; ++j) F[2] += g2[j] * z[j];
     }
+}
+
+// With ro.perm set, thread s reads particle perm[s] and writes the updated
+// particle (and its id) to position s of the output arrays: the kick also
+// moves the particles into the cell order of the deposit.  With ro.in_place
+// as well, thread s updates particle perm[s] where it is: the threads of a
+// wave then handle particles of neighbouring cells, whose potential loads
+// coincide, while the particles stay in their order.
+template <typename PP, int R>
+__global__ __launch_bounds__(256) void k_gather_kick(typename PP::Pos* __restrict__ X, double* __restrict__ P,
+                                                      long np, const typename PP::Mesh* __restrict__ phi,
+                                                      int lgN, PosArgs pa, double ck, double cd, int cd_on,
+                                                      double* __restrict__ Fout, Reorder<PP> ro) {
+    using M = typename PP::Mesh;
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np) return;
+    const bool move = ro.perm && !ro.in_place;
+    const long i = ro.perm ? (long)ro.perm[s] : s;  // input index
+    const long w = move ? s : i;                      // output index
+    typename PP::Pos* __restrict__ Xo = move ? ro.Xo : X;
+    double* __restrict__ Po = move ? ro.Po : P;
+    if (move) ro.ido[s] = ro.idi[i];
+    typename PP::Pos xs[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) xs[k] = X[k * np + i];
+    M F[R];
+    gather_force<PP, R>(xs, phi, lgN, pa, F);
     if (Fout) {
 #pragma unroll
         for (int k = 0; k < R; ++k) Fout[k * np + i] = (double)F[k];
@@ -689,6 +704,45 @@ __global__ __launch_bounds__(256) void k_gather_kick(typename PP::Pos* __restric
             }
         }
         if (cd_on || move) Xo[k * np + w] = xn;
+    }
+}
+
+
+// The FP64 kick in two passes.  k_gather_force_sorted: the force at every
+// particle in the deposit's cell order (positions copied into that order by
+// k_dep_frac), so the threads of a wave read the same potential lines and
+// their positions contiguously; k_kick_drift_inv: the kick (+ drift) in the
+// particles' own order, reading each force through the inverse order.  The
+// arithmetic is k_gather_kick's, operation for operation.
+template <typename PP, int R>
+__global__ __launch_bounds__(256) void k_gather_force_sorted(const typename PP::Pos* __restrict__ Xs, long np,
+                                                              const typename PP::Mesh* __restrict__ phi, int lgN,
+                                                              PosArgs pa, double* __restrict__ Fs) {
+    using M = typename PP::Mesh;
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np) return;
+    typename PP::Pos xs[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) xs[k] = Xs[k * np + s];
+    M F[R];
+    gather_force<PP, R>(xs, phi, lgN, pa, F);
+#pragma unroll
+    for (int k = 0; k < R; ++k) Fs[k * np + s] = (double)F[k];
+}
+
+template <typename PP, int R>
+__global__ __launch_bounds__(256) void k_kick_drift_inv(typename PP::Pos* __restrict__ X, double* __restrict__ P,
+                                                         long np, const double* __restrict__ Fs,
+                                                         const uint32_t* __restrict__ inv, double ck, double cd,
+                                                         int cd_on) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const long s = inv[i];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const double p = P[k * np + i] - Fs[k * np + s] * ck;
+        P[k * np + i] = p;
+        if (cd_on) X[k * np + i] = X[k * np + i] + p * cd;
     }
 }
 
@@ -932,13 +986,26 @@ void row_bounds(const uint32_t* keys, long np, int lgN, long nrows, uint32_t* ro
 template <typename PP>
 void ParticleOps<PP>::deposit(int R, const uint32_t* row_start, const uint32_t* keys, const uint32_t* pids,
                               const Pos* X, long np, int lgN, const PosArgs& pa, Mesh mass, Mesh* mesh,
-                              double* row_partial, long nrows, Mesh* frac, hipStream_t st) {
+                              double* row_partial, long nrows, Mesh* frac, Pos* Xs, uint32_t* inv,
+                              hipStream_t st) {
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_dep_frac<PP, RR>), dim3(nblk(np, 256)), dim3(256), 0, st, pids, X, np,
-                                         pa, frac));
+                                         pa, frac, Xs, inv));
     CPM_LAUNCH_CHECK();
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit<PP, RR>), dim3((unsigned)nrows), dim3(DEP_NT), 0, st,
                                          row_start, keys, pids, (const Mesh*)frac, np, lgN, pa, mass, mesh,
                                          row_partial));
+    CPM_LAUNCH_CHECK();
+}
+
+template <typename PP>
+void ParticleOps<PP>::gather_kick_sorted(int R, Pos* X, double* P, long np, const Mesh* phi, int lgN,
+                                         const PosArgs& pa, double ck, double cd, int cd_on, const Pos* Xs,
+                                         double* Fs, const uint32_t* inv, hipStream_t st) {
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_force_sorted<PP, RR>), dim3(nblk(np, 256)), dim3(256), 0, st, Xs,
+                                         np, phi, lgN, pa, Fs));
+    CPM_LAUNCH_CHECK();
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_kick_drift_inv<PP, RR>), dim3(nblk(np, 256)), dim3(256), 0, st, X, P, np,
+                                         (const double*)Fs, inv, ck, cd, cd_on));
     CPM_LAUNCH_CHECK();
 }
 

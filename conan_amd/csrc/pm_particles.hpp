@@ -112,9 +112,15 @@ struct ParticleOps {
                      hipStream_t st);
     static void deposit(int R, const uint32_t* row_start, const uint32_t* keys, const uint32_t* pids,
                         const Pos* X, long np, int lgN, const PosArgs& pa, Mesh mass, Mesh* mesh,
-                        double* row_partial, long nrows, Mesh* frac, hipStream_t st);
+                        double* row_partial, long nrows, Mesh* frac, Pos* Xs, uint32_t* inv, hipStream_t st);
     static void gather_kick(int R, Pos* X, double* P, long np, const Mesh* phi, int lgN, const PosArgs& pa,
                             double ck, double cd, int cd_on, double* Fout, const Reorder<PP>& ro, hipStream_t st);
+    // FP64 kick (+ drift) in two passes: forces in the deposit's cell order
+    // from Xs (positions in that order), then the update in storage order
+    // through inv (the inverse order); both from the last deposit.
+    static void gather_kick_sorted(int R, Pos* X, double* P, long np, const Mesh* phi, int lgN, const PosArgs& pa,
+                                   double ck, double cd, int cd_on, const Pos* Xs, double* Fs, const uint32_t* inv,
+                                   hipStream_t st);
     static void drift(Pos* X, const double* P, long n, double cd, hipStream_t st);
     static void zeldovich(int R, const double* phi, int lgNm, double L, double a_pos, double a_mom, Pos* X,
                           double* P, hipStream_t st);
