@@ -240,6 +240,30 @@ def test_gpu_leapfrog_convergence_order(pm, midpoint, order):
     assert abs(p - order) < 0.25
 
 
+@pytest.mark.parametrize("dim,Nm,Nf,wrap", [(3, 32, 64, False), (3, 32, 64, True), (2, 64, 64, False),
+                                            (3, 16, 8, False)])
+def test_kick_is_the_gather_of_its_potential_fp64(pm, orc, dim, Nm, Nf, wrap):
+    """pm_kick takes its forces by bricks with the potential in LDS and the
+    particles in the deposit's cell order; they must equal the direct gather
+    of the same potential (pm_gather_mesh) bit for bit.  Positions include
+    negative and out-of-box coordinates (a gather origin outside the brick)
+    and a dense blob (long row parts)."""
+    L, a, da = 100.0, 0.5, 0.02
+    X = _positions(dim, Nm, L, 3.0 * L / Nm)
+    X[1::3] = 37.0 + 0.05 * RNG.standard_normal(X[1::3].shape)
+    P = RNG.standard_normal(X.shape)
+    say("kick vs gather of its potential", dim, Nm, Nf, wrap)
+    with pm.Gravity(dim, Nm, Nf, L, wrap_gather=wrap) as g:
+        g.set_state(X, P)
+        phi = g.potential(a)
+        F = g.gather(phi)
+        g.kick(a, da)
+        X2, P2 = g.get_state()
+    ck = da / orc.adot(68.5, 1.0, 0.0, a)
+    assert np.array_equal(X2, X)
+    assert np.array_equal(P2, P - F * ck)
+
+
 def test_steps_fused_equals_unfused(pm, orc):
     dim, Nm, Nf, L, a0, da = 3, 16, 16, 100.0, 0.02, 0.0098
     noise = orc.white_noise(5, Nm ** dim)
