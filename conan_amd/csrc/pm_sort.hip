@@ -14,6 +14,9 @@ static constexpr int SORT_ITEMS = 16;
 static constexpr int SORT_TILE = SORT_NT * SORT_ITEMS;
 static constexpr int SORT_WAVES = SORT_NT / 64;
 static constexpr int MAX_RADIX = 512;
+#ifndef CPM_SORT_BITS
+#define CPM_SORT_BITS 8  // most bits per pass (27-bit keys: 4 passes of 7)
+#endif
 
 __global__ __launch_bounds__(SORT_NT) void k_rs_hist(const uint32_t* __restrict__ keys, long n, int shift,
                                                      int dbits, uint32_t* __restrict__ counts, int nblocks) {
@@ -215,7 +218,7 @@ void exclusive_scan(const uint32_t* in, uint32_t* out, long n, uint32_t* scratch
 
 // ------------------------------------------------------------ sort driver
 RadixSorter::RadixSorter(long n_, int keybits_) : n(n_), keybits(keybits_) {
-    passes = (keybits + 8) / 9;
+    passes = (keybits + CPM_SORT_BITS - 1) / CPM_SORT_BITS;
     if (passes < 1) passes = 1;
     dbits = (keybits + passes - 1) / passes;
     nblocks = (int)((n + SORT_TILE - 1) / SORT_TILE);
