@@ -10,7 +10,7 @@ size_t scan_scratch_elems(long n);
 void exclusive_scan(const uint32_t* in, uint32_t* out, long n, uint32_t* scratch, hipStream_t st);
 
 // Stable LSD radix sort of n (key, value) pairs on the low keybits bits,
-// 9 or fewer bits per pass.
+// 8 or fewer bits per pass.
 struct RadixSorter {
     long n;
     int keybits, passes, dbits, nblocks;
