@@ -242,6 +242,12 @@ struct Impl : ImplBase {
             tot.alloc(nsort_groups + 1);
             scan_scr.alloc(scan_scratch_elems(nsort_groups + 1));
         }
+        if (!FX) {  // the FP64 deposit's and kick's sorted-order copies
+            frac.alloc((size_t)R * Np);
+            xsort.alloc((size_t)R * Np);
+            fsort.alloc((size_t)R * Np);
+            inv.alloc(Np);
+        }
         mesh.alloc(M);
         spec.alloc(fft.spec_elems);
         nyq.alloc(fft.nyq_elems);
@@ -434,12 +440,6 @@ struct Impl : ImplBase {
             deposit_fx_brick(R, bstart, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p, st);
             npartials = brick_count(pa);
         } else {
-            if (frac.n < (size_t)R * Np) {
-                frac.alloc((size_t)R * Np);
-                xsort.alloc((size_t)R * Np);
-                fsort.alloc((size_t)R * Np);
-                inv.alloc(Np);
-            }
             ParticleOps<PP>::deposit(R, row_start.p, skeys, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p,
                                      nrows, frac.p, xsort.p, inv.p, st);
             npartials = nrows;
