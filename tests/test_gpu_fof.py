@@ -143,21 +143,23 @@ def _modulus(a, b):
     return a - int(a / b - 1) * b if a < 0 else a - int(a / b) * b
 
 
-def test_cli_fof_output(pm, orc, tmp_path):
+@pytest.mark.parametrize("dim,Nm,Nf", [(3, 32, 64), (2, 128, 256)])
+def test_cli_fof_output(pm, orc, tmp_path, dim, Nm, Nf):
     from conan_amd import conanfile
 
-    dim, Nm, Nf, L, a0, da, nsteps = 3, 32, 64, 100.0, 0.02, 0.0392, 25
+    L, a0, da, nsteps = 100.0, 0.02, 0.0392, 25
+    mbits = int(np.log2(Nm))
     noise = orc.white_noise(23, Nm ** dim)
-    phi = orc.compute_potential(dim, 5, L, orc.ic_scale_free(dim, 5, L, -2.0, noise))
+    phi = orc.compute_potential(dim, mbits, L, orc.ic_scale_free(dim, mbits, L, -2.0, noise))
     with pm.Gravity(dim, Nm, Nf, L) as g:
         g.init(phi, a0, a0 + da / 2)
         g.steps(a0, da, nsteps)
         X, P = g.get_state()
     hdr = {"id": "s", "N": Nm, "L": "100", "dim": dim, "a-mom": "1", "a-pos": "1"}
     conanfile.write(str(tmp_path / "s.pos.10000.conan"), hdr, [(1700000000, "ran nbody")],
-                    {"pos": ("3f8", X), "mom": ("3f8", P)})
+                    {"pos": (f"{dim}f8", X), "mom": (f"{dim}f8", P)})
     ll = 0.2 * L / Nm
-    say("conan fof")
+    say("conan fof", dim, Nm)
     r = subprocess.run([CONAN, "fof", "-i", "s", "-t", "1.0", "-l", repr(ll)], cwd=tmp_path, stdout=subprocess.PIPE,
                        stderr=subprocess.PIPE, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
@@ -170,11 +172,10 @@ def test_cli_fof_output(pm, orc, tmp_path):
     hl = head.splitlines()
     assert hl[0] == "# header" and hl[-1] == ""
     keys = dict(l[2:].split("=", 1) for l in hl[1:-1])
-    assert keys["linking-length"] == repr(ll) and keys["id"] == "s" and keys["N"] == "32"
+    assert keys["linking-length"] == repr(ll) and keys["id"] == "s" and keys["N"] == str(Nm)
     assert keys["time"] == "1.0" and keys["ascii"] == "true"
     hist, body = rest.split("\n\n", 1)
     assert hist.splitlines()[0] == "# 1700000000:ran nbody" and "fof" in hist.splitlines()[1]
-    first = {}
     members = [[] for _ in range(ng)]
     for i, t in enumerate(go):
         members[t].append(i)
