@@ -95,31 +95,51 @@ def run_form(pm, phi, prec, steps, warmup, wrap=False, profile=True, pk_bins=64,
         g.sync()
         if profile:
             g.profile(1)
+        # the first and the last 10 steps timed apart (nearly uniform vs clustered)
+        chunks = [10, steps - 20, 10] if steps >= 30 else [steps]
         t0 = time.perf_counter()
-        a = g.steps(A0, DA, steps)  # returns after the stream is synchronised
+        a, tc = A0, []
+        for n in chunks:
+            c0 = time.perf_counter()
+            a = g.steps(a, DA, n)  # returns after the stream is synchronised
+            tc.append((time.perf_counter() - c0) / n)
         t1 = time.perf_counter()
         ms, cnt = g.profile(0) if profile else (None, None)
         k, p, c = g.power_spectrum(pk_bins)
         diag = g.diagnostics()
-        X = g.get_state()[0] if keep_state else None
-        return {"seconds": t1 - t0, "a": a, "phase_ms": ms, "phase_cnt": cnt, "pk": (k, p, c), "diag": diag, "X": X}
+        state = g.get_state() if keep_state else None
+        return {"seconds": t1 - t0, "a": a, "phase_ms": ms, "phase_cnt": cnt, "pk": (k, p, c), "diag": diag,
+                "state": state, "first_ms": tc[0] * 1e3, "last_ms": tc[-1] * 1e3}
     finally:
         g.close()
 
 
-def cpu_baseline(phi, nsteps: int, threads: int):
+def cpu_baseline(phi, nsteps: int, threads: int, late=None):
     """The oracle (FP64 CPU restatement of the reference, OpenMP loops and the
-    per-add critical section of the reference's deposit) on a bounded sample."""
+    per-add critical section of the reference's deposit) on a bounded sample:
+    the first step(s) of the run, and, given late = (X, P, a), one step from
+    that clustered state."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle as orc
 
     sim = orc.Sim(DIM, NM, NF, L, nthreads=threads)
     sim.init(phi, A0, DA)
     secs = sim.steps(nsteps)  # steady_clock around the leapfrog loop only
-    return {"value": NP * nsteps / secs, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{nsteps} leapfrog step(s) (drift+deposit+FFT Poisson+gather+kick) of the same "
-                      f"256^3/512^3 FP64 run from the same potential, {threads} OpenMP threads, "
-                      f"{secs:.1f} s"}
+    out = {"value": NP * nsteps / secs, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+           "sample": f"{nsteps} leapfrog step(s) (drift+deposit+FFT Poisson+gather+kick) of the same "
+                     f"256^3/512^3 FP64 run from the same potential, {threads} OpenMP threads, "
+                     f"{secs:.1f} s"}
+    if late is not None:
+        X, P, a = late
+        sim.set_state(X, P)
+        t0 = time.perf_counter()
+        sim.drift(a, DA)
+        sim.kick(a + DA / 2, DA)
+        s2 = time.perf_counter() - t0
+        out["late_value"] = NP / s2
+        out["late_sample"] = (f"1 step from the state of the FP32 run at a = {a:.3f} (clustered), "
+                              f"{threads} OpenMP threads, {s2:.1f} s")
+    return out
 
 
 FOF_B = 0.2  # linking length in units of the mean particle spacing
@@ -262,7 +282,7 @@ def main():
     del noise
 
     log(f"FP32 form: {args.warmup} warmup + {args.steps} timed steps")
-    r32 = run_form(pm, phi, "fp32", args.steps, args.warmup, keep_state=not args.no_fof)
+    r32 = run_form(pm, phi, "fp32", args.steps, args.warmup, keep_state=not (args.no_fof and args.no_cpu))
     log(f"  {r32['seconds']:.3f} s -> {r32['seconds'] / args.steps * 1e3:.3f} ms/step")
     r64 = None
     if not args.no_fp64:
@@ -317,11 +337,13 @@ def main():
                    "particles": NP, "mesh": NF, "form": "fp32"},
         "roofline": roofline,
         "step_roofline": step_roof,
+        "ms_per_step_first10": round(r32["first_ms"], 4), "ms_per_step_last10": round(r32["last_ms"], 4),
         "phases_fp32": ph32,
     }
     if r64 is not None:
         t64 = r64["seconds"] / args.steps
         out["fp64"] = {"value": round(NP / t64, 1), "ms_per_step": round(t64 * 1e3, 4),
+                       "ms_per_step_first10": round(r64["first_ms"], 4), "ms_per_step_last10": round(r64["last_ms"], 4),
                        "step_roofline_frac": round(step_bytes("fp64") / t64 / PEAK_HBM, 4),
                        "phases": phase_report(r64, "fp64")}
         out["fp32_over_fp64"] = round(t64 / t32, 3)
@@ -351,7 +373,7 @@ def main():
         out["differentiable"] = runs
     if not args.no_fof:
         log(f"friends-of-friends on the final FP32 snapshot (b = {FOF_B})")
-        out["fof"] = fof_bench(pm, r32["X"], cpu=not args.no_cpu)
+        out["fof"] = fof_bench(pm, r32["state"][0], cpu=not args.no_cpu)
         log(f"  {out['fof']}")
     out["final_a"] = r32["a"]
     out["momentum_sum_fp32"] = r32["diag"]["momentum"]
@@ -359,7 +381,8 @@ def main():
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"CPU baseline: oracle, {args.cpu_steps} step(s), {threads} threads")
-        out["cpu_baseline"] = cpu_baseline(phi, args.cpu_steps, threads)
+        X32, P32 = r32["state"]
+        out["cpu_baseline"] = cpu_baseline(phi, args.cpu_steps, threads, late=(X32, P32, r32["a"]))
         log(f"  {out['cpu_baseline']['sample']}")
     else:
         out["cpu_baseline"] = None
