@@ -727,7 +727,7 @@ __global__ __launch_bounds__(256) void k_gather_force_sorted(const typename PP::
     M F[R];
     gather_force<PP, R>(xs, phi, lgN, pa, F);
 #pragma unroll
-    for (int k = 0; k < R; ++k) Fs[k * np + s] = (double)F[k];
+    for (int k = 0; k < R; ++k) Fs[s * R + k] = (double)F[k];
 }
 
 template <typename PP, int R>
@@ -740,7 +740,7 @@ __global__ __launch_bounds__(256) void k_kick_drift_inv(typename PP::Pos* __rest
     const long s = inv[i];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-        const double p = P[k * np + i] - Fs[k * np + s] * ck;
+        const double p = P[k * np + i] - Fs[s * R + k] * ck;
         P[k * np + i] = p;
         if (cd_on) X[k * np + i] = X[k * np + i] + p * cd;
     }
