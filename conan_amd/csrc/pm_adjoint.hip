@@ -74,7 +74,7 @@ __global__ __launch_bounds__(1024) void k_max_u32(const uint32_t* __restrict__ b
     red[threadIdx.x] = m;
     __syncthreads();
     for (int off = 512; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + off]);
+        if ((int)threadIdx.x < off) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + off]);
         __syncthreads();
     }
     if (threadIdx.x == 0) *lmax = red[0];
@@ -463,7 +463,7 @@ __global__ __launch_bounds__(DOT_NT) void k_dot_partial(const double* __restrict
     red[threadIdx.x] = s;
     __syncthreads();
     for (int off = DOT_NT / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
         __syncthreads();
     }
     if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
@@ -476,7 +476,7 @@ __global__ __launch_bounds__(DOT_NT) void k_dot_final(const double* __restrict__
     red[threadIdx.x] = s;
     __syncthreads();
     for (int off = DOT_NT / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
         __syncthreads();
     }
     if (threadIdx.x == 0) *out = red[0];
@@ -514,7 +514,7 @@ __global__ __launch_bounds__(256) void k_state_diff(const uint32_t* __restrict__
     r2[threadIdx.x] = mp;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < off) {
+        if ((int)threadIdx.x < off) {
             r0[threadIdx.x] += r0[threadIdx.x + off];
             r1[threadIdx.x] = fmax(r1[threadIdx.x], r1[threadIdx.x + off]);
             r2[threadIdx.x] = fmax(r2[threadIdx.x], r2[threadIdx.x + off]);

@@ -122,7 +122,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit(const uint32_t* __restrict__
                                                      const uint32_t* __restrict__ keys,
                                                      const uint32_t* __restrict__ pids,
                                                      const typename PP::Mesh* __restrict__ frac, long np,
-                                                     int lgN, PosArgs pa, typename PP::Mesh mass,
+                                                     int lgN, typename PP::Mesh mass,
                                                      typename PP::Mesh* __restrict__ mesh,
                                                      double* __restrict__ row_partial) {
     using M = typename PP::Mesh;
@@ -816,7 +816,7 @@ __global__ __launch_bounds__(RED_NT) void k_partial_sum(const T* __restrict__ a,
     red[threadIdx.x] = s;
     __syncthreads();
     for (int off = RED_NT / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
         __syncthreads();
     }
     if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
@@ -830,7 +830,7 @@ __global__ __launch_bounds__(RED_NT) void k_final_sum(const double* __restrict__
     red[threadIdx.x] = s;
     __syncthreads();
     for (int off = RED_NT / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
         __syncthreads();
     }
     if (threadIdx.x == 0) *out = red[0] * scale;
@@ -846,7 +846,7 @@ __global__ __launch_bounds__(RED_NT) void k_partial_max_abs(const double* __rest
     red[threadIdx.x] = m;
     __syncthreads();
     for (int off = RED_NT / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+        if ((int)threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
         __syncthreads();
     }
     if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
@@ -860,7 +860,7 @@ __global__ __launch_bounds__(RED_NT) void k_final_max(const double* __restrict__
     red[threadIdx.x] = m;
     __syncthreads();
     for (int off = RED_NT / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+        if ((int)threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
         __syncthreads();
     }
     if (threadIdx.x == 0) *out = red[0];
@@ -992,7 +992,7 @@ void ParticleOps<PP>::deposit(int R, const uint32_t* row_start, const uint32_t* 
                                          pa, frac, Xs, inv));
     CPM_LAUNCH_CHECK();
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit<PP, RR>), dim3((unsigned)nrows), dim3(DEP_NT), 0, st,
-                                         row_start, keys, pids, (const Mesh*)frac, np, lgN, pa, mass, mesh,
+                                         row_start, keys, pids, (const Mesh*)frac, np, lgN, mass, mesh,
                                          row_partial));
     CPM_LAUNCH_CHECK();
 }

@@ -184,7 +184,7 @@ __global__ __launch_bounds__(PK_NT) void k_pk_final(const double* __restrict__ p
     s[2][threadIdx.x] = W;
     __syncthreads();
     for (int off = PK_NT / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off)
+        if ((int)threadIdx.x < off)
             for (int q = 0; q < 3; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + off];
         __syncthreads();
     }
