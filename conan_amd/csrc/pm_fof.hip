@@ -198,6 +198,7 @@ __global__ __launch_bounds__(FOF_NT) void k_fof_place(const double* __restrict__
 // single thread runs a long serial loop.  Should the queue be full, the
 // thread scans the whole range itself.
 static constexpr uint32_t FOF_CAP = 32;
+static constexpr uint32_t FOF_UNROLL = 2;  // candidates loaded together per round (4 measured slower)
 // Counters and the queue are spread over many addresses: atomics on a single
 // address from every wave of the grid would serialise.
 static constexpr int FOF_NSLOT = 1024;  // counter slots (tests, links)
@@ -253,7 +254,26 @@ __global__ __launch_bounds__(FOF_NT) void k_fof_link(const double* __restrict__ 
                     hi = lo + FOF_CAP;
                 }
             }
-            for (uint32_t t = lo; t < hi; ++t) {
+            // FOF_UNROLL candidates per round: their positions in flight together
+            uint32_t t = lo;
+            for (; t + FOF_UNROLL <= hi; t += FOF_UNROLL) {
+                double p[FOF_UNROLL][3];
+#pragma unroll
+                for (int u = 0; u < FOF_UNROLL; ++u)
+#pragma unroll
+                    for (int k = 0; k < R; ++k) p[u][k] = spos[(size_t)(t + u) * R + k];
+                ntest += FOF_UNROLL;
+                bool near[FOF_UNROLL];
+#pragma unroll
+                for (int u = 0; u < FOF_UNROLL; ++u) near[u] = fof_near<R>(a, p[u], g);
+#pragma unroll
+                for (int u = 0; u < FOF_UNROLL; ++u)
+                    if (near[u]) {
+                        ++nlink;
+                        rs = uf_unite(parent, rs, sidx[t + u]);
+                    }
+            }
+            for (; t < hi; ++t) {
                 double p[3];
 #pragma unroll
                 for (int k = 0; k < R; ++k) p[k] = spos[(size_t)t * R + k];
