@@ -400,3 +400,59 @@ def test_fp32_incremental_resort_equals_full_sort(pm, orc, dim, Nm, Nf):
         Xb, Pb = g.get_state()
         rb = g.deposit()
     assert np.array_equal(Xa, Xb) and np.array_equal(Pa, Pb) and np.array_equal(ra, rb)
+
+
+def test_full_size_run(pm, orc):
+    """The 3-D mixed-precision configuration at its real size: 256^3 particles
+    on a 512^3 mesh, L = 100, P(k) ~ k^-1.5 from seeded white noise, a = 0.02
+    to 1 in 200 steps.  The oracle cannot run it in test time; parity is shown
+    through what does not depend on the size: the FP64 form's cell keys and
+    deposit of its final, clustered state are bit-exact against the oracle's;
+    the FP32 form's P(k) agrees with the FP64 form's (folded gather) within
+    1e-3 per bin, its positions and density track the FP64 ones; mass is
+    conserved; a second FP32 run ends bit-identical."""
+    dim, Nm, Nf, L, ns, a0, n = 3, 256, 512, 100.0, -1.5, 0.02, 200
+    da = (1.0 - a0) / n
+    say("full size: IC", Nm, Nf)
+    noise = np.random.default_rng(1).standard_normal(Nm ** dim)
+    _, phi = pm.ic_scale_free(dim, 8, L, ns, noise)
+    del noise
+    runs = {}
+    for name, prec, wrap in (("fp32", "fp32", False), ("fp32b", "fp32", False), ("fp64", "fp64", True)):
+        say("full size:", name, n, "steps")
+        with pm.Gravity(dim, Nm, Nf, L, precision=prec, wrap_gather=wrap) as g:
+            g.init(phi, a0, a0 + da / 2)
+            a = g.steps(a0, da, n)
+            assert abs(a - 1.0) < 1e-12
+            X, P = g.get_state()
+            pk = g.power_spectrum(64)
+            if prec == "fp64":
+                rho, keys = g.deposit(keys=True)
+                runs[name] = (X, P, pk, rho, keys)
+            else:
+                runs[name] = (X, P, pk, None if name == "fp32b" else g.deposit(), None)
+    X32, P32, (k32, p32, c32), r32, _ = runs["fp32"]
+    X32b, P32b = runs.pop("fp32b")[:2]
+    assert np.array_equal(X32, X32b) and np.array_equal(P32, P32b)
+    del X32b, P32b
+    X64, P64, (k64, p64, c64), r64, keys = runs["fp64"]
+
+    say("full size: oracle keys and deposit")
+    assert np.array_equal(keys, orc.cic_keys(dim, Nf, L, X64))
+    assert np.array_equal(r64, orc.md_cic(dim, Nf, L, X64, (Nf / Nm) ** dim))
+    assert abs(r64.sum() / Nf ** dim - 1) < 1e-12
+    assert abs(r32.sum() / Nf ** dim - 1) < 1e-6
+    assert r64.max() > 100  # the end state is clustered
+
+    m = (c32 > 0) & (p64 > 0)
+    assert m.sum() > 50 and np.array_equal(c32, c64) and np.array_equal(k32, k64)
+    pk_rel = np.max(np.abs(p32[m] / p64[m] - 1))
+    dx = np.abs(np.mod(X32, L) - np.mod(X64, L))
+    dx = np.minimum(dx, L - dx)
+    dx_rms = np.sqrt(np.mean(dx ** 2)) / (L / Nf)
+    rho_rms = np.sqrt(np.mean((r32 - r64) ** 2))
+    say(f"full size: P(k) max rel {pk_rel:.3g}, displacement rms {dx_rms:.3g} mesh cells, "
+        f"density rms {rho_rms:.3g}, momentum rel {rel(P32, P64):.3g}")
+    assert pk_rel < 1e-3
+    assert dx_rms < 1e-3
+    assert rho_rms < 2e-3
