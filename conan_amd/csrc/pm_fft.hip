@@ -21,6 +21,18 @@
 
 namespace cpm {
 
+// Workgroup barrier that orders LDS accesses only.  Every barrier of these
+// kernels separates LDS writes from LDS reads of other threads; global memory
+// is never exchanged between the threads of a workgroup (a tile's HBM loads
+// are consumed in registers before its first barrier, and its stores go to
+// addresses no thread reads again), so the barrier need not wait for
+// outstanding global stores or prefetch loads, as __syncthreads() does.
+__device__ __forceinline__ void lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // LDS slot of element n of pencil p: pencils are N apart, and the low bits
 // of n are XOR-swizzled with the pencil number so that both the column-wise
 // global<->LDS copies and the in-pencil Stockham accesses spread over banks.
@@ -43,7 +55,7 @@ __device__ __forceinline__ void lds_stage(cx<T>* sh, int Ns, const cx<T>* __rest
 #pragma unroll
         for (int t = 0; t < R; ++t) v[i * R + t] = sh[lidx<N>(p, j + t * BPP)];
     }
-    __syncthreads();
+    lds_sync();
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
         const int b = tid + i * NT;
@@ -52,7 +64,7 @@ __device__ __forceinline__ void lds_stage(cx<T>* sh, int Ns, const cx<T>* __rest
 #pragma unroll
         for (int t = 0; t < R; ++t) sh[lidx<N>(p, stock_out<N, R>(j, Ns, t))] = v[i * R + t];
     }
-    __syncthreads();
+    lds_sync();
 }
 
 // In-LDS transform of all pencils of the tile (E*NT elements, pencils of N).
@@ -112,7 +124,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_r2c(
             sh[lidx<H>(e / H, e % H)] = mk<T>(z[i].re - sft, z[i].im - sft);
         }
     }
-    __syncthreads();
+    lds_sync();
     lds_fft<T, H, C::E, NT, -1>(sh, stw, 2, tid);
     for (int e = tid; e < P * H; e += NT) {
         const int p = e / H, k = e % H;
@@ -135,11 +147,26 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_c2r(
     using C = RowCfg<T, N>;
     constexpr int H = C::H, P = C::P, NT = C::NT, E = C::E;
     __shared__ cx<T> sh[P * H];
-    __shared__ cx<T> shn[P];
     __shared__ cx<T> stw[N];
     const int tid = threadIdx.x;
     const long row0 = (long)blockIdx.x * P;
+    using S = Sched<H>;
+    constexpr int R0 = S::lead ? S::lead : 8;                // radix of the first stage
+    constexpr int NMID = S::n8 - (S::lead ? 1 : 2);          // radix-8 stages between first and last
+    constexpr bool fused = NMID >= 0 && E % R0 == 0 && E % 8 == 0;
     for (int m = tid; m < N; m += NT) stw[m] = tw[m];
+    // The Nyquist mode of row p (the X*_{H-k} of k = 0 in the merge): in the
+    // fused path held in a register by the thread whose first-stage butterfly
+    // i starts row p (j = 0); otherwise staged in LDS.  (A staged copy in the
+    // fused path would put the kernel's LDS just above a fraction of the
+    // CU's, one resident workgroup fewer.)
+    constexpr int NQ = fused ? E / R0 : 1;
+    cx<T> nq[NQ];
+    cx<T>* shn = nullptr;
+    if constexpr (!fused) {
+        __shared__ cx<T> shn_lds[P];
+        shn = shn_lds;
+    }
     {
         // all loads of the thread first, then the LDS stores
         cx<T> z[E];
@@ -149,19 +176,28 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_c2r(
             const long row = row0 + e / H;
             z[i] = row < nrows ? spec[row * H + e % H] : mk<T>(0, 0);
         }
-        for (int p = tid; p < P; p += NT) shn[p] = (row0 + p < nrows) ? nyq[row0 + p] : mk<T>(0, 0);
+        if constexpr (fused) {
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                const int b = tid + i * NT, p = b / (H / R0), j = b % (H / R0);
+                nq[i] = (j == 0 && row0 + p < nrows) ? nyq[row0 + p] : mk<T>(0, 0);
+            }
+        } else {
+            for (int p = tid; p < P; p += NT) shn[p] = (row0 + p < nrows) ? nyq[row0 + p] : mk<T>(0, 0);
+        }
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             const int e = tid + i * NT;
             sh[lidx<H>(e / H, e % H)] = z[i];
         }
     }
-    __syncthreads();
+    lds_sync();
     // Merge of the half spectrum into the H-point complex input of the
-    // inverse transform: z_k = (X_k + X*_{H-k}) + i conj(W_N^k) (X_k - X*_{H-k}).
-    auto merged = [&](int p, int k) -> cx<T> {
+    // inverse transform: z_k = (X_k + X*_{H-k}) + i conj(W_N^k) (X_k - X*_{H-k}),
+    // with X_H = xn the Nyquist mode at k = 0.
+    auto merged_with = [&](int p, int k, const cx<T>& xn) -> cx<T> {
         const cx<T> Xk = sh[lidx<H>(p, k)];
-        const cx<T> Xm = conj(k == 0 ? shn[p] : sh[lidx<H>(p, H - k)]);
+        const cx<T> Xm = conj(k == 0 ? xn : sh[lidx<H>(p, H - k)]);
         const cx<T> Ep = Xk + Xm;
         const cx<T> wO = conj(stw[k]) * (Xk - Xm);
         return mk<T>(Ep.re - wO.im, Ep.im + wO.re);
@@ -174,10 +210,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_c2r(
             return mk<T>(v.re * inv, v.im * inv);
         }
     };
-    using S = Sched<H>;
-    constexpr int R0 = S::lead ? S::lead : 8;                // radix of the first stage
-    constexpr int NMID = S::n8 - (S::lead ? 1 : 2);          // radix-8 stages between first and last
-    if constexpr (NMID >= 0 && E % R0 == 0 && E % 8 == 0) {
+    if constexpr (fused) {
         // the merge fused into the first stage (its inputs read from the
         // loaded spectrum), the last stage stored straight to HBM
         cx<T> v[E];
@@ -185,9 +218,9 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_c2r(
         for (int i = 0; i < E / R0; ++i) {
             const int b = tid + i * NT, p = b / (H / R0), j = b % (H / R0);
 #pragma unroll
-            for (int t = 0; t < R0; ++t) v[i * R0 + t] = merged(p, j + t * (H / R0));
+            for (int t = 0; t < R0; ++t) v[i * R0 + t] = merged_with(p, j + t * (H / R0), nq[i]);
         }
-        __syncthreads();
+        lds_sync();
 #pragma unroll
         for (int i = 0; i < E / R0; ++i) {
             const int b = tid + i * NT, p = b / (H / R0), j = b % (H / R0);
@@ -195,7 +228,7 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_c2r(
 #pragma unroll
             for (int t = 0; t < R0; ++t) sh[lidx<H>(p, stock_out<H, R0>(j, 1, t))] = v[i * R0 + t];
         }
-        __syncthreads();
+        lds_sync();
         int Ns = R0;
 #pragma unroll
         for (int m = 0; m < NMID; ++m) {
@@ -221,15 +254,15 @@ __global__ __launch_bounds__((RowCfg<T, N>::NT)) void k_rows_c2r(
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             const int e = tid + i * NT;
-            z[i] = merged(e / H, e % H);
+            z[i] = merged_with(e / H, e % H, shn[e / H]);
         }
-        __syncthreads();
+        lds_sync();
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             const int e = tid + i * NT;
             sh[lidx<H>(e / H, e % H)] = z[i];
         }
-        __syncthreads();
+        lds_sync();
         lds_fft<T, H, E, NT, +1>(sh, stw, 2, tid);
         for (int e = tid; e < P * H; e += NT) {
             const int p = e / H, n = e % H;
@@ -336,6 +369,17 @@ struct Fused {
     static constexpr int mid = S::n8 - (S::lead ? 1 : 2);  // radix-8 stages between first and last
 };
 
+// Pencil of lane p of a tile starting at pencil c0.  A tile wider than the
+// block (the last axis of a small 2-D mesh) repeats the block's last pencil:
+// such a lane transforms the same values as that pencil's own lane, in the
+// same operations, and stores the same results to the same addresses, so
+// the fused path needs no per-lane branch around its HBM loads and stores.
+template <typename T>
+__device__ __forceinline__ long col_of(long c0, int p, const ColGeom& g) {
+    const long c = c0 + p;
+    return c < g.nc ? c : g.nc - 1;
+}
+
 // HBM loads of the first-stage inputs of thread (p, q) of tile t.
 template <typename T, int N, int CP = 0>
 __device__ __forceinline__ void col_first_load(const ColTile<T, N, CP>& t, int tid, cx<T>* v) {
@@ -344,14 +388,12 @@ __device__ __forceinline__ void col_first_load(const ColTile<T, N, CP>& t, int t
     constexpr int R0 = Fused<N>::R0, NB = 8 / R0;
     const ColPart<T>& P = *t.part;
     const int p = tid % C::C, q = tid / C::C;
-    const long col = t.c0 + p;
-    const bool ok = col < P.g.nc;
-    const cx<T>* __restrict__ src = P.data + t.bbase + col * P.g.cs;
+    const long es = P.g.es;
+    const cx<T>* __restrict__ src = P.data + t.bbase + col_of<T>(t.c0, p, P.g) * P.g.cs + (long)q * es;
 #pragma unroll
     for (int u = 0; u < NB; ++u)
 #pragma unroll
-        for (int k = 0; k < R0; ++k)
-            v[u * R0 + k] = ok ? src[(long)(q + u * (N / 8) + k * (N / R0)) * P.g.es] : mk<T>(0, 0);
+        for (int k = 0; k < R0; ++k) v[u * R0 + k] = src[(long)(u * (N / 8) + k * (N / R0)) * es];
 }
 
 // The first stage (no twiddles) on those values, written to LDS.
@@ -374,19 +416,28 @@ __device__ __forceinline__ void col_first_store(cx<T>* sh, cx<T>* v, int tid) {
 // flight.  body(tile, v) gets the first-stage inputs in v.
 template <typename T, int N, int CP, typename Body>
 __device__ __forceinline__ void col_pipeline(const ColPart<T>& pa, const ColPart<T>& pb, int tid, Body body) {
-    const uint32_t ntiles = (uint32_t)(pa.nblk + pb.nblk);
+    const uint32_t ntiles = (uint32_t)(pa.nblk + pb.nblk), G = gridDim.x;
     uint32_t w = blockIdx.x;
-    cx<T> nv[8];
-    if (w < ntiles) col_first_load<T, N, CP>(col_tile_at<T, N, CP>(pa, pb, w), tid, nv);
-    for (; w < ntiles; w += gridDim.x) {
-        const ColTile<T, N, CP> t = col_tile_at<T, N, CP>(pa, pb, w);
-        cx<T> v[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] = nv[i];
-        if (w + gridDim.x < ntiles)
-            col_first_load<T, N, CP>(col_tile_at<T, N, CP>(pa, pb, w + gridDim.x), tid, nv);
-        body(t, v);
-        __syncthreads();  // LDS is refilled by the next tile
+    if (w >= ntiles) return;
+    // Two register sets in turn: no copy of a prefetched set waits for its
+    // loads.  The prefetch is unconditional (past the last tile it reads the
+    // current tile again, from the cache, and the values are dropped), so no
+    // branch merges a path without it and the waits before a body count only
+    // that body's own loads.
+    const uint32_t last = ntiles - 1;
+    cx<T> va[8], vb[8];
+    col_first_load<T, N, CP>(col_tile_at<T, N, CP>(pa, pb, w), tid, va);
+    for (;;) {
+        col_first_load<T, N, CP>(col_tile_at<T, N, CP>(pa, pb, w + G < ntiles ? w + G : w), tid, vb);
+        body(col_tile_at<T, N, CP>(pa, pb, w), va);
+        lds_sync();  // LDS is refilled by the next tile
+        w += G;
+        if (w >= ntiles) break;
+        col_first_load<T, N, CP>(col_tile_at<T, N, CP>(pa, pb, w + G < ntiles ? w + G : w), tid, va);
+        body(col_tile_at<T, N, CP>(pa, pb, w), vb);
+        lds_sync();
+        w += G;
+        if (w >= ntiles) break;
     }
 }
 
@@ -416,11 +467,10 @@ __device__ __forceinline__ void col_store_regs(const cx<T>* v, cx<T>* __restrict
                                                long c0, int tid) {
     using C = ColCfg<T, N, CP>;
     const int p = tid % C::C, j = tid / C::C;
-    const long col = c0 + p;
-    if (col >= g.nc) return;
-    cx<T>* __restrict__ dst = data + bbase + col * g.cs;
+    const long es = g.es;
+    cx<T>* __restrict__ dst = data + bbase + col_of<T>(c0, p, g) * g.cs + (long)j * es;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) dst[(long)(j + t * (N / 8)) * g.es] = v[t];
+    for (int t = 0; t < 8; ++t) dst[(long)(t * (N / 8)) * es] = v[t];
 }
 
 template <typename T, int N, int DIR>
@@ -438,7 +488,7 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols(ColPart<T> pa, ColP
         cx<T> v[8];
         col_first_load<T, N>(t, tid, v);
         col_first_store<T, N, DIR>(sh, v, tid);
-        __syncthreads();
+        lds_sync();
         col_middle<T, N, DIR>(sh, stw, tid);
         col_last_regs<T, N, DIR>(sh, stw, tid, v);
         col_store_regs<T, N>(v, t.part->data, t.part->g, t.bbase, t.c0, tid);
@@ -446,7 +496,7 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols(ColPart<T> pa, ColP
         const ColTile<T, N> t = col_tile<T, N>(pa, pb);
         const ColPart<T>& p = *t.part;
         col_load<T, N>(sh, p.data, p.g, t.bbase, t.c0, tid);
-        __syncthreads();
+        lds_sync();
         lds_fft<T, N, C::E, C::NT, DIR>(sh, stw, 1, tid);
         col_store<T, N>(sh, p.data, p.g, t.bbase, t.c0, tid);
     }
@@ -498,11 +548,11 @@ __global__ __launch_bounds__((ColCfg<T, N, CPM_GREEN_C>::NT)) void k_cols_green(
         const GreenGeom gg = t.part->gg;
         const T kf = (T)gg.kf;
         col_first_store<T, N, -1, CP>(sh, v, tid);
-        __syncthreads();
+        lds_sync();
         col_middle<T, N, -1, CP>(sh, stw, tid);
         col_last_regs<T, N, -1, CP>(sh, stw, tid, v);
         const int pc = tid % C::C, j = tid / C::C;
-        const long col = t.c0 + pc;
+        const long col = col_of<T>(t.c0, pc, t.part->g);
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int n = j + q * (N / 8);
@@ -518,16 +568,16 @@ __global__ __launch_bounds__((ColCfg<T, N, CPM_GREEN_C>::NT)) void k_cols_green(
             const T gv = green_value<T>(ix, iy, iz, gg.dim, N, kf);
             v[q] = (ix == 0 && iy == 0 && iz == 0) ? mk<T>(0, 0) : scale(v[q], gv);
         }
-        __syncthreads();  // every thread has read its last-stage inputs
+        lds_sync();  // every thread has read its last-stage inputs
         if constexpr (Fused<N>::R0 == 8) {
             dft<T, 8, +1>(v);  // backward first stage, butterfly j, no twiddles
 #pragma unroll
             for (int q = 0; q < 8; ++q) sh[lidx<N>(pc, j * 8 + q)] = v[q];
-            __syncthreads();
+            lds_sync();
         } else {
 #pragma unroll
             for (int q = 0; q < 8; ++q) sh[lidx<N>(pc, j + q * (N / 8))] = v[q];
-            __syncthreads();
+            lds_sync();
             lds_stage<T, N, C::E, C::NT, +1, Fused<N>::R0>(sh, 1, stw, 1, tid);
         }
         col_middle<T, N, +1, CP>(sh, stw, tid);
@@ -541,7 +591,7 @@ __global__ __launch_bounds__((ColCfg<T, N, CPM_GREEN_C>::NT)) void k_cols_green(
     const GreenGeom gg = p.gg;
     const T kf = (T)gg.kf;
     col_load<T, N>(sh, p.data, p.g, t.bbase, t.c0, tid);
-    __syncthreads();
+    lds_sync();
     lds_fft<T, N, C::E, C::NT, -1>(sh, stw, 1, tid);
     for (int e = tid; e < C::C * N; e += C::NT) {
         const int c = e / N, n = e % N;
@@ -559,7 +609,7 @@ __global__ __launch_bounds__((ColCfg<T, N, CPM_GREEN_C>::NT)) void k_cols_green(
         const int li = lidx<N>(c, n);
         sh[li] = (ix == 0 && iy == 0 && iz == 0) ? mk<T>(0, 0) : scale(sh[li], gv);
     }
-    __syncthreads();
+    lds_sync();
     lds_fft<T, N, C::E, C::NT, +1>(sh, stw, 1, tid);
     col_store<T, N>(sh, p.data, p.g, t.bbase, t.c0, tid);
 }
