@@ -424,7 +424,6 @@ __device__ __forceinline__ void col_pipeline(const ColPart<T>& pa, const ColPart
     // current tile again, from the cache, and the values are dropped), so no
     // branch merges a path without it and the waits before a body count only
     // that body's own loads.
-    const uint32_t last = ntiles - 1;
     cx<T> va[8], vb[8];
     col_first_load<T, N, CP>(col_tile_at<T, N, CP>(pa, pb, w), tid, va);
     for (;;) {
@@ -441,13 +440,79 @@ __device__ __forceinline__ void col_pipeline(const ColPart<T>& pa, const ColPart
     }
 }
 
-template <typename T, int N, int DIR, int CP = 0>
-__device__ __forceinline__ void col_middle(cx<T>* sh, const cx<T>* __restrict__ tw, int tid) {
-    using C = ColCfg<T, N, CP>;
-    int Ns = Fused<N>::R0;
+// Twiddles of the radix-8 middle stages of a fused column transform, stage
+// after stage: entry (t - 1) Ns + k of the stage of sub-length Ns holds
+// W_N^(k t N / (8 Ns)) (k < Ns, 1 <= t < 8), the value the full table holds
+// at k t N / (8 Ns).  The lanes of a middle stage read up to eight different
+// k at once: in the full table those lie N / (8 Ns) entries apart, in a few
+// LDS banks; here they are consecutive.
+template <int N>
+constexpr int mid_tw_size() {
+    int n = 0, ns = Fused<N>::R0;
+    for (int s = 0; s < Fused<N>::mid; ++s) {
+        n += 7 * ns;
+        ns *= 8;
+    }
+    return n > 0 ? n : 1;
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void fill_mid_tw(cx<T>* mt, const cx<T>* __restrict__ tw, int tid, int nt) {
+    int off = 0, ns = Fused<N>::R0;
 #pragma unroll
     for (int s = 0; s < Fused<N>::mid; ++s) {
-        lds_stage<T, N, C::E, C::NT, DIR, 8>(sh, Ns, tw, 1, tid);
+        for (int e = tid; e < 7 * ns; e += nt) {
+            const int t = e / ns + 1, k = e % ns;
+            mt[off + e] = tw[k * t * (N / (8 * ns))];
+        }
+        off += 7 * ns;
+        ns *= 8;
+    }
+}
+
+// A radix-8 Stockham stage in LDS (lds_stage with R = 8, tws = 1) whose
+// twiddles come from that stage's part of the middle-stage table.
+template <typename T, int N, int E, int NT, int DIR>
+__device__ __forceinline__ void lds_stage8_mt(cx<T>* sh, int Ns, const cx<T>* mt, int tid) {
+    constexpr int NB = E / 8;
+    constexpr int BPP = N / 8;
+    cx<T> v[E];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int b = tid + i * NT;
+        const int p = b / BPP, j = b % BPP;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) v[i * 8 + t] = sh[lidx<N>(p, j + t * BPP)];
+    }
+    lds_sync();
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int b = tid + i * NT;
+        const int p = b / BPP, j = b % BPP;
+        const int k = j & (Ns - 1);
+        if (k != 0) {
+#pragma unroll
+            for (int t = 1; t < 8; ++t) {
+                cx<T> w = mt[(t - 1) * Ns + k];
+                if (DIR > 0) w.im = -w.im;
+                v[i * 8 + t] = v[i * 8 + t] * w;
+            }
+        }
+        dft<T, 8, DIR>(&v[i * 8]);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) sh[lidx<N>(p, stock_out<N, 8>(j, Ns, t))] = v[i * 8 + t];
+    }
+    lds_sync();
+}
+
+template <typename T, int N, int DIR, int CP = 0>
+__device__ __forceinline__ void col_middle(cx<T>* sh, const cx<T>* mt, int tid) {
+    using C = ColCfg<T, N, CP>;
+    int Ns = Fused<N>::R0, off = 0;
+#pragma unroll
+    for (int s = 0; s < Fused<N>::mid; ++s) {
+        lds_stage8_mt<T, N, C::E, C::NT, DIR>(sh, Ns, mt + off, tid);
+        off += 7 * Ns;
         Ns *= 8;
     }
 }
@@ -479,9 +544,11 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols(ColPart<T> pa, ColP
     using C = ColCfg<T, N>;
     __shared__ cx<T> sh[C::C * N];
     __shared__ cx<T> stw[N];
+    __shared__ cx<T> smt[mid_tw_size<N>()];
     const int tid = threadIdx.x;
     for (int m = tid; m < N; m += C::NT) stw[m] = tw[m];
     if constexpr (Fused<N>::ok) {
+        fill_mid_tw<T, N>(smt, tw, tid, C::NT);
         // one tile per workgroup: with a single FFT per tile, prefetching
         // the next tile costs more in registers than it hides
         const ColTile<T, N> t = col_tile<T, N>(pa, pb);
@@ -489,7 +556,7 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols(ColPart<T> pa, ColP
         col_first_load<T, N>(t, tid, v);
         col_first_store<T, N, DIR>(sh, v, tid);
         lds_sync();
-        col_middle<T, N, DIR>(sh, stw, tid);
+        col_middle<T, N, DIR>(sh, smt, tid);
         col_last_regs<T, N, DIR>(sh, stw, tid, v);
         col_store_regs<T, N>(v, t.part->data, t.part->g, t.bbase, t.c0, tid);
     } else {
@@ -536,10 +603,12 @@ __global__ __launch_bounds__((ColCfg<T, N, CPM_GREEN_C>::NT)) void k_cols_green(
     using C = ColCfg<T, N, CP>;
     __shared__ cx<T> sh[C::C * N];
     __shared__ cx<T> stw[N];
+    __shared__ cx<T> smt[mid_tw_size<N>()];
     const int tid = threadIdx.x;
     const int H = N / 2;
     for (int m = tid; m < N; m += C::NT) stw[m] = tw[m];
     if constexpr (Fused<N>::ok) {
+        fill_mid_tw<T, N>(smt, tw, tid, C::NT);
         // forward: first stage on load, middle stages, last stage in
         // registers; the Green's function on the registers; backward: first
         // stage in registers (when it is radix 8 it reads exactly these
@@ -549,7 +618,7 @@ __global__ __launch_bounds__((ColCfg<T, N, CPM_GREEN_C>::NT)) void k_cols_green(
         const T kf = (T)gg.kf;
         col_first_store<T, N, -1, CP>(sh, v, tid);
         lds_sync();
-        col_middle<T, N, -1, CP>(sh, stw, tid);
+        col_middle<T, N, -1, CP>(sh, smt, tid);
         col_last_regs<T, N, -1, CP>(sh, stw, tid, v);
         const int pc = tid % C::C, j = tid / C::C;
         const long col = col_of<T>(t.c0, pc, t.part->g);
@@ -580,7 +649,7 @@ __global__ __launch_bounds__((ColCfg<T, N, CPM_GREEN_C>::NT)) void k_cols_green(
             lds_sync();
             lds_stage<T, N, C::E, C::NT, +1, Fused<N>::R0>(sh, 1, stw, 1, tid);
         }
-        col_middle<T, N, +1, CP>(sh, stw, tid);
+        col_middle<T, N, +1, CP>(sh, smt, tid);
         col_last_regs<T, N, +1, CP>(sh, stw, tid, v);
         col_store_regs<T, N, CP>(v, t.part->data, t.part->g, t.bbase, t.c0, tid);
         });
