@@ -198,6 +198,18 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit(const uint32_t* __restrict__
 // Particles move less than a cell per step, so `perm` (pm_brick.hpp) is close
 // to the identity and reads through it are nearly contiguous.
 
+// round(w 2^40) (ties to even) as the 64-bit deposit quantum, for
+// 0 <= w < 2^23, in FP32 operations: a = w 2^8 is exact, hi = floor(a) and
+// a - hi are exact, (a - hi) 2^32 < 2^32 is exact, and rounding it to an
+// integer rounds w 2^40, whose integer part hi 2^32 needs no rounding.
+// Equal to __double2ll_rn((double)w * 2^40), without FP64 operations.
+__device__ __forceinline__ unsigned long long fx_quantum(float w) {
+    const float a = w * 256.f;
+    const float hi = floorf(a);
+    const float lo = rintf((a - hi) * 4294967296.f);
+    return ((unsigned long long)(uint32_t)hi << 32) + (uint32_t)lo;
+}
+
 // Deposit, one workgroup per brick.  The brick's cells receive the CIC
 // weights of the particles of the brick and of its lower neighbours (a
 // particle reaches its origin cell and the next one along each axis); only
@@ -260,20 +272,32 @@ __global__ __launch_bounds__(FXB_NT) void k_deposit_fx_brick(const uint32_t* __r
                     float w = mass;
 #pragma unroll
                     for (int k = 0; k < R; ++k) w *= ((j >> k) & 1) ? f[k] : 1.f - f[k];
-                    const long long qv = __double2ll_rn((double)w * FX_ONE);
-                    atomicAdd(&acc[l[0] + bc.B[0] * (l[1] + bc.B[1] * l[2])], (unsigned long long)qv);
+                    atomicAdd(&acc[l[0] + bc.B[0] * (l[1] + bc.B[1] * l[2])], fx_quantum(w));
                 }
             }
         }
     }
     __syncthreads();
     double rsum = 0;
-    for (int e = tid; e < ncell; e += FXB_NT) {
-        const int lx = e & (bc.B[0] - 1), ly = (e >> pa.lb[0]) & (bc.B[1] - 1), lz = e >> (pa.lb[0] + pa.lb[1]);
-        const MT v = (MT)((double)(long long)acc[e] * (1.0 / FX_ONE));
-        const long idx = (long)(bc.x0[0] + lx) + (long)N * ((bc.x0[1] + ly) + (long)N * (bc.x0[2] + lz));
-        mesh[idx] = v;
-        rsum += (double)v;
+    if (R == 3 && bc.B[0] * bc.B[1] == FXB_NT) {
+        // a whole brick layer per round: thread tid owns the column (lx, ly)
+        // and walks it along the last axis (cells tid + FXB_NT lz, as below)
+        const int lx = tid & (bc.B[0] - 1), ly = tid >> pa.lb[0];
+        MT* __restrict__ out = mesh + (long)(bc.x0[0] + lx) + (long)N * ((bc.x0[1] + ly) + (long)N * bc.x0[2]);
+        const long plane = (long)N * N;
+        for (int lz = 0; lz < bc.B[2]; ++lz) {
+            const MT v = (MT)((double)(long long)acc[tid + lz * FXB_NT] * (1.0 / FX_ONE));
+            out[lz * plane] = v;
+            rsum += (double)v;
+        }
+    } else {
+        for (int e = tid; e < ncell; e += FXB_NT) {
+            const int lx = e & (bc.B[0] - 1), ly = (e >> pa.lb[0]) & (bc.B[1] - 1), lz = e >> (pa.lb[0] + pa.lb[1]);
+            const MT v = (MT)((double)(long long)acc[e] * (1.0 / FX_ONE));
+            const long idx = (long)(bc.x0[0] + lx) + (long)N * ((bc.x0[1] + ly) + (long)N * (bc.x0[2] + lz));
+            mesh[idx] = v;
+            rsum += (double)v;
+        }
     }
     const double tot = block_sum<FXB_NT>(rsum, red, tid);
     if (tid == 0) partial[bid] = tot;
