@@ -12,6 +12,7 @@
 // reference's expressions, so both sides see the same doubles.
 #pragma once
 #include "../../include/conanpm.h"
+#include "pm_adjoint.hpp"
 #include "pm_fft_api.hpp"
 #include "pm_particles.hpp"
 #include "pm_sort.hpp"
@@ -727,8 +728,19 @@ struct Impl : ImplBase {
         std::memset(d, 0, sizeof(*d));
         double h[8] = {0};
         if (have_deposit) sum_partials(row_partial.p, npartials, 1.0, red.p, 4096, scal.p + 2, st);
-        for (int k = 0; k < R; ++k) reduce_sum(P.p + (size_t)k * Np, Np, 0, 1.0, red.p, 4096, scal.p + 3 + k, st);
-        reduce_sum(P.p, (long)R * Np, 1, 1.0, red.p, 4096, scal.p + 6, st);
+        // The sums run in particle order: the FX storage order within a
+        // brick group depends on the order in which the re-sort's atomics
+        // placed the particles that changed group, and a sum in that order
+        // would differ from run to run in its last bits.
+        const double* Pd = P.p;
+        DevBuf<double> byid;
+        if (FX) {
+            byid.alloc((size_t)R * Np);
+            scatter_by_id<double>(P.p, id.p, Np, R, byid.p, st);
+            Pd = byid.p;
+        }
+        for (int k = 0; k < R; ++k) reduce_sum(Pd + (size_t)k * Np, Np, 0, 1.0, red.p, 4096, scal.p + 3 + k, st);
+        reduce_sum(Pd, (long)R * Np, 1, 1.0, red.p, 4096, scal.p + 6, st);
         CPM_HIP(hipMemcpyAsync(h, scal.p + 2, sizeof(double) * 5, hipMemcpyDeviceToHost, st));
         CPM_HIP(hipStreamSynchronize(st));
         d->mass = have_deposit ? h[0] : 0.0;

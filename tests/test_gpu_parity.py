@@ -357,7 +357,9 @@ def test_power_spectrum(pm, orc):
 def test_runs_are_bit_reproducible(pm, orc, prec):
     """The deposit has no order-dependent floating-point sums: two runs from
     the same state agree bit for bit (the reference's omp critical deposit
-    does not have this property)."""
+    does not have this property), and so do the FP64 diagnostics sums (in
+    the FX form they run in particle order, not in the storage order the
+    re-sort's atomics leave)."""
     dim, Nm, Nf, L, a0, da = 3, 32, 64, 100.0, 0.02, 0.0098
     noise = orc.white_noise(21, Nm ** dim)
     phi = orc.compute_potential(dim, 5, L, orc.ic_scale_free(dim, 5, L, -2.0, noise))
@@ -368,7 +370,9 @@ def test_runs_are_bit_reproducible(pm, orc, prec):
             g.init(phi, a0, a0 + da / 2)
             g.steps(a0, da, 8)
             X, P = g.get_state()
-            outs.append((X, P, g.deposit()))
+            rho = g.deposit()
+            d = g.diagnostics()
+            outs.append((X, P, rho, np.array(d["momentum"] + [d["p2"], d["mass"]])))
     for a, b in zip(outs[0], outs[1]):
         assert np.array_equal(a, b)
 
