@@ -622,20 +622,35 @@ __global__ __launch_bounds__((ColCfg<T, N, CPM_GREEN_C>::NT)) void k_cols_green(
         col_last_regs<T, N, -1, CP>(sh, stw, tid, v);
         const int pc = tid % C::C, j = tid / C::C;
         const long col = col_of<T>(t.c0, pc, t.part->g);
+        // green_value's sum kx^2 [+ ky^2] + kn^2 for the pencil's fixed modes
+        // and the mode n along it: the fixed part once per tile
+        int ix, iy = 0;
+        if (gg.dim == 3) {
+            ix = gg.nyq ? H : (int)(col & (H - 1));
+            iy = gg.nyq ? (int)col : (int)(col / H);
+        } else {
+            ix = gg.nyq ? H : (int)col;
+        }
+        const T kx = (T)signed_mode(ix, N) * kf;
+        T k2 = (T)0;
+        k2 += kx * kx;
+        if (gg.dim == 3) {
+            const T ky = (T)signed_mode(iy, N) * kf;
+            k2 += ky * ky;
+        }
+        const bool fixed_zero = ix == 0 && iy == 0;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int n = j + q * (N / 8);
-            int ix, iy, iz = 0;
-            if (gg.dim == 3) {
-                ix = gg.nyq ? H : (int)(col & (H - 1));
-                iy = gg.nyq ? (int)col : (int)(col / H);
-                iz = n;
+            const T kn = (T)signed_mode(n, N) * kf;
+            const T kk = k2 + kn * kn;
+            T gv;
+            if constexpr (sizeof(T) == 4) {
+                gv = -__builtin_amdgcn_rcpf(kk);
             } else {
-                ix = gg.nyq ? H : (int)col;
-                iy = n;
+                gv = (T)-1. / kk;
             }
-            const T gv = green_value<T>(ix, iy, iz, gg.dim, N, kf);
-            v[q] = (ix == 0 && iy == 0 && iz == 0) ? mk<T>(0, 0) : scale(v[q], gv);
+            v[q] = (fixed_zero && n == 0) ? mk<T>(0, 0) : scale(v[q], gv);
         }
         lds_sync();  // every thread has read its last-stage inputs
         if constexpr (Fused<N>::R0 == 8) {
