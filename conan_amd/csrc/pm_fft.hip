@@ -489,8 +489,8 @@ __device__ __forceinline__ void lds_stage8_mt(cx<T>* sh, int Ns, const cx<T>* mt
     for (int i = 0; i < NB; ++i) {
         const int b = tid + i * NT;
         const int p = b / BPP, j = b % BPP;
-        const int k = j & (Ns - 1);
-        if (k != 0) {
+        const int k = j & (Ns - 1);  // (all lanes multiply: see butterfly)
+        if (Ns > 1) {
 #pragma unroll
             for (int t = 1; t < 8; ++t) {
                 cx<T> w = mt[(t - 1) * Ns + k];

@@ -77,8 +77,11 @@ __host__ __device__ __forceinline__ void dft(cx<T>* v) {
 template <typename T, int N, int R, int DIR>
 __host__ __device__ __forceinline__ void butterfly(cx<T>* v, int j, int Ns, const cx<T>* tw,
                                                    int tws) {
+    // Every lane multiplies, also at k = 0 (twiddles exactly 1): a branch on
+    // k would diverge within the wave and cost register copies at the join.
+    // The first stage (Ns = 1, all k = 0) skips the multiplications.
     int k = j & (Ns - 1);
-    if (k != 0) {
+    if (Ns > 1) {
         const int mstep = k * (N / (Ns * R));
 #pragma unroll
         for (int t = 1; t < R; ++t) {
