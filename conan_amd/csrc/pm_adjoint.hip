@@ -267,7 +267,7 @@ __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) voi
                 for (int bx = 0; bx < 2; ++bx) {
                     const int j = bx | (by << 1) | (bz << 2);
                     g[0][j] = fd4(f[bx], f[bx + 1], f[bx + 3], f[bx + 4], inv12res);
-                    F[0] += g[0][j] * z[j];
+                    F[0] = fmaf(g[0][j], z[j], F[0]);  // as k_gather_kick_fx_brick
                 }
             }
 #pragma unroll
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) voi
                 for (int by = 0; by < 2; ++by) {
                     const int j = bx | (by << 1) | (bz << 2);
                     g[1][j] = fd4(f[by], f[by + 1], f[by + 3], f[by + 4], inv12res);
-                    F[1] += g[1][j] * z[j];
+                    F[1] = fmaf(g[1][j], z[j], F[1]);  // as k_gather_kick_fx_brick
                 }
             }
             if constexpr (R == 3) {
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) voi
                     for (int bz = 0; bz < 2; ++bz) {
                         const int j = bx | (by << 1) | (bz << 2);
                         g[2][j] = fd4(f[bz], f[bz + 1], f[bz + 3], f[bz + 4], inv12res);
-                        F[2] += g[2][j] * z[j];
+                        F[2] = fmaf(g[2][j], z[j], F[2]);  // as k_gather_kick_fx_brick
                     }
                 }
             }

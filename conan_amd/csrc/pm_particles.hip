@@ -496,7 +496,7 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
             for (int t = 0; t < 6; ++t) f[t] = T(lx + t, ly + 2 + by, lz + zc + bz);
 #pragma unroll
             for (int bx = 0; bx < 2; ++bx)
-                F[0] += fd4(f[bx], f[bx + 1], f[bx + 3], f[bx + 4], inv12res) * z[bx | (by << 1) | (bz << 2)];
+                F[0] = fmaf(fd4(f[bx], f[bx + 1], f[bx + 3], f[bx + 4], inv12res), z[bx | (by << 1) | (bz << 2)], F[0]);
         }
 #pragma unroll
         for (int l = 0; l < NC / 2; ++l) {
@@ -506,7 +506,7 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
             for (int t = 0; t < 6; ++t) f[t] = T(lx + 2 + bx, ly + t, lz + zc + bz);
 #pragma unroll
             for (int by = 0; by < 2; ++by)
-                F[1] += fd4(f[by], f[by + 1], f[by + 3], f[by + 4], inv12res) * z[bx | (by << 1) | (bz << 2)];
+                F[1] = fmaf(fd4(f[by], f[by + 1], f[by + 3], f[by + 4], inv12res), z[bx | (by << 1) | (bz << 2)], F[1]);
         }
         if constexpr (R == 3) {
 #pragma unroll
@@ -517,7 +517,7 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
                 for (int t = 0; t < 6; ++t) f[t] = T(lx + 2 + bx, ly + 2 + by, lz + t);
 #pragma unroll
                 for (int bz = 0; bz < 2; ++bz)
-                    F[2] += fd4(f[bz], f[bz + 1], f[bz + 3], f[bz + 4], inv12res) * z[bx | (by << 1) | (bz << 2)];
+                    F[2] = fmaf(fd4(f[bz], f[bz + 1], f[bz + 3], f[bz + 4], inv12res), z[bx | (by << 1) | (bz << 2)], F[2]);
             }
         }
         uint32_t xn[3] = {0u, 0u, 0u};

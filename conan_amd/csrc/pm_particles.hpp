@@ -44,9 +44,11 @@ __device__ __forceinline__ double fd4(double fm2, double fm1, double fp1, double
     double v = fm2 / 12. - 2 * fm1 / 3. + 2 * fp1 / 3. - fp2 / 12.;
     return v / res;
 }
-// Same operator in FP32, arranged for fewer roundings: (f[-2] - f[+2] + 8 (f[+1] - f[-1])) / (12 res).
+// Same operator in FP32, arranged for fewer roundings: (f[-2] - f[+2] + 8 (f[+1] - f[-1])) / (12 res),
+// the sum fused with the multiplication by 8 (explicitly, so every kernel that
+// uses it rounds the same way whatever its contraction setting).
 __device__ __forceinline__ float fd4(float fm2, float fm1, float fp1, float fp2, float inv12res) {
-    return ((fm2 - fp2) + 8.f * (fp1 - fm1)) * inv12res;
+    return fmaf(8.f, fp1 - fm1, fm2 - fp2) * inv12res;
 }
 
 // FX kick increment F * ck rounded to a multiple of the momentum quantum pq
