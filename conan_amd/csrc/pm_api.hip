@@ -82,6 +82,11 @@ static void validate(const pm_config* c) {
     if (c->flags & ~(PM_FLAG_WRAP_GATHER | PM_FLAG_MIDPOINT)) throw cpm::Error(PM_EINVAL, "unknown flags");
     double np = std::pow((double)c->Nm, c->dim);
     if (np > 2147483647.0) throw cpm::Error(PM_EUNSUP, "too many particles");
+    // FP32 deposit: 64-bit accumulators in units of 2^-40 of the mean cell
+    // mass hold less than 2^23, and the quantum of a weight is formed in FP32
+    // for weights below 2^23 (a particle's mass is (Nf / Nm)^dim)
+    if (c->precision == PM_FP32 && std::pow((double)c->Nf / c->Nm, c->dim) >= 8388608.0)
+        throw cpm::Error(PM_EUNSUP, "FP32 form: particle mass (Nf/Nm)^dim must be below 2^23");
     int dev = 0;
     hipError_t e = hipGetDeviceCount(&dev);
     if (e != hipSuccess || dev < 1) throw cpm::Error(PM_EDEVICE, "no HIP device available");

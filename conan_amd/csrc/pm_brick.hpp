@@ -92,7 +92,7 @@ __device__ __forceinline__ void stage_box(float* tile, const float* __restrict__
 // before an LDS-DMA load reads it (the s_nop).
 __device__ __forceinline__ void lds_dma_dword(const float* src, uint32_t lds_base) {
     const uint32_t a = __builtin_amdgcn_readfirstlane(lds_base);
-    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off" : : "v"(src), "s"(a) : "memory");
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off" : : "v"(src), "s"(a) : "memory", "m0");
 }
 // s_waitcnt vmcnt(0) with the other counters left alone (gfx9 encoding:
 // vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).

@@ -64,7 +64,10 @@ struct DiffPM {
         c.Omega_m = q.Omega_m;
         c.Omega_L = q.Omega_L;
         c.precision = PM_FP32;
-        c.flags = 0;
+        // the differentiable map gathers on folded coordinates: its force is
+        // a smooth function of the position on the torus (the adjoint kernels
+        // stage the tile of the folded origin)
+        c.flags = PM_FLAG_WRAP_GATHER;
         return c;
     }
 
@@ -427,6 +430,8 @@ int pm_diff_create(const pm_diff_params* p, pm_diff** out) {
     if (p->f < 1 || (p->f & (p->f - 1)) || (p->f << p->mbits) > 1024)
         throw cpm::Error(PM_EUNSUP, "f must be a power of two with f * 2^mbits <= 1024");
     if (p->nsteps < 1 || p->ckpt_every < 1) throw cpm::Error(PM_EINVAL, "nsteps and ckpt_every must be >= 1");
+    if (std::pow((double)p->f, p->dim) >= 8388608.0)
+        throw cpm::Error(PM_EUNSUP, "particle mass f^dim must be below 2^23 (FP32 deposit quantum)");
     if (p->precision != PM_FP32) throw cpm::Error(PM_EUNSUP, "the adjoint is implemented for the FP32 form");
     if (p->flags & ~(PM_DIFF_STORE_POTENTIALS | PM_DIFF_VERIFY_CHECKPOINTS))
         throw cpm::Error(PM_EINVAL, "unknown pm_diff flags");

@@ -211,9 +211,10 @@ struct Impl : ImplBase {
         pa.L = c.L;
         pa.res = c.L / c.Nf;
         pa.wrap = (c.flags & PM_FLAG_WRAP_GATHER) ? 1 : 0;
-        pa.fshift = 32 - lgNf;
+        pa.fshift = FX_BITS - lgNf;
         pa.fmask = (uint32_t)((1ull << pa.fshift) - 1);
         pa.fscale = (float)ldexp(1.0, -pa.fshift);
+        pa.cmask = c.Nf - 1;
         pa.brick = 0;
         for (int k = 0; k < 3; ++k) pa.lb[k] = pa.lnb[k] = 0;
         if (FX) brick_layout(R, lgNf, pa);
@@ -299,7 +300,7 @@ struct Impl : ImplBase {
         const double a = a_eval(a_in, da);
         double adot = cos.adot(a);
         double c = da / (a * a * adot);  // src/run/nbody.hh:124,129
-        return FX ? c * (4294967296.0 / cfg.L) : c;
+        return FX ? c * (FX_UNIT / cfg.L) : c;
     }
 
     // FX momenta live on a grid of pq = 2^(e - 36), max |P| < 2^e (2^-40 L
@@ -579,15 +580,25 @@ struct Impl : ImplBase {
 
     void set_state(const double* Xh, const double* Ph) override {
         std::vector<double> tmp((size_t)R * Np);
+        const bool had_state = have_state;
+        if (Xh) check_finite(Xh, (size_t)R * Np, "positions");
+        if (Ph) check_finite(Ph, (size_t)R * Np, "momenta");
         if (Xh) {
-            check_finite(Xh, (size_t)R * Np, "positions");
+            if (FX && !Ph && have_state) {
+                // the kept momenta follow the storage order of the last step;
+                // the new positions come in mass-grid order, which reset_ids
+                // makes the storage order: put the momenta there first
+                DevBuf<double> byid((size_t)R * Np);
+                scatter_by_id<double>(P.p, id.p, Np, R, byid.p, st);
+                CPM_HIP(hipMemcpyAsync(P.p, byid.p, sizeof(double) * R * Np, hipMemcpyDeviceToDevice, st));
+                CPM_HIP(hipStreamSynchronize(st));
+            }
             for (long i = 0; i < Np; ++i)
                 for (int k = 0; k < R; ++k) tmp[(size_t)k * Np + i] = Xh[i * R + k];
             upload_positions_soa(tmp.data(), false);
             reset_ids();
         }
         if (Ph) {
-            check_finite(Ph, (size_t)R * Np, "momenta");
             const std::vector<uint32_t> ids = host_ids();  // momenta follow the stored order
             for (long s = 0; s < Np; ++s)
                 for (int k = 0; k < R; ++k) tmp[(size_t)k * Np + s] = Ph[(size_t)ids[s] * R + k];
@@ -599,7 +610,8 @@ struct Impl : ImplBase {
         }
         if (Xh) have_state = true;
         have_deposit = false;
-        fx_quantize_momenta();
+        // new momenta (or zeros): choose their grid; kept momenta stay on theirs
+        if (Ph || !had_state) fx_quantize_momenta();
     }
 
     void set_positions_device(const double* Xsoa) override {

@@ -9,11 +9,13 @@
 //        reference's arithmetic in its order, so keys, weights, the gather,
 //        the kick and the drift are bit-identical to the reference for equal
 //        inputs.
-//   FX:  positions are unsigned 32-bit fractions of the box (x = q * L / 2^32),
-//        so the periodic wrap is exact integer overflow; mesh and gather in
-//        FP32, momenta and reductions in FP64.
+//   FX:  positions are signed 32-bit fixed-point coordinates (x = q * L / 2^30,
+//        unwrapped within [-2L, 2L)), so the periodic wrap is exact integer
+//        arithmetic; mesh and gather in FP32, momenta and reductions in FP64.
 // Particle arrays are SoA: component k of particle i at [k * np + i].
 #include "pm_brick.hpp"
+
+#include <type_traits>
 
 namespace cpm {
 
@@ -303,6 +305,91 @@ __global__ __launch_bounds__(FXB_NT) void k_deposit_fx_brick(const uint32_t* __r
     if (tid == 0) partial[bid] = tot;
 }
 
+// FX force at one particle: the trilinear interpolation (weights from the
+// fractions a1) of the 4-point FD gradient at the 2^R nodes around its origin
+// o.  ld(tx, ty, tz) returns the potential at node o - 2 + (tx, ty, tz)
+// (tz ignored in 2-D): the brick kernels read it from their LDS tile, the
+// others from the mesh, with the same operations in the same order, so every
+// FX path computes the same force bit for bit.
+template <int R, typename Ld>
+__device__ __forceinline__ void force_fx(const float* a1, Ld ld, float inv12res, float* F) {
+    constexpr int NC = 1 << R;
+    float z[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        float w = 1.f;
+#pragma unroll
+        for (int k = 0; k < R; ++k) w *= ((c >> k) & 1) ? a1[k] : 1.f - a1[k];
+        z[c] = w;
+    }
+    const int zc = (R == 3) ? 2 : 0;  // tz of the origin's own plane
+    F[0] = F[1] = F[2] = 0.f;
+    // axis 0: lines along x through the (y, z) corners
+#pragma unroll
+    for (int l = 0; l < NC / 2; ++l) {
+        const int by = l & 1, bz = (l >> 1) & 1;
+        float f[6];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) f[t] = ld(t, 2 + by, zc + bz);
+#pragma unroll
+        for (int bx = 0; bx < 2; ++bx)
+            F[0] = fmaf(fd4(f[bx], f[bx + 1], f[bx + 3], f[bx + 4], inv12res), z[bx | (by << 1) | (bz << 2)], F[0]);
+    }
+#pragma unroll
+    for (int l = 0; l < NC / 2; ++l) {
+        const int bx = l & 1, bz = (l >> 1) & 1;
+        float f[6];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) f[t] = ld(2 + bx, t, zc + bz);
+#pragma unroll
+        for (int by = 0; by < 2; ++by)
+            F[1] = fmaf(fd4(f[by], f[by + 1], f[by + 3], f[by + 4], inv12res), z[bx | (by << 1) | (bz << 2)], F[1]);
+    }
+    if constexpr (R == 3) {
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            const int bx = l & 1, by = (l >> 1) & 1;
+            float f[6];
+#pragma unroll
+            for (int t = 0; t < 6; ++t) f[t] = ld(2 + bx, 2 + by, t);
+#pragma unroll
+            for (int bz = 0; bz < 2; ++bz)
+                F[2] = fmaf(fd4(f[bz], f[bz + 1], f[bz + 3], f[bz + 4], inv12res), z[bx | (by << 1) | (bz << 2)], F[2]);
+        }
+    }
+}
+
+// The potential at node o - 2 + (tx, ty, tz) of an N^R mesh, wrapped.
+template <int R>
+struct MeshNodes;
+// force_fx from the mesh, kept out of line: the brick kernel calls it only for
+// the rare particles its tile does not cover, and inlined it would hold
+// registers across the whole kernel.
+struct F3 {
+    float v[3];
+};
+template <int R>
+__device__ __attribute__((noinline)) F3 force_fx_mesh(float a0, float a1, float a2, const float* __restrict__ phi,
+                                                      int o0, int o1, int o2, int lgN, float inv12res);
+template <int R>
+struct MeshNodes {
+    const float* __restrict__ phi;
+    int o0, o1, o2, lgN, msk;
+    __device__ __forceinline__ float operator()(int tx, int ty, int tz) const {
+        uint32_t idx = (uint32_t)((o0 - 2 + tx) & msk) + ((uint32_t)((o1 - 2 + ty) & msk) << lgN);
+        if (R == 3) idx += (uint32_t)((o2 - 2 + tz) & msk) << (2 * lgN);
+        return phi[idx];
+    }
+};
+template <int R>
+__device__ __attribute__((noinline)) F3 force_fx_mesh(float a0, float a1, float a2, const float* __restrict__ phi,
+                                                      int o0, int o1, int o2, int lgN, float inv12res) {
+    const float a[3] = {a0, a1, a2};
+    F3 r;
+    force_fx<R>(a, MeshNodes<R>{phi, o0, o1, o2, lgN, (1 << lgN) - 1}, inv12res, r.v);
+    return r;
+}
+
 // Gather + kick (+ drift).  A workgroup takes a segment of 2^lseg bricks
 // that follow each other along the last axis (same x0 and, in 3-D, y0).  A
 // brick reads the potential of its cells and a halo of 2 cells below / 3
@@ -476,49 +563,16 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
         // tile coordinates of the stencil origin (cell o - 2 along each axis)
         const int lx = (o[0] - bc.x0[0]) & msk, ly = (o[1] - bc.x0[1]) & msk;
         const int lz = (R == 3) ? ((o[2] - bc.x0[2]) & msk) : 0;
-        constexpr int NC = 1 << R;
-        float z[NC];
+        float F[3];
+        // A particle whose origin is the cell above its own (a negative
+        // unwrapped coordinate, see PosFX::gat) on the brick's upper face
+        // needs one plane beyond the tile: it reads the mesh instead.
+        if (lx >= bc.B[0] || ly >= bc.B[1] || (R == 3 && lz >= bc.B[2])) {
+            const F3 r = force_fx_mesh<R>(a1[0], a1[1], a1[2], phi, o[0], o[1], o[2], lgN, inv12res);
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            float w = 1.f;
-#pragma unroll
-            for (int k = 0; k < R; ++k) w *= ((c >> k) & 1) ? a1[k] : 1.f - a1[k];
-            z[c] = w;
-        }
-        const int zc = (R == 3) ? 2 : 0;  // tile z of the origin's own plane offset
-        float F[3] = {0.f, 0.f, 0.f};
-        // axis 0: lines along x through the (y, z) corners
-#pragma unroll
-        for (int l = 0; l < NC / 2; ++l) {
-            const int by = l & 1, bz = (l >> 1) & 1;
-            float f[6];
-#pragma unroll
-            for (int t = 0; t < 6; ++t) f[t] = T(lx + t, ly + 2 + by, lz + zc + bz);
-#pragma unroll
-            for (int bx = 0; bx < 2; ++bx)
-                F[0] = fmaf(fd4(f[bx], f[bx + 1], f[bx + 3], f[bx + 4], inv12res), z[bx | (by << 1) | (bz << 2)], F[0]);
-        }
-#pragma unroll
-        for (int l = 0; l < NC / 2; ++l) {
-            const int bx = l & 1, bz = (l >> 1) & 1;
-            float f[6];
-#pragma unroll
-            for (int t = 0; t < 6; ++t) f[t] = T(lx + 2 + bx, ly + t, lz + zc + bz);
-#pragma unroll
-            for (int by = 0; by < 2; ++by)
-                F[1] = fmaf(fd4(f[by], f[by + 1], f[by + 3], f[by + 4], inv12res), z[bx | (by << 1) | (bz << 2)], F[1]);
-        }
-        if constexpr (R == 3) {
-#pragma unroll
-            for (int l = 0; l < 4; ++l) {
-                const int bx = l & 1, by = (l >> 1) & 1;
-                float f[6];
-#pragma unroll
-                for (int t = 0; t < 6; ++t) f[t] = T(lx + 2 + bx, ly + 2 + by, lz + t);
-#pragma unroll
-                for (int bz = 0; bz < 2; ++bz)
-                    F[2] = fmaf(fd4(f[bz], f[bz + 1], f[bz + 3], f[bz + 4], inv12res), z[bx | (by << 1) | (bz << 2)], F[2]);
-            }
+            for (int k = 0; k < 3; ++k) F[k] = r.v[k];
+        } else {
+            force_fx<R>(a1, [&](int tx, int ty, int tz) { return T(lx + tx, ly + ty, lz + tz); }, inv12res, F);
         }
         uint32_t xn[3] = {0u, 0u, 0u};
 #pragma unroll
@@ -608,6 +662,17 @@ __device__ __forceinline__ void gather_force(const typename PP::Pos* xs, const t
                                              int lgN, const PosArgs& pa, typename PP::Mesh* F) {
     using M = typename PP::Mesh;
     const int N = 1 << lgN, msk = N - 1;
+    if constexpr (std::is_same<PP, PosFX>::value) {
+        // the brick kernel's operations (force_fx), from the mesh
+        int of[3] = {0, 0, 0};
+        float af[3] = {0.f, 0.f, 0.f}, Ff[3];
+#pragma unroll
+        for (int k = 0; k < R; ++k) PosFX::gat(xs[k], pa, of[k], af[k]);
+        force_fx<R>(af, MeshNodes<R>{phi, of[0], of[1], of[2], lgN, msk}, (float)(1.0 / (12.0 * pa.res)), Ff);
+#pragma unroll
+        for (int k = 0; k < R; ++k) F[k] = Ff[k];
+        return;
+    }
     int o[R];
     M a1[R];
 #pragma unroll
@@ -770,7 +835,7 @@ __global__ __launch_bounds__(256) void k_kick_drift_inv(typename PP::Pos* __rest
     }
 }
 
-// X += P * cd (src/run/nbody.hh:122-133); FX: cd is in units of L / 2^32.
+// X += P * cd (src/run/nbody.hh:122-133); FX: cd is in units of L / 2^30.
 template <typename PP>
 __global__ __launch_bounds__(256) void k_drift(typename PP::Pos* __restrict__ X, const double* __restrict__ P,
                                                 long n, double cd) {
@@ -816,9 +881,7 @@ __global__ __launch_bounds__(256) void k_zeldovich(const double* __restrict__ ph
         if constexpr (sizeof(typename PP::Mesh) == 8) {
             X[k * np + i] = x;
         } else {
-            double t = x / L;
-            t = t - floor(t);
-            X[k * np + i] = (uint32_t)(unsigned long long)(t * 4294967296.0);
+            X[k * np + i] = fx_word(x / L);
         }
     }
 }
@@ -895,24 +958,16 @@ __global__ __launch_bounds__(256) void k_quantize(double* __restrict__ a, long n
     if (i < n) a[i] = rint(a[i] * (1.0 / q)) * q;
 }
 
-// FX positions -> doubles in [0, L)
+// FX positions -> unwrapped doubles in [-2L, 2L)
 __global__ __launch_bounds__(256) void k_fx_to_f64(const uint32_t* __restrict__ q, long n, double L,
                                                    double* __restrict__ x) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = (double)q[i] * (L / 4294967296.0);
+    if (i < n) x[i] = fx_coord(q[i], L);
 }
 __global__ __launch_bounds__(256) void k_f64_to_fx(const double* __restrict__ x, long n, double L,
                                                    uint32_t* __restrict__ q) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double t = x[i] / L;
-    t = t - floor(t);
-    // floor(t 2^32), except within 2^-19 of an integer, which is taken: a
-    // fixed-point position written as a double (q L / 2^32, off by at most
-    // 2^-20 units after the division) comes back exactly
-    const double u = t * 4294967296.0, r = rint(u);
-    const double v = fabs(u - r) <= 0x1p-19 ? r : floor(u);
-    q[i] = (uint32_t)(unsigned long long)v;  // 2^32 wraps to 0
+    if (i < n) q[i] = fx_word(x[i] / L);
 }
 
 // ------------------------------------------------------------ conan cic -s

@@ -8,10 +8,11 @@ namespace cpm {
 // Parameters of the position -> (cell, fraction) maps.
 struct PosArgs {
     double L, res;   // box size and mesh spacing (F64)
-    int wrap;        // F64 gather: 0 = truncate the unwrapped coordinate (reference), 1 = fold first
-    int fshift;      // FX: 32 - log2 N
+    int wrap;        // gather: 0 = truncate the unwrapped coordinate (reference), 1 = fold first
+    int fshift;      // FX: FX_BITS - log2 N (bits of the fraction within a cell)
     uint32_t fmask;  // FX: (1 << fshift) - 1
     float fscale;    // FX: 2^-fshift
+    int cmask;       // FX: N - 1 (the cell bits above the fraction)
     int brick;       // sort keys: 0 = cell index idx(origin), 1 = brick index (FX)
     int lb[3];       // log2 brick size per axis
     int lnb[3];      // log2 bricks per axis
@@ -60,7 +61,7 @@ __host__ __device__ __forceinline__ double fx_kick_increment(double f, double ck
 }
 // Low 32 bits of rint(y) (round to nearest even), for |y| < 2^51: adding
 // 1.5 * 2^52 leaves rint(y) + 2^51 in the low mantissa bits.  The FX drift
-// adds this to a 32-bit position (y: the step in units of L / 2^32).
+// adds this to a 32-bit position (y: the step in units of L / 2^30).
 __device__ __forceinline__ uint32_t fx_round_u32(double y) {
     return (uint32_t)__double_as_longlong(y + 6755399441055744.0);
 }
@@ -84,14 +85,56 @@ struct PosF64 {
     }
 };
 
+// FX positions: a signed 32-bit fixed-point coordinate of the *unwrapped*
+// position in units of L / 2^FX_BITS, so one word spans [-2L, 2L): the low
+// FX_BITS bits are the position in the box (cell bits above the fraction),
+// the top two bits count box images mod 4.  The periodic wrap is exact
+// integer arithmetic (cell = bits masked to the box), and the sign of the
+// unwrapped coordinate, which the reference's gather depends on (Q1), is the
+// word's sign bit.  A particle stays representable while it moves less than
+// L from where it started in [0, L).
+constexpr int FX_BITS = 30;
+constexpr double FX_UNIT = 1073741824.0;  // 2^FX_BITS positions per box
+
+// Unit coordinate t = x / L -> FX word: t folded into [-2, 2) (whole
+// multiples of 4 boxes), then floor(t 2^30), except within 2^-19 of an
+// integer, which is taken: an FX position written as a double (q L / 2^30)
+// comes back exactly.
+__host__ __device__ __forceinline__ uint32_t fx_word(double t) {
+    t = t - 4.0 * floor((t + 2.0) * 0.25);
+    const double u = t * FX_UNIT, r = rint(u);
+    const double v = fabs(u - r) <= 0x1p-19 ? r : floor(u);
+    return (uint32_t)(long long)v;  // 2^31 (t = 2 after rounding) wraps to -2^31, the same point
+}
+// FX word -> unwrapped coordinate in [-2L, 2L)
+__host__ __device__ __forceinline__ double fx_coord(uint32_t q, double L) {
+    return (double)(int32_t)q * (L / FX_UNIT);
+}
+
 struct PosFX {
     using Pos = uint32_t;
     using Mesh = float;
+    // deposit: the cell of the folded coordinate and the fraction within it
+    // (the reference wraps before the deposit, src/run/numeric.hh:114)
     static __device__ __forceinline__ void dep(Pos x, const PosArgs& a, int& o, float& f) {
-        o = (int)(x >> a.fshift);
+        o = (int)(x >> a.fshift) & a.cmask;
         f = (float)(x & a.fmask) * a.fscale;
     }
-    static __device__ __forceinline__ void gat(Pos x, const PosArgs& a, int& o, float& f) { dep(x, a, o, f); }
+    // gather: the reference truncates the unwrapped coordinate towards zero
+    // (src/run/numeric.hh:19,26-28), so for a negative coordinate with a
+    // nonzero fraction the origin is the next cell up and the weight of its
+    // upper neighbour is fraction - 1 < 0 (an extrapolation).  wrap != 0
+    // folds first, like the deposit.
+    static __device__ __forceinline__ void gat(Pos x, const PosArgs& a, int& o, float& f) {
+        const uint32_t fr = x & a.fmask;
+        o = (int)(x >> a.fshift) & a.cmask;
+        if (!a.wrap && (int32_t)x < 0 && fr != 0u) {
+            o = (o + 1) & a.cmask;
+            f = (float)((int32_t)fr - (int32_t)(a.fmask + 1u)) * a.fscale;  // fraction - 1, rounded once
+        } else {
+            f = (float)fr * a.fscale;
+        }
+    }
 };
 
 // Optional reordering of the gather + kick: read particle perm[s], write it

@@ -49,6 +49,10 @@ class DiffPM:
         _check(lib.pm_diff_create(C.byref(p), C.byref(h)), None)
         self._h = h
         self._lib = lib
+        # forward passes run so far: the handle's backward differentiates the
+        # last one only, so a graph whose forward is no longer the last one
+        # runs it again before its backward (see density)
+        self.forwards = 0
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -64,6 +68,7 @@ class DiffPM:
         if w.size != self.Np:
             raise ValueError(f"noise must hold {self.Np} values")
         rho = np.empty(self.M)
+        self.forwards += 1
         _check(self._lib.pm_diff_forward(self._h, w.ctypes.data, rho.ctypes.data, 0), self._h)
         return rho
 
@@ -84,6 +89,7 @@ class DiffPM:
             raise ValueError(f"noise must be a device tensor of {self.Np} values")
         rho = torch.empty(self.M, dtype=torch.float64, device=w.device)
         torch.cuda.current_stream(w.device).synchronize()
+        self.forwards += 1
         _check(self._lib.pm_diff_forward(self._h, w.data_ptr(), rho.data_ptr(), 1), self._h)
         return rho
 
@@ -113,11 +119,21 @@ def _autograd_function():
         def forward(ctx, noise, engine):
             ctx.engine = engine
             ctx.shape = noise.shape
-            return engine.forward_device(noise)
+            rho = engine.forward_device(noise)
+            ctx.generation = engine.forwards
+            ctx.save_for_backward(noise)
+            return rho
 
         @staticmethod
         def backward(ctx, grad_rho):
-            return ctx.engine.backward_device(grad_rho).reshape(ctx.shape), None
+            engine = ctx.engine
+            if engine.forwards != ctx.generation:
+                # another forward ran on this engine since this graph's one
+                # (e.g. a second density() call, or a validation pass): the
+                # handle holds that run's checkpoints, so replay this one
+                (noise,) = ctx.saved_tensors
+                engine.forward_device(noise)
+            return engine.backward_device(grad_rho).reshape(ctx.shape), None
 
     return PMDensity
 

@@ -36,12 +36,13 @@ extern "C" {
 
 /* precision forms */
 #define PM_FP64 0 /* plain form: FP64 everywhere, reference arithmetic (the drop-in) */
-#define PM_FP32 1 /* mixed form: 32-bit fixed-point positions, FP32 mesh/FFT/gather, FP64 momenta */
+#define PM_FP32 1 /* mixed form: 32-bit fixed-point positions (unwrapped, 2^30 units per box,
+                     so [-2L, 2L) is representable), FP32 mesh/FFT/gather, FP64 momenta */
 
 /* flags */
-#define PM_FLAG_WRAP_GATHER 1u /* FP64: fold positions into [0, L) before the gather, as the
-                                  deposit does; the reference truncates the unwrapped
-                                  coordinate (src/run/numeric.hh:19,26-28) */
+#define PM_FLAG_WRAP_GATHER 1u /* fold positions into [0, L) before the gather, as the
+                                  deposit does; without it both forms truncate the unwrapped
+                                  coordinate like the reference (src/run/numeric.hh:19,26-28) */
 #define PM_FLAG_MIDPOINT 2u    /* drift and kick factors (and the 1/a of the potential) at the
                                   middle a + da/2 of the interval [a, a + da] each integrates
                                   over: second order in da.  Without it they are taken at the
@@ -93,7 +94,11 @@ int pm_kick(pm_handle* h, double a, double da);
 int pm_steps(pm_handle* h, double a, double da, int n, double* a_out);
 
 /* State in the reference layout (Array<dVector<R>> X, P; src/run/nbody.hh:65).
- * PM_FP32 returns positions folded into [0, L). */
+ * Positions are unwrapped, as the reference keeps them.  PM_FP32 holds them
+ * in [-2L, 2L) (2^30 fixed-point units per box): pm_set_state folds a
+ * coordinate outside that range by whole multiples of 4L, and a particle
+ * that moves more than L from where it started in [0, L) wraps the same way.
+ * Only the sign of a coordinate (the gather's truncation, Q1) depends on it. */
 int pm_get_state(pm_handle* h, double* X, double* P);
 int pm_set_state(pm_handle* h, const double* X, const double* P);
 
@@ -242,7 +247,9 @@ int pm_fof(int dim, long long n, double L, double ll, const double* pos, long lo
  *        (pm_init_zeldovich with a_mom = a0 + da/2, as LeapFrog::run,
  *        src/run/leapfrog.hh:36-37) -> nsteps leapfrog steps of da (pm_steps)
  *     -> final CIC density rho on the force mesh (pm_deposit)
- * run in the FP32 form, and its adjoint: given dL/drho, dL/dw by reverse-mode
+ * run in the FP32 form with the gather on folded coordinates
+ * (PM_FLAG_WRAP_GATHER: the force is then a smooth function of the position
+ * on the torus), and its adjoint: given dL/drho, dL/dw by reverse-mode
  * differentiation with hand-written adjoint kernels.  The backward pass
  * integrates the leapfrog backwards (the drift is undone exactly in the
  * fixed-point positions, the kick with the recomputed force) and restores the

@@ -111,10 +111,14 @@ def test_fp32_precision_flag(sim_dir):
     from conan_amd import conanfile
 
     d, _ = sim_dir
+    run(["run", "-i", "t", "-f", "2", "--a0", "0.02", "--da", "0.0098", "-n", "10"], d)
+    X64 = conanfile.read(str(d / "t.pos.1.conan"))[2]["pos"][1]
     run(["run", "-i", "t", "-f", "2", "--a0", "0.02", "--da", "0.0098", "-n", "10", "--precision", "fp32"], d)
     h, _, rec = conanfile.read(str(d / "t.pos.1.conan"))
     X = rec["pos"][1]
-    assert X.min() >= 0 and X.max() < 100.0  # the fixed-point form keeps positions folded
+    # the fixed-point form keeps unwrapped coordinates too, and tracks the FP64 run
+    assert X.min() < 0 and X64.min() < 0
+    assert np.sqrt(np.mean((X - X64) ** 2)) < 1e-4 * 100.0 / 16
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])

@@ -25,7 +25,7 @@ def test_forward_equals_step_sequence(pm, orc, dim, mbits, f, n):
     eng = _engine(dim, mbits, f, ns, a0, da, n, 4)
     rho = eng.forward(w)
     _, phi = pm.ic_scale_free(dim, mbits, L, ns, w)
-    g = pm.Gravity(dim, Nm, f * Nm, L, precision="fp32")
+    g = pm.Gravity(dim, Nm, f * Nm, L, precision="fp32", wrap_gather=True)  # the differentiable map's gather
     g.init(phi, a0, a0 + da / 2)
     g.steps(a0, da, n)
     # the same fixed-point sums: FP64 here, rounded to FP32 by the step's deposit

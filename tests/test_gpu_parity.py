@@ -265,6 +265,80 @@ def test_kick_is_the_gather_of_its_potential_fp64(pm, orc, dim, Nm, Nf, wrap):
     assert np.array_equal(P2, P - F * ck)
 
 
+@pytest.mark.parametrize("dim,Nm,Nf,wrap", [(3, 32, 64, False), (3, 32, 64, True), (2, 64, 64, False),
+                                            (3, 16, 8, False), (3, 64, 256, False), (2, 128, 512, False)])
+def test_kick_is_the_gather_of_its_potential_fp32(pm, orc, dim, Nm, Nf, wrap):
+    """The FP32 kick reads the potential from an LDS tile per brick; a
+    particle with a negative unwrapped coordinate on a brick's upper face
+    (its truncated origin is the next cell up, outside the tile) reads the
+    mesh instead.  Either way its kick must be the direct gather of the same
+    potential (pm_gather_mesh) rounded to the momentum grid, bit for bit.  A
+    third of the particles sit one box below [0, L), so every brick holds
+    such particles."""
+    L, a, da = 100.0, 0.5, 0.02
+    X = _positions(dim, Nm, L, 3.0 * L / Nm, special=False)
+    X[::3] -= L
+    X[1::7] = 37.0 + 0.05 * RNG.standard_normal(X[1::7].shape)
+    P = RNG.standard_normal(X.shape)
+    say("fp32 kick vs gather of its potential", dim, Nm, Nf, wrap)
+    with pm.Gravity(dim, Nm, Nf, L, precision="fp32", wrap_gather=wrap) as g:
+        g.set_state(X, P)
+        X0, P0 = g.get_state()
+        pq = g.momentum_quantum()
+        phi = g.potential(a)
+        F = g.gather(phi)
+        g.kick(a, da)
+        X2, P2 = g.get_state()
+    ck = da / orc.adot(68.5, 1.0, 0.0, a)
+    assert (X0 < 0).any() and np.array_equal(X0, np.round(X0 / (L / 2 ** 30)) * (L / 2 ** 30))
+    assert np.array_equal(X2, X0)
+    assert np.array_equal(P2, P0 - np.rint(F * (ck * (1.0 / pq))) * pq)
+
+
+@pytest.mark.parametrize("dim,Nm,Nf", [(2, 32, 64), (3, 16, 32), (3, 32, 64), (3, 16, 8)])
+def test_fp32_gather_truncates_like_the_reference(pm, orc, dim, Nm, Nf):
+    """The FP32 gather of a given potential at unwrapped positions in
+    [-2L, 2L) against the oracle's FP64 gather (src/run/numeric.hh:17-44,
+    which truncates the unwrapped coordinate towards zero and extrapolates
+    for a negative one), on the same fixed-point positions and the same
+    FP32-rounded potential: FP32 rounding only.  The folded gather differs
+    from it by O(1) on these positions, so a gather that ignored the sign
+    would fail."""
+    L = 100.0
+    X = _positions(dim, Nm, L, 3.0 * L / Nm, special=False)
+    X[::2] -= L
+    X[1::10] += 0.5 * L  # all within [-2L, 2L)
+    phi = RNG.standard_normal(Nf ** dim).astype(np.float32).astype(np.float64)
+    say("fp32 gather, truncation", dim, Nm, Nf)
+    with pm.Gravity(dim, Nm, Nf, L, precision="fp32") as g:
+        g.set_state(X)
+        Xq, _ = g.get_state()
+        F32 = g.gather(phi)
+    assert np.max(np.abs(Xq - X)) < 1e-6 and (Xq < 0).mean() > 0.3
+    F64 = orc.gather(dim, Nf, L, phi, Xq, 0)
+    Ffold = orc.gather(dim, Nf, L, phi, Xq, 1)
+    say(f"  rel {rel(F32, F64):.3g}, folded vs truncated {rel(Ffold, F64):.3g}")
+    assert rel(F32, F64) < 1e-5
+    assert rel(Ffold, F64) > 1e-1
+
+
+def test_fp32_set_positions_keeps_momenta(pm, orc):
+    """pm_set_state(X, NULL) on an FP32 handle that has stepped (particles
+    stored in brick order) keeps every particle's momentum."""
+    dim, Nm, Nf, L, a0, da = 3, 32, 64, 100.0, 0.02, 0.0098
+    noise = orc.white_noise(12, Nm ** dim)
+    phi = orc.compute_potential(dim, 5, L, orc.ic_scale_free(dim, 5, L, -2.0, noise))
+    say("fp32 set_state(X) after steps")
+    with pm.Gravity(dim, Nm, Nf, L, precision="fp32") as g:
+        g.init(phi, a0, a0 + da / 2)
+        g.steps(a0, da, 5)
+        X, P = g.get_state()
+        g.set_state(X)  # the same positions, uploaded in mass-grid order; momenta kept
+        X2, P2 = g.get_state()
+        g.steps(a0 + 5 * da, da, 2)  # and the run goes on
+    assert np.array_equal(X2, X) and np.array_equal(P2, P)
+
+
 def test_steps_fused_equals_unfused(pm, orc):
     dim, Nm, Nf, L, a0, da = 3, 16, 16, 100.0, 0.02, 0.0098
     noise = orc.white_noise(5, Nm ** dim)
@@ -284,14 +358,15 @@ def test_steps_fused_equals_unfused(pm, orc):
     assert np.array_equal(X1, X2) and np.array_equal(P1, P2)
 
 
-def test_fp32_form_tracks_fp64(pm, orc):
+@pytest.mark.parametrize("wrap", [False, True])
+def test_fp32_form_tracks_fp64(pm, orc, wrap):
     dim, Nm, Nf, L, a0, da, n = 3, 32, 32, 100.0, 0.02, 0.0098, 20
     noise = orc.white_noise(11, Nm ** dim)
     phi = orc.compute_potential(dim, 5, L, orc.ic_scale_free(dim, 5, L, -2.0, noise))
-    say("fp32 vs fp64")
+    say("fp32 vs fp64, wrap", wrap)
     out = {}
     for prec in ("fp64", "fp32"):
-        with pm.Gravity(dim, Nm, Nf, L, precision=prec, wrap_gather=True) as g:
+        with pm.Gravity(dim, Nm, Nf, L, precision=prec, wrap_gather=wrap) as g:
             g.init(phi, a0, a0 + da / 2)
             g.steps(a0, da, n)
             X, P = g.get_state()
@@ -305,11 +380,12 @@ def test_fp32_form_tracks_fp64(pm, orc):
     assert np.sqrt(np.mean((r32 - r64) ** 2)) < 1e-3
 
 
+@pytest.mark.parametrize("wrap", [False, True])
 @pytest.mark.parametrize("dim,Nm,Nf", [(3, 128, 256), (3, 64, 256), (3, 128, 512)])
-def test_fp32_kick_matches_fp64_on_large_meshes(pm, orc, dim, Nm, Nf):
+def test_fp32_kick_matches_fp64_on_large_meshes(pm, orc, dim, Nm, Nf, wrap):
     """On meshes with enough bricks, a gather workgroup walks several bricks
     along the last axis and keeps their potential planes in an LDS ring; the
-    FP32 kicks must then still match the FP64 form's (folded gather) to FP32
+    FP32 kicks must then still match the FP64 form's (same gather mode) to FP32
     accuracy: a misplaced plane would show as an O(1) error."""
     L, a0, da, n = 100.0, 0.02, 0.0098, 3
     mbits = int(np.log2(Nm))
@@ -318,7 +394,7 @@ def test_fp32_kick_matches_fp64_on_large_meshes(pm, orc, dim, Nm, Nf):
     say("fp32 vs fp64 kicks", dim, Nm, Nf)
     out = {}
     for prec in ("fp64", "fp32"):
-        with pm.Gravity(dim, Nm, Nf, L, precision=prec, wrap_gather=True) as g:
+        with pm.Gravity(dim, Nm, Nf, L, precision=prec, wrap_gather=wrap) as g:
             g.init(phi, a0, a0 + da / 2)
             _, P0 = g.get_state()
             g.steps(a0, da, n)
@@ -413,9 +489,14 @@ def test_full_size_run(pm, orc):
     to 1 in 200 steps.  The oracle cannot run it in test time; parity is shown
     through what does not depend on the size: the FP64 form's cell keys and
     deposit of its final, clustered state are bit-exact against the oracle's;
-    the FP32 form's P(k) agrees with the FP64 form's (folded gather) within
-    1e-3 per bin, its positions and density track the FP64 ones; mass is
-    conserved; a second FP32 run ends bit-identical."""
+    the FP32 form, against the FP64 form with the reference's own gather (the
+    unwrapped coordinate truncated, src/run/numeric.hh:19,26-28): P(k) within
+    1e-3 per bin over 64 log bins, displacement rms below 1e-3 mesh cell,
+    density rms below 2e-3 (mean 1), force rms below 5e-3 of the force rms
+    (measured 1.8e-3: the trajectories of the clustered end state diverge at
+    the rounding level and Q1 makes the force jump at negative cell
+    boundaries), and, at the same positions (the FP32 operator alone), below
+    1e-4; mass is conserved; a second FP32 run ends bit-identical."""
     dim, Nm, Nf, L, ns, a0, n = 3, 256, 512, 100.0, -1.5, 0.02, 200
     da = (1.0 - a0) / n
     say("full size: IC", Nm, Nf)
@@ -423,24 +504,28 @@ def test_full_size_run(pm, orc):
     _, phi = pm.ic_scale_free(dim, 8, L, ns, noise)
     del noise
     runs = {}
-    for name, prec, wrap in (("fp32", "fp32", False), ("fp32b", "fp32", False), ("fp64", "fp64", True)):
+    for name, prec in (("fp32", "fp32"), ("fp32b", "fp32"), ("fp64", "fp64")):
         say("full size:", name, n, "steps")
-        with pm.Gravity(dim, Nm, Nf, L, precision=prec, wrap_gather=wrap) as g:
+        with pm.Gravity(dim, Nm, Nf, L, precision=prec) as g:
             g.init(phi, a0, a0 + da / 2)
             a = g.steps(a0, da, n)
             assert abs(a - 1.0) < 1e-12
             X, P = g.get_state()
+            if name == "fp32b":
+                runs[name] = (X, P)
+                continue
             pk = g.power_spectrum(64)
+            F = g.forces(a)
             if prec == "fp64":
                 rho, keys = g.deposit(keys=True)
-                runs[name] = (X, P, pk, rho, keys)
+                runs[name] = (X, P, pk, rho, keys, F)
             else:
-                runs[name] = (X, P, pk, None if name == "fp32b" else g.deposit(), None)
-    X32, P32, (k32, p32, c32), r32, _ = runs["fp32"]
-    X32b, P32b = runs.pop("fp32b")[:2]
+                runs[name] = (X, P, pk, g.deposit(), None, F)
+    X32, P32, (k32, p32, c32), r32, _, F32 = runs["fp32"]
+    X32b, P32b = runs.pop("fp32b")
     assert np.array_equal(X32, X32b) and np.array_equal(P32, P32b)
     del X32b, P32b
-    X64, P64, (k64, p64, c64), r64, keys = runs["fp64"]
+    X64, P64, (k64, p64, c64), r64, keys, F64 = runs["fp64"]
 
     say("full size: oracle keys and deposit")
     assert np.array_equal(keys, orc.cic_keys(dim, Nf, L, X64))
@@ -448,16 +533,28 @@ def test_full_size_run(pm, orc):
     assert abs(r64.sum() / Nf ** dim - 1) < 1e-12
     assert abs(r32.sum() / Nf ** dim - 1) < 1e-6
     assert r64.max() > 100  # the end state is clustered
+    assert (X64 < 0).any()  # some coordinates are negative: the gather's truncation matters
 
     m = (c32 > 0) & (p64 > 0)
     assert m.sum() > 50 and np.array_equal(c32, c64) and np.array_equal(k32, k64)
     pk_rel = np.max(np.abs(p32[m] / p64[m] - 1))
-    dx = np.abs(np.mod(X32, L) - np.mod(X64, L))
-    dx = np.minimum(dx, L - dx)
+    dx = np.abs(X32 - X64)  # both unwrapped
+    dx = np.minimum(dx, np.abs(dx - L))
     dx_rms = np.sqrt(np.mean(dx ** 2)) / (L / Nf)
     rho_rms = np.sqrt(np.mean((r32 - r64) ** 2))
+    f_scale = np.sqrt(np.mean(F64 ** 2))
+    f_rms = np.sqrt(np.mean((F32 - F64) ** 2)) / f_scale
+    p_rel = rel(P32, P64)
+    del X32, P32, F32
+    # the FP32 operator alone: its force at the FP64 run's final positions
+    with pm.Gravity(dim, Nm, Nf, L, precision="fp32") as g:
+        g.set_state(X64)
+        f_op = np.sqrt(np.mean((g.forces(1.0) - F64) ** 2)) / f_scale
     say(f"full size: P(k) max rel {pk_rel:.3g}, displacement rms {dx_rms:.3g} mesh cells, "
-        f"density rms {rho_rms:.3g}, momentum rel {rel(P32, P64):.3g}")
+        f"density rms {rho_rms:.3g}, force rms rel {f_rms:.3g} (same positions: {f_op:.3g}), "
+        f"momentum max rel {p_rel:.3g}")
     assert pk_rel < 1e-3
     assert dx_rms < 1e-3
     assert rho_rms < 2e-3
+    assert f_rms < 5e-3
+    assert f_op < 1e-4
