@@ -15,6 +15,8 @@
 // Particle arrays are SoA: component k of particle i at [k * np + i].
 #include "pm_brick.hpp"
 
+#include <algorithm>
+#include <cstddef>
 #include <type_traits>
 
 namespace cpm {
@@ -361,17 +363,6 @@ __device__ __forceinline__ void force_fx(const float* a1, Ld ld, float inv12res,
 
 // The potential at node o - 2 + (tx, ty, tz) of an N^R mesh, wrapped.
 template <int R>
-struct MeshNodes;
-// force_fx from the mesh, kept out of line: the brick kernel calls it only for
-// the rare particles its tile does not cover, and inlined it would hold
-// registers across the whole kernel.
-struct F3 {
-    float v[3];
-};
-template <int R>
-__device__ __attribute__((noinline)) F3 force_fx_mesh(float a0, float a1, float a2, const float* __restrict__ phi,
-                                                      int o0, int o1, int o2, int lgN, float inv12res);
-template <int R>
 struct MeshNodes {
     const float* __restrict__ phi;
     int o0, o1, o2, lgN, msk;
@@ -381,13 +372,53 @@ struct MeshNodes {
         return phi[idx];
     }
 };
+
+// Kick (and drift, with the new sort key and mover counts) of one particle
+// whose force k_gather_kick_fx_brick cannot take from its tile: input index
+// i, sorted position s, the force read from the mesh by the same operations
+// as in the tile (force_fx), the rest as in the tile kernel.
 template <int R>
-__device__ __attribute__((noinline)) F3 force_fx_mesh(float a0, float a1, float a2, const float* __restrict__ phi,
-                                                      int o0, int o1, int o2, int lgN, float inv12res) {
-    const float a[3] = {a0, a1, a2};
-    F3 r;
-    force_fx<R>(a, MeshNodes<R>{phi, o0, o1, o2, lgN, (1 << lgN) - 1}, inv12res, r.v);
-    return r;
+__device__ __forceinline__ void kick_from_mesh(uint32_t s, uint32_t i, const uint32_t* __restrict__ X,
+                                               const double* __restrict__ P, uint32_t* __restrict__ Xo,
+                                               double* __restrict__ Po, long np, const float* __restrict__ phi,
+                                               int lgN, const PosArgs& pa, double ckq, double pq, double cd, int cd_on,
+                                               uint32_t* __restrict__ keys, uint32_t* __restrict__ in_count,
+                                               uint32_t* __restrict__ out_count) {
+    const int msk = (1 << lgN) - 1;
+    uint32_t xs[3] = {0u, 0u, 0u};
+    double pv[3] = {0, 0, 0};
+    int o[3] = {0, 0, 0};
+    float a1[3] = {0.f, 0.f, 0.f}, F[3];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        xs[k] = X[k * np + i];
+        pv[k] = P[k * np + i];
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) PosFX::gat(xs[k], pa, o[k], a1[k]);
+    force_fx<R>(a1, MeshNodes<R>{phi, o[0], o[1], o[2], lgN, msk}, (float)(1.0 / (12.0 * pa.res)), F);
+    uint32_t xn[3] = {0u, 0u, 0u};
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const double p = pv[k] - fx_kick_increment((double)F[k], ckq, pq);
+        Po[k * np + s] = p;
+        xn[k] = cd_on ? xs[k] + fx_round_u32(p * cd) : xs[k];
+        Xo[k * np + s] = xn[k];
+    }
+    if (keys) {
+        int co[3] = {0, 0, 0}, cn[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            co[k] = (int)(xs[k] >> pa.fshift) & msk;
+            cn[k] = (int)(xn[k] >> pa.fshift) & msk;
+        }
+        const uint32_t ko = brick_key(co[0], co[1], co[2], R, pa), kn = brick_key(cn[0], cn[1], cn[2], R, pa);
+        keys[s] = kn;
+        if (kn != ko) {
+            atomicAdd(&in_count[kn], 1u);
+            atomicAdd(&out_count[ko], 1u);
+        }
+    }
 }
 
 // Gather + kick (+ drift).  A workgroup takes a segment of 2^lseg bricks
@@ -417,18 +448,22 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
     const double* __restrict__ P, const uint32_t* __restrict__ idi, uint32_t* __restrict__ Xo,
     double* __restrict__ Po, uint32_t* __restrict__ ido, long np, const float* __restrict__ phi, int lgN,
     PosArgs pa, double ckq, double pq, double cd, int cd_on, int lseg, uint32_t* __restrict__ keys,
-    uint32_t* __restrict__ in_count, uint32_t* __restrict__ out_count) {
+    uint32_t* __restrict__ in_count, uint32_t* __restrict__ out_count, uint32_t* __restrict__ defer, int qpar) {
     constexpr int LA = R - 1;                     // the axis the tile slides along
     constexpr int BL = (R == 3) ? FXB_Z : FXB_X;  // brick extent along it, at most
     constexpr int PSMAX = (R == 3) ? (FXB_X + 5) * (FXB_Y + 5) : FXB_X + 5;
     // The guard keeps the ring at least 256 bytes above LDS address 0, so the
     // base of the second part of a straddling wave (up to 63 cells before the
     // ring) is a valid address.
-    __shared__ struct {
+    constexpr int DQ = 512;  // LDS queue of particles whose force is read from the mesh
+    struct Lds {
         float guard[64];
         float ring[(2 * BL + 5) * PSMAX];
-    } sm;
-    static_assert(sizeof(sm) <= 65536, "the ring must stay addressable below 64 KiB");
+        uint32_t dq[DQ];
+        uint32_t dqn;
+    };
+    __shared__ Lds sm;
+    static_assert(offsetof(Lds, dq) <= 65536, "the ring must stay addressable below 64 KiB");
     float* const ring = sm.ring;
     const int N = 1 << lgN, msk = N - 1;
     const int tid = threadIdx.x;
@@ -550,6 +585,7 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
         }
     };
     if (s0 + tid < s1) fetch(perm[s0 + tid]);
+    if (tid == 0) sm.dqn = 0u;
     load_planes(0, TP, bc.x0[LA], 0);
     __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
     __syncthreads();
@@ -563,17 +599,20 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
         // tile coordinates of the stencil origin (cell o - 2 along each axis)
         const int lx = (o[0] - bc.x0[0]) & msk, ly = (o[1] - bc.x0[1]) & msk;
         const int lz = (R == 3) ? ((o[2] - bc.x0[2]) & msk) : 0;
-        float F[3];
         // A particle whose origin is the cell above its own (a negative
         // unwrapped coordinate, see PosFX::gat) on the brick's upper face
-        // needs one plane beyond the tile: it reads the mesh instead.
+        // needs one plane beyond the tile: it is queued, and kicked from the
+        // mesh when the segment is done (kick_from_mesh).
         if (lx >= bc.B[0] || ly >= bc.B[1] || (R == 3 && lz >= bc.B[2])) {
-            const F3 r = force_fx_mesh<R>(a1[0], a1[1], a1[2], phi, o[0], o[1], o[2], lgN, inv12res);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) F[k] = r.v[k];
-        } else {
-            force_fx<R>(a1, [&](int tx, int ty, int tz) { return T(lx + tx, ly + ty, lz + tz); }, inv12res, F);
+            const uint32_t q = atomicAdd(&sm.dqn, 1u);
+            if (q < (uint32_t)DQ)
+                sm.dq[q] = s;
+            else
+                defer[2 + atomicAdd(&defer[qpar], 1u)] = s;  // for k_gather_deferred
+            return;
         }
+        float F[3];
+        force_fx<R>(a1, [&](int tx, int ty, int tz) { return T(lx + tx, ly + ty, lz + tz); }, inv12res, F);
         uint32_t xn[3] = {0u, 0u, 0u};
 #pragma unroll
         for (int k = 0; k < R; ++k) {
@@ -641,6 +680,35 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
         rot = rot >= TR ? rot - TR : rot;
         bid += cstride;
         bc.x0[LA] += B;
+    }
+    // the queued particles of the segment, from the mesh
+    __syncthreads();
+    const uint32_t nq = sm.dqn < (uint32_t)DQ ? sm.dqn : (uint32_t)DQ;
+    for (uint32_t q = tid; q < nq; q += FXG_NT) {
+        const uint32_t s = sm.dq[q];
+        kick_from_mesh<R>(s, perm[s], X, P, Xo, Po, np, phi, lgN, pa, ckq, pq, cd, cd_on, keys, in_count, out_count);
+    }
+}
+
+// The particles that k_gather_kick_fx_brick's workgroups could not queue in
+// LDS (defer[qpar] of them, sorted positions from defer[2]), kicked from the
+// mesh.  The two counts alternate between launches: this one clears the
+// other, which the next gather fills (the previous launch's queue is done
+// with by now).
+template <int R>
+__global__ __launch_bounds__(256) void k_gather_deferred(uint32_t* __restrict__ defer, int qpar,
+                                                          const uint32_t* __restrict__ perm,
+                                                          const uint32_t* __restrict__ X, const double* __restrict__ P,
+                                                          uint32_t* __restrict__ Xo, double* __restrict__ Po, long np,
+                                                          const float* __restrict__ phi, int lgN, PosArgs pa,
+                                                          double ckq, double pq, double cd, int cd_on,
+                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ in_count,
+                                                          uint32_t* __restrict__ out_count) {
+    const uint32_t n = defer[qpar];
+    if (blockIdx.x == 0 && threadIdx.x == 0) defer[qpar ^ 1] = 0u;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const uint32_t s = defer[2 + q];
+        kick_from_mesh<R>(s, perm[s], X, P, Xo, Po, np, phi, lgN, pa, ckq, pq, cd, cd_on, keys, in_count, out_count);
     }
 }
 
@@ -1125,7 +1193,7 @@ template void deposit_fx_brick<double>(int, const uint32_t*, const uint32_t*, co
 void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, const double* P,
                           const uint32_t* idi, uint32_t* Xo, double* Po, uint32_t* ido, long np, const float* phi,
                           int lgN, const PosArgs& pa, double ck, double pq, double cd, int cd_on, uint32_t* keys,
-                          uint32_t* in_count, uint32_t* out_count, hipStream_t st) {
+                          uint32_t* in_count, uint32_t* out_count, uint32_t* defer, int qpar, hipStream_t st) {
     // segments of up to 2^CPM_GATHER_LSEG bricks along the last axis, as long
     // as that leaves at least 4 workgroups per compute unit
     const long nbr = brick_count(pa), minwg = 4 * device_cus();
@@ -1134,7 +1202,14 @@ void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, c
     const unsigned nb = (unsigned)(nbr >> lseg);
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_kick_fx_brick<RR>), dim3(nb), dim3(FXG_NT), 0, st, bstart, perm,
                                          X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck * (1.0 / pq), pq, cd, cd_on, lseg,
-                                         keys, in_count, out_count));
+                                         keys, in_count, out_count, defer, qpar));
+    CPM_LAUNCH_CHECK();
+    if (pa.wrap) return;  // folded coordinates: every origin lies in its brick, nothing is queued
+    // the overflow of the workgroups' LDS queues: usually empty
+    const long nbd = std::min<long>(256, (np + 255) / 256);
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_deferred<RR>), dim3((unsigned)nbd), dim3(256), 0, st, defer, qpar,
+                                         perm, X, P, Xo, Po, np, phi, lgN, pa, ck * (1.0 / pq), pq, cd, cd_on, keys,
+                                         in_count, out_count));
     CPM_LAUNCH_CHECK();
 }
 

@@ -265,22 +265,28 @@ def test_kick_is_the_gather_of_its_potential_fp64(pm, orc, dim, Nm, Nf, wrap):
     assert np.array_equal(P2, P - F * ck)
 
 
-@pytest.mark.parametrize("dim,Nm,Nf,wrap", [(3, 32, 64, False), (3, 32, 64, True), (2, 64, 64, False),
-                                            (3, 16, 8, False), (3, 64, 256, False), (2, 128, 512, False)])
-def test_kick_is_the_gather_of_its_potential_fp32(pm, orc, dim, Nm, Nf, wrap):
+@pytest.mark.parametrize("dim,Nm,Nf,wrap,corner", [(3, 32, 64, False, False), (3, 32, 64, True, False),
+                                                   (2, 64, 64, False, False), (3, 16, 8, False, False),
+                                                   (3, 64, 256, False, False), (2, 128, 512, False, False),
+                                                   (3, 32, 64, False, True), (2, 64, 128, False, True)])
+def test_kick_is_the_gather_of_its_potential_fp32(pm, orc, dim, Nm, Nf, wrap, corner):
     """The FP32 kick reads the potential from an LDS tile per brick; a
     particle with a negative unwrapped coordinate on a brick's upper face
     (its truncated origin is the next cell up, outside the tile) reads the
     mesh instead.  Either way its kick must be the direct gather of the same
     potential (pm_gather_mesh) rounded to the momentum grid, bit for bit.  A
     third of the particles sit one box below [0, L), so every brick holds
-    such particles."""
+    such particles.  corner: every particle just below the origin of the box
+    on every axis, all in the one corner cell of one brick, which overflows
+    the workgroup's queue of them."""
     L, a, da = 100.0, 0.5, 0.02
     X = _positions(dim, Nm, L, 3.0 * L / Nm, special=False)
     X[::3] -= L
     X[1::7] = 37.0 + 0.05 * RNG.standard_normal(X[1::7].shape)
+    if corner:
+        X = -(L / Nf) * RNG.uniform(0.05, 0.95, X.shape)
     P = RNG.standard_normal(X.shape)
-    say("fp32 kick vs gather of its potential", dim, Nm, Nf, wrap)
+    say("fp32 kick vs gather of its potential", dim, Nm, Nf, wrap, corner)
     with pm.Gravity(dim, Nm, Nf, L, precision="fp32", wrap_gather=wrap) as g:
         g.set_state(X, P)
         X0, P0 = g.get_state()
@@ -307,7 +313,8 @@ def test_fp32_gather_truncates_like_the_reference(pm, orc, dim, Nm, Nf):
     L = 100.0
     X = _positions(dim, Nm, L, 3.0 * L / Nm, special=False)
     X[::2] -= L
-    X[1::10] += 0.5 * L  # all within [-2L, 2L)
+    X[1::10] += 0.5 * L
+    X = np.clip(X, -1.9 * L, 1.9 * L)  # all within [-2L, 2L)
     phi = RNG.standard_normal(Nf ** dim).astype(np.float32).astype(np.float64)
     say("fp32 gather, truncation", dim, Nm, Nf)
     with pm.Gravity(dim, Nm, Nf, L, precision="fp32") as g:
