@@ -29,7 +29,8 @@ EXPORTS = [
     "pm_last_error", "pm_create", "pm_destroy", "pm_init_zeldovich", "pm_drift", "pm_kick",
     "pm_steps", "pm_get_state", "pm_set_state", "pm_get_times", "pm_get_momentum_quantum",
     "pm_set_momentum_quantum", "pm_deposit", "pm_potential",
-    "pm_forces", "pm_gather_mesh", "pm_poisson_mesh", "pm_power_spectrum", "pm_diagnostics", "pm_profile", "pm_sync",
+    "pm_forces", "pm_gather_mesh", "pm_poisson_mesh", "pm_power_spectrum", "pm_power_spectrum_noalloc",
+    "pm_diagnostics", "pm_profile", "pm_sync",
     "pm_ic_scale_free", "pm_ic_cdm", "pm_ic_potential", "pm_ic_displacement", "pm_fft_r2c", "pm_fft_c2r",
     "pm_cic", "pm_diff_create", "pm_diff_destroy", "pm_diff_last_error", "pm_diff_forward", "pm_diff_backward",
     "pm_diff_get_stats", "pm_fof_create", "pm_fof_destroy", "pm_fof_last_error", "pm_fof_run", "pm_fof_get_stats",
@@ -116,6 +117,7 @@ def load() -> C.CDLL:
         "pm_gather_mesh": (C.c_int, [V, _dp, _dp]),
         "pm_poisson_mesh": (C.c_int, [V, _dp, C.c_double, _dp]),
         "pm_power_spectrum": (C.c_int, [V, C.c_int, _dp, _dp, _dp, _dp]),
+        "pm_power_spectrum_noalloc": (C.c_int, [V, C.c_int, _dp, _dp, _dp, _dp]),
         "pm_diagnostics": (C.c_int, [V, C.POINTER(pm_diag)]),
         "pm_profile": (C.c_int, [V, C.c_int, _dp, C.POINTER(C.c_longlong)]),
         "pm_sync": (C.c_int, [V]),

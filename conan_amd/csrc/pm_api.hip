@@ -221,6 +221,14 @@ int pm_power_spectrum(pm_handle* h, int nbins, const double* edges, double* kmea
     PM_CATCH(h)
 }
 
+int pm_power_spectrum_noalloc(pm_handle* h, int nbins, const double* edges, double* kmean, double* pk,
+                              double* count) {
+    PM_NEED(h)
+    PM_TRY
+    h->impl->power_spectrum_noalloc(nbins, edges, kmean, pk, count);
+    PM_CATCH(h)
+}
+
 int pm_diagnostics(pm_handle* h, pm_diag* d) {
     PM_NEED(h)
     PM_TRY

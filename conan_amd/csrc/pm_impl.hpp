@@ -138,6 +138,7 @@ struct ImplBase {
     virtual void poisson_mesh(const double* rho, double a, double* phi) = 0;
     virtual void gather_mesh(const double* phi, double* F) = 0;
     virtual void power_spectrum(int nbins, const double* edges, double* k, double* pk, double* cnt) = 0;
+    virtual void power_spectrum_noalloc(int nbins, const double* edges, double* k, double* pk, double* cnt) = 0;
     virtual void diagnostics(pm_diag* d) = 0;
     virtual double momentum_quantum() const = 0;
     virtual void set_momentum_quantum(double pq) = 0;
@@ -739,6 +740,34 @@ struct Impl : ImplBase {
             if (kmean) kmean[b] = h[b];
             if (pk) pk[b] = h[nbins + b];
             if (cnt) cnt[b] = h[2 * nbins + b];
+        }
+        have_deposit = false;  // the mesh was consumed
+    }
+
+    // power_spectrum with no allocation of any kind: the handle's mesh and
+    // spectrum, the estimator's fixed device scratch, host arrays on the stack.
+    void power_spectrum_noalloc(int nbins, const double* edges, double* kmean, double* pk, double* cnt) override {
+        need_state();
+        if (nbins < 1 || nbins > PKS_MAXBINS)
+            throw Error(PM_EINVAL, "pm_power_spectrum_noalloc: nbins must lie in [1, " + std::to_string(PKS_MAXBINS) +
+                                       "]");
+        double e[PKS_MAXBINS + 1], out[3 * PKS_MAXBINS];
+        const double kf = 2 * M_PI / cfg.L, kn = M_PI * cfg.Nf / cfg.L;
+        for (int b = 0; b <= nbins; ++b) e[b] = edges ? edges[b] : kf * pow(kn / kf, (double)b / nbins);
+        do_deposit((Mesh)mass);
+        const double* mean_ptr = nullptr;
+        if (FX) {
+            sum_partials(row_partial.p, npartials, 1.0 / (double)M, red.p, 4096, scal.p, st);
+            mean_ptr = scal.p;
+        }
+        fft.forward(mesh.p, spec.p, nyq.p, (Mesh)1.0, mean_ptr, st);
+        const double norm = pow(cfg.L, R) / ((double)M * (double)M);
+        cpm::power_spectrum_noalloc<Mesh>(spec.p, nyq.p, (long)fft.spec_elems, (long)fft.nyq_elems, R, lgNf, kf, e,
+                                          nbins, norm, out, st);
+        for (int b = 0; b < nbins; ++b) {
+            if (kmean) kmean[b] = out[b];
+            if (pk) pk[b] = out[nbins + b];
+            if (cnt) cnt[b] = out[2 * nbins + b];
         }
         have_deposit = false;  // the mesh was consumed
     }

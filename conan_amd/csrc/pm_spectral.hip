@@ -12,6 +12,9 @@
 // (`fft.in[0] = 0`, ic.cc:68,78,113).
 #include "pm_spectral.hpp"
 
+#include <mutex>
+#include <string>
+
 namespace cpm {
 
 struct ModeIdx {
@@ -216,6 +219,136 @@ void power_spectrum(const cx<T>* spec, const cx<T>* nyq, long nmain, long nnyq, 
                        nb, nbins, norm, kmean, pk, cnt);
     CPM_LAUNCH_CHECK();
 }
+
+// ------------------------------------------------------------ P(k) without allocation
+// pm_power_spectrum_noalloc keeps everything the estimator needs beyond the
+// handle's spectrum in one file-scope device array of fixed size: the bin
+// edges, the partial sums of a fixed grid of PKS_BLOCKS workgroups, and the
+// per-bin results.  Each device holds its own copy; its address comes from
+// hipGetSymbolAddress (pk_scratch), never from the array's name in host code.
+static constexpr int PKS_BLOCKS = 512;
+static constexpr long PKS_EDGES = 0;
+static constexpr long PKS_PART = PKS_EDGES + PKS_MAXBINS + 1;
+static constexpr long PKS_OUT = PKS_PART + (long)PKS_BLOCKS * PKS_MAXBINS * 3;
+static constexpr long PKS_WORDS = PKS_OUT + 3L * PKS_MAXBINS;
+static __device__ __align__(256) double scratch[PKS_WORDS];
+
+double* pk_scratch() {
+    static std::mutex mu;
+    static double* ptr[64] = {nullptr};
+    int dev = 0;
+    CPM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw Error(-2, "device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error(-3, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        ptr[dev] = static_cast<double*>(p);
+    }
+    return ptr[dev];
+}
+
+// As k_pk_partial, over a fixed grid: workgroup g takes the chunks g,
+// g + gridDim.x, ... in order and sums each bin over them in that order.
+template <typename T>
+__global__ __launch_bounds__(PK_NT) void k_pk_partial_grid(const cx<T>* __restrict__ spec,
+                                                           const cx<T>* __restrict__ nyq, long nmain, long nnyq,
+                                                           int dim, int lgN, double kf,
+                                                           const double* __restrict__ edges, int nbins,
+                                                           double* __restrict__ part) {
+    __shared__ short sb[PK_CH];
+    __shared__ double sp[PK_CH];
+    __shared__ double sk[PK_CH];
+    __shared__ float sw[PK_CH];
+    const int N = 1 << lgN;
+    const long n = nmain + nnyq;
+    const long nch = (n + PK_CH - 1) / PK_CH;
+    double P = 0, Kk = 0, W = 0;  // bin threadIdx.x (nbins <= PK_NT)
+    for (long c = blockIdx.x; c < nch; c += gridDim.x) {
+        const long e0 = c * PK_CH;
+        for (int t = threadIdx.x; t < PK_CH; t += PK_NT) {
+            const long e = e0 + t;
+            short bin = -1;
+            double p2 = 0, kk = 0;
+            float w = 0;
+            if (e < n) {
+                const ModeIdx m = mode_of(e, nmain, dim, lgN);
+                double K[3];
+                K[0] = kcomp(m.ix, N, kf);
+                K[1] = kcomp(m.iy, N, kf);
+                K[2] = dim == 3 ? kcomp(m.iz, N, kf) : 0.0;
+                double k2 = 0;
+                for (int k = 0; k < dim; ++k) k2 += K[k] * K[k];
+                const double k = sqrt(k2);
+                const bool dc = (m.ix == 0 && m.iy == 0 && m.iz == 0);
+                if (!dc && k >= edges[0] && k <= edges[nbins]) {
+                    int lo = 0, hi = nbins - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (edges[mid] <= k) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    bin = (short)lo;
+                    const cx<T> z = e < nmain ? spec[e] : nyq[e - nmain];
+                    w = (m.ix == 0 || m.ix == N / 2) ? 1.f : 2.f;
+                    const double zr = (double)z.re, zi = (double)z.im;
+                    p2 = (double)w * (zr * zr + zi * zi);
+                    kk = (double)w * k;
+                }
+            }
+            sb[t] = bin;
+            sp[t] = p2;
+            sk[t] = kk;
+            sw[t] = w;
+        }
+        __syncthreads();
+        const int b = threadIdx.x;
+        if (b < nbins) {
+            double cP = 0, cK = 0, cW = 0;
+            for (int t = 0; t < PK_CH; ++t)
+                if (sb[t] == b) {
+                    cP += sp[t];
+                    cK += sk[t];
+                    cW += (double)sw[t];
+                }
+            P += cP;
+            Kk += cK;
+            W += cW;
+        }
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < nbins) {
+        double* o = part + ((long)blockIdx.x * nbins + threadIdx.x) * 3;
+        o[0] = P;
+        o[1] = Kk;
+        o[2] = W;
+    }
+}
+
+template <typename T>
+void power_spectrum_noalloc(const cx<T>* spec, const cx<T>* nyq, long nmain, long nnyq, int dim, int lgN, double kf,
+                            const double* edges_host, int nbins, double norm, double* out_host, hipStream_t st) {
+    if (nbins < 1 || nbins > PKS_MAXBINS) throw Error(-1, "pm_power_spectrum_noalloc: nbins must lie in [1, 128]");
+    double* const scr = pk_scratch();
+    double* const edges = scr + PKS_EDGES;
+    double* const part = scr + PKS_PART;
+    double* const out = scr + PKS_OUT;
+    CPM_HIP(hipMemcpyAsync(edges, edges_host, sizeof(double) * (nbins + 1), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL((k_pk_partial_grid<T>), dim3(PKS_BLOCKS), dim3(PK_NT), 0, st, spec, nyq, nmain, nnyq, dim, lgN,
+                       kf, (const double*)edges, nbins, part);
+    CPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_pk_final, dim3((unsigned)nbins), dim3(PK_NT), 0, st, (const double*)part, (long)PKS_BLOCKS,
+                       nbins, norm, out, out + nbins, out + 2 * nbins);
+    CPM_LAUNCH_CHECK();
+    CPM_HIP(hipMemcpyAsync(out_host, out, sizeof(double) * 3 * nbins, hipMemcpyDeviceToHost, st));
+    CPM_HIP(hipStreamSynchronize(st));
+}
+template void power_spectrum_noalloc<float>(const cx<float>*, const cx<float>*, long, long, int, int, double,
+                                            const double*, int, double, double*, hipStream_t);
+template void power_spectrum_noalloc<double>(const cx<double>*, const cx<double>*, long, long, int, int, double,
+                                             const double*, int, double, double*, hipStream_t);
 
 template void power_spectrum<float>(const cx<float>*, const cx<float>*, long, long, int, int, double,
                                     const double*, int, double, double*, double*, double*, double*, hipStream_t);

@@ -22,6 +22,15 @@ void apply_filter(cx<double>* spec, cx<double>* nyq, long nmain, long nnyq, cons
                   hipStream_t st);
 
 long pk_partial_elems(long nmain, long nnyq, int nbins);
+// The estimator with no device allocation: its scratch (edges, partials,
+// results) is a fixed file-scope device array, so at most PKS_MAXBINS bins.
+// edges_host: nbins + 1 host doubles; out_host: 3 nbins host doubles (mean
+// |k|, P(k), mode count per bin).  Calls that overlap on one device share
+// the array and race.
+constexpr int PKS_MAXBINS = 128;
+template <typename T>
+void power_spectrum_noalloc(const cx<T>* spec, const cx<T>* nyq, long nmain, long nnyq, int dim, int lgN, double kf,
+                            const double* edges_host, int nbins, double norm, double* out_host, hipStream_t st);
 template <typename T>
 void power_spectrum(const cx<T>* spec, const cx<T>* nyq, long nmain, long nnyq, int dim, int lgN, double kf,
                     const double* edges, int nbins, double norm, double* part, double* kmean, double* pk,

@@ -137,6 +137,15 @@ class Gravity:
         self._ck(self._lib.pm_power_spectrum(self._h, nbins, ptr(e), ptr(k), ptr(p), ptr(n)))
         return k, p, n
 
+    def power_spectrum_noalloc(self, nbins: int = 64, edges=None):
+        """power_spectrum through pm_power_spectrum_noalloc (no allocation; nbins <= 128)."""
+        k = np.empty(nbins)
+        p = np.empty(nbins)
+        n = np.empty(nbins)
+        e = None if edges is None else _c(edges)
+        self._ck(self._lib.pm_power_spectrum_noalloc(self._h, nbins, ptr(e), ptr(k), ptr(p), ptr(n)))
+        return k, p, n
+
     def diagnostics(self):
         d = _lib.pm_diag()
         self._ck(self._lib.pm_diagnostics(self._h, _lib.C.byref(d)))

@@ -147,6 +147,19 @@ int pm_poisson_mesh(pm_handle* h, const double* rho, double a, double* phi);
 int pm_power_spectrum(pm_handle* h, int nbins, const double* edges, double* kmean, double* pk,
                       double* count);
 
+/* pm_power_spectrum without any allocation: no hipMalloc / hipFree, no
+ * host heap.  No counterpart in the reference; for callers that cannot
+ * allocate (code under HIP graph capture, tight serving or monitoring loops,
+ * harnesses that forbid allocation) and want P(k) every step.  Its scratch
+ * (bin edges, the partial sums of a fixed grid of 512 workgroups, results:
+ * 1.6 MB) is one file-scope device array per GPU, held until the process
+ * ends; so nbins <= 128 (PM_EINVAL beyond).  Same outputs as
+ * pm_power_spectrum, summed in another fixed order (equal to ~1e-14).  Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU, from
+ * any handle or thread, race. */
+int pm_power_spectrum_noalloc(pm_handle* h, int nbins, const double* edges, double* kmean, double* pk,
+                              double* count);
+
 /* FP64 reductions of the state and the last deposit. */
 typedef struct pm_diag {
     double mass;          /* sum of the last deposit (= Nm^dim * (Nf/Nm)^dim) */
