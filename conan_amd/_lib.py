@@ -52,7 +52,7 @@ class pm_config(C.Structure):
 
 class pm_ic_params(C.Structure):
     _fields_ = [("dim", C.c_int), ("mbits", C.c_int), ("L", C.c_double), ("ns", C.c_double),
-                ("smooth", C.c_int), ("scale", C.c_double), ("sigma", C.c_double)]
+                ("smooth", C.c_int), ("scale", C.c_double), ("sigma", C.c_double), ("nyquist_cutoff", C.c_int)]
 
 
 class pm_diag(C.Structure):
@@ -122,7 +122,8 @@ def load() -> C.CDLL:
         "pm_profile": (C.c_int, [V, C.c_int, _dp, C.POINTER(C.c_longlong)]),
         "pm_sync": (C.c_int, [V]),
         "pm_ic_scale_free": (C.c_int, [C.POINTER(pm_ic_params), _dp, _dp, _dp]),
-        "pm_ic_cdm": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp]),
+        "pm_ic_cdm": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp,
+                                _dp]),
         "pm_ic_potential": (C.c_int, [C.c_int, C.c_int, C.c_double, _dp]),
         "pm_ic_displacement": (C.c_int, [C.c_int, C.c_int, C.c_double, _dp, _dp]),
         "pm_fft_r2c": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp]),

@@ -70,7 +70,9 @@ void cmd_ic(int argc, char** argv) {
          {0, "p", "potential", "false", "include the potential in the result."},
          {0, "z", "displacement", "false", "include the Zel'dovich displacement in the result."},
          {O::VALUED | O::CHECK, "", "noise", "",
-          "read the white noise from record 'noise' of this .conan file instead of drawing it from --seed."}});
+          "read the white noise from record 'noise' of this .conan file instead of drawing it from --seed."},
+         {0, "", "nyquist-cutoff", "false",
+          "leave out the modes beyond the Nyquist wave number (|k| > pi N / L) in the initial conditions."}});
     if (C.get<bool>("help")) help_and_exit(C);
 
     Header H;
@@ -106,11 +108,11 @@ void cmd_ic(int argc, char** argv) {
     std::vector<double> D(size);
     if (H.get<bool>("scale-free")) {
         pm_ic_params p{dim, mbits, L, H.get<double>("ns"), H.get<bool>("smooth") ? 1 : 0, H.get<double>("scale"),
-                       H.get<double>("sigma")};
+                       H.get<double>("sigma"), H.get<bool>("nyquist-cutoff") ? 1 : 0};
         pm_ok(pm_ic_scale_free(&p, noise.data(), D.data(), nullptr));
     } else {
         pm_ok(pm_ic_cdm(dim, mbits, L, H.get<double>("ns"), H.get<double>("Omega0"), H.get<double>("H0"),
-                        noise.data(), D.data()));
+                        H.get<bool>("nyquist-cutoff") ? 1 : 0, noise.data(), D.data()));
     }
 
     std::ofstream fo(C["id"] + ".density.init.conan", std::ios::binary);

@@ -326,6 +326,7 @@ int pm_ic_scale_free(const pm_ic_params* p, const double* noise, double* density
     cpm::FilterArgs fa = c.fargs(cpm::FILT_PS, 2 * M_PI / (double)N);  // kspace<R>(N, N)
     fa.ns = p->ns;
     fa.t = p->scale * N / p->L;  // Fourier::scale<R>(sigma * N/L), ic.cc:63
+    fa.kcut2 = p->nyquist_cutoff ? M_PI * M_PI : 0.0;  // grid units: |K| > pi
     cpm::apply_filter(c.spec.p, c.nyq.p, (long)c.fft.spec_elems, (long)c.fft.nyq_elems, fa, c.st);
     c.fft.backward(c.spec.p, c.nyq.p, c.mesh.p, (double)c.M, c.st);
     cpm::reduce_sum(c.mesh.p, c.M, 1, 1.0 / (double)c.M, c.red.p, 4096, c.scal.p, c.st);
@@ -345,8 +346,8 @@ int pm_ic_scale_free(const pm_ic_params* p, const double* noise, double* density
     PM_CATCH(h)
 }
 
-int pm_ic_cdm(int dim, int mbits, double L, double ns, double Omega0, double H0, const double* noise,
-              double* density) {
+int pm_ic_cdm(int dim, int mbits, double L, double ns, double Omega0, double H0, int nyquist_cutoff,
+              const double* noise, double* density) {
     pm_handle* h = nullptr;
     PM_TRY
     if (!noise || !density) throw cpm::Error(PM_EINVAL, "null argument");
@@ -361,6 +362,7 @@ int pm_ic_cdm(int dim, int mbits, double L, double ns, double Omega0, double H0,
     fa.ns = ns;
     fa.omega0 = Omega0;
     fa.h = H0 / 100.0;
+    fa.kcut2 = nyquist_cutoff ? (M_PI * N / L) * (M_PI * N / L) : 0.0;
     cpm::apply_filter(c.spec.p, c.nyq.p, (long)c.fft.spec_elems, (long)c.fft.nyq_elems, fa, c.st);
     c.fft.backward(c.spec.p, c.nyq.p, c.mesh.p, c.M * pow(res, 0.5 * dim), c.st);  // ic.cc:92
     c.download(density);

@@ -9,7 +9,9 @@
 //   CDM   sqrt(A k^ns T0(k)^2), the BBKS-like transfer function of CDM<R>
 //         (ic.cc:14-35), k physical
 // The first three zero the k = 0 mode as the reference does after filtering
-// (`fft.in[0] = 0`, ic.cc:68,78,113).
+// (`fft.in[0] = 0`, ic.cc:68,78,113).  With kcut2 > 0 (conan ic
+// --nyquist-cutoff, no reference counterpart) PS, P and CDM also zero every
+// mode beyond the Nyquist sphere |k| > pi N / L.
 #include "pm_spectral.hpp"
 
 #include <mutex>
@@ -67,6 +69,10 @@ __global__ __launch_bounds__(256) void k_filter(cx<double>* __restrict__ spec, c
     }
     double k2 = 0;
     for (int k = 0; k < fa.dim; ++k) k2 += K[k] * K[k];
+    if (fa.kind != FILT_POT && fa.kcut2 > 0 && k2 > fa.kcut2) {  // --nyquist-cutoff
+        *p = mk<double>(0, 0);
+        return;
+    }
     double f;
     if (fa.kind == FILT_POT) {
         f = -1. / k2;

@@ -16,6 +16,7 @@ struct FilterArgs {
     int axis;       // FILT_DERIV axis
     double omega0;  // FILT_CDM: matter density
     double h;       // FILT_CDM: H0 / 100
+    double kcut2;   // PS, P, CDM: zero modes with |k|^2 > kcut2 (kscale units); 0: no cutoff
 };
 
 void apply_filter(cx<double>* spec, cx<double>* nyq, long nmain, long nnyq, const FilterArgs& fa,

@@ -184,12 +184,22 @@ class LeapFrog:
         return a
 
 
+def ic_cdm(dim: int, mbits: int, L: float, noise, ns: float = 0.96, Omega0: float = 0.32, H0: float = 69.0,
+           nyquist_cutoff: bool = False):
+    """conan ic without --scale-free (the CDM branch, src/ic/ic.cc:14-35,84-97) on the GPU."""
+    lib = _lib.load()
+    dens = np.empty((1 << mbits) ** dim)
+    check(lib.pm_ic_cdm(dim, mbits, L, ns, Omega0, H0, 1 if nyquist_cutoff else 0, ptr(_c(noise).ravel()),
+                        ptr(dens)))
+    return dens
+
+
 def ic_scale_free(dim: int, mbits: int, L: float, ns: float, noise, smooth: bool = False,
-                  scale: float = 8.0, sigma: float = 0.83, potential: bool = True):
-    """conan ic --scale-free [-p] on the GPU (src/ic/ic.cc:37-83,100-117)."""
+                  scale: float = 8.0, sigma: float = 0.83, potential: bool = True, nyquist_cutoff: bool = False):
+    """conan ic --scale-free [-p] [--nyquist-cutoff] on the GPU (src/ic/ic.cc:37-83,100-117)."""
     lib = _lib.load()
     N = 1 << mbits
-    p = _lib.pm_ic_params(dim, mbits, L, ns, 1 if smooth else 0, scale, sigma)
+    p = _lib.pm_ic_params(dim, mbits, L, ns, 1 if smooth else 0, scale, sigma, 1 if nyquist_cutoff else 0)
     dens = np.empty(N ** dim)
     phi = np.empty(N ** dim) if potential else None
     check(lib.pm_ic_scale_free(_lib.C.byref(p), ptr(_c(noise).ravel()), ptr(dens), ptr(phi)))

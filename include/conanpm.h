@@ -192,14 +192,18 @@ typedef struct pm_ic_params {
     int smooth;       /* apply the smoothing to the final field too */
     double scale;     /* smoothing scale (Mpc/h) */
     double sigma;     /* normalisation */
+    int nyquist_cutoff; /* zero the modes with |k| > pi N / L (beyond the Nyquist sphere) in
+                           the filters; no counterpart in the reference (conan ic
+                           --nyquist-cutoff); 0: every mode, as the reference */
 } pm_ic_params;
 int pm_ic_scale_free(const pm_ic_params* p, const double* noise, double* density, double* potential);
 
 /* _generate_random_field<R>, CDM branch (src/ic/ic.cc:84-97 with CDM<R>,
  * ic.cc:14-35): noise filtered by sqrt(P_CDM(k)), physical k, divided by
- * N^dim * res^(dim/2).  density: N^dim doubles. */
-int pm_ic_cdm(int dim, int mbits, double L, double ns, double Omega0, double H0, const double* noise,
-              double* density);
+ * N^dim * res^(dim/2).  density: N^dim doubles.  nyquist_cutoff: as in
+ * pm_ic_params. */
+int pm_ic_cdm(int dim, int mbits, double L, double ns, double Omega0, double H0, int nyquist_cutoff,
+              const double* noise, double* density);
 
 /* _compute_potential<R> alone (src/ic/ic.cc:100-117): in-place. */
 int pm_ic_potential(int dim, int mbits, double L, double* field);

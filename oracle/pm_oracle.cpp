@@ -144,17 +144,24 @@ void fft_nd(const Box& box, const cplx* in, cplx* out, int sgn, int nthreads) {
     for (int d = 0; d < box.R; ++d) {
         const size_t st = box.stride[d];
         const size_t npencil = S / (size_t)N;
+        // pencils with consecutive low indices are copied out and back in
+        // groups of G, so the strided axes are read in runs of G elements
+        // (the same transform of every pencil)
+        const size_t G = st >= 16 ? 16 : 1;
 #pragma omp parallel num_threads(nthreads)
         {
-            std::vector<cplx> buf(N);
+            std::vector<cplx> buf((size_t)N * G);
 #pragma omp for schedule(static)
-            for (long long p = 0; p < (long long)npencil; ++p) {
-                // pencil p: low = p % st, high = p / st
-                size_t lo = (size_t)p % st, hi = (size_t)p / st;
-                size_t base = lo + hi * st * (size_t)N;
-                for (int n = 0; n < N; ++n) buf[n] = out[base + (size_t)n * st];
-                f.run(buf.data(), sgn);
-                for (int n = 0; n < N; ++n) out[base + (size_t)n * st] = buf[n];
+            for (long long pg = 0; pg < (long long)(npencil / G); ++pg) {
+                // pencils p = pg G + g: low = p % st, high = p / st
+                const size_t p0 = (size_t)pg * G;
+                const size_t lo = p0 % st, hi = p0 / st;
+                const size_t base = lo + hi * st * (size_t)N;
+                for (int n = 0; n < N; ++n)
+                    for (size_t g = 0; g < G; ++g) buf[g * N + n] = out[base + (size_t)n * st + g];
+                for (size_t g = 0; g < G; ++g) f.run(buf.data() + g * N, sgn);
+                for (int n = 0; n < N; ++n)
+                    for (size_t g = 0; g < G; ++g) out[base + (size_t)n * st + g] = buf[g * N + n];
             }
         }
     }
@@ -297,6 +304,9 @@ struct Sim {
     Cosmology cos;
     double mass;
     int nthreads, wrap;
+    // the deposit on one thread (the reference's serial summation order, which
+    // the FP64 GPU deposit reproduces) while FFT and particle loops use nthreads
+    int serial_deposit = 0;
     std::vector<double> X, P, delta;
     std::vector<cplx> in, out;
     double a, da;
@@ -330,7 +340,7 @@ struct Sim {
     }
     void kick(double ak, double dak) {
         const int R = mbox.R;
-        md_cic(fbox, delta.data(), X.data(), mbox.size, mass, nthreads, nullptr);
+        md_cic(fbox, delta.data(), X.data(), mbox.size, mass, serial_deposit ? 1 : nthreads, nullptr);
         const double ae = midpoint ? ak + dak / 2 : ak;
         poisson(fbox, delta.data(), cos, ae, nthreads, in, out);
         double adot = cos.adot(ae);
@@ -529,6 +539,7 @@ void* orc_sim_create(int R, int Nm, int Nf, double L, double H0, double Om, doub
 }
 void orc_sim_destroy(void* s) { delete static_cast<Sim*>(s); }
 void orc_sim_set_midpoint(void* s, int on) { static_cast<Sim*>(s)->midpoint = on; }
+void orc_sim_set_serial_deposit(void* s, int on) { static_cast<Sim*>(s)->serial_deposit = on; }
 void orc_sim_init(void* sp, const double* phi_m, double a0, double da) {
     Sim* s = static_cast<Sim*>(sp);
     s->init(phi_m, a0, a0 + da / 2);
@@ -573,18 +584,33 @@ void orc_sim_mesh(void* sp, double* out) {
 }
 
 // ---- initial conditions (src/ic/ic.cc)
+// Planted mistakes (ORC_PLANT_*): compiled only into copies of the oracle
+// that tests/test_oracle.py builds in a scratch folder to show that the
+// closed-form checks of these functions fail on a plausible error.  The
+// oracle itself is built without them.
+//
+// --nyquist-cutoff (no reference counterpart; `conan ic` of this build): the
+// initial-condition filters are zeroed for |k| above the Nyquist wave number
+// pi N / L (pi in grid units), i.e. outside the sphere inscribed in the cube
+// of modes.  Without it, the reference's filters over every mode.
+
 // _generate_random_field<R>, scale-free branch (ic.cc:37-83).
 void orc_ic_scale_free(int R, int mbits, double L, double slope, int smooth, double sigma_scale,
-                       double norm, const double* noise, double* dens, int nthreads) {
+                       double norm, int nyq_cut, const double* noise, double* dens, int nthreads) {
     size_t N = size_t(1) << mbits;
     Box b(R, (int)N, L);
     const size_t size = b.size;
     std::vector<cplx> in(size), out(size);
     for (size_t i = 0; i < size; ++i) in[i] = cplx(noise[i], 0.0);
+#ifdef ORC_PLANT_SMOOTH_T
+    const double t = sigma_scale * L / N;
+#else
     const double t = sigma_scale * N / L;  // Fourier::scale<R>(sigma * N/L)
+#endif
     auto filt = [&](size_t i, bool withS) {
         double K[4];
         kvec(b, i, (double)N, K);  // kspace<R>(N, N): grid units
+        if (nyq_cut && ksqr(K, R) > M_PI * M_PI) return 0.0;
         double P = sqrt(pow(sqrt(ksqr(K, R)), slope));
         if (!withS) return P;
         double v = 1.0;
@@ -605,6 +631,54 @@ void orc_ic_scale_free(int R, int mbits, double L, double slope, int smooth, dou
     in[0] = 0;
     fft_nd(b, in.data(), out.data(), +1, nthreads);
     const double s = size * sqrt(var) / norm;
+    for (size_t i = 0; i < size; ++i) dens[i] = out[i].real() / s;
+}
+
+// CDM<R> (ic.cc:14-35): sqrt(P(k)), P = A k^ns T0(q)^2 with the zero-baryon
+// transfer function T0 = L0 / (L0 + C0 q^2), L0 = ln(2e + 1.8 q),
+// C0 = 14.2 + 731 / (1 + 62.5 q), q = k Theta^2 / (Omega0 h),
+// Theta = 2.7255 / 2.7, A = 1122670; k physical.
+double cdm_amplitude(double k, double ns, double Omega0, double h) {
+    const double e = exp(1), Theta_CMB = 2.7255 / 2.7, A = 1122670;
+    const double q = k * pow(Theta_CMB, 2) / (Omega0 * h), L0 = log(2 * e + 1.8 * q),
+                 C0 = 14.2 + 731.0 / (1 + 62.5 * q), T0 = L0 / (L0 + C0 * pow(q, 2));
+    return sqrt(A * pow(k, ns) * pow(T0, 2));
+}
+extern "C" double orc_cdm_amplitude(double k, double ns, double Omega0, double h) {
+    return cdm_amplitude(k, ns, Omega0, h);
+}
+
+// _generate_random_field<R>, CDM branch (ic.cc:84-97): the noise filtered by
+// sqrt(P_CDM(|k|)) with physical k (kspace<R>(N, L)), the k = 0 mode zeroed,
+// divided by size res^(R/2).
+void orc_ic_cdm(int R, int mbits, double L, double ns, double Omega0, double H0, int nyq_cut, const double* noise,
+                double* dens, int nthreads) {
+    size_t N = size_t(1) << mbits;
+    Box b(R, (int)N, L);
+    const size_t size = b.size;
+    const double res = L / N;
+    std::vector<cplx> in(size), out(size);
+    for (size_t i = 0; i < size; ++i) in[i] = cplx(noise[i], 0.0);
+    fft_nd(b, in.data(), out.data(), -1, nthreads);
+    const double kn = M_PI * N / L;
+    for (size_t i = 0; i < size; ++i) {
+        double K[4];
+#ifdef ORC_PLANT_CDM_GRIDK
+        kvec(b, i, (double)N, K);
+#else
+        kvec(b, i, L, K);  // kspace<R>(N, L)
+#endif
+        const double k2 = ksqr(K, R);
+        const double f = (nyq_cut && k2 > kn * kn) ? 0.0 : cdm_amplitude(sqrt(k2), ns, Omega0, H0 / 100.0);
+        in[i] = mul_real(out[i], f);
+    }
+    in[0] = 0;
+    fft_nd(b, in.data(), out.data(), +1, nthreads);
+#ifdef ORC_PLANT_CDM_NORM
+    const double s = size;
+#else
+    const double s = size * pow(res, 0.5 * R);  // real_part(size * pow(res, 0.5*R)), ic.cc:92
+#endif
     for (size_t i = 0; i < size; ++i) dens[i] = out[i].real() / s;
 }
 
@@ -650,12 +724,11 @@ void orc_compute_displacement(int R, int mbits, double L, const double* potentia
 }
 
 // ---- conan cic (src/cic/cic.hh:49-81)
-void orc_cic_tool(int R, int Nm, int Nf, double L, int s, const double* X, double* rho) {
-    Box bm(R, Nm, L), bf(R, Nf, L);
-    if (s == 1) {
-        md_cic(bf, rho, X, bm.size, 1.0, 1, nullptr);
-        return;
-    }
+// The sub-sampled Lagrangian grid of cic_run<R> with s > 1 (cic.hh:63-77): the
+// displacements D = X - G on the mass grid, interpolated (LinearInterpolation,
+// numeric.hh:17-44) at the points y of the (s Nm)^R grid: Y = y + D(y), AoS.
+void orc_cic_subsample(int R, int Nm, double L, int s, const double* X, double* Y) {
+    Box bm(R, Nm, L);
     std::vector<double> D(bm.size * R);
     int q[4];
     for (size_t i = 0; i < bm.size; ++i) {
@@ -664,7 +737,6 @@ void orc_cic_tool(int R, int Nm, int Nf, double L, int s, const double* X, doubl
     }
     const int Ns = Nm * s;
     Box bs(R, Ns, L);
-    std::vector<double> Y(bs.size * R);
     for (size_t i = 0; i < bs.size; ++i) {
         bs.decode(i, q);
         double y[4], x[4], A[2][4], v[4] = {0, 0, 0, 0};
@@ -673,8 +745,13 @@ void orc_cic_tool(int R, int Nm, int Nf, double L, int s, const double* X, doubl
         for (int k = 0; k < R; ++k) x[k] = y[k] / bm.res;
         for (int k = 0; k < R; ++k) {
             origin[k] = (int)(x[k]);
+#ifdef ORC_PLANT_CIC_SWAP
+            A[0][k] = x[k] - origin[k];
+            A[1][k] = 1 - A[0][k];
+#else
             A[1][k] = x[k] - origin[k];
             A[0][k] = 1 - A[1][k];
+#endif
         }
         for (int j = 0; j < (1 << R); ++j) {
             double z = 1;
@@ -688,6 +765,17 @@ void orc_cic_tool(int R, int Nm, int Nf, double L, int s, const double* X, doubl
         }
         for (int k = 0; k < R; ++k) Y[i * R + k] = y[k] + v[k];
     }
+}
+
+void orc_cic_tool(int R, int Nm, int Nf, double L, int s, const double* X, double* rho) {
+    Box bm(R, Nm, L), bf(R, Nf, L);
+    if (s == 1) {
+        md_cic(bf, rho, X, bm.size, 1.0, 1, nullptr);
+        return;
+    }
+    Box bs(R, Nm * s, L);
+    std::vector<double> Y(bs.size * R);
+    orc_cic_subsample(R, Nm, L, s, X, Y.data());
     double a = pow(double(s * Nm) / double(Nf), -float(R));
     md_cic(bf, rho, Y.data(), bs.size, a, 1, nullptr);
 }

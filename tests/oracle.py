@@ -23,7 +23,20 @@ def load():
         return _lib
     if not os.path.exists(LIB):
         subprocess.run(["make", "-C", ODIR], check=True, stdout=subprocess.DEVNULL)
-    lib = C.CDLL(LIB)
+    _lib = _bind(C.CDLL(LIB))
+    return _lib
+
+
+def build_planted(define: str, outdir: str):
+    """A copy of the oracle compiled with the planted mistake ORC_PLANT_<define>
+    (see pm_oracle.cpp), for tests that show a pin fails on it."""
+    out = os.path.join(outdir, f"liboracle_{define.lower()}.so")
+    subprocess.run(["g++", "-std=c++14", "-O2", "-ffp-contract=off", "-frounding-math", "-fopenmp", "-fPIC",
+                    "-shared", f"-DORC_PLANT_{define}", "-o", out, os.path.join(ODIR, "pm_oracle.cpp")], check=True)
+    return _bind(C.CDLL(out))
+
+
+def _bind(lib):
     V = C.c_void_p
     i, d, sz = C.c_int, C.c_double, C.c_size_t
     sig = {
@@ -42,6 +55,7 @@ def load():
         "orc_sim_create": (V, [i, i, i, d, d, d, d, i, i]),
         "orc_sim_destroy": (None, [V]),
         "orc_sim_set_midpoint": (None, [V, i]),
+        "orc_sim_set_serial_deposit": (None, [V, i]),
         "orc_sim_init": (None, [V, _dp, d, d]),
         "orc_sim_steps": (d, [V, i]),
         "orc_sim_drift": (None, [V, d, d]),
@@ -49,7 +63,10 @@ def load():
         "orc_sim_get": (None, [V, _dp, _dp, _dp]),
         "orc_sim_set": (None, [V, _dp, _dp]),
         "orc_sim_mesh": (None, [V, _dp]),
-        "orc_ic_scale_free": (None, [i, i, d, d, i, d, d, _dp, _dp, i]),
+        "orc_ic_scale_free": (None, [i, i, d, d, i, d, d, i, _dp, _dp, i]),
+        "orc_ic_cdm": (None, [i, i, d, d, d, d, i, _dp, _dp, i]),
+        "orc_cdm_amplitude": (d, [d, d, d, d]),
+        "orc_cic_subsample": (None, [i, i, d, i, _dp, _dp]),
         "orc_compute_potential": (None, [i, i, d, _dp, i]),
         "orc_compute_displacement": (None, [i, i, d, _dp, _dp, i]),
         "orc_cic_tool": (None, [i, i, i, d, i, _dp, _dp]),
@@ -60,7 +77,6 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _lib = lib
     return lib
 
 
@@ -150,12 +166,15 @@ def adot(H0, Om, OL, a):
 class Sim:
     """Gravity<R> + LeapFrog on the CPU (src/run/nbody.hh, leapfrog.hh)."""
 
-    def __init__(self, dim, Nm, Nf, L, H0=68.5, Om=1.0, OL=0.0, nthreads=1, wrap=0, midpoint=False):
+    def __init__(self, dim, Nm, Nf, L, H0=68.5, Om=1.0, OL=0.0, nthreads=1, wrap=0, midpoint=False,
+                 serial_deposit=False):
         self.lib = load()
         self.dim, self.Nm, self.Nf = dim, Nm, Nf
         self.h = self.lib.orc_sim_create(dim, Nm, Nf, L, H0, Om, OL, nthreads, wrap)
         if midpoint:
             self.lib.orc_sim_set_midpoint(self.h, 1)
+        if serial_deposit:
+            self.lib.orc_sim_set_serial_deposit(self.h, 1)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -191,11 +210,29 @@ class Sim:
         return out
 
 
-def ic_scale_free(dim, mbits, L, ns, noise, smooth=False, scale=8.0, sigma=0.83, nthreads=1):
+def ic_scale_free(dim, mbits, L, ns, noise, smooth=False, scale=8.0, sigma=0.83, nthreads=1, nyquist_cutoff=False,
+                  lib=None):
     out = np.empty((1 << mbits) ** dim)
-    load().orc_ic_scale_free(dim, mbits, L, ns, 1 if smooth else 0, scale, sigma, _p(_c(noise).ravel()), _p(out),
-                             nthreads)
+    (lib or load()).orc_ic_scale_free(dim, mbits, L, ns, 1 if smooth else 0, scale, sigma, 1 if nyquist_cutoff else 0,
+                                      _p(_c(noise).ravel()), _p(out), nthreads)
     return out
+
+
+def ic_cdm(dim, mbits, L, ns, Omega0, H0, noise, nyquist_cutoff=False, nthreads=1, lib=None):
+    out = np.empty((1 << mbits) ** dim)
+    (lib or load()).orc_ic_cdm(dim, mbits, L, ns, Omega0, H0, 1 if nyquist_cutoff else 0, _p(_c(noise).ravel()),
+                               _p(out), nthreads)
+    return out
+
+
+def cdm_amplitude(k, ns, Omega0, h):
+    return load().orc_cdm_amplitude(k, ns, Omega0, h)
+
+
+def cic_subsample(dim, Nm, L, s, X, lib=None):
+    Y = np.empty(((Nm * s) ** dim, dim))
+    (lib or load()).orc_cic_subsample(dim, Nm, L, s, _p(_c(X)), _p(Y))
+    return Y
 
 
 def compute_potential(dim, mbits, L, dens, nthreads=1):

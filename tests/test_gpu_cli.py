@@ -51,6 +51,21 @@ def test_ic_file_matches_oracle(sim_dir, orc):
     assert np.max(np.abs(rec["potential"][1] - phi)) < 1e-12 * np.max(np.abs(phi))
 
 
+@pytest.mark.parametrize("cut", [False, True])
+def test_ic_cdm_file_matches_oracle(sim_dir, orc, cut):
+    """conan ic without --scale-free (the CDM branch, src/ic/ic.cc:84-97),
+    with and without --nyquist-cutoff."""
+    from conan_amd import conanfile
+
+    d, noise = sim_dir
+    run(["ic", "-i", "c", "-d", "3", "-b", "4", "-L", "100", "--noise", "noise.conan"]
+        + (["--nyquist-cutoff"] if cut else []), d)
+    h, hist, rec = conanfile.read(str(d / "c.density.init.conan"))
+    assert h["scale-free"] == "false" and h["nyquist-cutoff"] == ("1" if cut else "false")
+    want = orc.ic_cdm(3, 4, 100.0, 0.96, 0.32, 69.0, noise, nyquist_cutoff=cut)
+    assert np.max(np.abs(rec["density"][1] - want)) < 1e-12 * np.max(np.abs(want))
+
+
 def test_run_snapshots_match_oracle(sim_dir, orc):
     from conan_amd import conanfile
 

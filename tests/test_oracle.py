@@ -506,3 +506,143 @@ def test_leapfrog_convergence_order(orc, midpoint, order):
     s = [_plane_wave_amplitude(orc, da, midpoint) for da in (0.004, 0.002, 0.001)]
     p = math.log2(abs(s[0] - s[1]) / abs(s[1] - s[2]))
     assert abs(p - order) < 0.25
+
+
+# ---------------------------------------------------------------- IC filters, cic -s, CDM
+# Each closed-form check below is also run against copies of the oracle with
+# a plausible mistake planted (ORC_PLANT_*, pm_oracle.cpp) and must fail there
+# (test_pins_fail_on_planted_mistakes).
+
+def _two_mode_noise(dim, N, n1, n2, A1, A2):
+    x = np.stack(np.meshgrid(*([np.arange(N)] * dim), indexing="ij"), -1).reshape(-1, dim)[:, ::-1]
+    ph1 = 2 * np.pi * (x @ np.array(n1[:dim])) / N
+    ph2 = 2 * np.pi * (x @ np.array(n2[:dim])) / N + 0.3
+    return A1 * np.cos(ph1) + A2 * np.cos(ph2), np.cos(ph1), np.cos(ph2 )
+
+
+def _smoothing_error(orc, lib=None):
+    """_generate_random_field's scale-free branch (src/ic/ic.cc:58-80) on a
+    two-mode noise: with grid-unit wave vectors K = 2 pi n / N, the filter
+    f(K) = |K|^(ns/2) S(K), S = prod_i exp(t (cos K_i - 1)), t = scale N / L
+    (src/base/fourier.hh:79-90), and the normalisation by the realised
+    variance of the smoothed field, the output is, for smooth = true,
+    sigma (A1 f1 c1 + A2 f2 c2) / sqrt((A1^2 f1^2 + A2^2 f2^2) / 2), and for
+    smooth = false the same with the unsmoothed |K|^(ns/2) in the numerator.
+    Returns the largest relative error over both modes, 2-D and 3-D."""
+    err = 0.0
+    for dim, mbits, n1, n2 in ((2, 5, (1, 2, 0), (3, 0, 0)), (3, 4, (1, 0, 2), (2, 3, 1))):
+        N, L, ns, scale, sigma, A1, A2 = 1 << mbits, 100.0, -1.3, 8.0, 0.7, 1.7, -0.6
+        w, c1, c2 = _two_mode_noise(dim, N, n1, n2, A1, A2)
+        t = scale * N / L
+        K1, K2 = (2 * np.pi * np.array(n[:dim]) / N for n in (n1, n2))
+        g1, g2 = (np.linalg.norm(K) ** (ns / 2) for K in (K1, K2))
+        f1, f2 = (g * np.prod(np.exp(t * (np.cos(K) - 1))) for g, K in ((g1, K1), (g2, K2)))
+        var = (A1 ** 2 * f1 ** 2 + A2 ** 2 * f2 ** 2) / 2
+        for smooth, (h1, h2) in ((True, (f1, f2)), (False, (g1, g2))):
+            want = sigma * (A1 * h1 * c1 + A2 * h2 * c2) / np.sqrt(var)
+            got = orc.ic_scale_free(dim, mbits, L, ns, w, smooth=smooth, scale=scale, sigma=sigma, lib=lib)
+            err = max(err, float(np.max(np.abs(got - want)) / np.max(np.abs(want))))
+    return err
+
+
+def test_ic_smoothing_filter_two_modes(orc):
+    assert _smoothing_error(orc) < 1e-11
+
+
+def _subsample_error(orc, lib=None):
+    """cic_run's sub-sampling (src/cic/cic.hh:63-77): Y = y + D(y), D the
+    trilinear interpolation (numeric.hh:17-44) of X - G on the mass grid.
+    (a) D linear in q: exact wherever the interpolation does not wrap;
+    (b) D nonzero at one mass node q0 only: Y - y = D0 prod_k max(0, 1 - |y_k /
+    res - q0_k|), e.g. 0.375 D0 at y = (q0 + (0.5, 0.25)) res by hand.
+    Returns the largest absolute error."""
+    L, Nm, s = 100.0, 8, 4
+    res = L / Nm
+    q = np.stack(np.meshgrid(*([np.arange(Nm)] * 2), indexing="ij"), -1).reshape(-1, 2)[:, ::-1]
+    G = q * res
+    Ns = Nm * s
+    y = np.stack(np.meshgrid(*([np.arange(Ns)] * 2), indexing="ij"), -1).reshape(-1, 2)[:, ::-1] * (L / Ns)
+    alpha, B = np.array([0.3, -0.2]), np.array([[0.01, -0.02], [0.015, 0.005]])
+    Y = orc.cic_subsample(2, Nm, L, s, G + alpha + (q * res) @ B.T, lib=lib)
+    inner = np.all(y / res < Nm - 1, axis=1)
+    err = float(np.max(np.abs((Y - y - alpha - y @ B.T)[inner])))
+    q0, D0 = np.array([1, 2]), np.array([0.4, -0.9])
+    X = G.copy()
+    X[q0[0] + Nm * q0[1]] += D0
+    Y = orc.cic_subsample(2, Nm, L, s, X, lib=lib)
+    w = np.prod(np.clip(1 - np.abs(y / res - q0), 0, None), axis=1)
+    err = max(err, float(np.max(np.abs(Y - y - w[:, None] * D0))))
+    pt = np.flatnonzero(np.all(np.isclose(y, (q0 + [0.5, 0.25]) * res), axis=1))[0]
+    err = max(err, float(np.max(np.abs(Y[pt] - y[pt] - 0.375 * D0))))
+    return err
+
+
+def test_cic_subsample_interpolation(orc):
+    assert _subsample_error(orc) < 1e-12
+
+
+# T0(q) of the zero-baryon transfer function (Eisenstein & Hu 1998, ApJ 496,
+# 605, eqs. 28-29, the form CDM<R> uses: L0 = ln(2e + 1.8 q), C0 = 14.2 +
+# 731 / (1 + 62.5 q), T0 = L0 / (L0 + C0 q^2)), evaluated once in 40-digit
+# decimal arithmetic.
+T0_KNOWN = {0.01: 0.9733744025219708362, 0.1: 0.6000425918333350036, 1.0: 0.07147266257664810312,
+            10.0: 0.002048341518435641791}
+
+
+def test_cdm_transfer_function_known_answers(orc):
+    """sqrt(P(k)) of CDM<R> (src/ic/ic.cc:14-35) with P = A k^ns T0(q)^2,
+    q = k Theta^2 / (Omega0 h), Theta = 2.7255 / 2.7, A = 1122670."""
+    ns, Om, h = 0.96, 0.32, 0.69
+    theta2 = (2.7255 / 2.7) ** 2
+    for q, T0 in T0_KNOWN.items():
+        k = q * Om * h / theta2
+        got = orc.cdm_amplitude(k, ns, Om, h)
+        assert abs(got / np.sqrt(1122670 * k ** ns * T0 ** 2) - 1) < 1e-13
+
+
+def _cdm_mode_error(orc, lib=None):
+    """The CDM branch of _generate_random_field (ic.cc:84-97) on a single
+    mode n: physical k = 2 pi |n| / L (kspace<R>(N, L)), output
+    sqrt(P(k)) cos(2 pi n.x / N) / res^(R/2)."""
+    err = 0.0
+    for dim, mbits, n in ((2, 5, (2, 1, 0)), (3, 4, (1, 1, 3))):
+        N, L, ns, Om, H0 = 1 << mbits, 200.0, 0.96, 0.32, 69.0
+        w, c, _ = _two_mode_noise(dim, N, n, n, 1.0, 0.0)
+        k = 2 * np.pi * np.linalg.norm(n[:dim]) / L
+        want = orc.cdm_amplitude(k, ns, Om, H0 / 100) * c / (L / N) ** (dim / 2)
+        got = orc.ic_cdm(dim, mbits, L, ns, Om, H0, w, lib=lib)
+        err = max(err, float(np.max(np.abs(got - want)) / np.max(np.abs(want))))
+    return err
+
+
+def test_ic_cdm_single_mode(orc):
+    assert _cdm_mode_error(orc) < 1e-11
+
+
+def test_nyquist_cutoff_removes_only_modes_beyond_nyquist(orc):
+    """--nyquist-cutoff: a mode with |k| above pi N / L (the corner mode
+    n = (N/2, N/2): |K| = pi sqrt 2 in grid units) is removed, one below it
+    (n = (N/2 - 1, 1)) passes, in both initial-condition branches."""
+    dim, mbits, L = 2, 4, 100.0
+    N = 1 << mbits
+    w, c_keep, c_cut = _two_mode_noise(dim, N, (N // 2 - 1, 1, 0), (N // 2, N // 2, 0), 1.0, 1.0)
+    c_cut = np.cos(2 * np.pi * (np.stack(np.meshgrid(np.arange(N), np.arange(N), indexing="ij"), -1)
+                                .reshape(-1, 2)[:, ::-1] @ np.array([N // 2, N // 2])) / N)
+    for f in (lambda cut: orc.ic_scale_free(dim, mbits, L, -1.0, w, smooth=True, nyquist_cutoff=cut),
+              lambda cut: orc.ic_cdm(dim, mbits, L, 0.96, 0.32, 69.0, w, nyquist_cutoff=cut)):
+        plain, cut = f(False), f(True)
+        # the corner mode's share of the field, and the kept mode's
+        assert abs(plain @ c_cut) > 1e-3 * np.abs(plain).sum()
+        assert abs(cut @ c_cut) < 1e-12 * np.abs(cut).sum()
+        assert abs(cut @ c_keep) > 1e-3 * np.abs(cut).sum()
+
+
+@pytest.mark.parametrize("plant,check", [("SMOOTH_T", _smoothing_error), ("CIC_SWAP", _subsample_error),
+                                         ("CDM_GRIDK", _cdm_mode_error), ("CDM_NORM", _cdm_mode_error)])
+def test_pins_fail_on_planted_mistakes(orc, tmp_path, plant, check):
+    """Each pin above, run on a copy of the oracle with one plausible mistake
+    planted (t = scale L / N; the interpolation weights A0 and A1 swapped; the
+    CDM filter in grid-unit k; the CDM normalisation without res^(R/2)),
+    reports an error far above its tolerance."""
+    assert check(orc) < 1e-11
+    assert check(orc, orc.build_planted(plant, str(tmp_path))) > 1e-3
