@@ -56,7 +56,8 @@ class pm_ic_params(C.Structure):
 
 
 class pm_diag(C.Structure):
-    _fields_ = [("mass", C.c_double), ("momentum", C.c_double * 3), ("p2", C.c_double)]
+    _fields_ = [("mass", C.c_double), ("momentum", C.c_double * 3), ("p2", C.c_double), ("kinetic", C.c_double),
+                ("potential", C.c_double), ("a_pos", C.c_double), ("a_mom", C.c_double)]
 
 
 class pm_diff_params(C.Structure):

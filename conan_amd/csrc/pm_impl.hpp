@@ -776,7 +776,20 @@ struct Impl : ImplBase {
         need_state();
         std::memset(d, 0, sizeof(*d));
         double h[8] = {0};
-        if (have_deposit) sum_partials(row_partial.p, npartials, 1.0, red.p, 4096, scal.p + 2, st);
+        // the deposit of the current positions, its mass, and the potential
+        // energy from its spectrum (the mesh is consumed)
+        do_deposit((Mesh)mass);
+        sum_partials(row_partial.p, npartials, 1.0, red.p, 4096, scal.p + 2, st);
+        const double* mean_ptr = nullptr;
+        if (FX) {
+            sum_partials(row_partial.p, npartials, 1.0 / (double)M, red.p, 4096, scal.p, st);
+            mean_ptr = scal.p;
+        }
+        fft.forward(mesh.p, spec.p, nyq.p, (Mesh)1.0, mean_ptr, st);
+        have_deposit = false;
+        const long ne = spectrum_energy_partials<Mesh>(spec.p, nyq.p, (long)fft.spec_elems, (long)fft.nyq_elems, R,
+                                                      lgNf, 2 * M_PI / cfg.L, row_partial.p, nrows, st);
+        sum_partials(row_partial.p, ne, 1.0, red.p, 4096, scal.p + 7, st);
         // The sums run in particle order: the FX storage order within a
         // brick group depends on the order in which the re-sort's atomics
         // placed the particles that changed group, and a sum in that order
@@ -790,11 +803,15 @@ struct Impl : ImplBase {
         }
         for (int k = 0; k < R; ++k) reduce_sum(Pd + (size_t)k * Np, Np, 0, 1.0, red.p, 4096, scal.p + 3 + k, st);
         reduce_sum(Pd, (long)R * Np, 1, 1.0, red.p, 4096, scal.p + 6, st);
-        CPM_HIP(hipMemcpyAsync(h, scal.p + 2, sizeof(double) * 5, hipMemcpyDeviceToHost, st));
+        CPM_HIP(hipMemcpyAsync(h, scal.p + 2, sizeof(double) * 6, hipMemcpyDeviceToHost, st));
         CPM_HIP(hipStreamSynchronize(st));
-        d->mass = have_deposit ? h[0] : 0.0;
+        d->mass = h[0];
         for (int k = 0; k < R; ++k) d->momentum[k] = h[1 + k];
         d->p2 = h[4];
+        d->a_pos = last_apos;
+        d->a_mom = last_avel;
+        d->kinetic = mass * d->p2 / (2 * last_avel * last_avel);
+        d->potential = -cos.G / (2 * last_apos * (double)M) * h[5];
     }
 };
 

@@ -226,6 +226,56 @@ void power_spectrum(const cx<T>* spec, const cx<T>* nyq, long nmain, long nnyq, 
     CPM_LAUNCH_CHECK();
 }
 
+// ------------------------------------------------------------ potential energy
+// S = sum over modes k != 0 of w |z_k|^2 / |k|^2 (w = 2 for the interior modes
+// 0 < kx < N/2 that stand for k and -k, 1 on the kx = 0 and N/2 planes), k
+// physical (kf per index step), in FP64: block b sums the modes
+// [b chunk, (b + 1) chunk) with a fixed per-thread stride and a fixed tree.
+template <typename T>
+__global__ __launch_bounds__(PK_NT) void k_energy_partial(const cx<T>* __restrict__ spec, const cx<T>* __restrict__ nyq,
+                                                          long nmain, long nnyq, int dim, int lgN, double kf, long chunk,
+                                                          double* __restrict__ part) {
+    __shared__ double red[PK_NT];
+    const int N = 1 << lgN;
+    const long n = nmain + nnyq;
+    const long e0 = (long)blockIdx.x * chunk, e1 = e0 + chunk < n ? e0 + chunk : n;
+    double acc = 0;
+    for (long e = e0 + threadIdx.x; e < e1; e += PK_NT) {
+        const ModeIdx m = mode_of(e, nmain, dim, lgN);
+        if (m.ix == 0 && m.iy == 0 && m.iz == 0) continue;
+        const double kx = kcomp(m.ix, N, kf), ky = kcomp(m.iy, N, kf), kz = dim == 3 ? kcomp(m.iz, N, kf) : 0.0;
+        const cx<T> z = e < nmain ? spec[e] : nyq[e - nmain];
+        const double zr = (double)z.re, zi = (double)z.im;
+        const double w = (m.ix == 0 || m.ix == N / 2) ? 1.0 : 2.0;
+        acc += w * (zr * zr + zi * zi) / (kx * kx + ky * ky + kz * kz);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = PK_NT / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+template <typename T>
+long spectrum_energy_partials(const cx<T>* spec, const cx<T>* nyq, long nmain, long nnyq, int dim, int lgN, double kf,
+                              double* part, long max_partials, hipStream_t st) {
+    const long n = nmain + nnyq;
+    long nb = (n + PK_NT - 1) / PK_NT;
+    if (nb > max_partials) nb = max_partials;
+    const long chunk = (n + nb - 1) / nb;
+    nb = (n + chunk - 1) / chunk;
+    hipLaunchKernelGGL((k_energy_partial<T>), dim3((unsigned)nb), dim3(PK_NT), 0, st, spec, nyq, nmain, nnyq, dim,
+                       lgN, kf, chunk, part);
+    CPM_LAUNCH_CHECK();
+    return nb;
+}
+template long spectrum_energy_partials<float>(const cx<float>*, const cx<float>*, long, long, int, int, double, double*,
+                                              long, hipStream_t);
+template long spectrum_energy_partials<double>(const cx<double>*, const cx<double>*, long, long, int, int, double,
+                                               double*, long, hipStream_t);
+
 // ------------------------------------------------------------ P(k) without allocation
 // pm_power_spectrum_noalloc keeps everything the estimator needs beyond the
 // handle's spectrum in one file-scope device array of fixed size: the bin

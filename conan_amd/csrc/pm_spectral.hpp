@@ -23,6 +23,11 @@ void apply_filter(cx<double>* spec, cx<double>* nyq, long nmain, long nnyq, cons
                   hipStream_t st);
 
 long pk_partial_elems(long nmain, long nnyq, int nbins);
+// Partial sums (at most max_partials, returned: how many) of sum_k w |z_k|^2 /
+// |k|^2 over the half spectrum, k != 0, physical k (kf per index step).
+template <typename T>
+long spectrum_energy_partials(const cx<T>* spec, const cx<T>* nyq, long nmain, long nnyq, int dim, int lgN, double kf,
+                              double* part, long max_partials, hipStream_t st);
 // The estimator with no device allocation: its scratch (edges, partials,
 // results) is a fixed file-scope device array, so at most PKS_MAXBINS bins.
 // edges_host: nbins + 1 host doubles; out_host: 3 nbins host doubles (mean

@@ -149,7 +149,8 @@ class Gravity:
     def diagnostics(self):
         d = _lib.pm_diag()
         self._ck(self._lib.pm_diagnostics(self._h, _lib.C.byref(d)))
-        return {"mass": d.mass, "momentum": list(d.momentum)[: self.dim], "p2": d.p2}
+        return {"mass": d.mass, "momentum": list(d.momentum)[: self.dim], "p2": d.p2, "kinetic": d.kinetic,
+                "potential": d.potential, "a_pos": d.a_pos, "a_mom": d.a_mom}
 
     def profile(self, enable: int = -1):
         ms = np.zeros(_lib.PM_NPHASES)

@@ -160,11 +160,23 @@ int pm_power_spectrum(pm_handle* h, int nbins, const double* edges, double* kmea
 int pm_power_spectrum_noalloc(pm_handle* h, int nbins, const double* edges, double* kmean, double* pk,
                               double* count);
 
-/* FP64 reductions of the state and the last deposit. */
+/* FP64 reductions of the state (no counterpart in the reference), for
+ * conservation checks.  m = (Nf/Nm)^dim is the particle mass in units of the
+ * mean cell mass, a_pos / a_mom the scale factors of the positions and
+ * momenta (pm_get_times).  The energies are those of the Layzer-Irvine
+ * equation d[a (K + W)]/da = -K, with P = a^2 dx/dt and phi the peculiar
+ * potential the kick uses (nabla^2 phi = G delta / a, src/run/nbody.hh:
+ * 99-110): K = m sum_i |P_i|^2 / (2 a_mom^2), W = 1/2 sum_cells rho phi at
+ * a_pos, evaluated in Fourier space as -(G / (2 a_pos M)) sum_k |rho_k|^2 /
+ * |k|^2.  The call deposits the current positions (one deposit and one
+ * forward FFT). */
 typedef struct pm_diag {
-    double mass;          /* sum of the last deposit (= Nm^dim * (Nf/Nm)^dim) */
+    double mass;          /* sum of the deposit of the current positions (= Nm^dim * m) */
     double momentum[3];   /* sum_i P_i */
     double p2;            /* sum_i |P_i|^2 */
+    double kinetic;       /* K */
+    double potential;     /* W */
+    double a_pos, a_mom;
 } pm_diag;
 int pm_diagnostics(pm_handle* h, pm_diag* d);
 

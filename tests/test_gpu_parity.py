@@ -455,7 +455,7 @@ def test_runs_are_bit_reproducible(pm, orc, prec):
             X, P = g.get_state()
             rho = g.deposit()
             d = g.diagnostics()
-            outs.append((X, P, rho, np.array(d["momentum"] + [d["p2"], d["mass"]])))
+            outs.append((X, P, rho, np.array(d["momentum"] + [d["p2"], d["mass"], d["potential"]])))
     for a, b in zip(outs[0], outs[1]):
         assert np.array_equal(a, b)
 
