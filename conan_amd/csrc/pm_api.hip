@@ -118,9 +118,17 @@ int pm_init_zeldovich(pm_handle* h, const double* phi, double a_pos, double a_mo
     PM_CATCH(h)
 }
 
+// The scale factor must be positive and finite (adot(a) and 1/a enter every
+// factor), the step finite.
+static void check_times(double a, double da) {
+    if (!std::isfinite(a) || !std::isfinite(da)) throw cpm::Error(PM_ENONFINITE, "non-finite scale factor or step");
+    if (!(a > 0)) throw cpm::Error(PM_EINVAL, "the scale factor must be positive");
+}
+
 int pm_drift(pm_handle* h, double a, double da) {
     PM_NEED(h)
     PM_TRY
+    check_times(a, da);
     h->impl->drift(a, da);
     PM_CATCH(h)
 }
@@ -128,6 +136,7 @@ int pm_drift(pm_handle* h, double a, double da) {
 int pm_kick(pm_handle* h, double a, double da) {
     PM_NEED(h)
     PM_TRY
+    check_times(a, da);
     h->impl->kick(a, da);
     PM_CATCH(h)
 }
@@ -135,6 +144,8 @@ int pm_kick(pm_handle* h, double a, double da) {
 int pm_steps(pm_handle* h, double a, double da, int n, double* a_out) {
     PM_NEED(h)
     PM_TRY
+    check_times(a, da);
+    if (n > 0) check_times(a + (n - 1) * da, da);
     double r = h->impl->steps(a, da, n);
     if (a_out) *a_out = r;
     PM_CATCH(h)

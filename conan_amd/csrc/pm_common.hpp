@@ -2,8 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 
@@ -64,6 +66,38 @@ struct Error : std::runtime_error {
     } while (0)
 
 #define CPM_LAUNCH_CHECK() CPM_HIP(hipGetLastError())
+
+// Device allocations of the library go through device_alloc / device_free.
+// The environment variable CONANPM_MAX_DEVICE_BYTES, when set, caps the
+// bytes the library holds at once: a request beyond it fails like a
+// hipMalloc that runs out of memory (PM_EDEVICE), which lets callers and
+// tests exercise that path without exhausting the GPU.
+inline std::atomic<unsigned long long>& device_bytes_held() {
+    static std::atomic<unsigned long long> held{0};
+    return held;
+}
+inline void device_alloc(void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) return;
+    if (const char* cap = std::getenv("CONANPM_MAX_DEVICE_BYTES")) {
+        const unsigned long long lim = std::strtoull(cap, nullptr, 10);
+        if (device_bytes_held().load() + bytes > lim)
+            throw Error(-3, "device allocation of " + std::to_string(bytes) + " bytes exceeds CONANPM_MAX_DEVICE_BYTES = " +
+                                std::to_string(lim) + " (" + std::to_string(device_bytes_held().load()) + " held)");
+    }
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        throw Error(-3, "hipMalloc of " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+    }
+    device_bytes_held() += bytes;
+}
+inline void device_free(void* p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    device_bytes_held() -= bytes;
+}
 
 // Compute units of the current device (cached per device), for grid sizing.
 inline long device_cus() {

@@ -837,13 +837,20 @@ FFTPlan<T>::FFTPlan(int dim_, int N_) : dim(dim_), N(N_), tw(nullptr) {
         long double a = twopi * (long double)m / (long double)N;
         h[m] = mk<T>((T)cosl(a), (T)(-sinl(a)));
     }
-    CPM_HIP(hipMalloc(&tw, sizeof(cx<T>) * N));
-    CPM_HIP(hipMemcpy(tw, h.data(), sizeof(cx<T>) * N, hipMemcpyHostToDevice));
+    void* q = nullptr;
+    device_alloc(&q, sizeof(cx<T>) * N);
+    tw = static_cast<cx<T>*>(q);
+    const hipError_t e = hipMemcpy(tw, h.data(), sizeof(cx<T>) * N, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        device_free(tw, sizeof(cx<T>) * N);
+        tw = nullptr;
+        throw Error(-3, std::string("hipMemcpy of the FFT twiddles: ") + hipGetErrorString(e));
+    }
 }
 
 template <typename T>
 FFTPlan<T>::~FFTPlan() {
-    if (tw) (void)hipFree(tw);
+    device_free(tw, sizeof(cx<T>) * N);
 }
 
 template <typename T>

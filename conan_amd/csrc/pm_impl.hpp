@@ -41,11 +41,13 @@ struct DevBuf {
     explicit DevBuf(size_t n_) { alloc(n_); }
     void alloc(size_t n_) {
         release();
+        void* q = nullptr;
+        device_alloc(&q, sizeof(T) * n_);
+        p = static_cast<T*>(q);
         n = n_;
-        if (n) CPM_HIP(hipMalloc(&p, sizeof(T) * n));
     }
     void release() {
-        if (p) (void)hipFree(p);
+        device_free(p, sizeof(T) * n);
         p = nullptr;
         n = 0;
     }
@@ -226,6 +228,15 @@ struct Impl : ImplBase {
         if (FX) brick_layout(R, lgNf, pa);
         nsort_groups = FX ? (brick_count(pa) << R) : nrows;
         CPM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        try {
+            allocate();
+        } catch (...) {
+            (void)hipStreamDestroy(st);  // the buffers allocated so far go with the members
+            st = nullptr;
+            throw;
+        }
+    }
+    void allocate() {
         X.alloc((size_t)R * Np);
         P.alloc((size_t)R * Np);
         if (FX) {
@@ -810,8 +821,13 @@ struct Impl : ImplBase {
         d->p2 = h[4];
         d->a_pos = last_apos;
         d->a_mom = last_avel;
-        d->kinetic = mass * d->p2 / (2 * last_avel * last_avel);
-        d->potential = -cos.G / (2 * last_apos * (double)M) * h[5];
+        // the energies need the scale factors (none yet after a bare pm_set_state)
+        if (last_avel > 0) d->kinetic = mass * d->p2 / (2 * last_avel * last_avel);
+        if (last_apos > 0) d->potential = -cos.G / (2 * last_apos * (double)M) * h[5];
+        for (double v : {d->mass, d->momentum[0], d->momentum[1], d->momentum[2], d->p2, d->kinetic, d->potential, h[5]})
+            if (!std::isfinite(v))
+                throw Error(PM_ENONFINITE, "the state holds non-finite values: a momentum, position or density sum "
+                                           "overflowed");
     }
 };
 

@@ -224,13 +224,21 @@ RadixSorter::RadixSorter(long n_, int keybits_) : n(n_), keybits(keybits_) {
     nblocks = (int)((n + SORT_TILE - 1) / SORT_TILE);
     if (nblocks < 1) nblocks = 1;
     ncounts = (long)(1 << dbits) * nblocks;
-    CPM_HIP(hipMalloc(&counts, sizeof(uint32_t) * (ncounts + 1)));
-    CPM_HIP(hipMalloc(&scratch, sizeof(uint32_t) * scan_scratch_elems(ncounts)));
+    void* c = nullptr;
+    device_alloc(&c, sizeof(uint32_t) * (ncounts + 1));
+    counts = static_cast<uint32_t*>(c);
+    try {
+        device_alloc(&c, sizeof(uint32_t) * scan_scratch_elems(ncounts));
+    } catch (...) {
+        device_free(counts, sizeof(uint32_t) * (ncounts + 1));
+        throw;
+    }
+    scratch = static_cast<uint32_t*>(c);
 }
 
 RadixSorter::~RadixSorter() {
-    if (counts) (void)hipFree(counts);
-    if (scratch) (void)hipFree(scratch);
+    device_free(counts, sizeof(uint32_t) * (ncounts + 1));
+    device_free(scratch, sizeof(uint32_t) * scan_scratch_elems(ncounts));
 }
 
 int RadixSorter::sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, hipStream_t st) {

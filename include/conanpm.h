@@ -168,8 +168,10 @@ int pm_power_spectrum_noalloc(pm_handle* h, int nbins, const double* edges, doub
  * potential the kick uses (nabla^2 phi = G delta / a, src/run/nbody.hh:
  * 99-110): K = m sum_i |P_i|^2 / (2 a_mom^2), W = 1/2 sum_cells rho phi at
  * a_pos, evaluated in Fourier space as -(G / (2 a_pos M)) sum_k |rho_k|^2 /
- * |k|^2.  The call deposits the current positions (one deposit and one
- * forward FFT). */
+ * |k|^2 (K, W = 0 while the handle has no scale factors: a state set with
+ * pm_set_state before any step).  The call deposits the current positions
+ * (one deposit and one forward FFT).  A sum that overflows returns
+ * PM_ENONFINITE. */
 typedef struct pm_diag {
     double mass;          /* sum of the deposit of the current positions (= Nm^dim * m) */
     double momentum[3];   /* sum_i P_i */
