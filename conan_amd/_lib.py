@@ -18,6 +18,7 @@ PM_FP64 = 0
 PM_FP32 = 1
 PM_FLAG_WRAP_GATHER = 1
 PM_FLAG_MIDPOINT = 2
+PM_FLAG_SERIAL_DEPOSIT = 4
 PM_DIFF_STORE_POTENTIALS = 1
 PM_DIFF_VERIFY_CHECKPOINTS = 2
 PM_NPHASES = 10

@@ -227,6 +227,9 @@ void cmd_run(int argc, char** argv) {
          {0, "", "wrap-gather", "false",
           "fold positions into the box before the force interpolation (the reference truncates the "
           "unwrapped coordinate)."},
+         {0, "", "serial-deposit", "false",
+          "fp64: sum each cell of the deposit in particle order like the reference (bit-identical density) "
+          "instead of in exact fixed point."},
          {0, "", "midpoint", "false",
           "evaluate the drift and kick factors at the middle of each step, a second-order leapfrog (the "
           "reference evaluates them at the start of the step)."},
@@ -266,6 +269,7 @@ void cmd_run(int argc, char** argv) {
 
     pm_config cfg{dim, (int)N, (int)(f * N), L, H0, Om, OL, prec == "fp64" ? PM_FP64 : PM_FP32,
                   (H.get<bool>("wrap-gather") ? PM_FLAG_WRAP_GATHER : 0u) |
+                  (H.get<bool>("serial-deposit") ? PM_FLAG_SERIAL_DEPOSIT : 0u) |
                       (H.get<bool>("midpoint") ? PM_FLAG_MIDPOINT : 0u)};
     pm_handle* h = nullptr;
     pm_ok(pm_create(&cfg, &h));

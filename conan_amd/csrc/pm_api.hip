@@ -79,7 +79,8 @@ static void validate(const pm_config* c) {
         throw cpm::Error(PM_EUNSUP, "Nf must be a power of two in [4, 1024]");
     if (!(c->L > 0) || !std::isfinite(c->L)) throw cpm::Error(PM_EINVAL, "L must be positive");
     if (c->precision != PM_FP64 && c->precision != PM_FP32) throw cpm::Error(PM_EINVAL, "unknown precision");
-    if (c->flags & ~(PM_FLAG_WRAP_GATHER | PM_FLAG_MIDPOINT)) throw cpm::Error(PM_EINVAL, "unknown flags");
+    if (c->flags & ~(PM_FLAG_WRAP_GATHER | PM_FLAG_MIDPOINT | PM_FLAG_SERIAL_DEPOSIT))
+        throw cpm::Error(PM_EINVAL, "unknown flags");
     double np = std::pow((double)c->Nm, c->dim);
     if (np > 2147483647.0) throw cpm::Error(PM_EUNSUP, "too many particles");
     // FP32 deposit: 64-bit accumulators in units of 2^-40 of the mean cell
@@ -453,6 +454,7 @@ int pm_cic(int dim, int Nm, int Nf, double L, int s, const double* X, double* rh
     c.H0 = 1;
     c.Omega_m = 1;
     c.precision = PM_FP64;
+    c.flags = PM_FLAG_SERIAL_DEPOSIT;  // md_cic_T's serial order: bit-identical density
     validate(&c);
     cpm::Impl<cpm::PosF64> im(c);
     if (s == 1) {

@@ -462,7 +462,7 @@ struct Impl : ImplBase {
             npartials = brick_count(pa);
         } else {
             ParticleOps<PP>::deposit(R, row_start.p, skeys, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p,
-                                     nrows, frac.p, xsort.p, inv.p, st);
+                                     nrows, frac.p, xsort.p, inv.p, (cfg.flags & PM_FLAG_SERIAL_DEPOSIT) != 0, st);
             npartials = nrows;
         }
         prof.end(1, st);

@@ -157,7 +157,8 @@ struct ParticleOps {
                      hipStream_t st);
     static void deposit(int R, const uint32_t* row_start, const uint32_t* keys, const uint32_t* pids,
                         const Pos* X, long np, int lgN, const PosArgs& pa, Mesh mass, Mesh* mesh,
-                        double* row_partial, long nrows, Mesh* frac, Pos* Xs, uint32_t* inv, hipStream_t st);
+                        double* row_partial, long nrows, Mesh* frac, Pos* Xs, uint32_t* inv, bool serial_order,
+                        hipStream_t st);
     static void gather_kick(int R, Pos* X, double* P, long np, const Mesh* phi, int lgN, const PosArgs& pa,
                             double ck, double cd, int cd_on, double* Fout, const Reorder<PP>& ro, hipStream_t st);
     // FP64 kick (+ drift) in two passes: forces in the deposit's cell order

@@ -26,14 +26,15 @@ class Gravity:
 
     def __init__(self, dim: int, Nm: int, Nf: int, L: float, H0: float = 68.5,
                  Omega_m: float = 1.0, Omega_L: float = 0.0, precision: str = "fp64",
-                 wrap_gather: bool = False, midpoint: bool = False):
+                 wrap_gather: bool = False, midpoint: bool = False, serial_deposit: bool = False):
         lib = _lib.load()
         self.dim, self.Nm, self.Nf, self.L = dim, Nm, Nf, L
         self.Np = Nm ** dim
         self.M = Nf ** dim
         cfg = _lib.pm_config(dim, Nm, Nf, L, H0, Omega_m, Omega_L,
                              PM_FP64 if precision == "fp64" else PM_FP32,
-                             (PM_FLAG_WRAP_GATHER if wrap_gather else 0) | (PM_FLAG_MIDPOINT if midpoint else 0))
+                             (PM_FLAG_WRAP_GATHER if wrap_gather else 0) | (PM_FLAG_MIDPOINT if midpoint else 0)
+                             | (_lib.PM_FLAG_SERIAL_DEPOSIT if serial_deposit else 0))
         h = _lib.C.c_void_p()
         check(lib.pm_create(_lib.C.byref(cfg), _lib.C.byref(h)))
         self._h = h

@@ -51,6 +51,14 @@ extern "C" {
                                   the reference; for runs that want the accuracy of a proper
                                   leapfrog at the same step count. */
 
+#define PM_FLAG_SERIAL_DEPOSIT 4u /* FP64: sum every cell of the deposit in ascending particle
+                                    order, the order of the reference's serial loop (md_cic,
+                                    src/run/numeric.hh:106-139), so that the density is bit-identical
+                                    to it.  Without it the FP64 deposit adds the same weights in
+                                    128-bit fixed point (2^-96 units): exact, order-independent and
+                                    faster, within an ulp or two of the serial sum.  The FP32 form's
+                                    deposit is fixed point in any case. */
+
 /* Configuration of a run: Gravity<R>(mbox, fbox, cosmos) (src/run/nbody.hh:75-86),
  * Cosmology(H0, Omega_m, Omega_L) (src/run/cosmology.hh:11-14). */
 typedef struct pm_config {

@@ -43,7 +43,7 @@ def _cfg(**kw):
 
 @pytest.mark.parametrize("kw,code", [({"dim": 4}, -2), ({"dim": 1}, -2), ({"Nf": 12}, -2), ({"Nf": 2048}, -2),
                                      ({"L": 0.0}, -1), ({"L": float("nan")}, -1), ({"Nm": 0}, -1),
-                                     ({"precision": 7}, -1), ({"flags": 4}, -1)])
+                                     ({"precision": 7}, -1), ({"flags": 8}, -1)])
 def test_create_rejects_bad_configs(kw, code):
     import ctypes
 

@@ -7,9 +7,10 @@ C1: 2-D, 128^2 particles on a 128^2 mesh, ns = -1 with the Nyquist cutoff,
 C2: 3-D, 128^3 particles on a 128^3 mesh, ns = -2, a = 0.02 -> 0.9996 in 100
     steps of 0.0098; FP64 form against the oracle run on the box's cores.
 
-Same inputs give bit-exact cell keys and deposits (the oracle sums each cell
-in the reference's serial order, as the FP64 form does) and forces equal to
-FFT rounding; the two trajectories, which see different FFT roundings every
+Same inputs give bit-exact cell keys and deposits (with
+PM_FLAG_SERIAL_DEPOSIT, each cell summed in the reference's serial order, as
+the oracle does; the default fixed-point deposit within 1e-13) and forces
+equal to FFT rounding; the two trajectories, which see different FFT roundings every
 step, are compared by the rms density difference, the rms force difference
 and P(k) per logarithmic bin (the oracle's estimator on both densities)."""
 import os
@@ -85,10 +86,14 @@ def _gpu_vs_oracle(pm, orc, dim, Nm, Nf, L, phi, a0, da, n):
         g.set_state(Xo, Po)
         rho_same, keys_same = g.deposit(keys=True)
         F_same = g.forces(a_pos)
+    with pm.Gravity(dim, Nm, Nf, L, serial_deposit=True) as g:
+        g.set_state(Xo, Po)
+        rho_serial = g.deposit()
     m = (Nf / Nm) ** dim
     rho_o = orc.md_cic(dim, Nf, L, Xo, m)
     assert np.array_equal(keys_same, orc.cic_keys(dim, Nf, L, Xo))
-    assert np.array_equal(rho_same, rho_o)
+    assert np.array_equal(rho_serial, rho_o)
+    assert np.all(np.abs(rho_same - rho_o) <= 1e-13 * np.abs(rho_o) + 1e-24)
     F_o = orc.gather(dim, Nf, L, orc.poisson(dim, Nf, L, rho_o, a_pos, nthreads=nt), Xo, 0)
     edges = _log_edges(Nf, L)
     _, pk, c = orc.power_spectrum(dim, Nf, L, rho - 1, edges, nthreads=nt)

@@ -52,33 +52,49 @@ def _positions(dim, Nm, L, disp, special=True):
     return X
 
 
+def _check_fp64_deposit(rho, ro, serial):
+    if serial:
+        assert np.array_equal(rho, ro)  # same weights, summed in the same particle order
+    else:
+        # exact sum of the weights rounded to 2^-96, against the serial double sum
+        err = np.abs(rho - ro) / np.maximum(np.abs(ro), 1e-300)
+        say(f"  fixed-point deposit vs serial sum: max rel {float(err[ro > 1e-12].max()):.3g}")
+        assert np.all(np.abs(rho - ro) <= 1e-13 * np.abs(ro) + 1e-24)
+
+
+@pytest.mark.parametrize("serial", [True, False])
 @pytest.mark.parametrize("dim,Nm,Nf", [(2, 16, 16), (2, 32, 64), (3, 8, 8), (3, 16, 32), (3, 32, 32)])
-def test_keys_and_deposit_bit_exact_fp64(pm, orc, dim, Nm, Nf):
+def test_keys_and_deposit_fp64(pm, orc, dim, Nm, Nf, serial):
+    """Cell keys bit-exact; the deposit bit-exact with PM_FLAG_SERIAL_DEPOSIT
+    (the reference's serial summation order), else the fixed-point sum
+    within 1e-13 of it."""
     L = 100.0
     X = _positions(dim, Nm, L, 3.0 * L / Nm)
-    say("deposit fp64", dim, Nm, Nf)
-    with pm.Gravity(dim, Nm, Nf, L) as g:
+    say("deposit fp64", dim, Nm, Nf, serial)
+    with pm.Gravity(dim, Nm, Nf, L, serial_deposit=serial) as g:
         g.set_state(X)
         rho, keys = g.deposit(keys=True)
     m = (Nf / Nm) ** dim
     ko = orc.cic_keys(dim, Nf, L, X)
     ro = orc.md_cic(dim, Nf, L, X, m)
     assert np.array_equal(keys, ko)
-    assert np.array_equal(rho, ro)  # same weights, summed in the same particle order
+    _check_fp64_deposit(rho, ro, serial)
 
 
+@pytest.mark.parametrize("serial", [True, False])
 @pytest.mark.parametrize("dim,Nm,Nf", [(2, 32, 32), (3, 16, 32), (3, 32, 32)])
-def test_deposit_clustered_bit_exact_fp64(pm, orc, dim, Nm, Nf):
+def test_deposit_clustered_fp64(pm, orc, dim, Nm, Nf, serial):
     """Many particles per cell (late-time clustering): the merge order still
-    reproduces the serial sum exactly."""
+    reproduces the serial sum exactly; the fixed-point sum stays within
+    1e-13 of it."""
     L = 10.0
     X = L * 0.5 + 0.02 * L * RNG.standard_normal((Nm ** dim, dim))
-    say("deposit clustered", dim, Nm, Nf)
-    with pm.Gravity(dim, Nm, Nf, L) as g:
+    say("deposit clustered", dim, Nm, Nf, serial)
+    with pm.Gravity(dim, Nm, Nf, L, serial_deposit=serial) as g:
         g.set_state(X)
         rho = g.deposit()
     ro = orc.md_cic(dim, Nf, L, X, (Nf / Nm) ** dim)
-    assert np.array_equal(rho, ro)
+    _check_fp64_deposit(rho, ro, serial)
     assert abs(rho.sum() - Nm ** dim * (Nf / Nm) ** dim) < 1e-9 * Nm ** dim
 
 
@@ -494,8 +510,9 @@ def test_full_size_run(pm, orc):
     """The 3-D mixed-precision configuration at its real size: 256^3 particles
     on a 512^3 mesh, L = 100, P(k) ~ k^-1.5 from seeded white noise, a = 0.02
     to 1 in 200 steps.  The oracle cannot run it in test time; parity is shown
-    through what does not depend on the size: the FP64 form's cell keys and
-    deposit of its final, clustered state are bit-exact against the oracle's;
+    through what does not depend on the size: the FP64 form's cell keys of its
+    final, clustered state are bit-exact against the oracle's, and so is its
+    deposit with PM_FLAG_SERIAL_DEPOSIT (the fixed-point one within 1e-13);
     the FP32 form, against the FP64 form with the reference's own gather (the
     unwrapped coordinate truncated, src/run/numeric.hh:19,26-28): P(k) within
     1e-3 per bin over 64 log bins, displacement rms below 1e-3 mesh cell,
@@ -536,7 +553,12 @@ def test_full_size_run(pm, orc):
 
     say("full size: oracle keys and deposit")
     assert np.array_equal(keys, orc.cic_keys(dim, Nf, L, X64))
-    assert np.array_equal(r64, orc.md_cic(dim, Nf, L, X64, (Nf / Nm) ** dim))
+    ro = orc.md_cic(dim, Nf, L, X64, (Nf / Nm) ** dim)
+    _check_fp64_deposit(r64, ro, False)
+    with pm.Gravity(dim, Nm, Nf, L, serial_deposit=True) as g:
+        g.set_state(X64)
+        assert np.array_equal(g.deposit(), ro)
+    del ro
     assert abs(r64.sum() / Nf ** dim - 1) < 1e-12
     assert abs(r32.sum() / Nf ** dim - 1) < 1e-6
     assert r64.max() > 100  # the end state is clustered
