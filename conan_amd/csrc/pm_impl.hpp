@@ -214,7 +214,10 @@ struct Impl : ImplBase {
         : cfg(c), R(c.dim), lgNf(ilog2_exact(c.Nf, "Nf")), Np(ipow(c.Nm, c.dim)), M(ipow(c.Nf, c.dim)),
           nrows(M / c.Nf), cos(c.H0, c.Omega_m, c.Omega_L), fft(c.dim, c.Nf),
           // FP64: (cell, id) pairs sorted by the full cell index; FX: by brick
-          sorter(Np, FX ? fx_keybits(c.dim, lgNf) : c.dim * lgNf) {
+          // FP64: by the full cell index for the serial-order deposit, by the
+          // mesh row of the origin cell for the fixed-point one
+          sorter(Np, FX ? fx_keybits(c.dim, lgNf)
+                        : ((c.flags & PM_FLAG_SERIAL_DEPOSIT) ? c.dim * lgNf : (c.dim - 1) * lgNf)) {
         mass = pow(double(c.Nf) / c.Nm, R);  // src/run/nbody.hh:85
         pa.L = c.L;
         pa.res = c.L / c.Nf;
@@ -224,6 +227,7 @@ struct Impl : ImplBase {
         pa.fscale = (float)ldexp(1.0, -pa.fshift);
         pa.cmask = c.Nf - 1;
         pa.brick = 0;
+        pa.rowkey = (!FX && !(c.flags & PM_FLAG_SERIAL_DEPOSIT)) ? 1 : 0;
         for (int k = 0; k < 3; ++k) pa.lb[k] = pa.lnb[k] = 0;
         if (FX) brick_layout(R, lgNf, pa);
         nsort_groups = FX ? (brick_count(pa) << R) : nrows;
@@ -264,7 +268,7 @@ struct Impl : ImplBase {
             zero_u32(defer.p, 2, st);
         }
         if (!FX) {  // the FP64 deposit's and kick's sorted-order copies
-            frac.alloc((size_t)R * Np);
+            if (!pa.rowkey) frac.alloc((size_t)R * Np);
             xsort.alloc((size_t)R * Np);
             fsort.alloc((size_t)R * Np);
             inv.alloc(Np);
@@ -452,7 +456,7 @@ struct Impl : ImplBase {
             const int where = sorter.sort(k0.p, v0.p, k1.p, v1.p, st);
             skeys = where ? k1.p : k0.p;
             spids = where ? v1.p : v0.p;
-            row_bounds(skeys, Np, lgNf, nsort_groups, row_start.p, st);
+            row_bounds(skeys, Np, pa.rowkey ? 0 : lgNf, nsort_groups, row_start.p, st);
             bstart = row_start.p;
         }
         prof.end(0, st);
@@ -673,6 +677,7 @@ struct Impl : ImplBase {
         if (keys) {
             PosArgs pk = pa;
             pk.brick = 0;  // the full cell index idx(origin)
+            pk.rowkey = 0;
             ParticleOps<PP>::keys(R, X.p, Np, lgNf, pk, k1.p, v1.p, st);
             std::vector<uint32_t> k(Np);
             CPM_HIP(hipMemcpyAsync(k.data(), k1.p, sizeof(uint32_t) * Np, hipMemcpyDeviceToHost, st));

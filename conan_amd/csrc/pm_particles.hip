@@ -39,7 +39,7 @@ __global__ __launch_bounds__(256) void k_keys(const typename PP::Pos* __restrict
         oc[k] = o & mask;
         key |= (uint32_t)oc[k] << (k * lgN);  // Box::idx of the origin (box.hh:75-81)
     }
-    keys[i] = pa.brick ? brick_key(oc[0], oc[1], oc[2], R, pa) : key;
+    keys[i] = pa.brick ? brick_key(oc[0], oc[1], oc[2], R, pa) : (pa.rowkey ? key >> lgN : key);
     pids[i] = (uint32_t)i;
 }
 
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void k_dep_frac(const uint32_t* __restrict__ p
         typename PP::Mesh f;
         const typename PP::Pos x = X[k * np + m];
         PP::dep(x, pa, o, f);
-        frac[k * np + s] = f;
+        if (frac) frac[k * np + s] = f;
         if (Xs) Xs[k * np + s] = x;
     }
     if (inv) inv[m] = (uint32_t)s;
@@ -206,7 +206,8 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit(const uint32_t* __restrict__
 // are exact sums of the rounded weights in any order: deterministic, and
 // within an ulp or two of the reference's serial double sum (which rounds
 // at every addition).  Each particle of the 2^(R-1) origin rows adds its two
-// corners along x that fall in this row.
+// corners along x that fall in this row.  The particles come grouped by the
+// mesh row of their origin cell (row_start), positions in that order (Xs).
 static constexpr double FX128_ONE = 79228162514264337593543950336.0;  // 2^96
 __device__ __forceinline__ void fx128_split(double z, unsigned long long& hi, unsigned long long& lo) {
     // z in [0, 2^31): hi = floor(z 2^32), lo = floor(frac(z 2^32) 2^64)
@@ -221,9 +222,8 @@ __device__ __forceinline__ double fx128_value(unsigned long long hi, unsigned lo
 
 template <int R>
 __global__ __launch_bounds__(DEP_NT) void k_deposit_f64_fx(const uint32_t* __restrict__ row_start,
-                                                            const uint32_t* __restrict__ keys,
-                                                            const double* __restrict__ frac, long np, int lgN,
-                                                            double mass, double* __restrict__ mesh,
+                                                            const double* __restrict__ Xs, long np, int lgN,
+                                                            PosArgs pa, double mass, double* __restrict__ mesh,
                                                             double* __restrict__ row_partial) {
     constexpr int NL = 1 << (R - 1);
     __shared__ unsigned long long alo[1024], ahi[1024];
@@ -242,9 +242,14 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_f64_fx(const uint32_t* __res
         if (R == 3) orow += (long)((cz - bz) & msk) << lgN;
         const uint32_t s0 = row_start[orow], s1 = row_start[orow + 1];
         for (uint32_t s = s0 + tid; s < s1; s += DEP_NT) {
-            const int x = (int)(keys[s] & msk);
-            const double fx = frac[s], fy = frac[np + s];
-            const double fz = (R == 3) ? frac[2 * np + s] : 0.0;
+            // the cell and fractions of md_cic (numeric.hh:114-122), from the
+            // position in the sort's order
+            int x, oy, oz = 0;
+            double fx, fy, fz = 0.0;
+            PosF64::dep(Xs[s], pa, x, fx);
+            PosF64::dep(Xs[np + s], pa, oy, fy);
+            if (R == 3) PosF64::dep(Xs[2 * np + s], pa, oz, fz);
+            x &= msk;
 #pragma unroll
             for (int bx = 0; bx < 2; ++bx) {
                 double z = mass;
@@ -1214,7 +1219,7 @@ void ParticleOps<PP>::deposit(int R, const uint32_t* row_start, const uint32_t* 
         if (!serial_order) {
             if (lgN > 10) throw Error(-2, "the fixed-point FP64 deposit takes meshes of at most 1024 per side");
             CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit_f64_fx<RR>), dim3((unsigned)nrows), dim3(DEP_NT), 0, st,
-                                                 row_start, keys, (const double*)frac, np, lgN, (double)mass,
+                                                 row_start, (const double*)Xs, np, lgN, pa, (double)mass,
                                                  (double*)mesh, row_partial));
             CPM_LAUNCH_CHECK();
             return;

@@ -14,6 +14,7 @@ struct PosArgs {
     float fscale;    // FX: 2^-fshift
     int cmask;       // FX: N - 1 (the cell bits above the fraction)
     int brick;       // sort keys: 0 = cell index idx(origin), 1 = brick index (FX)
+    int rowkey;      // brick == 0: 1 = the mesh row of the origin cell, idx(origin) >> log2 N
     int lb[3];       // log2 brick size per axis
     int lnb[3];      // log2 bricks per axis
 };
