@@ -109,17 +109,66 @@ def test_cic_tool_matches_oracle(sim_dir, orc):
 
 
 def test_ifrit_and_ascii_outputs(sim_dir):
+    """The IFRIT and ASCII snapshots of a run hold the values of its .conan
+    snapshot: IFRIT (src/run/nbody.hh:135-166) uint32-framed blocks of the
+    count, the box (0, 0, 0, L, L, L) and the six float32 arrays X_k, P_k;
+    ASCII (nbody.hh:204-208) one line per particle, `X P/a` with the
+    default stream format and the vectors' space-terminated components."""
     import struct
 
+    from conan_amd import conanfile
+
     d, _ = sim_dir
+    run(["run", "-i", "t", "-f", "1", "--da", "0.0098", "-n", "10"], d)
+    _, _, rec = conanfile.read(str(d / "t.pos.1.conan"))
+    X, P = rec["pos"][1], rec["mom"][1]
     run(["run", "-i", "t", "-f", "1", "--da", "0.0098", "-n", "10", "--ifrit"], d)
     raw = (d / "t.pos.1.bin").read_bytes()
-    n = struct.unpack("<I", raw[:4])[0]
-    assert n == 4 and struct.unpack("<I", raw[4:8])[0] == 4096  # uint32 framing, particle count
     assert len(raw) == (4 + 4 + 4) + (4 + 24 + 4) + 6 * (4 + 4096 * 4 + 4)
+    off = 0
+
+    def block():
+        nonlocal off
+        n = struct.unpack_from("<I", raw, off)[0]
+        body = raw[off + 4: off + 4 + n]
+        assert struct.unpack_from("<I", raw, off + 4 + n)[0] == n  # framed on both sides
+        off += 8 + n
+        return body
+
+    assert struct.unpack("<I", block())[0] == 4096
+    assert np.array_equal(np.frombuffer(block(), "<f4"), np.array([0, 0, 0, 100, 100, 100], np.float32))
+    for A in (X, P):
+        for k in range(3):
+            assert np.array_equal(np.frombuffer(block(), "<f4"), A[:, k].astype(np.float32))
     run(["run", "-i", "t", "-f", "1", "--da", "0.0098", "-n", "10", "--ascii"], d)
     lines = (d / "t.pos.1.txt").read_text().splitlines()
-    assert len(lines) == 4096 and len(lines[0].split()) == 6
+    a = 0.02
+    for _ in range(10):  # LeapFrog::run's a after the 10th step (src/run/leapfrog.hh:38-44)
+        a += 0.0098
+    want = ["".join(f"{v:g} " for v in X[i]) + " " + "".join(f"{v / a:g} " for v in P[i]) for i in range(len(X))]
+    assert lines == want
+
+
+def test_concordance_run_matches_oracle(sim_dir, orc):
+    """run --concordance (H0 68.5, Omega_m 0.31, Omega_L 0.69, src/run/main.cc:
+    94-101): a LambdaCDM expansion in every drift and kick factor."""
+    from conan_amd import conanfile
+
+    d, _ = sim_dir
+    run(["run", "-i", "t", "-f", "2", "--a0", "0.05", "--da", "0.02", "-n", "10", "--concordance"], d)
+    _, _, rec = conanfile.read(str(d / "t.pos.1.conan"))
+    _, _, irec = conanfile.read(str(d / "t.density.init.conan"))
+    sim = orc.Sim(3, 16, 32, 100.0, H0=68.5, Om=0.31, OL=0.69)
+    sim.init(irec["potential"][1], 0.05, 0.02)
+    sim.steps(10)
+    Xo, Po, _ = sim.state()
+    assert np.max(np.abs(rec["pos"][1] - Xo)) < 1e-9 * 100
+    assert np.max(np.abs(rec["mom"][1] - Po)) < 1e-9 * np.max(np.abs(Po))
+    # and it differs from the Einstein-de Sitter run
+    sim = orc.Sim(3, 16, 32, 100.0)
+    sim.init(irec["potential"][1], 0.05, 0.02)
+    sim.steps(10)
+    assert np.max(np.abs(sim.state()[1] - Po)) > 1e-3 * np.max(np.abs(Po))
 
 
 def test_fp32_precision_flag(sim_dir):

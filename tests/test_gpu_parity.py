@@ -181,19 +181,24 @@ def test_ic_displacement(pm, orc):
 
 
 # ------------------------------------------------------------------- leapfrog
+@pytest.mark.parametrize("cosmo", [(68.5, 1.0, 0.0), (68.5, 0.31, 0.69), (70.0, 0.3, 0.0)])
 @pytest.mark.parametrize("dim,Nm,Nf,nsteps", [(2, 32, 32, 10), (3, 16, 16, 10), (3, 16, 32, 5)])
-def test_leapfrog_fp64_vs_oracle(pm, orc, dim, Nm, Nf, nsteps):
+def test_leapfrog_fp64_vs_oracle(pm, orc, dim, Nm, Nf, nsteps, cosmo):
+    """Einstein-de Sitter, LambdaCDM (--concordance) and an open universe
+    (Omega_k = 0.7): the expansion enters every factor (src/run/cosmology.hh:
+    16-22)."""
     L, a0, da = 100.0, 0.02, 0.0098
+    H0, Om, OL = cosmo
     mbits = int(np.log2(Nm))
     noise = orc.white_noise(3, Nm ** dim)
     dens = orc.ic_scale_free(dim, mbits, L, -2.0, noise)
     phi = orc.compute_potential(dim, mbits, L, dens)
-    say("leapfrog", dim, Nm, Nf, nsteps)
-    sim = orc.Sim(dim, Nm, Nf, L)
+    say("leapfrog", dim, Nm, Nf, nsteps, cosmo)
+    sim = orc.Sim(dim, Nm, Nf, L, H0=H0, Om=Om, OL=OL)
     sim.init(phi, a0, da)
     sim.steps(nsteps)
     Xo, Po, t = sim.state()
-    with pm.Gravity(dim, Nm, Nf, L) as g:
+    with pm.Gravity(dim, Nm, Nf, L, H0=H0, Omega_m=Om, Omega_L=OL) as g:
         g.init(phi, a0, a0 + da / 2)
         a = g.steps(a0, da, nsteps)
         X, P = g.get_state()
