@@ -51,6 +51,20 @@ def test_ic_file_matches_oracle(sim_dir, orc):
     assert np.max(np.abs(rec["potential"][1] - phi)) < 1e-12 * np.max(np.abs(phi))
 
 
+def test_ic_seed_matches_oracle_noise(sim_dir, orc):
+    """conan ic --seed S draws the white noise with std::mt19937(S) and
+    std::normal_distribution<double> in linear order (Gaussian_white_noise,
+    src/base/misc.hh:179-188; ic.cc:55-57), as the oracle's white_noise does."""
+    from conan_amd import conanfile
+
+    d, _ = sim_dir
+    run(["ic", "-i", "s", "-d", "3", "-b", "4", "-L", "100", "--scale-free", "-n", "-2", "--seed", "4242"], d)
+    h, _, rec = conanfile.read(str(d / "s.density.init.conan"))
+    assert h["seed"] == "4242"
+    want = orc.ic_scale_free(3, 4, 100.0, -2.0, orc.white_noise(4242, 16 ** 3))
+    assert np.max(np.abs(rec["density"][1] - want)) < 1e-12 * np.max(np.abs(want))
+
+
 @pytest.mark.parametrize("cut", [False, True])
 def test_ic_cdm_file_matches_oracle(sim_dir, orc, cut):
     """conan ic without --scale-free (the CDM branch, src/ic/ic.cc:84-97),
