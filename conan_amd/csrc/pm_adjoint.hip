@@ -25,10 +25,37 @@
 
 namespace cpm {
 
+// The adjoint state (lY, lP) is stored in FP32; every update is computed in
+// FP64 from the stored values and rounded once.  The drift's adjoint
+// lP + cd lY, in one place so that the fused path (k_adj_gather_fx_brick) and
+// the separate one (k_rev_drift_keys) round it identically.
+__device__ __forceinline__ float adj_drift(float lp, double cd, float ly) {
+    return (float)((double)lp + cd * (double)ly);
+}
+
+// splitmix64's output function (Steele, Lea, Flood 2014; constants of
+// Stafford's "Mix13"): a bijective 64-bit mixer.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ull;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// The checksum term of one particle: id, then position and momentum bits
+// axis by axis (k_state_hash and the fused adjoint gather).
+__device__ __forceinline__ unsigned long long hash_start(uint32_t id) {
+    return mix64((unsigned long long)id + 0x9E3779B97F4A7C15ull);
+}
+__device__ __forceinline__ unsigned long long hash_axis(unsigned long long h, uint32_t x, double p) {
+    h = mix64(h ^ x);
+    return mix64(h ^ (unsigned long long)__double_as_longlong(p));
+}
+
 // ------------------------------------------------------------ reverse drift + keys
 template <int R>
 __global__ __launch_bounds__(256) void k_rev_drift_keys(uint32_t* __restrict__ Y, const double* __restrict__ P,
-                                                         const double* __restrict__ lY, double* __restrict__ lP,
+                                                         const float* __restrict__ lY, float* __restrict__ lP,
                                                          long np, double cd_fx, double cd, int lgN, PosArgs pa,
                                                          const uint32_t* __restrict__ prevkeys,
                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ pids,
@@ -47,9 +74,9 @@ __global__ __launch_bounds__(256) void k_rev_drift_keys(uint32_t* __restrict__ Y
                 y -= fx_round_u32(P[k * np + i] * cd_fx);
                 Y[k * np + i] = y;
             }
-            const double lp = lP[k * np + i] + cd * lY[k * np + i];
+            const float lp = adj_drift(lP[k * np + i], cd, lY[k * np + i]);
             lP[k * np + i] = lp;
-            mx = fmaxf(mx, fabsf((float)lp));
+            mx = fmaxf(mx, fabsf(lp));
             oc[k] = (int)(y >> pa.fshift) & mask;
         }
         const uint32_t key = brick_key(oc[0], oc[1], oc[2], R, pa);
@@ -91,7 +118,7 @@ template <int R>
 __global__ __launch_bounds__(VEC_NT) void k_deposit_fx_vec(const uint32_t* __restrict__ bstart,
                                                            const uint32_t* __restrict__ perm,
                                                            const uint32_t* __restrict__ Y,
-                                                           const double* __restrict__ lP, long np, int lgN,
+                                                           const float* __restrict__ lP, long np, int lgN,
                                                            PosArgs pa, const uint32_t* __restrict__ lmax,
                                                            float* __restrict__ s, long M) {
     __shared__ unsigned long long acc[R][FXB_CELLS];
@@ -121,7 +148,7 @@ __global__ __launch_bounds__(VEC_NT) void k_deposit_fx_vec(const uint32_t* __res
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             PosFX::dep(Y[k * np + i], pa, o[k], f[k]);
-            lam[k] = lP[k * np + i] * scale;
+            lam[k] = (double)lP[k * np + i] * scale;
         }
 #pragma unroll
         for (int j = 0; j < (1 << R); ++j) {
@@ -183,16 +210,19 @@ __global__ __launch_bounds__(256) void k_neg_div_f32(const float* __restrict__ s
 // CIC weights dW(c)/dY_k = +-1/res * prod_{m != k} A_m(c).
 static constexpr int FXB_CTILE = (FXB_X + 1) * (FXB_Y + 1) * (FXB_Z + 1);
 
-template <int R>
+// HASH: dr.hash is set (a separate instance, so that the checksum's
+// registers do not weigh on the launches without it).
+template <int R, bool HASH>
 __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_gather_fx_brick(
     const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ Y,
-    const double* __restrict__ P, const double* __restrict__ lY, const double* __restrict__ lP,
+    const double* __restrict__ P, const float* __restrict__ lY, const float* __restrict__ lP,
     const uint32_t* __restrict__ idi, uint32_t* __restrict__ Yo, double* __restrict__ Po,
-    double* __restrict__ lYo, double* __restrict__ lPo, uint32_t* __restrict__ ido, long np,
+    float* __restrict__ lYo, float* __restrict__ lPo, uint32_t* __restrict__ ido, long np,
     const float* __restrict__ phi, const float* __restrict__ chi, int lgN, PosArgs pa, AdjCoef cf, AdjDrift dr) {
     __shared__ float tile[FXB_TILE];
     __shared__ float ctile[FXB_CTILE];
     __shared__ float wmax[FXG_NT / 64];
+    __shared__ unsigned long long whash[FXG_NT / 64];
     const double cFq = cf.cF * cf.ipq;  // the forward's ck / pq for cF = ck (exact: ipq is a power of two)
     const int N = 1 << lgN, msk = N - 1;
     const uint32_t bid = blockIdx.x;
@@ -205,14 +235,14 @@ __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) voi
     // latencies overlap; each later one at the end of the previous iteration.
     const uint32_t s0 = bstart[bid << R], s1 = bstart[(bid + 1) << R];
     uint32_t ys[3] = {0u, 0u, 0u}, idv = 0u, iv = 0u;
-    double lam[3] = {0, 0, 0};
+    double lam[3] = {0, 0, 0};  // lP of the particle (FP32 values)
     auto fetch = [&](uint32_t sp) {
         iv = perm[sp];
         idv = idi[iv];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             ys[k] = Y[k * np + iv];
-            lam[k] = lP[k * np + iv];
+            lam[k] = (double)lP[k * np + iv];
         }
     };
     if (s0 + tid < s1) fetch(s0 + tid);
@@ -223,6 +253,8 @@ __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) voi
     const double ires = 1.0 / pa.res;
     constexpr int NC = 1 << R;
     float mx = 0.f;
+    constexpr bool hh = HASH;
+    unsigned long long hsum = 0;  // checksum of the state before the drift's undo (dr.hash)
     for (uint32_t s = s0 + tid; s < s1; s += FXG_NT) {
         ido[s] = idv;
         int o[3] = {0, 0, 0};
@@ -317,22 +349,25 @@ __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) voi
             }
         }
         uint32_t yn[3] = {0u, 0u, 0u};
+        unsigned long long h = hh ? hash_start(idv) : 0ull;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const double p = P[k * np + iv] + fx_kick_increment((double)F[k], cFq, cf.pq);
-            const double lyk = lY[k * np + iv] + cf.cT * T[k] + cf.cU * U[k];
-            double lpk = lam[k];
+            const float lyk = (float)((double)lY[k * np + iv] + cf.cT * T[k] + cf.cU * U[k]);
+            float lpk = (float)lam[k];
+            if (hh) h = hash_axis(h, ys[k], p);
             yn[k] = ys[k];
             if (dr.on) {  // the previous step's drift, undone, and its adjoint
                 yn[k] -= fx_round_u32(p * dr.cd_fx);
-                lpk += dr.cd * lyk;
+                lpk = adj_drift(lpk, dr.cd, lyk);
             }
             Yo[k * np + s] = yn[k];
             Po[k * np + s] = p;
             lYo[k * np + s] = lyk;
             lPo[k * np + s] = lpk;
-            mx = fmaxf(mx, fabsf((float)lpk));
+            mx = fmaxf(mx, fabsf(lpk));
         }
+        hsum += h;
         if (dr.keys) {
             int co[3] = {0, 0, 0}, cn[3] = {0, 0, 0};
 #pragma unroll
@@ -349,6 +384,18 @@ __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) voi
         }
         if (s + FXG_NT < s1) fetch(s + FXG_NT);
     }
+    if constexpr (HASH) {  // integer sum: the same for any order of the particles
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) hsum += __shfl_xor(hsum, off);
+        if ((tid & 63) == 0) whash[tid >> 6] = hsum;
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long t = 0;
+#pragma unroll
+            for (int w = 0; w < FXG_NT / 64; ++w) t += whash[w];
+            atomicAdd(dr.hash, t);
+        }
+    }
     if (dr.bmax) {  // block max of |lP| out -> bmax[block]
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
@@ -364,15 +411,6 @@ __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) voi
 }
 
 // ------------------------------------------------------------ state checksum
-// splitmix64's output function (Steele, Lea, Flood 2014; constants of
-// Stafford's "Mix13"): a bijective 64-bit mixer.
-__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
-    z ^= z >> 30;
-    z *= 0xBF58476D1CE4E5B9ull;
-    z ^= z >> 27;
-    z *= 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
 // *sum += sum over particles of a hash of (id, positions, momentum bits):
 // integer addition, so the value does not depend on the storage order.  With
 // Xc != nullptr the state is also copied to (Xc, Pc, idc) in the same pass.
@@ -385,13 +423,12 @@ __global__ __launch_bounds__(256) void k_state_hash(const uint32_t* __restrict__
     unsigned long long acc = 0;
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < np; i += (long)gridDim.x * 256) {
         const uint32_t d = id[i];
-        unsigned long long h = mix64((unsigned long long)d + 0x9E3779B97F4A7C15ull);
+        unsigned long long h = hash_start(d);
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const uint32_t x = X[k * np + i];
             const double pk = P[k * np + i];
-            h = mix64(h ^ x);
-            h = mix64(h ^ (unsigned long long)__double_as_longlong(pk));
+            h = hash_axis(h, x, pk);
             if (Xc) {
                 Xc[k * np + i] = x;
                 Pc[k * np + i] = pk;
@@ -446,6 +483,11 @@ __global__ __launch_bounds__(256) void k_neg_div_f64(const double* __restrict__ 
 }
 
 // ------------------------------------------------------------ small vector kernels
+__global__ __launch_bounds__(256) void k_axpby_f32(long n, double a, const float* __restrict__ x, double b,
+                                                   const float* __restrict__ y, double* __restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a * (double)x[i] + b * (double)y[i];
+}
 __global__ __launch_bounds__(256) void k_axpby(long n, double a, const double* __restrict__ x, double b,
                                                const double* __restrict__ y, double* __restrict__ out) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -529,7 +571,7 @@ __global__ __launch_bounds__(256) void k_state_diff(const uint32_t* __restrict__
 }
 
 // ------------------------------------------------------------ launchers
-void rev_drift_keys(int R, uint32_t* Y, const double* P, const double* lY, double* lP, long np, double cd_fx,
+void rev_drift_keys(int R, uint32_t* Y, const double* P, const float* lY, float* lP, long np, double cd_fx,
                     double cd, int lgN, const PosArgs& pa, const uint32_t* prevkeys, uint32_t* keys, uint32_t* pids,
                     uint32_t* in_count, uint32_t* out_count, uint32_t* bmax, uint32_t* lmax, hipStream_t st) {
     const unsigned nb = nblk(np, 256);
@@ -540,7 +582,7 @@ void rev_drift_keys(int R, uint32_t* Y, const double* P, const double* lY, doubl
     CPM_LAUNCH_CHECK();
 }
 
-void deposit_fx_vec(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* Y, const double* lP,
+void deposit_fx_vec(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* Y, const float* lP,
                     long np, int lgN, const PosArgs& pa, const uint32_t* lmax, float* s, hipStream_t st) {
     const unsigned nb = (unsigned)brick_count(pa);
     const long M = 1L << (R * lgN);
@@ -558,12 +600,20 @@ void neg_div_f32(int R, const float* s, int lgN, double res, float* psi, hipStre
 }
 
 void adj_gather_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* Y, const double* P,
-                         const double* lY, const double* lP, const uint32_t* idi, uint32_t* Yo, double* Po,
-                         double* lYo, double* lPo, uint32_t* ido, long np, const float* phi, const float* chi,
+                         const float* lY, const float* lP, const uint32_t* idi, uint32_t* Yo, double* Po,
+                         float* lYo, float* lPo, uint32_t* ido, long np, const float* phi, const float* chi,
                          int lgN, const PosArgs& pa, AdjCoef c, AdjDrift dr, uint32_t* lmax, hipStream_t st) {
     const unsigned nb = (unsigned)brick_count(pa);
-    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_adj_gather_fx_brick<RR>), dim3(nb), dim3(FXG_NT), 0, st, bstart, perm,
-                                         Y, P, lY, lP, idi, Yo, Po, lYo, lPo, ido, np, phi, chi, lgN, pa, c, dr));
+    if (dr.hash) zero_u32(reinterpret_cast<uint32_t*>(dr.hash), 2, st);
+    if (dr.hash) {
+        CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_adj_gather_fx_brick<RR, true>), dim3(nb), dim3(FXG_NT), 0, st, bstart,
+                                             perm, Y, P, lY, lP, idi, Yo, Po, lYo, lPo, ido, np, phi, chi, lgN, pa, c,
+                                             dr));
+    } else {
+        CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_adj_gather_fx_brick<RR, false>), dim3(nb), dim3(FXG_NT), 0, st, bstart,
+                                             perm, Y, P, lY, lP, idi, Yo, Po, lYo, lPo, ido, np, phi, chi, lgN, pa, c,
+                                             dr));
+    }
     CPM_LAUNCH_CHECK();
     if (dr.bmax) {
         hipLaunchKernelGGL(k_max_u32, dim3(1), dim3(1024), 0, st, (const uint32_t*)dr.bmax, (long)nb, lmax);
@@ -596,6 +646,8 @@ template void scatter_by_id<double>(const double*, const uint32_t*, long, int, d
 template void scatter_by_id<uint32_t>(const uint32_t*, const uint32_t*, long, int, uint32_t*, hipStream_t);
 template void gather_by_id<double>(const double*, const uint32_t*, long, int, double*, hipStream_t);
 template void gather_by_id<uint32_t>(const uint32_t*, const uint32_t*, long, int, uint32_t*, hipStream_t);
+template void scatter_by_id<float>(const float*, const uint32_t*, long, int, float*, hipStream_t);
+template void gather_by_id<float>(const float*, const uint32_t*, long, int, float*, hipStream_t);
 
 void neg_div_f64(int R, const double* u, int lgN, double res, double* psi, hipStream_t st) {
     const long n = 1L << (R * lgN);
@@ -604,6 +656,10 @@ void neg_div_f64(int R, const double* u, int lgN, double res, double* psi, hipSt
     CPM_LAUNCH_CHECK();
 }
 
+void axpby_f32(long n, double a, const float* x, double b, const float* y, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_axpby_f32, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, x, b, y, out);
+    CPM_LAUNCH_CHECK();
+}
 void axpby(long n, double a, const double* x, double b, const double* y, double* out, hipStream_t st) {
     hipLaunchKernelGGL(k_axpby, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, x, b, y, out);
     CPM_LAUNCH_CHECK();

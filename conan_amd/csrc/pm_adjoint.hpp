@@ -6,20 +6,23 @@
 
 namespace cpm {
 
+// The adjoint state lY = dL/dY, lP = dL/dP is held in FP32 (each update
+// computed in FP64 and rounded once).
+//
 // Reverse of the drift that a fused kick appends, and the drift's adjoint:
 //   Y -= round(P * cd_fx)        (exact inverse of the forward's integer step)
 //   lP += cd * lY                (lY: dL/dY in 1/length, cd: drift factor)
 // then the FX sort keys of the new Y (as keys_fx: full sort when prevkeys is
 // null, else incremental), and max |lP| (float bits) into *lmax (bmax:
 // scratch of one word per 256 particles).  cd_fx == 0 leaves Y as it is.
-void rev_drift_keys(int R, uint32_t* Y, const double* P, const double* lY, double* lP, long np, double cd_fx,
+void rev_drift_keys(int R, uint32_t* Y, const double* P, const float* lY, float* lP, long np, double cd_fx,
                     double cd, int lgN, const PosArgs& pa, const uint32_t* prevkeys, uint32_t* keys, uint32_t* pids,
                     uint32_t* in_count, uint32_t* out_count, uint32_t* bmax, uint32_t* lmax, hipStream_t st);
 
 // s_k(c) = sum_p lP_k(p) * W(Y_p, c), k < R: the CIC deposit of the R
 // components of lP onto R meshes of M cells (s + k * M), accumulated in
 // 64-bit fixed point with a scale taken from *lmax (deterministic).
-void deposit_fx_vec(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* Y, const double* lP,
+void deposit_fx_vec(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* Y, const float* lP,
                     long np, int lgN, const PosArgs& pa, const uint32_t* lmax, float* s, hipStream_t st);
 
 // psi = -sum_k D_k s_k with D_k the FP32 4-point derivative of the gather:
@@ -39,17 +42,20 @@ struct AdjCoef {
 // lY): on != 0 undoes Y -= round(P cd_fx) and adds lP += cd lY.  keys !=
 // nullptr writes the incremental re-sort's keys and mover counts (in_count,
 // out_count zeroed) of the output positions against the input ones; bmax !=
-// nullptr takes max |lP| per workgroup, reduced into *lmax.
+// nullptr takes max |lP| per workgroup, reduced into *lmax.  hash !=
+// nullptr: *hash = the state_hash checksum of (id, Y, P) after the kick's
+// undo and before the drift's (the state a checkpoint holds).
 struct AdjDrift {
     double cd_fx, cd;
     int on;
     uint32_t *keys, *in_count, *out_count, *bmax;
+    unsigned long long* hash;
 };
 // One workgroup per brick; reads particle perm[s] and writes it, its id and
 // its adjoints to position s.  phi == nullptr: F = T = 0; chi == nullptr: U = 0.
 void adj_gather_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* Y, const double* P,
-                         const double* lY, const double* lP, const uint32_t* idi, uint32_t* Yo, double* Po,
-                         double* lYo, double* lPo, uint32_t* ido, long np, const float* phi, const float* chi,
+                         const float* lY, const float* lP, const uint32_t* idi, uint32_t* Yo, double* Po,
+                         float* lYo, float* lPo, uint32_t* ido, long np, const float* phi, const float* chi,
                          int lgN, const PosArgs& pa, AdjCoef c, AdjDrift dr, uint32_t* lmax, hipStream_t st);
 
 // *sum = sum over particles of a 64-bit hash of (id, X, P bits), independent
@@ -69,6 +75,8 @@ void gather_by_id(const T* src, const uint32_t* id, long np, int nc, T* dst, hip
 void neg_div_f64(int R, const double* u, int lgN, double res, double* psi, hipStream_t st);
 
 void axpby(long n, double a, const double* x, double b, const double* y, double* out, hipStream_t st);
+// out = a x + b y in FP64 from FP32 x, y
+void axpby_f32(long n, double a, const float* x, double b, const float* y, double* out, hipStream_t st);
 // Deterministic FP64 dot product (fixed block tree): *out = sum a[i] * b[i].
 void dot(const double* a, const double* b, long n, double* partials, long max_partials, double* out,
          hipStream_t st);

@@ -11,10 +11,13 @@
 // Backward, from lY = m dW/dY . dL/drho at the final positions:
 //   for i = n-1 .. 0:  sort, phi_i (stored, or deposit + Poisson:
 //     bit-identical to the forward), deposit of the momentum adjoint, -div,
-//     Poisson (chi), fused adjoint gather that undoes kick i and, between
-//     checkpoints, the drift before it (exact) with the drift's adjoint; at a
-//     checkpoint, compare with the stored state (checksum, or value by value
-//     with PM_DIFF_VERIFY_CHECKPOINTS) and continue from it.
+//     Poisson (chi), fused adjoint gather that undoes kick i and (i > 0)
+//     the drift before it (exact) with the drift's adjoint; at a checkpoint
+//     the same pass takes the checksum of the state between the two undos
+//     and compares it with the stored one; on a mismatch, or with
+//     PM_DIFF_VERIFY_CHECKPOINTS, the drift is redone on the positions, the
+//     state compared value by value and the pass continues from the stored
+//     state.
 //   then the Zel'dovich start and the IC field in FP64 on the mass grid:
 //     X = q - a0 u, P = -a_mom u, u = D phi_IC / res  (src/run/nbody.hh:40-50)
 //     phi_IC = Pi delta, Pi = IFFT[-1/k^2 FFT] / n       (src/ic/ic.cc:100-117)
@@ -46,7 +49,7 @@ struct DiffPM {
     DevBuf<unsigned long long> ckhash, hbuf;        // checkpoint checksums, scratch
     std::vector<std::unique_ptr<DevBuf<float>>> phis;  // PM_DIFF_STORE_POTENTIALS: phi of every kick
     bool store_phi, verify;
-    DevBuf<double> lY, lP, lY2, lP2;
+    DevBuf<float> lY, lP, lY2, lP2;  // adjoint state (FP32), in the storage order and the next
     DevBuf<uint32_t> tmpY, bmax, lmax;
     DevBuf<float> svec, psi, chi, grho;
     DevBuf<double> dpart, dscal;
@@ -238,13 +241,15 @@ struct DiffPM {
 
     // Adjoint gather (undoes the kick).  fuse: also undo the drift before it
     // (cd_fx, cd; both 0 when there is none) and produce the next sort's keys
-    // and max |lP| (into lmax).
-    void adj_gather(const float* phi, const float* chiv, AdjCoef c, bool fuse, double cd_fx, double cd) {
-        AdjDrift dr{0.0, 0.0, 0, nullptr, nullptr, nullptr, nullptr};
+    // and max |lP| (into lmax).  hash != nullptr: the checksum of the state
+    // between the two undos, into *hash.
+    void adj_gather(const float* phi, const float* chiv, AdjCoef c, bool fuse, double cd_fx, double cd,
+                    unsigned long long* hash = nullptr) {
+        AdjDrift dr{0.0, 0.0, 0, nullptr, nullptr, nullptr, nullptr, hash};
         if (fuse) {
             zero_u32(sim.cnt3.p, 3 * sim.nsort_groups, st);
             dr = AdjDrift{cd_fx, cd, (cd_fx != 0.0 || cd != 0.0) ? 1 : 0, sim.knew.p, sim.cnt3.p,
-                          sim.cnt3.p + sim.nsort_groups, bmax.p};
+                          sim.cnt3.p + sim.nsort_groups, bmax.p, hash};
         }
         adj_gather_fx_brick(R, sim.bstart, sim.spids, sim.X.p, sim.P.p, lY.p, lP.p, sim.id.p, sim.X2.p, sim.P2.p,
                             lY2.p, lP2.p, sim.id2.p, Np, phi, chiv, sim.lgNf, sim.pa, c, dr, lmax.p, st);
@@ -253,37 +258,39 @@ struct DiffPM {
         sim.keys_ready = fuse;
     }
 
-    // Compare the reversed state with checkpoint c.  Without verify, equal
-    // checksums end it: the state is the checkpoint's up to the storage
-    // order, which nothing downstream depends on.  Otherwise (or on a
-    // mismatch) compare value by value and continue from the checkpoint: the
-    // adjoints are moved into its particle order via the ids.
+    // Whether the checksum in hbuf equals checkpoint c's: the reversed state
+    // is then the checkpoint's up to the storage order, which nothing
+    // downstream depends on.
+    bool hash_matches(int c) {
+        unsigned long long hv[2] = {0, 1};
+        CPM_HIP(hipMemcpyAsync(&hv[0], hbuf.p, sizeof(hv[0]), hipMemcpyDeviceToHost, st));
+        CPM_HIP(hipMemcpyAsync(&hv[1], ckhash.p + c, sizeof(hv[1]), hipMemcpyDeviceToHost, st));
+        CPM_HIP(hipStreamSynchronize(st));
+        return hv[0] == hv[1];
+    }
+
+    // Compare the reversed state with checkpoint c value by value and continue
+    // from the checkpoint: the adjoints are moved into its particle order via
+    // the ids.
     void restore(int c) {
-        if (!verify) {
-            state_hash(R, sim.X.p, sim.P.p, sim.id.p, Np, nullptr, nullptr, nullptr, hbuf.p, st);
-            unsigned long long hv[2] = {0, 1};
-            CPM_HIP(hipMemcpyAsync(&hv[0], hbuf.p, sizeof(hv[0]), hipMemcpyDeviceToHost, st));
-            CPM_HIP(hipMemcpyAsync(&hv[1], ckhash.p + c, sizeof(hv[1]), hipMemcpyDeviceToHost, st));
-            CPM_HIP(hipStreamSynchronize(st));
-            if (hv[0] == hv[1]) return;
-        }
         const DevBuf<uint32_t>& cX = *ckX[c];
         const DevBuf<double>& cP = *ckP[c];
         const DevBuf<uint32_t>& cid = *ckid[c];
         // reversed state in the checkpoint's order: X2 / P2
+        DevBuf<double> tP((size_t)R * Np);
         scatter_by_id<uint32_t>(sim.X.p, sim.id.p, Np, R, tmpY.p, st);
         gather_by_id<uint32_t>(tmpY.p, cid.p, Np, R, sim.X2.p, st);
-        scatter_by_id<double>(sim.P.p, sim.id.p, Np, R, lP2.p, st);
-        gather_by_id<double>(lP2.p, cid.p, Np, R, sim.P2.p, st);
+        scatter_by_id<double>(sim.P.p, sim.id.p, Np, R, tP.p, st);
+        gather_by_id<double>(tP.p, cid.p, Np, R, sim.P2.p, st);
         const int nb = 1024;
         state_diff(sim.X2.p, cX.p, sim.P2.p, cP.p, (long)R * Np, dpart.p, nb, st);
         std::vector<double> h(3 * nb);
         CPM_HIP(hipMemcpyAsync(h.data(), dpart.p, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, st));
         // adjoints into the checkpoint's order
-        scatter_by_id<double>(lY.p, sim.id.p, Np, R, lY2.p, st);
-        gather_by_id<double>(lY2.p, cid.p, Np, R, lY.p, st);
-        scatter_by_id<double>(lP.p, sim.id.p, Np, R, lP2.p, st);
-        gather_by_id<double>(lP2.p, cid.p, Np, R, lP.p, st);
+        scatter_by_id<float>(lY.p, sim.id.p, Np, R, lY2.p, st);
+        gather_by_id<float>(lY2.p, cid.p, Np, R, lY.p, st);
+        scatter_by_id<float>(lP.p, sim.id.p, Np, R, lP2.p, st);
+        gather_by_id<float>(lP2.p, cid.p, Np, R, lP.p, st);
         CPM_HIP(hipMemcpyAsync(sim.X.p, cX.p, sizeof(uint32_t) * R * Np, hipMemcpyDeviceToDevice, st));
         CPM_HIP(hipMemcpyAsync(sim.P.p, cP.p, sizeof(double) * R * Np, hipMemcpyDeviceToDevice, st));
         CPM_HIP(hipMemcpyAsync(sim.id.p, cid.p, sizeof(uint32_t) * Np, hipMemcpyDeviceToDevice, st));
@@ -316,14 +323,15 @@ struct DiffPM {
             f64_to_f32(src, sim.M, grho.p, st);
             CPM_HIP(hipStreamSynchronize(st));
         }
-        CPM_HIP(hipMemsetAsync(lY.p, 0, sizeof(double) * R * Np, st));
-        CPM_HIP(hipMemsetAsync(lP.p, 0, sizeof(double) * R * Np, st));
+        CPM_HIP(hipMemsetAsync(lY.p, 0, sizeof(float) * R * Np, st));
+        CPM_HIP(hipMemsetAsync(lP.p, 0, sizeof(float) * R * Np, st));
         const double m = sim.mass;
         // final density: rho = m sum W(Y, c)  ->  lY = m dW/dY . grad_rho;
         // the last kick has no drift after it: only the keys and max |lP|.
         // The forward's final sort still describes the particles.
         adj_gather(nullptr, grho.p, AdjCoef{0.0, 0.0, m, sim.pq, 1.0 / sim.pq}, true, 0.0, 0.0);
-        bool drifted = true;  // the drift after kick i is undone (keys, lmax ready)
+        bool drifted = true;      // the drift after kick i is undone (keys, lmax ready)
+        bool lam_drifted = false;  // its adjoint is already in lP (a checkpoint restored after a fused undo)
         for (int i = p.nsteps - 1; i >= 0; --i) {
             const double ak = a_seq[i] + p.da / 2;
             const double ck = p.da / sim.cos.adot(ak);  // src/run/nbody.hh:112-116
@@ -331,8 +339,9 @@ struct DiffPM {
                 // the drift after kick i (at a_{i+1}, fused into kick i forward)
                 const Impl<PosFX>::FxKeyArgs ka = sim.fx_key_args();
                 rev_drift_keys(R, sim.X.p, sim.P.p, lY.p, lP.p, Np, sim.drift_factor(a_seq[i + 1], p.da),
-                               drift_adj(a_seq[i + 1]), sim.lgNf, sim.pa, ka.prevkeys, ka.keys, ka.pids,
-                               ka.in_count, ka.out_count, bmax.p, lmax.p, st);
+                               lam_drifted ? 0.0 : drift_adj(a_seq[i + 1]), sim.lgNf, sim.pa, ka.prevkeys, ka.keys,
+                               ka.pids, ka.in_count, ka.out_count, bmax.p, lmax.p, st);
+                lam_drifted = false;
             }
             sim.fx_sort_current();
             const float* phi = nullptr;
@@ -350,10 +359,24 @@ struct DiffPM {
             poisson_into(psi.p, chi.p, ak);
             const AdjCoef coef{ck, -ck, -ck * m, sim.pq, 1.0 / sim.pq};
             if (i % p.ckpt_every == 0) {
-                // the checkpoint holds the state before kick i: compare first
-                adj_gather(phi, chi.p, coef, false, 0.0, 0.0);
-                restore(i / p.ckpt_every);
-                drifted = false;
+                // the checkpoint holds the state before kick i: its checksum is
+                // taken in the pass that undoes kick i and the drift before it
+                const double cd_fx = i > 0 ? sim.drift_factor(a_seq[i], p.da) : 0.0;
+                adj_gather(phi, chi.p, coef, true, cd_fx, i > 0 ? drift_adj(a_seq[i]) : 0.0,
+                           verify ? nullptr : hbuf.p);
+                drifted = true;
+                if (verify || !hash_matches(i / p.ckpt_every)) {
+                    // redo the drift on the positions (Y + round(P cd_fx), exact:
+                    // rounding to nearest is odd), compare value by value and
+                    // continue from the checkpoint; lP already holds the drift's
+                    // adjoint
+                    const Impl<PosFX>::FxKeyArgs ka = sim.fx_key_args();
+                    rev_drift_keys(R, sim.X.p, sim.P.p, lY.p, lP.p, Np, -cd_fx, 0.0, sim.lgNf, sim.pa, ka.prevkeys,
+                                   ka.keys, ka.pids, ka.in_count, ka.out_count, bmax.p, lmax.p, st);
+                    restore(i / p.ckpt_every);
+                    drifted = false;
+                    lam_drifted = true;
+                }
             } else {
                 adj_gather(phi, chi.p, coef, true, sim.drift_factor(a_seq[i], p.da), drift_adj(a_seq[i]));
                 drifted = true;
@@ -363,9 +386,10 @@ struct DiffPM {
         // Zel'dovich start: Y0 = X0 + cd0 P0, X0 = q - a0 u, P0 = -a_mom u
         const double a0 = a_seq[0], amom = p.a0 + p.da / 2;
         const double cd0 = p.da / (a0 * a0 * sim.cos.adot(a0));
-        axpby((long)R * Np, -a0 - amom * cd0, lY.p, -amom, lP.p, lY2.p, st);  // l_u, storage order
-        scatter_by_id<double>(lY2.p, sim.id.p, Np, R, lP2.p, st);            // l_u, mass-grid order
-        neg_div_f64(R, lP2.p, lgNm, p.L / Nm, icm.p, st);                     // l_phi
+        // (the state's momenta are consumed: P2 and P take l_u)
+        axpby_f32((long)R * Np, -a0 - amom * cd0, lY.p, -amom, lP.p, sim.P2.p, st);  // l_u, storage order
+        scatter_by_id<double>(sim.P2.p, sim.id.p, Np, R, sim.P.p, st);              // l_u, mass-grid order
+        neg_div_f64(R, sim.P.p, lgNm, p.L / Nm, icm.p, st);                          // l_phi
         ic_potential();                                                        // l_delta (Pi symmetric)
         dot(icm.p, delta.p, Np, icred.p, 4096, icscal.p, st);
         double ld = 0;

@@ -289,8 +289,9 @@ int pm_fof(int dim, long long n, double L, double ll, const double* pos, long lo
  * run in the FP32 form with the gather on folded coordinates
  * (PM_FLAG_WRAP_GATHER: the force is then a smooth function of the position
  * on the torus), and its adjoint: given dL/drho, dL/dw by reverse-mode
- * differentiation with hand-written adjoint kernels.  The backward pass
- * integrates the leapfrog backwards (the drift is undone exactly in the
+ * differentiation with hand-written adjoint kernels (the particles' adjoint
+ * state in FP32, each update computed in FP64; the IC part in FP64).  The
+ * backward pass integrates the leapfrog backwards (the drift is undone exactly in the
  * fixed-point positions, the kick with the recomputed force) and restores the
  * exact state from the checkpoint stored every ckpt_every steps of the
  * forward pass, instead of storing the trajectory.
