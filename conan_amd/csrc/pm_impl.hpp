@@ -166,7 +166,8 @@ struct Impl : ImplBase {
     RadixSorter sorter;
     DevBuf<Pos> X, X2;
     DevBuf<double> P, P2;
-    // FX keeps particles in the cell order of the last deposit; id[s] is the
+    // The brick forms (FX, and FP64 without PM_FLAG_SERIAL_DEPOSIT) keep the
+    // particles in the brick order of the last deposit; id[s] is the
     // mass-grid index of the particle stored at s (X2/P2/id2: the next order).
     DevBuf<uint32_t> id, id2;
     DevBuf<uint32_t> k0, v0, k1, v1, row_start;
@@ -180,7 +181,7 @@ struct Impl : ImplBase {
     DevBuf<uint32_t> defer;
     int dpar = 0;
     const uint32_t* bstart = nullptr;  // group starts of the last sort (row_start or bs2)
-    bool resort_ok = false;            // storage order == output order of the last FX sort
+    bool resort_ok = false;            // storage order == output order of the last brick sort
     bool keys_ready = false;           // knew / cnt3 hold the keys of the current positions (fused gather)
     DevBuf<Mesh> mesh;
     DevBuf<Mesh> frac;  // FP64 deposit: fractional cell coordinates in sorted order
@@ -197,10 +198,13 @@ struct Impl : ImplBase {
     double pq = 0;          // FX: momentum quantum (a power of two), see fx_quantize_momenta
     bool pq_pinned = false; // FX: pq was set by the caller (kept by set_state)
     Mesh* phi_out = nullptr;  // when set, the potential of a kick goes here instead of into mesh
+    bool bricks;              // the brick order (brick_form)
 
+    // the brick order (FX always; FP64 unless the serial-order deposit is asked for)
+    static bool brick_form(const pm_config& c) { return FX || !(c.flags & PM_FLAG_SERIAL_DEPOSIT); }
     static int fx_keybits(int dim, int lgN) {
         PosArgs t{};
-        brick_layout(dim, lgN, t);
+        brick_layout(dim, lgN, t, !FX);
         return t.lnb[0] + t.lnb[1] + t.lnb[2] + dim;  // brick, then face class
     }
 
@@ -213,11 +217,9 @@ struct Impl : ImplBase {
     explicit Impl(const pm_config& c)
         : cfg(c), R(c.dim), lgNf(ilog2_exact(c.Nf, "Nf")), Np(ipow(c.Nm, c.dim)), M(ipow(c.Nf, c.dim)),
           nrows(M / c.Nf), cos(c.H0, c.Omega_m, c.Omega_L), fft(c.dim, c.Nf),
-          // FP64: (cell, id) pairs sorted by the full cell index; FX: by brick
-          // FP64: by the full cell index for the serial-order deposit, by the
-          // mesh row of the origin cell for the fixed-point one
-          sorter(Np, FX ? fx_keybits(c.dim, lgNf)
-                        : ((c.flags & PM_FLAG_SERIAL_DEPOSIT) ? c.dim * lgNf : (c.dim - 1) * lgNf)) {
+          // (key, index) pairs sorted by brick and face class, or (FP64 with
+          // the serial-order deposit) by the full cell index
+          sorter(Np, brick_form(c) ? fx_keybits(c.dim, lgNf) : c.dim * lgNf), bricks(brick_form(c)) {
         mass = pow(double(c.Nf) / c.Nm, R);  // src/run/nbody.hh:85
         pa.L = c.L;
         pa.res = c.L / c.Nf;
@@ -227,10 +229,10 @@ struct Impl : ImplBase {
         pa.fscale = (float)ldexp(1.0, -pa.fshift);
         pa.cmask = c.Nf - 1;
         pa.brick = 0;
-        pa.rowkey = (!FX && !(c.flags & PM_FLAG_SERIAL_DEPOSIT)) ? 1 : 0;
+        pa.rowkey = 0;
         for (int k = 0; k < 3; ++k) pa.lb[k] = pa.lnb[k] = 0;
-        if (FX) brick_layout(R, lgNf, pa);
-        nsort_groups = FX ? (brick_count(pa) << R) : nrows;
+        if (bricks) brick_layout(R, lgNf, pa, !FX);
+        nsort_groups = bricks ? (brick_count(pa) << R) : nrows;
         CPM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         try {
             allocate();
@@ -243,7 +245,7 @@ struct Impl : ImplBase {
     void allocate() {
         X.alloc((size_t)R * Np);
         P.alloc((size_t)R * Np);
-        if (FX) {
+        if (bricks) {
             X2.alloc((size_t)R * Np);
             P2.alloc((size_t)R * Np);
             id.alloc(Np);
@@ -254,7 +256,7 @@ struct Impl : ImplBase {
         k1.alloc(Np);
         v1.alloc(Np);
         row_start.alloc((nsort_groups > nrows ? nsort_groups : nrows) + 1);
-        if (FX) {
+        if (bricks) {
             knew.alloc(Np);
             skb0.alloc(Np);
             skb1.alloc(Np);
@@ -267,8 +269,8 @@ struct Impl : ImplBase {
             defer.alloc(Np + 2);
             zero_u32(defer.p, 2, st);
         }
-        if (!FX) {  // the FP64 deposit's and kick's sorted-order copies
-            if (!pa.rowkey) frac.alloc((size_t)R * Np);
+        if (!bricks) {  // the serial-order deposit's and its kick's sorted-order copies
+            frac.alloc((size_t)R * Np);
             xsort.alloc((size_t)R * Np);
             fsort.alloc((size_t)R * Np);
             inv.alloc(Np);
@@ -276,7 +278,7 @@ struct Impl : ImplBase {
         mesh.alloc(M);
         spec.alloc(fft.spec_elems);
         nyq.alloc(fft.nyq_elems);
-        row_partial.alloc(FX && brick_count(pa) > nrows ? brick_count(pa) : nrows);
+        row_partial.alloc(bricks && brick_count(pa) > nrows ? brick_count(pa) : nrows);
         red.alloc(4096);
         scal.alloc(16);
     }
@@ -294,7 +296,7 @@ struct Impl : ImplBase {
     // Storage order = mass-grid order (set after ZA / set_state).
     void reset_ids() {
         invalidate_order();
-        if (!FX) return;
+        if (!bricks) return;
         std::vector<uint32_t> h(Np);
         for (long i = 0; i < Np; ++i) h[i] = (uint32_t)i;
         CPM_HIP(hipMemcpyAsync(id.p, h.data(), sizeof(uint32_t) * Np, hipMemcpyHostToDevice, st));
@@ -303,7 +305,7 @@ struct Impl : ImplBase {
     // Mass-grid index of each stored particle.
     std::vector<uint32_t> host_ids() {
         std::vector<uint32_t> h(Np);
-        if (FX) {
+        if (bricks) {
             CPM_HIP(hipMemcpyAsync(h.data(), id.p, sizeof(uint32_t) * Np, hipMemcpyDeviceToHost, st));
             CPM_HIP(hipStreamSynchronize(st));
         } else {
@@ -441,22 +443,27 @@ struct Impl : ImplBase {
     void fx_sort_current() {
         if (!keys_ready) {
             const FxKeyArgs ka = fx_key_args();
-            keys_fx(R, reinterpret_cast<const uint32_t*>(X.p), Np, lgNf, pa, ka.prevkeys, ka.keys, ka.pids,
-                    ka.in_count, ka.out_count, st);
+            if constexpr (FX) {
+                keys_fx(R, reinterpret_cast<const uint32_t*>(X.p), Np, lgNf, pa, ka.prevkeys, ka.keys, ka.pids,
+                        ka.in_count, ka.out_count, st);
+            } else {
+                keys_f64_brick(R, reinterpret_cast<const double*>(X.p), Np, lgNf, pa, ka.prevkeys, ka.keys, ka.pids,
+                               ka.in_count, ka.out_count, st);
+            }
         }
         fx_sort();
     }
 
     void do_deposit(Mesh m) {
         prof.begin(0, st);
-        if constexpr (FX) {
+        if (bricks) {
             fx_sort_current();
         } else {
             ParticleOps<PP>::keys(R, X.p, Np, lgNf, pa, k0.p, v0.p, st);
             const int where = sorter.sort(k0.p, v0.p, k1.p, v1.p, st);
             skeys = where ? k1.p : k0.p;
             spids = where ? v1.p : v0.p;
-            row_bounds(skeys, Np, pa.rowkey ? 0 : lgNf, nsort_groups, row_start.p, st);
+            row_bounds(skeys, Np, lgNf, nsort_groups, row_start.p, st);
             bstart = row_start.p;
         }
         prof.end(0, st);
@@ -464,9 +471,13 @@ struct Impl : ImplBase {
         if constexpr (FX) {
             deposit_fx_brick(R, bstart, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p, st);
             npartials = brick_count(pa);
+        } else if (bricks) {
+            deposit_f64_brick(R, bstart, spids, reinterpret_cast<const double*>(X.p), Np, lgNf, pa, (double)m,
+                              reinterpret_cast<double*>(mesh.p), row_partial.p, st);
+            npartials = brick_count(pa);
         } else {
             ParticleOps<PP>::deposit(R, row_start.p, skeys, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p,
-                                     nrows, frac.p, xsort.p, inv.p, (cfg.flags & PM_FLAG_SERIAL_DEPOSIT) != 0, st);
+                                     nrows, frac.p, xsort.p, inv.p, true, st);
             npartials = nrows;
         }
         prof.end(1, st);
@@ -525,8 +536,21 @@ struct Impl : ImplBase {
             std::swap(id.p, id2.p);
             resort_ok = true;
             keys_ready = fuse;
+        } else if (bricks) {
+            // the same with the FP64 tile and the reference's arithmetic
+            if (fuse) zero_u32(cnt3.p, 3 * nsort_groups, st);
+            gather_kick_f64_brick(R, bstart, spids, reinterpret_cast<const double*>(X.p), P.p, id.p,
+                                  reinterpret_cast<double*>(X2.p), P2.p, id2.p, Np,
+                                  reinterpret_cast<const double*>(phi_dst()), lgNf, pa, ck, cd, fuse ? 1 : 0,
+                                  fuse ? knew.p : nullptr, cnt3.p, cnt3.p + nsort_groups, defer.p, dpar, st);
+            dpar ^= 1;
+            std::swap(X.p, X2.p);
+            std::swap(P.p, P2.p);
+            std::swap(id.p, id2.p);
+            resort_ok = true;
+            keys_ready = fuse;
         } else {
-            // forces in the deposit's cell order, then the update in storage order
+            // serial-order deposit: forces in its cell order, then the update in storage order
             ParticleOps<PP>::gather_kick_sorted(R, X.p, P.p, Np, phi_dst(), lgNf, pa, ck, cd, fuse ? 1 : 0, xsort.p,
                                                 fsort.p, inv.p, st);
         }
@@ -608,7 +632,7 @@ struct Impl : ImplBase {
         if (Xh) check_finite(Xh, (size_t)R * Np, "positions");
         if (Ph) check_finite(Ph, (size_t)R * Np, "momenta");
         if (Xh) {
-            if (FX && !Ph && have_state) {
+            if (bricks && !Ph && have_state) {
                 // the kept momenta follow the storage order of the last step;
                 // the new positions come in mass-grid order, which reset_ids
                 // makes the storage order: put the momenta there first
@@ -812,7 +836,7 @@ struct Impl : ImplBase {
         // would differ from run to run in its last bits.
         const double* Pd = P.p;
         DevBuf<double> byid;
-        if (FX) {
+        if (bricks) {
             byid.alloc((size_t)R * Np);
             scatter_by_id<double>(P.p, id.p, Np, R, byid.p, st);
             Pd = byid.p;

@@ -438,56 +438,165 @@ __device__ __forceinline__ void force_fx(const float* a1, Ld ld, float inv12res,
     }
 }
 
+// FP64 force at one particle: the trilinear interpolation (weights from the
+// fractions a1) of the 4-point FD gradient at the 2^R nodes around its origin
+// o (src/run/numeric.hh:17-44,86-102), in the reference's operation order:
+// corner weights z_j = prod_k A[bit_k(j)][k], F_0 summed over the corners in
+// order as its lines are read, F_1 and F_2 from per-corner gradients summed
+// in corner order.  ld(tx, ty, tz) returns the potential at node
+// o - 2 + (tx, ty, tz) (tz ignored in 2-D): from the mesh (MeshNodes) or
+// from a brick's LDS tile, with the same operations either way.
+template <int R, typename Ld>
+__device__ __forceinline__ void force_f64(const double* a1, Ld ld, double res, double* F) {
+    constexpr int NC = 1 << R;
+    double z[NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+        double w = 1.0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) w *= ((j >> k) & 1) ? a1[k] : 1.0 - a1[k];
+        z[j] = w;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) F[k] = 0;
+#pragma unroll
+    for (int l = 0; l < NC / 2; ++l) {
+        const int by = l & 1, bz = (l >> 1) & 1;
+        double f[6];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) f[t] = ld(t, 2 + by, 2 + bz);
+#pragma unroll
+        for (int bx = 0; bx < 2; ++bx) {
+            const int j = bx | (by << 1) | (bz << 2);
+            const double g = fd4_div3(f[bx], f[bx + 1], f[bx + 3], f[bx + 4], res);
+            F[0] += g * z[j];
+        }
+    }
+    {
+        double g1[NC];
+#pragma unroll
+        for (int l = 0; l < NC / 2; ++l) {
+            const int bx = l & 1, bz = (l >> 1) & 1;
+            double f[6];
+#pragma unroll
+            for (int t = 0; t < 6; ++t) f[t] = ld(2 + bx, t, 2 + bz);
+#pragma unroll
+            for (int by = 0; by < 2; ++by) g1[bx | (by << 1) | (bz << 2)] = fd4_div3(f[by], f[by + 1], f[by + 3], f[by + 4], res);
+        }
+#pragma unroll
+        for (int j = 0; j < NC; ++j) F[1] += g1[j] * z[j];
+    }
+    if constexpr (R == 3) {
+        double g2[NC];
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            const int bx = l & 1, by = (l >> 1) & 1;
+            double f[6];
+#pragma unroll
+            for (int t = 0; t < 6; ++t) f[t] = ld(2 + bx, 2 + by, t);
+#pragma unroll
+            for (int bz = 0; bz < 2; ++bz) g2[bx | (by << 1) | (bz << 2)] = fd4_div3(f[bz], f[bz + 1], f[bz + 3], f[bz + 4], res);
+        }
+#pragma unroll
+        for (int j = 0; j < NC; ++j) F[2] += g2[j] * z[j];
+    }
+}
+
 // The potential at node o - 2 + (tx, ty, tz) of an N^R mesh, wrapped.
-template <int R>
+template <int R, typename T = float>
 struct MeshNodes {
-    const float* __restrict__ phi;
+    const T* __restrict__ phi;
     int o0, o1, o2, lgN, msk;
-    __device__ __forceinline__ float operator()(int tx, int ty, int tz) const {
+    __device__ __forceinline__ T operator()(int tx, int ty, int tz) const {
         uint32_t idx = (uint32_t)((o0 - 2 + tx) & msk) + ((uint32_t)((o1 - 2 + ty) & msk) << lgN);
         if (R == 3) idx += (uint32_t)((o2 - 2 + tz) & msk) << (2 * lgN);
         return phi[idx];
     }
 };
 
+// The per-particle steps of the brick gathers, for both position forms.
+// FX: the kick adds F ck rounded to the momentum grid (ckq = ck / pq), the
+// drift is integer; FP64: the reference's P - F ck and X + P cd (ckq = ck,
+// pq unused).  The sort cell of a position is the deposit's (PP::dep).
+template <typename PP>
+__device__ __forceinline__ double brick_kick(double p, double F, double ckq, double pq) {
+    if constexpr (std::is_same<PP, PosFX>::value) {
+        return p - fx_kick_increment(F, ckq, pq);
+    } else {
+        return p - F * ckq;
+    }
+}
+template <typename PP>
+__device__ __forceinline__ typename PP::Pos brick_drift(typename PP::Pos x, double p, double cd) {
+    if constexpr (std::is_same<PP, PosFX>::value) {
+        return x + fx_round_u32(p * cd);
+    } else {
+        return x + p * cd;
+    }
+}
+template <typename PP>
+__device__ __forceinline__ int brick_cell(typename PP::Pos x, const PosArgs& pa, int msk) {
+    if constexpr (std::is_same<PP, PosFX>::value) {
+        return (int)(x >> pa.fshift) & msk;
+    } else {
+        int o;
+        double f;
+        PosF64::dep(x, pa, o, f);
+        return o & msk;
+    }
+}
+// The force of one particle from node values ld (force_fx or force_f64).
+template <typename PP, int R, typename Ld>
+__device__ __forceinline__ void brick_force(const typename PP::Mesh* a1, Ld ld, const PosArgs& pa,
+                                            typename PP::Mesh* F) {
+    if constexpr (std::is_same<PP, PosFX>::value) {
+        force_fx<R>(a1, ld, (float)(1.0 / (12.0 * pa.res)), F);
+    } else {
+        force_f64<R>(a1, ld, pa.res, F);
+    }
+}
+
 // Kick (and drift, with the new sort key and mover counts) of one particle
-// whose force k_gather_kick_fx_brick cannot take from its tile: input index
+// whose force k_gather_kick_brick cannot take from its tile: input index
 // i, sorted position s, the force read from the mesh by the same operations
-// as in the tile (force_fx), the rest as in the tile kernel.
-template <int R>
-__device__ __forceinline__ void kick_from_mesh(uint32_t s, uint32_t i, const uint32_t* __restrict__ X,
-                                               const double* __restrict__ P, uint32_t* __restrict__ Xo,
-                                               double* __restrict__ Po, long np, const float* __restrict__ phi,
-                                               int lgN, const PosArgs& pa, double ckq, double pq, double cd, int cd_on,
+// as in the tile (brick_force), the rest as in the tile kernel.
+template <typename PP, int R>
+__device__ __forceinline__ void kick_from_mesh(uint32_t s, uint32_t i, const typename PP::Pos* __restrict__ X,
+                                               const double* __restrict__ P, typename PP::Pos* __restrict__ Xo,
+                                               double* __restrict__ Po, long np,
+                                               const typename PP::Mesh* __restrict__ phi, int lgN, const PosArgs& pa,
+                                               double ckq, double pq, double cd, int cd_on,
                                                uint32_t* __restrict__ keys, uint32_t* __restrict__ in_count,
                                                uint32_t* __restrict__ out_count) {
+    using Pos = typename PP::Pos;
+    using T = typename PP::Mesh;
     const int msk = (1 << lgN) - 1;
-    uint32_t xs[3] = {0u, 0u, 0u};
+    Pos xs[3] = {0, 0, 0};
     double pv[3] = {0, 0, 0};
     int o[3] = {0, 0, 0};
-    float a1[3] = {0.f, 0.f, 0.f}, F[3];
+    T a1[3] = {0, 0, 0}, F[3];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         xs[k] = X[k * np + i];
         pv[k] = P[k * np + i];
     }
 #pragma unroll
-    for (int k = 0; k < R; ++k) PosFX::gat(xs[k], pa, o[k], a1[k]);
-    force_fx<R>(a1, MeshNodes<R>{phi, o[0], o[1], o[2], lgN, msk}, (float)(1.0 / (12.0 * pa.res)), F);
-    uint32_t xn[3] = {0u, 0u, 0u};
+    for (int k = 0; k < R; ++k) PP::gat(xs[k], pa, o[k], a1[k]);
+    brick_force<PP, R>(a1, MeshNodes<R, T>{phi, o[0], o[1], o[2], lgN, msk}, pa, F);
+    Pos xn[3] = {0, 0, 0};
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-        const double p = pv[k] - fx_kick_increment((double)F[k], ckq, pq);
+        const double p = brick_kick<PP>(pv[k], (double)F[k], ckq, pq);
         Po[k * np + s] = p;
-        xn[k] = cd_on ? xs[k] + fx_round_u32(p * cd) : xs[k];
+        xn[k] = cd_on ? brick_drift<PP>(xs[k], p, cd) : xs[k];
         Xo[k * np + s] = xn[k];
     }
     if (keys) {
         int co[3] = {0, 0, 0}, cn[3] = {0, 0, 0};
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            co[k] = (int)(xs[k] >> pa.fshift) & msk;
-            cn[k] = (int)(xn[k] >> pa.fshift) & msk;
+            co[k] = brick_cell<PP>(xs[k], pa, msk);
+            cn[k] = brick_cell<PP>(xn[k], pa, msk);
         }
         const uint32_t ko = brick_key(co[0], co[1], co[2], R, pa), kn = brick_key(cn[0], cn[1], cn[2], R, pa);
         keys[s] = kn;
@@ -519,29 +628,33 @@ __device__ __forceinline__ void kick_from_mesh(uint32_t s, uint32_t i, const uin
 // its new group (in_count) and out of its old one (out_count): what keys_fx
 // computes for an incremental re-sort, whose previous keys are those of the
 // positions read here.
-template <int R>
-__global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
-    const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ X,
-    const double* __restrict__ P, const uint32_t* __restrict__ idi, uint32_t* __restrict__ Xo,
-    double* __restrict__ Po, uint32_t* __restrict__ ido, long np, const float* __restrict__ phi, int lgN,
+template <typename PP, int R, int NT>
+__global__ __launch_bounds__(NT) void k_gather_kick_brick(
+    const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ perm, const typename PP::Pos* __restrict__ X,
+    const double* __restrict__ P, const uint32_t* __restrict__ idi, typename PP::Pos* __restrict__ Xo,
+    double* __restrict__ Po, uint32_t* __restrict__ ido, long np, const typename PP::Mesh* __restrict__ phi, int lgN,
     PosArgs pa, double ckq, double pq, double cd, int cd_on, int lseg, uint32_t* __restrict__ keys,
     uint32_t* __restrict__ in_count, uint32_t* __restrict__ out_count, uint32_t* __restrict__ defer, int qpar) {
+    using Pos = typename PP::Pos;
+    using T = typename PP::Mesh;                  // potential (float or double)
+    using BD = BrickDims<PP>;
+    constexpr int DW = (int)(sizeof(T) / 4);      // dwords per cell
     constexpr int LA = R - 1;                     // the axis the tile slides along
-    constexpr int BL = (R == 3) ? FXB_Z : FXB_X;  // brick extent along it, at most
-    constexpr int PSMAX = (R == 3) ? (FXB_X + 5) * (FXB_Y + 5) : FXB_X + 5;
+    constexpr int BL = (R == 3) ? BD::Z : BD::X2;   // brick extent along it, at most
+    constexpr int PSMAX = (R == 3) ? (BD::X + 5) * (BD::Y + 5) : BD::X2 + 5;
     // The guard keeps the ring at least 256 bytes above LDS address 0, so the
     // base of the second part of a straddling wave (up to 63 cells before the
     // ring) is a valid address.
     constexpr int DQ = 512;  // LDS queue of particles whose force is read from the mesh
     struct Lds {
         float guard[64];
-        float ring[(2 * BL + 5) * PSMAX];
+        T ring[(2 * BL + 5) * PSMAX];
         uint32_t dq[DQ];
         uint32_t dqn;
     };
     __shared__ Lds sm;
     static_assert(offsetof(Lds, dq) <= 65536, "the ring must stay addressable below 64 KiB");
-    float* const ring = sm.ring;
+    T* const ring = sm.ring;
     const int N = 1 << lgN, msk = N - 1;
     const int tid = threadIdx.x;
     const int lcol = pa.lnb[0] + ((R == 3) ? pa.lnb[1] : 0);
@@ -566,25 +679,28 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
         return q >= TR ? q - TR : q;
     };
     // (index products in 24-bit multiplies: every factor is below 2^12)
-    auto T = [&](int tx, int ty, int tz) -> float {
+    auto Tn = [&](int tx, int ty, int tz) -> T {
         return (R == 3) ? ring[__umul24(PS, slot(tz)) + tx + __umul24(TX, ty)] : ring[__umul24(PS, slot(ty)) + tx];
     };
     // Planes p0 .. p0 + npl - 1 of the tile of the brick at last-axis origin
     // zb, into slots (p + rp) mod TR.  Cell e of the run (plane e / PS) goes
     // to LDS slot-start * PS + e before the wrap and e - E after it.
+    // (In dwords: a cell is DW of them, dword d of the run belongs to cell
+    // d / DW, and half d mod DW = lane mod DW of it.)
     auto load_planes = [&](int p0, int npl, int zb, int rp) {
-        constexpr int NLMAX = ((BL + 5) * PSMAX + FXG_NT - 1) / FXG_NT;
-        const int ne = npl * PS;
+        constexpr int NLMAX = ((BL + 5) * PSMAX * DW + NT - 1) / NT;
+        const int nd = npl * PS * DW;
         int start = p0 + rp;
         start = start >= TR ? start - TR : start;
-        const int E = (TR - start) * PS;  // first cell whose plane wraps to slot 0
+        const int E = (TR - start) * PS * DW;  // first dword whose plane wraps to slot 0
         const int wbase = tid & ~63, lane = tid & 63;
 #pragma unroll 1
         for (int l = 0; l < NLMAX; ++l) {
-            const int e0 = wbase + l * FXG_NT;  // the wave's first cell
-            if (e0 >= ne) break;
-            const int e = e0 + lane;
-            if (e < ne) {
+            const int d0 = wbase + l * NT;  // the wave's first dword
+            if (d0 >= nd) break;
+            const int d = d0 + lane;
+            if (d < nd) {
+                const int e = d / DW;
                 const int q = (int)__umulhi((uint32_t)e, psinv), r = e - q * PS;
                 const int ty = (R == 3) ? (int)__umulhi((uint32_t)r, txinv) : 0;
                 const int tx = r - ty * TX;
@@ -592,53 +708,55 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
                 uint32_t off = (uint32_t)((bc.x0[0] - 2 + tx) & msk);
                 off += (R == 3) ? (uint32_t)N * ((uint32_t)((bc.x0[1] - 2 + ty) & msk) + (uint32_t)N * gl)
                                 : (uint32_t)N * gl;
-                if (e < E)
-                    lds_dma_dword(phi + off, ring_lds + 4u * (uint32_t)(start * PS + e0));
+                const float* src = reinterpret_cast<const float*>(phi + off) + (d & (DW - 1));
+                if (d < E)
+                    lds_dma_dword(src, ring_lds + 4u * (uint32_t)(start * PS * DW + d0));
                 else
-                    lds_dma_dword(phi + off, ring_lds + 4u * (uint32_t)(e0 - E));
+                    lds_dma_dword(src, ring_lds + 4u * (uint32_t)(d0 - E));
             }
         }
     };
-    // The next brick's B new planes: cell e = wbase + l FXG_NT + lane of the
+    // The next brick's B new planes: cell e = wbase + l NT + lane of the
     // run (plane e / PS) has the same in-plane mesh offset for every brick of
     // the segment, so it is computed once, packed as (plane << 27) | offset
     // (~0u: no cell).
-    constexpr int NLN = (BL * PSMAX + FXG_NT - 1) / FXG_NT;
+    constexpr int NLN = (BL * PSMAX * DW + NT - 1) / NT;
     const int wbase = __builtin_amdgcn_readfirstlane(tid & ~63), lane = tid & 63;
     const int lgP = (R == 3) ? 2 * lgN : lgN;  // log2 of the mesh stride between planes
     uint32_t noff[NLN];
 #pragma unroll
     for (int l = 0; l < NLN; ++l) {
-        const int e = wbase + l * FXG_NT + lane;
+        const int d = wbase + l * NT + lane, e = d / DW;
         const int q = (int)__umulhi((uint32_t)e, psinv), r = e - q * PS;
         const int ty = (R == 3) ? (int)__umulhi((uint32_t)r, txinv) : 0;
         const int tx = r - ty * TX;
         uint32_t xy = (uint32_t)((bc.x0[0] - 2 + tx) & msk);
         if (R == 3) xy += (uint32_t)N * (uint32_t)((bc.x0[1] - 2 + ty) & msk);
-        noff[l] = (e < B * PS) ? ((uint32_t)q << 27) | xy : ~0u;
+        noff[l] = (d < B * PS * DW) ? ((uint32_t)q << 27) | xy : ~0u;
     }
     // the next brick's new planes (last-axis origin zb) into slots
     // (p + rp) mod TR, p = 5 .. TP - 1
     auto load_new_planes = [&](int zb, int rp) {
         int start = 5 + rp;
         start = start >= TR ? start - TR : start;
-        const int E = (TR - start) * PS;  // first cell whose plane wraps to slot 0
+        const int E = (TR - start) * PS * DW;  // first dword whose plane wraps to slot 0
 #pragma unroll
         for (int l = 0; l < NLN; ++l) {
             const uint32_t v = noff[l];
             if (v != ~0u) {
                 const uint32_t gl = (uint32_t)((zb + 3 + (int)(v >> 27)) & msk);
                 const uint32_t off = (v & 0x7FFFFFFu) + (gl << lgP);
-                const int e0 = wbase + l * FXG_NT;
-                if (e0 + lane < E)
-                    lds_dma_dword(phi + off, ring_lds + 4u * (uint32_t)(start * PS + e0));
+                const int d0 = wbase + l * NT;
+                const float* src = reinterpret_cast<const float*>(phi + off) + (lane & (DW - 1));
+                if (d0 + lane < E)
+                    lds_dma_dword(src, ring_lds + 4u * (uint32_t)(start * PS * DW + d0));
                 else
-                    lds_dma_dword(phi + off, ring_lds + 4u * (uint32_t)(e0 - E));
+                    lds_dma_dword(src, ring_lds + 4u * (uint32_t)(d0 - E));
             }
         }
     };
-    // Particles are processed in rounds of FXG_NT (round r: sorted position
-    // s0 + r FXG_NT + tid); a thread's particle of round r + 1 is loaded at
+    // Particles are processed in rounds of NT (round r: sorted position
+    // s0 + r NT + tid); a thread's particle of round r + 1 is loaded at
     // the end of round r, its first one of the next brick before the barrier
     // that ends the brick.  The next brick's planes are issued at the start of
     // the last round, after a wait for everything loaded so far: no load the
@@ -651,7 +769,8 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
         nn0 = bstart[(bid + cstride) << R];
         nn1 = bstart[(bid + cstride + 1) << R];
     }
-    uint32_t xs[3] = {0u, 0u, 0u}, idv = 0u;
+    Pos xs[3] = {0, 0, 0};
+    uint32_t idv = 0u;
     double pv[3] = {0, 0, 0};
     auto fetch = [&](uint32_t i) {
         idv = idi[i];
@@ -666,20 +785,20 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
     load_planes(0, TP, bc.x0[LA], 0);
     __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
     __syncthreads();
-    const float inv12res = (float)(1.0 / (12.0 * pa.res));
     auto process = [&](uint32_t s) {
         ido[s] = idv;
         int o[3] = {0, 0, 0};
-        float a1[3] = {0.f, 0.f, 0.f};
+        T a1[3] = {0, 0, 0};
 #pragma unroll
-        for (int k = 0; k < R; ++k) PosFX::gat(xs[k], pa, o[k], a1[k]);
+        for (int k = 0; k < R; ++k) PP::gat(xs[k], pa, o[k], a1[k]);
         // tile coordinates of the stencil origin (cell o - 2 along each axis)
         const int lx = (o[0] - bc.x0[0]) & msk, ly = (o[1] - bc.x0[1]) & msk;
         const int lz = (R == 3) ? ((o[2] - bc.x0[2]) & msk) : 0;
         // A particle whose origin is the cell above its own (a negative
-        // unwrapped coordinate, see PosFX::gat) on the brick's upper face
-        // needs one plane beyond the tile: it is queued, and kicked from the
-        // mesh when the segment is done (kick_from_mesh).
+        // unwrapped coordinate, see PosFX::gat; FP64: also a rounding of the
+        // unwrapped coordinate that differs from the folded one's) on the
+        // brick's upper face needs one plane beyond the tile: it is queued,
+        // and kicked from the mesh when the segment is done (kick_from_mesh).
         if (lx >= bc.B[0] || ly >= bc.B[1] || (R == 3 && lz >= bc.B[2])) {
             const uint32_t q = atomicAdd(&sm.dqn, 1u);
             if (q < (uint32_t)DQ)
@@ -688,22 +807,22 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
                 defer[2 + atomicAdd(&defer[qpar], 1u)] = s;  // for k_gather_deferred
             return;
         }
-        float F[3];
-        force_fx<R>(a1, [&](int tx, int ty, int tz) { return T(lx + tx, ly + ty, lz + tz); }, inv12res, F);
-        uint32_t xn[3] = {0u, 0u, 0u};
+        T F[3];
+        brick_force<PP, R>(a1, [&](int tx, int ty, int tz) { return Tn(lx + tx, ly + ty, lz + tz); }, pa, F);
+        Pos xn[3] = {0, 0, 0};
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            const double p = pv[k] - fx_kick_increment((double)F[k], ckq, pq);
+            const double p = brick_kick<PP>(pv[k], (double)F[k], ckq, pq);
             Po[k * np + s] = p;
-            xn[k] = cd_on ? xs[k] + fx_round_u32(p * cd) : xs[k];
+            xn[k] = cd_on ? brick_drift<PP>(xs[k], p, cd) : xs[k];
             Xo[k * np + s] = xn[k];
         }
         if (keys) {
             int co[3] = {0, 0, 0}, cn[3] = {0, 0, 0};
 #pragma unroll
             for (int k = 0; k < R; ++k) {
-                co[k] = (int)(xs[k] >> pa.fshift) & msk;
-                cn[k] = (int)(xn[k] >> pa.fshift) & msk;
+                co[k] = brick_cell<PP>(xs[k], pa, msk);
+                cn[k] = brick_cell<PP>(xn[k], pa, msk);
             }
             const uint32_t ko = brick_key(co[0], co[1], co[2], R, pa), kn = brick_key(cn[0], cn[1], cn[2], R, pa);
             keys[s] = kn;
@@ -716,7 +835,7 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
     const uint32_t plast = (uint32_t)(np - 1);
     for (int j = 0;; ++j) {
         const bool more = j + 1 < nseg;
-        const int nr = (int)((s1 - s0 + FXG_NT - 1) / FXG_NT);
+        const int nr = (int)((s1 - s0 + NT - 1) / NT);
         uint32_t npi;
         // (npi is loaded unconditionally: a value merged from two paths would
         // be copied, and the copy would wait for the load at once)
@@ -732,15 +851,15 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
         };
         // all rounds but the last: every thread has a particle
         for (int r = 0; r + 1 < nr; ++r) {
-            const uint32_t s = s0 + (uint32_t)r * FXG_NT + tid;
+            const uint32_t s = s0 + (uint32_t)r * NT + tid;
             process(s);
-            if (s + FXG_NT < s1) fetch(perm[s + FXG_NT]);
+            if (s + NT < s1) fetch(perm[s + NT]);
         }
         // the last round, a separate copy of the body so that the compiler
         // sees no outstanding load of its own in it
         issue_next();
         if (nr > 0) {
-            const uint32_t s = s0 + (uint32_t)(nr - 1) * FXG_NT + tid;
+            const uint32_t s = s0 + (uint32_t)(nr - 1) * NT + tid;
             if (s < s1) process(s);
         }
         if (!more) break;
@@ -761,23 +880,26 @@ __global__ __launch_bounds__(FXG_NT) void k_gather_kick_fx_brick(
     // the queued particles of the segment, from the mesh
     __syncthreads();
     const uint32_t nq = sm.dqn < (uint32_t)DQ ? sm.dqn : (uint32_t)DQ;
-    for (uint32_t q = tid; q < nq; q += FXG_NT) {
+    for (uint32_t q = tid; q < nq; q += NT) {
         const uint32_t s = sm.dq[q];
-        kick_from_mesh<R>(s, perm[s], X, P, Xo, Po, np, phi, lgN, pa, ckq, pq, cd, cd_on, keys, in_count, out_count);
+        kick_from_mesh<PP, R>(s, perm[s], X, P, Xo, Po, np, phi, lgN, pa, ckq, pq, cd, cd_on, keys, in_count,
+                              out_count);
     }
 }
 
-// The particles that k_gather_kick_fx_brick's workgroups could not queue in
+// The particles that k_gather_kick_brick's workgroups could not queue in
 // LDS (defer[qpar] of them, sorted positions from defer[2]), kicked from the
 // mesh.  The two counts alternate between launches: this one clears the
 // other, which the next gather fills (the previous launch's queue is done
 // with by now).
-template <int R>
+template <typename PP, int R>
 __global__ __launch_bounds__(256) void k_gather_deferred(uint32_t* __restrict__ defer, int qpar,
                                                           const uint32_t* __restrict__ perm,
-                                                          const uint32_t* __restrict__ X, const double* __restrict__ P,
-                                                          uint32_t* __restrict__ Xo, double* __restrict__ Po, long np,
-                                                          const float* __restrict__ phi, int lgN, PosArgs pa,
+                                                          const typename PP::Pos* __restrict__ X,
+                                                          const double* __restrict__ P,
+                                                          typename PP::Pos* __restrict__ Xo, double* __restrict__ Po,
+                                                          long np, const typename PP::Mesh* __restrict__ phi, int lgN,
+                                                          PosArgs pa,
                                                           double ckq, double pq, double cd, int cd_on,
                                                           uint32_t* __restrict__ keys, uint32_t* __restrict__ in_count,
                                                           uint32_t* __restrict__ out_count) {
@@ -785,8 +907,116 @@ __global__ __launch_bounds__(256) void k_gather_deferred(uint32_t* __restrict__ 
     if (blockIdx.x == 0 && threadIdx.x == 0) defer[qpar ^ 1] = 0u;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
         const uint32_t s = defer[2 + q];
-        kick_from_mesh<R>(s, perm[s], X, P, Xo, Po, np, phi, lgN, pa, ckq, pq, cd, cd_on, keys, in_count, out_count);
+        kick_from_mesh<PP, R>(s, perm[s], X, P, Xo, Po, np, phi, lgN, pa, ckq, pq, cd, cd_on, keys, in_count,
+                              out_count);
     }
+}
+
+// ------------------------------------------------------------ FP64 brick order
+static constexpr int F64B_NT = 256;  // FP64 brick deposit workgroup size
+static constexpr int F64G_NT = 256;  // FP64 brick gather workgroup size
+static constexpr int F64B_CELLS = F64B_X * F64B_Y * F64B_Z;
+static_assert(F64B_X2 * F64B_X2 == F64B_CELLS, "2-D and 3-D FP64 bricks hold the same number of cells");
+
+// FP64 brick keys (keys_fx for FP64 positions: the brick of the deposit's cell).
+template <int R>
+__global__ __launch_bounds__(256) void k_keys_f64_brick(const double* __restrict__ X, long np, int lgN, PosArgs pa,
+                                                        const uint32_t* __restrict__ prevkeys,
+                                                        uint32_t* __restrict__ keys, uint32_t* __restrict__ pids,
+                                                        uint32_t* __restrict__ in_count,
+                                                        uint32_t* __restrict__ out_count) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np) return;
+    const int mask = (1 << lgN) - 1;
+    int oc[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < R; ++k) oc[k] = brick_cell<PosF64>(X[k * np + s], pa, mask);
+    const uint32_t key = brick_key(oc[0], oc[1], oc[2], R, pa);
+    keys[s] = key;
+    fx_note_key(key, s, prevkeys, pids, in_count, out_count);
+}
+
+// FP64 deposit in exact fixed point, one workgroup per brick: the particles
+// of the brick and the face classes of its lower neighbours (brick_ranges),
+// each adding the weights of its 2^R corners that land in the brick.  The
+// weights and their 128-bit fixed-point split are k_deposit_f64_fx's
+// (z = ((m A_x) A_y) A_z, numeric.hh:124-131), and the cell sums are exact,
+// so every cell is the row deposit's, bit for bit.
+template <int R>
+__global__ __launch_bounds__(F64B_NT) void k_deposit_f64_brick(const uint32_t* __restrict__ bstart,
+                                                               const uint32_t* __restrict__ perm,
+                                                               const double* __restrict__ X, long np, int lgN,
+                                                               PosArgs pa, double mass, double* __restrict__ mesh,
+                                                               double* __restrict__ partial) {
+    __shared__ unsigned long long alo[F64B_CELLS], ahi[F64B_CELLS];
+    __shared__ double red[F64B_NT / 64];
+    __shared__ BrickRanges rg;
+    const int N = 1 << lgN, msk = N - 1;
+    const uint32_t bid = blockIdx.x;
+    const BrickCoord bc = brick_coord(bid, pa);
+    const int ncell = bc.B[0] * bc.B[1] * bc.B[2];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < ncell; e += F64B_NT) alo[e] = ahi[e] = 0ull;
+    brick_ranges<R>(rg, bstart, bc, pa, tid);
+    const uint32_t total = rg.roff[rg.n];
+    // two particles per thread at a time: their perm and position loads are
+    // issued before the first is used
+    constexpr int BATCH = 2;
+    for (uint32_t t0 = tid; t0 < total; t0 += BATCH * F64B_NT) {
+        uint32_t pi[BATCH];
+#pragma unroll
+        for (int q = 0; q < BATCH; ++q) {
+            const uint32_t t = t0 + q * F64B_NT;
+            pi[q] = t < total ? perm[brick_range_pos(rg, t)] : 0u;
+        }
+        double xq[BATCH][3];
+#pragma unroll
+        for (int q = 0; q < BATCH; ++q)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) xq[q][k] = (k < R && t0 + q * F64B_NT < total) ? X[k * np + pi[q]] : 0.0;
+#pragma unroll
+        for (int q = 0; q < BATCH; ++q) {
+            if (t0 + q * F64B_NT >= total) break;
+            int o[3] = {0, 0, 0};
+            double f[3] = {0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                PosF64::dep(xq[q][k], pa, o[k], f[k]);
+                o[k] &= msk;
+            }
+#pragma unroll
+            for (int j = 0; j < (1 << R); ++j) {
+                int l[3];
+                bool in = true;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    l[k] = (k < R) ? (((o[k] + ((j >> k) & 1)) - bc.x0[k]) & msk) : 0;
+                    in &= l[k] < bc.B[k];
+                }
+                if (!in) continue;
+                double z = mass;
+                z *= (j & 1) ? f[0] : 1 - f[0];
+                z *= (j & 2) ? f[1] : 1 - f[1];
+                if (R == 3) z *= (j & 4) ? f[2] : 1 - f[2];
+                unsigned long long hi, lo;
+                fx128_split(z, hi, lo);
+                const int c = l[0] + bc.B[0] * (l[1] + bc.B[1] * l[2]);
+                const unsigned long long old = atomicAdd(&alo[c], lo);
+                atomicAdd(&ahi[c], hi + (old + lo < old ? 1ull : 0ull));
+            }
+        }
+    }
+    __syncthreads();
+    double rsum = 0;
+    for (int e = tid; e < ncell; e += F64B_NT) {
+        const int lx = e & (bc.B[0] - 1), ly = (e >> pa.lb[0]) & (bc.B[1] - 1), lz = e >> (pa.lb[0] + pa.lb[1]);
+        const long idx = (long)(bc.x0[0] + lx) + (long)N * ((bc.x0[1] + ly) + (long)N * (bc.x0[2] + lz));
+        const double v = fx128_value(ahi[e], alo[e]);
+        mesh[idx] = v;
+        rsum += v;
+    }
+    const double tot = block_sum<F64B_NT>(rsum, red, tid);
+    if (tid == 0) partial[bid] = tot;
 }
 
 // ------------------------------------------------------------ gather + kick (+ drift)
@@ -805,7 +1035,6 @@ __device__ __forceinline__ T load_mesh(const T* __restrict__ phi, long idx) {
 template <typename PP, int R>
 __device__ __forceinline__ void gather_force(const typename PP::Pos* xs, const typename PP::Mesh* __restrict__ phi,
                                              int lgN, const PosArgs& pa, typename PP::Mesh* F) {
-    using M = typename PP::Mesh;
     const int N = 1 << lgN, msk = N - 1;
     if constexpr (std::is_same<PP, PosFX>::value) {
         // the brick kernel's operations (force_fx), from the mesh
@@ -816,84 +1045,15 @@ __device__ __forceinline__ void gather_force(const typename PP::Pos* xs, const t
         force_fx<R>(af, MeshNodes<R>{phi, of[0], of[1], of[2], lgN, msk}, (float)(1.0 / (12.0 * pa.res)), Ff);
 #pragma unroll
         for (int k = 0; k < R; ++k) F[k] = Ff[k];
-        return;
-    }
-    int o[R];
-    M a1[R];
+    } else {
+        int o[3] = {0, 0, 0};
+        double a1[3] = {0, 0, 0};
 #pragma unroll
-    for (int k = 0; k < R; ++k) PP::gat(xs[k], pa, o[k], a1[k]);
-    // corner weights z_j = prod_k A[bit_k(j)][k] in the reference's order
-    constexpr int NC = 1 << R;
-    M z[NC];
-#pragma unroll
-    for (int j = 0; j < NC; ++j) {
-        M w = (M)1;
-#pragma unroll
-        for (int k = 0; k < R; ++k) w *= ((j >> k) & 1) ? a1[k] : (M)1 - a1[k];
-        z[j] = w;
-    }
-    // wrapped node coordinates o + t - 2, t = 0..5
-    int w0[6], w1[6], w2[6];
-#pragma unroll
-    for (int t = 0; t < 6; ++t) {
-        w0[t] = (o[0] + t - 2) & msk;
-        w1[t] = (o[1] + t - 2) & msk;
-        w2[t] = (R == 3) ? ((o[2] + t - 2) & msk) : 0;
-    }
-    const long sN = N, sNN = (long)N * N;
-    M resv;
-    if constexpr (sizeof(M) == 8) resv = (M)pa.res;
-    else resv = (M)(1.0 / (12.0 * pa.res));
-#pragma unroll
-    for (int k = 0; k < R; ++k) F[k] = 0;
-    // axis 0: lines along x through the (y, z) corners
-#pragma unroll
-    for (int l = 0; l < NC / 2; ++l) {
-        const int by = l & 1, bz = (l >> 1) & 1;
-        const long base = w1[2 + by] * sN + (R == 3 ? w2[2 + bz] * sNN : 0);
-        M f[6];
-#pragma unroll
-        for (int t = 0; t < 6; ++t) f[t] = load_mesh(phi, base + w0[t]);
-#pragma unroll
-        for (int bx = 0; bx < 2; ++bx) {
-            const int j = bx | (by << 1) | (bz << 2);
-            const M g = fd4(f[bx], f[bx + 1], f[bx + 3], f[bx + 4], resv);
-            F[0] += g * z[j];
-        }
-    }
-    // axes 1 and 2: gradients per corner first, then accumulated in corner
-    // order so that every component sums j = 0 .. 2^R-1 like the reference.
-    {
-        M g1[NC];
-#pragma unroll
-        for (int l = 0; l < NC / 2; ++l) {
-            const int bx = l & 1, bz = (l >> 1) & 1;
-            const long base = w0[2 + bx] + (R == 3 ? w2[2 + bz] * sNN : 0);
-            M f[6];
-#pragma unroll
-            for (int t = 0; t < 6; ++t) f[t] = load_mesh(phi, base + w1[t] * sN);
-#pragma unroll
-            for (int by = 0; by < 2; ++by) g1[bx | (by << 1) | (bz << 2)] = fd4(f[by], f[by + 1], f[by + 3], f[by + 4], resv);
-        }
-#pragma unroll
-        for (int j = 0; j < NC; ++j) F[1] += g1[j] * z[j];
-    }
-    if constexpr (R == 3) {
-        M g2[NC];
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-            const int bx = l & 1, by = (l >> 1) & 1;
-            const long base = w0[2 + bx] + w1[2 + by] * sN;
-            M f[6];
-#pragma unroll
-            for (int t = 0; t < 6; ++t) f[t] = load_mesh(phi, base + w2[t] * sNN);
-#pragma unroll
-            for (int bz = 0; bz < 2; ++bz) g2[bx | (by << 1) | (bz << 2)] = fd4(f[bz], f[bz + 1], f[bz + 3], f[bz + 4], resv);
-        }
-#pragma unroll
-        for (int j = 0; j < NC; ++j) F[2] += g2[j] * z[j];
+        for (int k = 0; k < R; ++k) PosF64::gat(xs[k], pa, o[k], a1[k]);
+        force_f64<R>(a1, MeshNodes<R, double>{phi, o[0], o[1], o[2], lgN, msk}, pa.res, F);
     }
 }
+
 
 // With ro.perm set, thread s reads particle perm[s] and writes the updated
 // particle (and its id) to position s of the output arrays: the kick also
@@ -1252,8 +1412,11 @@ void ParticleOps<PP>::gather_kick(int R, Pos* X, double* P, long np, const Mesh*
     CPM_LAUNCH_CHECK();
 }
 
-void brick_layout(int R, int lgN, PosArgs& pa) {
-    const int want[3] = {ilog2c(FXB_X), R == 3 ? ilog2c(FXB_Y) : ilog2c(FXB_X), R == 3 ? ilog2c(FXB_Z) : 0};
+void brick_layout(int R, int lgN, PosArgs& pa, bool f64) {
+    const int bx = f64 ? (R == 3 ? F64B_X : F64B_X2) : FXB_X;
+    const int by = f64 ? (R == 3 ? F64B_Y : F64B_X2) : (R == 3 ? FXB_Y : FXB_X);
+    const int bz = f64 ? F64B_Z : FXB_Z;
+    const int want[3] = {ilog2c(bx), ilog2c(by), R == 3 ? ilog2c(bz) : 0};
     for (int k = 0; k < 3; ++k) {
         const int lg = (k < R) ? lgN : 0;
         pa.lb[k] = want[k] < lg ? want[k] : lg;
@@ -1277,26 +1440,60 @@ template void deposit_fx_brick<float>(int, const uint32_t*, const uint32_t*, con
 template void deposit_fx_brick<double>(int, const uint32_t*, const uint32_t*, const uint32_t*, long, int,
                                        const PosArgs&, float, double*, double*, hipStream_t);
 
-void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, const double* P,
-                          const uint32_t* idi, uint32_t* Xo, double* Po, uint32_t* ido, long np, const float* phi,
-                          int lgN, const PosArgs& pa, double ck, double pq, double cd, int cd_on, uint32_t* keys,
-                          uint32_t* in_count, uint32_t* out_count, uint32_t* defer, int qpar, hipStream_t st) {
+template <typename PP, int NT>
+static void launch_gather_brick(int R, const uint32_t* bstart, const uint32_t* perm, const typename PP::Pos* X,
+                                const double* P, const uint32_t* idi, typename PP::Pos* Xo, double* Po, uint32_t* ido,
+                                long np, const typename PP::Mesh* phi, int lgN, const PosArgs& pa, double ckq,
+                                double pq, double cd, int cd_on, uint32_t* keys, uint32_t* in_count,
+                                uint32_t* out_count, uint32_t* defer, int qpar, hipStream_t st) {
     // segments of up to 2^CPM_GATHER_LSEG bricks along the last axis, as long
     // as that leaves at least 4 workgroups per compute unit
     const long nbr = brick_count(pa), minwg = 4 * device_cus();
     int lseg = 0;
     while (lseg < CPM_GATHER_LSEG && lseg < pa.lnb[R - 1] && (nbr >> (lseg + 1)) >= minwg) ++lseg;
     const unsigned nb = (unsigned)(nbr >> lseg);
-    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_kick_fx_brick<RR>), dim3(nb), dim3(FXG_NT), 0, st, bstart, perm,
-                                         X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck * (1.0 / pq), pq, cd, cd_on, lseg,
-                                         keys, in_count, out_count, defer, qpar));
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_kick_brick<PP, RR, NT>), dim3(nb), dim3(NT), 0, st, bstart, perm,
+                                         X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ckq, pq, cd, cd_on, lseg, keys,
+                                         in_count, out_count, defer, qpar));
     CPM_LAUNCH_CHECK();
-    if (pa.wrap) return;  // folded coordinates: every origin lies in its brick, nothing is queued
+    // folded coordinates: every origin is the deposit's cell, in its brick, and nothing is queued
+    if (pa.wrap) return;
     // the overflow of the workgroups' LDS queues: usually empty
     const long nbd = std::min<long>(256, (np + 255) / 256);
-    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_deferred<RR>), dim3((unsigned)nbd), dim3(256), 0, st, defer, qpar,
-                                         perm, X, P, Xo, Po, np, phi, lgN, pa, ck * (1.0 / pq), pq, cd, cd_on, keys,
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_deferred<PP, RR>), dim3((unsigned)nbd), dim3(256), 0, st, defer,
+                                         qpar, perm, X, P, Xo, Po, np, phi, lgN, pa, ckq, pq, cd, cd_on, keys,
                                          in_count, out_count));
+    CPM_LAUNCH_CHECK();
+}
+
+void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, const double* P,
+                          const uint32_t* idi, uint32_t* Xo, double* Po, uint32_t* ido, long np, const float* phi,
+                          int lgN, const PosArgs& pa, double ck, double pq, double cd, int cd_on, uint32_t* keys,
+                          uint32_t* in_count, uint32_t* out_count, uint32_t* defer, int qpar, hipStream_t st) {
+    launch_gather_brick<PosFX, FXG_NT>(R, bstart, perm, X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck * (1.0 / pq), pq,
+                                       cd, cd_on, keys, in_count, out_count, defer, qpar, st);
+}
+
+void gather_kick_f64_brick(int R, const uint32_t* bstart, const uint32_t* perm, const double* X, const double* P,
+                           const uint32_t* idi, double* Xo, double* Po, uint32_t* ido, long np, const double* phi,
+                           int lgN, const PosArgs& pa, double ck, double cd, int cd_on, uint32_t* keys,
+                           uint32_t* in_count, uint32_t* out_count, uint32_t* defer, int qpar, hipStream_t st) {
+    launch_gather_brick<PosF64, F64G_NT>(R, bstart, perm, X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck, 0.0, cd,
+                                         cd_on, keys, in_count, out_count, defer, qpar, st);
+}
+
+void keys_f64_brick(int R, const double* X, long np, int lgN, const PosArgs& pa, const uint32_t* prevkeys,
+                    uint32_t* keys, uint32_t* pids, uint32_t* in_count, uint32_t* out_count, hipStream_t st) {
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_keys_f64_brick<RR>), dim3(nblk(np, 256)), dim3(256), 0, st, X, np, lgN,
+                                         pa, prevkeys, keys, pids, in_count, out_count));
+    CPM_LAUNCH_CHECK();
+}
+
+void deposit_f64_brick(int R, const uint32_t* bstart, const uint32_t* perm, const double* X, long np, int lgN,
+                       const PosArgs& pa, double mass, double* mesh, double* partial, hipStream_t st) {
+    const unsigned nb = (unsigned)brick_count(pa);
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit_f64_brick<RR>), dim3(nb), dim3(F64B_NT), 0, st, bstart, perm, X,
+                                         np, lgN, pa, mass, mesh, partial));
     CPM_LAUNCH_CHECK();
 }
 

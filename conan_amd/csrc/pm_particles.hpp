@@ -46,6 +46,26 @@ __device__ __forceinline__ double fd4(double fm2, double fm1, double fp1, double
     double v = fm2 / 12. - 2 * fm1 / 3. + 2 * fp1 / 3. - fp2 / 12.;
     return v / res;
 }
+// x / 3 rounded to nearest, without a division: q0 = x RN(1/3) is within an
+// ulp of x / 3, r = x - 3 q0 is exact, and q0 + r RN(1/3) lies within
+// 2^-52 ulp of x / 3.  x / 3 is k / 3 ulps past a representable number (k =
+// 0, 1, 2), never within 1/6 ulp of a midpoint between two, so rounding
+// q0 + r RN(1/3) once gives RN(x / 3) for every x with |x| >= 2^-1018 (and
+// zeros, whose sign copysign keeps).  x / 12 = RN((x / 4) / 3) and 2 x / 3 =
+// RN((2 x) / 3): scaling by a power of two is exact.
+__device__ __forceinline__ double div3(double x) {
+    const double c = 0x1.5555555555555p-2;  // RN(1/3)
+    const double q0 = x * c;
+    const double r = fma(-q0, 3.0, x);
+    return __builtin_copysign(fma(r, c, q0), x);
+}
+// fd4 with its divisions by 12 and 3 done by div3: the same value, bit for
+// bit (a divide costs some ten instructions, div3 four).  The hot force
+// paths use this one.
+__device__ __forceinline__ double fd4_div3(double fm2, double fm1, double fp1, double fp2, double res) {
+    double v = div3(fm2 * 0.25) - div3(2 * fm1) + div3(2 * fp1) - div3(fp2 * 0.25);
+    return v / res;
+}
 // Same operator in FP32, arranged for fewer roundings: (f[-2] - f[+2] + 8 (f[+1] - f[-1])) / (12 res),
 // the sum fused with the multiplication by 8 (explicitly, so every kernel that
 // uses it rounds the same way whatever its contraction setting).
@@ -191,7 +211,21 @@ void keys_fx(int R, const uint32_t* X, long np, int lgN, const PosArgs& pa, cons
 #define CPM_FXB_Z 8
 #endif
 constexpr int FXB_X = CPM_FXB_X, FXB_Y = CPM_FXB_Y, FXB_Z = CPM_FXB_Z;
-void brick_layout(int R, int lgN, PosArgs& pa);
+// FP64 brick order (the plain form's brick deposit and tile gather, unless
+// PM_FLAG_SERIAL_DEPOSIT): bricks of F64B_X x F64B_Y x F64B_Z cells (2-D:
+// F64B_X2 x F64B_X2), so that the gather's ring of FP64 potential planes
+// (2 F64B_Z + 5 planes of (F64B_X + 5)(F64B_Y + 5) cells) stays below 64 KiB.
+constexpr int F64B_X = 16, F64B_Y = 16, F64B_Z = 4, F64B_X2 = 32;
+template <typename PP>
+struct BrickDims {
+    static constexpr int X = FXB_X, Y = FXB_Y, Z = FXB_Z, X2 = FXB_X;
+};
+template <>
+struct BrickDims<PosF64> {
+    static constexpr int X = F64B_X, Y = F64B_Y, Z = F64B_Z, X2 = F64B_X2;
+};
+// f64: the FP64 brick sizes instead of the FX ones
+void brick_layout(int R, int lgN, PosArgs& pa, bool f64 = false);
 long brick_count(const PosArgs& pa);
 // FX deposit / gather+kick(+drift) over particles grouped by brick (perm:
 // sorted position -> storage index; bstart: brick_count + 1 offsets).  The
@@ -211,6 +245,20 @@ void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, c
                           const uint32_t* idi, uint32_t* Xo, double* Po, uint32_t* ido, long np, const float* phi,
                           int lgN, const PosArgs& pa, double ck, double pq, double cd, int cd_on, uint32_t* keys,
                           uint32_t* in_count, uint32_t* out_count, uint32_t* defer, int qpar, hipStream_t st);
+// The FP64 brick pipeline: keys_f64_brick is keys_fx for FP64 positions
+// (the brick of the deposit's cell, PosF64::dep); deposit_f64_brick is the
+// FP64 fixed-point deposit (128-bit accumulators, the row deposit's cell sums
+// exactly) per brick, one FP64 partial sum per brick; gather_kick_f64_brick
+// is gather_kick_fx_brick with the FP64 potential tile and the reference's
+// force, kick and drift arithmetic (k_gather_kick's, operation for operation).
+void keys_f64_brick(int R, const double* X, long np, int lgN, const PosArgs& pa, const uint32_t* prevkeys,
+                    uint32_t* keys, uint32_t* pids, uint32_t* in_count, uint32_t* out_count, hipStream_t st);
+void deposit_f64_brick(int R, const uint32_t* bstart, const uint32_t* perm, const double* X, long np, int lgN,
+                       const PosArgs& pa, double mass, double* mesh, double* partial, hipStream_t st);
+void gather_kick_f64_brick(int R, const uint32_t* bstart, const uint32_t* perm, const double* X, const double* P,
+                           const uint32_t* idi, double* Xo, double* Po, uint32_t* ido, long np, const double* phi,
+                           int lgN, const PosArgs& pa, double ck, double cd, int cd_on, uint32_t* keys,
+                           uint32_t* in_count, uint32_t* out_count, uint32_t* defer, int qpar, hipStream_t st);
 // max |a[i]| (deterministic), and a[i] := rint(a[i] / q) * q.
 void reduce_max_abs(const double* a, long n, double* partials, long max_partials, double* out, hipStream_t st);
 void quantize(double* a, long n, double q, hipStream_t st);
