@@ -235,7 +235,10 @@ __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) voi
     // latencies overlap; each later one at the end of the previous iteration.
     const uint32_t s0 = bstart[bid << R], s1 = bstart[(bid + 1) << R];
     uint32_t ys[3] = {0u, 0u, 0u}, idv = 0u, iv = 0u;
-    double lam[3] = {0, 0, 0};  // lP of the particle (FP32 values)
+    // the particle's state, loaded one iteration ahead: lP (FP32 values), P, lY
+    double lam[3] = {0, 0, 0};
+    double pin[3] = {0, 0, 0};
+    float lyin[3] = {0.f, 0.f, 0.f};
     auto fetch = [&](uint32_t sp) {
         iv = perm[sp];
         idv = idi[iv];
@@ -243,6 +246,8 @@ __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) voi
         for (int k = 0; k < R; ++k) {
             ys[k] = Y[k * np + iv];
             lam[k] = (double)lP[k * np + iv];
+            pin[k] = P[k * np + iv];
+            lyin[k] = lY[k * np + iv];
         }
     };
     if (s0 + tid < s1) fetch(s0 + tid);
@@ -271,18 +276,14 @@ __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) voi
             for (int k = 0; k < R; ++k) w *= ((j >> k) & 1) ? a1[k] : 1.f - a1[k];
             z[j] = w;
         }
-        // dW(c_j)/dY_k
-        double dz[3][NC];
+        // dW(c_j)/dY_k, formed where it is used
+        auto dzf = [&](int k, int j) -> double {
+            double w = ((j >> k) & 1) ? ires : -ires;
 #pragma unroll
-        for (int k = 0; k < R; ++k)
-#pragma unroll
-            for (int j = 0; j < NC; ++j) {
-                double w = ((j >> k) & 1) ? ires : -ires;
-#pragma unroll
-                for (int m = 0; m < R; ++m)
-                    if (m != k) w *= ((j >> m) & 1) ? (double)a1[m] : 1.0 - (double)a1[m];
-                dz[k][j] = w;
-            }
+            for (int m = 0; m < R; ++m)
+                if (m != k) w *= ((j >> m) & 1) ? (double)a1[m] : 1.0 - (double)a1[m];
+            return w;
+        };
         float F[3] = {0.f, 0.f, 0.f};
         double T[3] = {0, 0, 0}, U[3] = {0, 0, 0};
         if (hp) {
@@ -336,7 +337,7 @@ __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
                 for (int q = 0; q < R; ++q) h += lam[q] * (double)g[q][j];
 #pragma unroll
-                for (int k = 0; k < R; ++k) T[k] += dz[k][j] * h;
+                for (int k = 0; k < R; ++k) T[k] += dzf(k, j) * h;
             }
         }
         if (hc) {
@@ -345,15 +346,15 @@ __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) voi
                 const int bx = j & 1, by = (j >> 1) & 1, bz = (j >> 2) & 1;
                 const double cv = (double)ctile[(lx + bx) + CX * ((ly + by) + CY * (lz + (R == 3 ? bz : 0)))];
 #pragma unroll
-                for (int k = 0; k < R; ++k) U[k] += dz[k][j] * cv;
+                for (int k = 0; k < R; ++k) U[k] += dzf(k, j) * cv;
             }
         }
         uint32_t yn[3] = {0u, 0u, 0u};
         unsigned long long h = hh ? hash_start(idv) : 0ull;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            const double p = P[k * np + iv] + fx_kick_increment((double)F[k], cFq, cf.pq);
-            const float lyk = (float)((double)lY[k * np + iv] + cf.cT * T[k] + cf.cU * U[k]);
+            const double p = pin[k] + fx_kick_increment((double)F[k], cFq, cf.pq);
+            const float lyk = (float)((double)lyin[k] + cf.cT * T[k] + cf.cU * U[k]);
             float lpk = (float)lam[k];
             if (hh) h = hash_axis(h, ys[k], p);
             yn[k] = ys[k];
