@@ -194,7 +194,7 @@ struct Impl : ImplBase {
     const uint32_t* spids = nullptr;
     bool have_deposit = false;
     long npartials = 0;     // FP64 mesh-sum partials written by the last deposit
-    long nsort_groups = 0;  // rows (FP64) or bricks (FX) delimited by row_start
+    long nsort_groups = 0;  // key groups of the sort: brick key groups, or mesh rows (serial-order deposit)
     double pq = 0;          // FX: momentum quantum (a power of two), see fx_quantize_momenta
     bool pq_pinned = false; // FX: pq was set by the caller (kept by set_state)
     Mesh* phi_out = nullptr;  // when set, the potential of a kick goes here instead of into mesh
@@ -229,7 +229,6 @@ struct Impl : ImplBase {
         pa.fscale = (float)ldexp(1.0, -pa.fshift);
         pa.cmask = c.Nf - 1;
         pa.brick = 0;
-        pa.rowkey = 0;
         for (int k = 0; k < 3; ++k) pa.lb[k] = pa.lnb[k] = 0;
         if (bricks) brick_layout(R, lgNf, pa, !FX);
         nsort_groups = bricks ? (brick_count(pa) << R) : nrows;
@@ -477,7 +476,7 @@ struct Impl : ImplBase {
             npartials = brick_count(pa);
         } else {
             ParticleOps<PP>::deposit(R, row_start.p, skeys, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p,
-                                     nrows, frac.p, xsort.p, inv.p, true, st);
+                                     nrows, frac.p, xsort.p, inv.p, st);
             npartials = nrows;
         }
         prof.end(1, st);
@@ -701,7 +700,6 @@ struct Impl : ImplBase {
         if (keys) {
             PosArgs pk = pa;
             pk.brick = 0;  // the full cell index idx(origin)
-            pk.rowkey = 0;
             ParticleOps<PP>::keys(R, X.p, Np, lgNf, pk, k1.p, v1.p, st);
             std::vector<uint32_t> k(Np);
             CPM_HIP(hipMemcpyAsync(k.data(), k1.p, sizeof(uint32_t) * Np, hipMemcpyDeviceToHost, st));

@@ -39,7 +39,7 @@ __global__ __launch_bounds__(256) void k_keys(const typename PP::Pos* __restrict
         oc[k] = o & mask;
         key |= (uint32_t)oc[k] << (k * lgN);  // Box::idx of the origin (box.hh:75-81)
     }
-    keys[i] = pa.brick ? brick_key(oc[0], oc[1], oc[2], R, pa) : (pa.rowkey ? key >> lgN : key);
+    keys[i] = pa.brick ? brick_key(oc[0], oc[1], oc[2], R, pa) : key;
     pids[i] = (uint32_t)i;
 }
 
@@ -198,16 +198,12 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit(const uint32_t* __restrict__
     if (tid == 0) row_partial[row] = tot;
 }
 
-// FP64 deposit in exact fixed point, one workgroup per mesh row.  The
-// weights are the reference's, z = ((m A_x) A_y) A_z (numeric.hh:124-131),
-// each rounded down to a multiple of 2^-96 and added to a 128-bit
-// accumulator per cell (two 64-bit LDS words, the carry of the low word
-// added to the high one).  Integer addition is associative, so the cell sums
-// are exact sums of the rounded weights in any order: deterministic, and
-// within an ulp or two of the reference's serial double sum (which rounds
-// at every addition).  Each particle of the 2^(R-1) origin rows adds its two
-// corners along x that fall in this row.  The particles come grouped by the
-// mesh row of their origin cell (row_start), positions in that order (Xs).
+// 128-bit fixed point of the FP64 deposit (k_deposit_f64_brick): a weight z
+// in [0, 2^31) rounded down to a multiple of 2^-96, held as two 64-bit words
+// (the carry of the low word added to the high one).  Integer addition is
+// associative, so a cell's sum is the exact sum of its rounded weights in
+// any order: deterministic, and within an ulp or two of the reference's
+// serial double sum (which rounds at every addition).
 static constexpr double FX128_ONE = 79228162514264337593543950336.0;  // 2^96
 __device__ __forceinline__ void fx128_split(double z, unsigned long long& hi, unsigned long long& lo) {
     // z in [0, 2^31): hi = floor(z 2^32), lo = floor(frac(z 2^32) 2^64)
@@ -218,61 +214,6 @@ __device__ __forceinline__ void fx128_split(double z, unsigned long long& hi, un
 }
 __device__ __forceinline__ double fx128_value(unsigned long long hi, unsigned long long lo) {
     return (double)hi * (1.0 / 4294967296.0) + (double)lo * (1.0 / FX128_ONE);
-}
-
-template <int R>
-__global__ __launch_bounds__(DEP_NT) void k_deposit_f64_fx(const uint32_t* __restrict__ row_start,
-                                                            const double* __restrict__ Xs, long np, int lgN,
-                                                            PosArgs pa, double mass, double* __restrict__ mesh,
-                                                            double* __restrict__ row_partial) {
-    constexpr int NL = 1 << (R - 1);
-    __shared__ unsigned long long alo[1024], ahi[1024];
-    __shared__ double red[DEP_NT / 64];
-    const int N = 1 << lgN, msk = N - 1;
-    const long row = blockIdx.x;
-    const int cy = (int)(row & msk);
-    const int cz = (R == 3) ? (int)(row >> lgN) : 0;
-    const int tid = threadIdx.x;
-    for (int c = tid; c < N; c += DEP_NT) alo[c] = ahi[c] = 0ull;
-    __syncthreads();
-#pragma unroll
-    for (int l = 0; l < NL; ++l) {
-        const int by = l & 1, bz = (l >> 1) & 1;
-        long orow = (long)((cy - by) & msk);
-        if (R == 3) orow += (long)((cz - bz) & msk) << lgN;
-        const uint32_t s0 = row_start[orow], s1 = row_start[orow + 1];
-        for (uint32_t s = s0 + tid; s < s1; s += DEP_NT) {
-            // the cell and fractions of md_cic (numeric.hh:114-122), from the
-            // position in the sort's order
-            int x, oy, oz = 0;
-            double fx, fy, fz = 0.0;
-            PosF64::dep(Xs[s], pa, x, fx);
-            PosF64::dep(Xs[np + s], pa, oy, fy);
-            if (R == 3) PosF64::dep(Xs[2 * np + s], pa, oz, fz);
-            x &= msk;
-#pragma unroll
-            for (int bx = 0; bx < 2; ++bx) {
-                double z = mass;
-                z *= bx ? fx : 1 - fx;
-                z *= by ? fy : 1 - fy;
-                if (R == 3) z *= bz ? fz : 1 - fz;
-                unsigned long long hi, lo;
-                fx128_split(z, hi, lo);
-                const int c = (x + bx) & msk;
-                const unsigned long long old = atomicAdd(&alo[c], lo);
-                atomicAdd(&ahi[c], hi + (old + lo < old ? 1ull : 0ull));
-            }
-        }
-    }
-    __syncthreads();
-    double rsum = 0;
-    for (int c = tid; c < N; c += DEP_NT) {
-        const double v = fx128_value(ahi[c], alo[c]);
-        mesh[row * N + c] = v;
-        rsum += v;
-    }
-    const double tot = block_sum<DEP_NT>(rsum, red, tid);
-    if (tid == 0) row_partial[row] = tot;
 }
 
 // ------------------------------------------------------------ FX: persistent brick order
@@ -938,10 +879,9 @@ __global__ __launch_bounds__(256) void k_keys_f64_brick(const double* __restrict
 
 // FP64 deposit in exact fixed point, one workgroup per brick: the particles
 // of the brick and the face classes of its lower neighbours (brick_ranges),
-// each adding the weights of its 2^R corners that land in the brick.  The
-// weights and their 128-bit fixed-point split are k_deposit_f64_fx's
-// (z = ((m A_x) A_y) A_z, numeric.hh:124-131), and the cell sums are exact,
-// so every cell is the row deposit's, bit for bit.
+// each adding the weights of its 2^R corners that land in the brick: the
+// reference's z = ((m A_x) A_y) A_z (numeric.hh:124-131), split by
+// fx128_split into the cell's two accumulator words.
 template <int R>
 __global__ __launch_bounds__(F64B_NT) void k_deposit_f64_brick(const uint32_t* __restrict__ bstart,
                                                                const uint32_t* __restrict__ perm,
@@ -1371,20 +1311,10 @@ template <typename PP>
 void ParticleOps<PP>::deposit(int R, const uint32_t* row_start, const uint32_t* keys, const uint32_t* pids,
                               const Pos* X, long np, int lgN, const PosArgs& pa, Mesh mass, Mesh* mesh,
                               double* row_partial, long nrows, Mesh* frac, Pos* Xs, uint32_t* inv,
-                              bool serial_order, hipStream_t st) {
+                              hipStream_t st) {
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_dep_frac<PP, RR>), dim3(nblk(np, 256)), dim3(256), 0, st, pids, X, np,
                                          pa, frac, Xs, inv));
     CPM_LAUNCH_CHECK();
-    if constexpr (sizeof(Mesh) == 8) {
-        if (!serial_order) {
-            if (lgN > 10) throw Error(-2, "the fixed-point FP64 deposit takes meshes of at most 1024 per side");
-            CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit_f64_fx<RR>), dim3((unsigned)nrows), dim3(DEP_NT), 0, st,
-                                                 row_start, (const double*)Xs, np, lgN, pa, (double)mass,
-                                                 (double*)mesh, row_partial));
-            CPM_LAUNCH_CHECK();
-            return;
-        }
-    }
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit<PP, RR>), dim3((unsigned)nrows), dim3(DEP_NT), 0, st,
                                          row_start, keys, pids, (const Mesh*)frac, np, lgN, mass, mesh,
                                          row_partial));

@@ -13,8 +13,7 @@ struct PosArgs {
     uint32_t fmask;  // FX: (1 << fshift) - 1
     float fscale;    // FX: 2^-fshift
     int cmask;       // FX: N - 1 (the cell bits above the fraction)
-    int brick;       // sort keys: 0 = cell index idx(origin), 1 = brick index (FX)
-    int rowkey;      // brick == 0: 1 = the mesh row of the origin cell, idx(origin) >> log2 N
+    int brick;       // sort keys: 0 = cell index idx(origin), 1 = brick key (brick_key)
     int lb[3];       // log2 brick size per axis
     int lnb[3];      // log2 bricks per axis
 };
@@ -178,7 +177,7 @@ struct ParticleOps {
                      hipStream_t st);
     static void deposit(int R, const uint32_t* row_start, const uint32_t* keys, const uint32_t* pids,
                         const Pos* X, long np, int lgN, const PosArgs& pa, Mesh mass, Mesh* mesh,
-                        double* row_partial, long nrows, Mesh* frac, Pos* Xs, uint32_t* inv, bool serial_order,
+                        double* row_partial, long nrows, Mesh* frac, Pos* Xs, uint32_t* inv,
                         hipStream_t st);
     static void gather_kick(int R, Pos* X, double* P, long np, const Mesh* phi, int lgN, const PosArgs& pa,
                             double ck, double cd, int cd_on, double* Fout, const Reorder<PP>& ro, hipStream_t st);

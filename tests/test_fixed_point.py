@@ -4,7 +4,7 @@ fx_quantum): round(w 2^40), ties to even, computed with FP32 operations
 FP64 rounding of the exact product w 2^40 that the accumulators were
 specified with.  numpy's float32 floor / rint / multiply are the IEEE
 operations the kernel compiles to (round-to-nearest-even products, exact
-here)."""
+here).  Also the FP64 force's division by 3 (div3, below)."""
 import numpy as np
 
 
@@ -47,3 +47,52 @@ def test_fp32_rounding_special_values():
     assert fx_quantum_f32(np.float32(0.5 * 2.0 ** -40)) == 0
     assert fx_quantum_f32(np.float32(1.5 * 2.0 ** -40)) == 2
     assert fx_quantum_f32(np.float32(2.5 * 2.0 ** -40)) == 2
+
+
+# ---------------------------------------------------------------- div3
+# The FP64 force (pm_particles.hpp, div3 / fd4_div3) forms the reference's
+# f / 12 and 2 f / 3 (src/run/numeric.hh:98,101) as RN(x / 3) of x = f / 4 and
+# x = 2 f, from q0 = RN(x RN(1/3)), r = fma(-q0, 3, x), RN(q0 + r RN(1/3)),
+# with the sign of x.  Here fma is evaluated exactly with fractions and
+# rounded once (float(Fraction) rounds to nearest even).
+from fractions import Fraction
+
+_C3 = float.fromhex("0x1.5555555555555p-2")  # RN(1/3)
+
+
+def _fma(a, b, c):
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def _div3(x, corrected=True):
+    q0 = x * _C3
+    if not corrected:
+        return q0
+    r = _fma(-q0, 3.0, x)
+    return float(np.copysign(_fma(r, _C3, q0), x))
+
+
+def _div3_cases():
+    r = np.random.default_rng(11)
+    xs = list(r.standard_normal(3000) * 2.0 ** r.integers(-300, 300, 3000))
+    # x = (3k + d) ulp: quotients just off a representable number, both signs
+    for e in (-3, 0, 7, 40):
+        for k in r.integers(2 ** 50, 2 ** 51, 300):
+            for d in (-1, 0, 1, 2):
+                m = int(3 * k + d)
+                if 2 ** 52 <= m < 2 ** 53:
+                    xs.append(float(m) * 2.0 ** (e - 52))
+                    xs.append(-float(m) * 2.0 ** (e - 52))
+    return [x for x in xs if x != 0.0 and abs(x) >= 2.0 ** -1000] + [0.0, -0.0, 1.0, 3.0, 2.0 ** -1000]
+
+
+def test_div3_is_the_rounded_quotient():
+    for x in _div3_cases():
+        for y, ref in ((x, x / 3.0), (x * 0.25, x / 12.0), (2.0 * x, 2.0 * x / 3.0)):
+            got = _div3(y)
+            assert got == ref and np.signbit(got) == np.signbit(ref), (x, y, got, ref)
+
+
+def test_div3_without_its_correction_fails():
+    # planted mistake: the bare product x RN(1/3) is off by an ulp for some x
+    assert any(_div3(x, corrected=False) != x / 3.0 for x in _div3_cases())
