@@ -223,6 +223,8 @@ struct Impl : ImplBase {
         mass = pow(double(c.Nf) / c.Nm, R);  // src/run/nbody.hh:85
         pa.L = c.L;
         pa.res = c.L / c.Nf;
+        pa.rinv = 1.0 / pa.res;
+        pa.fastdiv = res_divides_by_fma(pa.res) ? 1 : 0;
         pa.wrap = (c.flags & PM_FLAG_WRAP_GATHER) ? 1 : 0;
         pa.fshift = FX_BITS - lgNf;
         pa.fmask = (uint32_t)((1ull << pa.fshift) - 1);

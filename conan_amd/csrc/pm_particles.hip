@@ -388,7 +388,7 @@ __device__ __forceinline__ void force_fx(const float* a1, Ld ld, float inv12res,
 // o - 2 + (tx, ty, tz) (tz ignored in 2-D): from the mesh (MeshNodes) or
 // from a brick's LDS tile, with the same operations either way.
 template <int R, typename Ld>
-__device__ __forceinline__ void force_f64(const double* a1, Ld ld, double res, double* F) {
+__device__ __forceinline__ void force_f64(const double* a1, Ld ld, const PosArgs& pa, double* F) {
     constexpr int NC = 1 << R;
     double z[NC];
 #pragma unroll
@@ -409,7 +409,7 @@ __device__ __forceinline__ void force_f64(const double* a1, Ld ld, double res, d
 #pragma unroll
         for (int bx = 0; bx < 2; ++bx) {
             const int j = bx | (by << 1) | (bz << 2);
-            const double g = fd4_div3(f[bx], f[bx + 1], f[bx + 3], f[bx + 4], res);
+            const double g = fd4_div3(f[bx], f[bx + 1], f[bx + 3], f[bx + 4], pa);
             F[0] += g * z[j];
         }
     }
@@ -422,7 +422,7 @@ __device__ __forceinline__ void force_f64(const double* a1, Ld ld, double res, d
 #pragma unroll
             for (int t = 0; t < 6; ++t) f[t] = ld(2 + bx, t, 2 + bz);
 #pragma unroll
-            for (int by = 0; by < 2; ++by) g1[bx | (by << 1) | (bz << 2)] = fd4_div3(f[by], f[by + 1], f[by + 3], f[by + 4], res);
+            for (int by = 0; by < 2; ++by) g1[bx | (by << 1) | (bz << 2)] = fd4_div3(f[by], f[by + 1], f[by + 3], f[by + 4], pa);
         }
 #pragma unroll
         for (int j = 0; j < NC; ++j) F[1] += g1[j] * z[j];
@@ -436,7 +436,7 @@ __device__ __forceinline__ void force_f64(const double* a1, Ld ld, double res, d
 #pragma unroll
             for (int t = 0; t < 6; ++t) f[t] = ld(2 + bx, 2 + by, t);
 #pragma unroll
-            for (int bz = 0; bz < 2; ++bz) g2[bx | (by << 1) | (bz << 2)] = fd4_div3(f[bz], f[bz + 1], f[bz + 3], f[bz + 4], res);
+            for (int bz = 0; bz < 2; ++bz) g2[bx | (by << 1) | (bz << 2)] = fd4_div3(f[bz], f[bz + 1], f[bz + 3], f[bz + 4], pa);
         }
 #pragma unroll
         for (int j = 0; j < NC; ++j) F[2] += g2[j] * z[j];
@@ -493,7 +493,7 @@ __device__ __forceinline__ void brick_force(const typename PP::Mesh* a1, Ld ld, 
     if constexpr (std::is_same<PP, PosFX>::value) {
         force_fx<R>(a1, ld, (float)(1.0 / (12.0 * pa.res)), F);
     } else {
-        force_f64<R>(a1, ld, pa.res, F);
+        force_f64<R>(a1, ld, pa, F);
     }
 }
 
@@ -990,7 +990,7 @@ __device__ __forceinline__ void gather_force(const typename PP::Pos* xs, const t
         double a1[3] = {0, 0, 0};
 #pragma unroll
         for (int k = 0; k < R; ++k) PosF64::gat(xs[k], pa, o[k], a1[k]);
-        force_f64<R>(a1, MeshNodes<R, double>{phi, o[0], o[1], o[2], lgN, msk}, pa.res, F);
+        force_f64<R>(a1, MeshNodes<R, double>{phi, o[0], o[1], o[2], lgN, msk}, pa, F);
     }
 }
 

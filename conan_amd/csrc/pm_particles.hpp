@@ -3,11 +3,15 @@
 #pragma once
 #include "pm_common.hpp"
 
+#include <cmath>
+
 namespace cpm {
 
 // Parameters of the position -> (cell, fraction) maps.
 struct PosArgs {
     double L, res;   // box size and mesh spacing (F64)
+    double rinv;     // RN(1 / res)
+    int fastdiv;     // 1: v / res as div_fma (res_divides_by_fma(res)); 0: a divide
     int wrap;        // gather: 0 = truncate the unwrapped coordinate (reference), 1 = fold first
     int fshift;      // FX: FX_BITS - log2 N (bits of the fraction within a cell)
     uint32_t fmask;  // FX: (1 << fshift) - 1
@@ -58,12 +62,34 @@ __device__ __forceinline__ double div3(double x) {
     const double r = fma(-q0, 3.0, x);
     return __builtin_copysign(fma(r, c, q0), x);
 }
-// fd4 with its divisions by 12 and 3 done by div3: the same value, bit for
-// bit (a divide costs some ten instructions, div3 four).  The hot force
-// paths use this one.
-__device__ __forceinline__ double fd4_div3(double fm2, double fm1, double fp1, double fp2, double res) {
+// v / b rounded to nearest, from rinv = RN(1/b), without a divide.  Exact
+// for every divisor b = m 2^e whose odd significand m is below 2^50
+// (res_divides_by_fma): for m = 1 the quotient is v rinv itself; for m >= 3,
+// v / b is A 2^s / m ulps (A the significand of v, s >= 1), a fraction k / m
+// of an ulp past a representable number, never within 1/(2m) > 2^-51 ulp of
+// a midpoint; q0 = RN(v rinv) lies within 1.5 ulp of v / b, r = v - b q0 is
+// formed with one rounding, and q0 + r rinv lies within 3 2^-53 ulp of v / b,
+// so rounding it once gives RN(v / b) (normal v, v / b; zeros keep their sign).
+__device__ __forceinline__ double div_fma(double v, double b, double rinv) {
+    const double q0 = v * rinv;
+    const double r = fma(-q0, b, v);
+    return __builtin_copysign(fma(r, rinv, q0), v);
+}
+// Whether div_fma divides by b exactly (the odd part of b's significand
+// below 2^50; b positive and normal).
+inline bool res_divides_by_fma(double b) {
+    if (!(b > 0) || !std::isnormal(b)) return false;
+    int e = 0;
+    unsigned long long m = (unsigned long long)std::ldexp(std::frexp(b, &e), 53);
+    while (m && !(m & 1ull)) m >>= 1;
+    return m < (1ull << 50);
+}
+// fd4 with its divisions by 12 and 3 done by div3, and the one by res by
+// div_fma where that is exact: the same value, bit for bit (a divide costs
+// some ten instructions, div3 and div_fma four).  The FP64 force uses this one.
+__device__ __forceinline__ double fd4_div3(double fm2, double fm1, double fp1, double fp2, const PosArgs& a) {
     double v = div3(fm2 * 0.25) - div3(2 * fm1) + div3(2 * fp1) - div3(fp2 * 0.25);
-    return v / res;
+    return a.fastdiv ? div_fma(v, a.res, a.rinv) : v / a.res;
 }
 // Same operator in FP32, arranged for fewer roundings: (f[-2] - f[+2] + 8 (f[+1] - f[-1])) / (12 res),
 // the sum fused with the multiplication by 8 (explicitly, so every kernel that

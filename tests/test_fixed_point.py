@@ -96,3 +96,43 @@ def test_div3_is_the_rounded_quotient():
 def test_div3_without_its_correction_fails():
     # planted mistake: the bare product x RN(1/3) is off by an ulp for some x
     assert any(_div3(x, corrected=False) != x / 3.0 for x in _div3_cases())
+
+
+# ---------------------------------------------------------------- div_fma
+# v / res in the FP64 force (pm_particles.hpp, div_fma), taken when the odd
+# part of res's significand is below 2^50 (res_divides_by_fma): q0 = RN(v
+# RN(1/res)), r = fma(-q0, res, v), RN(q0 + r RN(1/res)) with v's sign.
+def _odd_significand(b):
+    m, e = np.frexp(b)
+    m = int(float(m) * 2.0 ** 53)
+    while m and m % 2 == 0:
+        m //= 2
+    return m
+
+
+def _div_fma(v, b):
+    y = 1.0 / b
+    q0 = v * y
+    r = _fma(-q0, b, v)
+    return float(np.copysign(_fma(r, y, q0), v))
+
+
+def test_div_fma_is_the_rounded_quotient():
+    r = np.random.default_rng(5)
+    vs = list(r.standard_normal(1500) * 2.0 ** r.integers(-40, 40, 1500)) + [0.0, -0.0, 1.0]
+    for L in (100.0, 50.0, 1000.0, 250.0, 1.0, 2.5, 0.75, 3.0):
+        for N in (16, 128, 512, 1024):
+            b = L / N
+            assert _odd_significand(b) < 2 ** 50
+            for v in vs:
+                # values just off the midpoints of quotients near v / b
+                for w in (v, b * (float(np.nextafter(v / b, np.inf)) + 0.0)):
+                    got, ref = _div_fma(w, b), w / b
+                    assert got == ref and np.signbit(got) == np.signbit(ref), (L, N, w, got, ref)
+
+
+def test_div_fma_is_not_taken_for_long_odd_significands():
+    # a spacing like 137.3 / 512 has an odd significand far above 2^50: the
+    # force divides there instead
+    assert _odd_significand(137.3 / 512) >= 2 ** 50
+    assert _odd_significand(100.0 / 512) == 25
