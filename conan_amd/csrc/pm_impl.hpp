@@ -224,7 +224,9 @@ struct Impl : ImplBase {
         pa.L = c.L;
         pa.res = c.L / c.Nf;
         pa.rinv = 1.0 / pa.res;
-        pa.fastdiv = res_divides_by_fma(pa.res) ? 1 : 0;
+        pa.fastdiv = divides_by_fma(pa.res) ? 1 : 0;
+        pa.Linv = 1.0 / pa.L;
+        pa.fastdivL = divides_by_fma(pa.L) ? 1 : 0;
         pa.wrap = (c.flags & PM_FLAG_WRAP_GATHER) ? 1 : 0;
         pa.fshift = FX_BITS - lgNf;
         pa.fmask = (uint32_t)((1ull << pa.fshift) - 1);

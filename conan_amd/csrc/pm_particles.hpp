@@ -11,7 +11,9 @@ namespace cpm {
 struct PosArgs {
     double L, res;   // box size and mesh spacing (F64)
     double rinv;     // RN(1 / res)
-    int fastdiv;     // 1: v / res as div_fma (res_divides_by_fma(res)); 0: a divide
+    int fastdiv;     // 1: v / res as div_fma (divides_by_fma(res)); 0: a divide
+    double Linv;     // RN(1 / L)
+    int fastdivL;    // the same for v / L
     int wrap;        // gather: 0 = truncate the unwrapped coordinate (reference), 1 = fold first
     int fshift;      // FX: FX_BITS - log2 N (bits of the fraction within a cell)
     uint32_t fmask;  // FX: (1 << fshift) - 1
@@ -37,11 +39,6 @@ __host__ __device__ __forceinline__ uint32_t brick_key(int x, int y, int z, int 
     return (brick_of(x, y, z, a) << R) | c;
 }
 
-// src/base/misc.hh:144-148
-__device__ __forceinline__ double modulus_d(double a, double b) {
-    if (a < 0) return a - (double)(int)(a / b - 1) * b;
-    return a - (double)(int)(a / b) * b;
-}
 
 // The 4-point derivative of the reference (src/run/numeric.hh:98,101):
 // ((f[-2]/12 - 2 f[-1]/3) + 2 f[+1]/3) - f[+2]/12, then / res.
@@ -64,7 +61,7 @@ __device__ __forceinline__ double div3(double x) {
 }
 // v / b rounded to nearest, from rinv = RN(1/b), without a divide.  Exact
 // for every divisor b = m 2^e whose odd significand m is below 2^50
-// (res_divides_by_fma): for m = 1 the quotient is v rinv itself; for m >= 3,
+// (divides_by_fma): for m = 1 the quotient is v rinv itself; for m >= 3,
 // v / b is A 2^s / m ulps (A the significand of v, s >= 1), a fraction k / m
 // of an ulp past a representable number, never within 1/(2m) > 2^-51 ulp of
 // a midpoint; q0 = RN(v rinv) lies within 1.5 ulp of v / b, r = v - b q0 is
@@ -77,7 +74,7 @@ __device__ __forceinline__ double div_fma(double v, double b, double rinv) {
 }
 // Whether div_fma divides by b exactly (the odd part of b's significand
 // below 2^50; b positive and normal).
-inline bool res_divides_by_fma(double b) {
+inline bool divides_by_fma(double b) {
     if (!(b > 0) || !std::isnormal(b)) return false;
     int e = 0;
     unsigned long long m = (unsigned long long)std::ldexp(std::frexp(b, &e), 53);
@@ -116,16 +113,28 @@ __device__ __forceinline__ uint32_t fx_round_u32(double y) {
 struct PosF64 {
     using Pos = double;
     using Mesh = double;
+    // x / L and x / res, by div_fma where that rounds like the divide
+    static __device__ __forceinline__ double over_L(double x, const PosArgs& a) {
+        return a.fastdivL ? div_fma(x, a.L, a.Linv) : x / a.L;
+    }
+    static __device__ __forceinline__ double over_res(double x, const PosArgs& a) {
+        return a.fastdiv ? div_fma(x, a.res, a.rinv) : x / a.res;
+    }
+    // modulus(x, L) of src/base/misc.hh:144-148, its quotient from over_L
+    static __device__ __forceinline__ double fold(double x, const PosArgs& a) {
+        if (x < 0) return x - (double)(int)(over_L(x, a) - 1) * a.L;
+        return x - (double)(int)(over_L(x, a)) * a.L;
+    }
     // deposit coordinate: wrapped into [0, L) first (src/run/numeric.hh:114-122)
     static __device__ __forceinline__ void dep(Pos x, const PosArgs& a, int& o, double& f) {
-        double p0 = modulus_d(x, a.L) / a.res;
+        double p0 = over_res(fold(x, a), a);
         o = (int)(p0);
         f = p0 - o;
     }
     // gather coordinate: the reference truncates the unwrapped coordinate
     // (src/run/numeric.hh:19,26-28); wrap != 0 folds it like the deposit.
     static __device__ __forceinline__ void gat(Pos x, const PosArgs& a, int& o, double& f) {
-        double p = (a.wrap ? modulus_d(x, a.L) : x) / a.res;
+        double p = over_res(a.wrap ? fold(x, a) : x, a);
         o = (int)(p);
         f = p - o;
     }

@@ -100,7 +100,7 @@ def test_div3_without_its_correction_fails():
 
 # ---------------------------------------------------------------- div_fma
 # v / res in the FP64 force (pm_particles.hpp, div_fma), taken when the odd
-# part of res's significand is below 2^50 (res_divides_by_fma): q0 = RN(v
+# part of res's significand is below 2^50 (divides_by_fma): q0 = RN(v
 # RN(1/res)), r = fma(-q0, res, v), RN(q0 + r RN(1/res)) with v's sign.
 def _odd_significand(b):
     m, e = np.frexp(b)
