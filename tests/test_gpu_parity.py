@@ -481,24 +481,26 @@ def test_runs_are_bit_reproducible(pm, orc, prec):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
 @pytest.mark.parametrize("dim,Nm,Nf", [(3, 32, 64), (2, 128, 256)])
-def test_fp32_incremental_resort_equals_full_sort(pm, orc, dim, Nm, Nf):
-    """The FX step re-sorts incrementally (only particles that changed brick or
-    face class are placed anew).  A run that forces a full radix sort before
-    every kick (a deposit between steps breaks the sorted storage order) ends
-    in the same state bit for bit: the deposit sums are exact integers, so no
-    result depends on the order within a brick."""
+def test_incremental_resort_equals_full_sort(pm, orc, dim, Nm, Nf, precision):
+    """Both forms keep their particles in brick order and re-sort
+    incrementally (only particles that changed brick or face class are placed
+    anew).  A run that forces a full radix sort before every kick (a deposit
+    between steps breaks the sorted storage order) ends in the same state bit
+    for bit: the deposit sums are exact integers, so no result depends on the
+    order within a brick."""
     L, a0, da, n = 100.0, 0.02, 0.0196, 12
     mbits = int(np.log2(Nm))
     noise = orc.white_noise(31, Nm ** dim)
     phi = orc.compute_potential(dim, mbits, L, orc.ic_scale_free(dim, mbits, L, -2.0, noise))
-    say("resort", dim, Nm, Nf)
-    with pm.Gravity(dim, Nm, Nf, L, precision="fp32") as g:
+    say("resort", dim, Nm, Nf, precision)
+    with pm.Gravity(dim, Nm, Nf, L, precision=precision) as g:
         g.init(phi, a0, a0 + da / 2)
         g.steps(a0, da, n)
         Xa, Pa = g.get_state()
         ra = g.deposit()
-    with pm.Gravity(dim, Nm, Nf, L, precision="fp32") as g:
+    with pm.Gravity(dim, Nm, Nf, L, precision=precision) as g:
         g.init(phi, a0, a0 + da / 2)
         a = a0
         for i in range(n):
