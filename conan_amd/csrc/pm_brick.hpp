@@ -17,7 +17,10 @@ static constexpr int FXB_NT = 512;  // deposit workgroup size
 #endif
 static constexpr int FXG_NT = CPM_FXG_NT;  // gather workgroup size
 #ifndef CPM_GATHER_LSEG
-#define CPM_GATHER_LSEG 3  // log2 of the bricks one gather workgroup slides through
+#define CPM_GATHER_LSEG 3  // log2 of the bricks one gather workgroup slides through (FP32)
+#endif
+#ifndef CPM_GATHER_LSEG64
+#define CPM_GATHER_LSEG64 4  // the same for the FP64 gather (its bricks are 4 planes deep)
 #endif
 static constexpr int FXB_CELLS = FXB_X * FXB_Y * FXB_Z;
 // potential tile of a brick: 2 cells of halo below and 3 above along each axis

@@ -1370,17 +1370,17 @@ template void deposit_fx_brick<float>(int, const uint32_t*, const uint32_t*, con
 template void deposit_fx_brick<double>(int, const uint32_t*, const uint32_t*, const uint32_t*, long, int,
                                        const PosArgs&, float, double*, double*, hipStream_t);
 
-template <typename PP, int NT>
+template <typename PP, int NT, int LSEG>
 static void launch_gather_brick(int R, const uint32_t* bstart, const uint32_t* perm, const typename PP::Pos* X,
                                 const double* P, const uint32_t* idi, typename PP::Pos* Xo, double* Po, uint32_t* ido,
                                 long np, const typename PP::Mesh* phi, int lgN, const PosArgs& pa, double ckq,
                                 double pq, double cd, int cd_on, uint32_t* keys, uint32_t* in_count,
                                 uint32_t* out_count, uint32_t* defer, int qpar, hipStream_t st) {
-    // segments of up to 2^CPM_GATHER_LSEG bricks along the last axis, as long
-    // as that leaves at least 4 workgroups per compute unit
+    // segments of up to 2^LSEG bricks along the last axis, as long as that
+    // leaves at least 4 workgroups per compute unit
     const long nbr = brick_count(pa), minwg = 4 * device_cus();
     int lseg = 0;
-    while (lseg < CPM_GATHER_LSEG && lseg < pa.lnb[R - 1] && (nbr >> (lseg + 1)) >= minwg) ++lseg;
+    while (lseg < LSEG && lseg < pa.lnb[R - 1] && (nbr >> (lseg + 1)) >= minwg) ++lseg;
     const unsigned nb = (unsigned)(nbr >> lseg);
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_kick_brick<PP, RR, NT>), dim3(nb), dim3(NT), 0, st, bstart, perm,
                                          X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ckq, pq, cd, cd_on, lseg, keys,
@@ -1400,7 +1400,7 @@ void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, c
                           const uint32_t* idi, uint32_t* Xo, double* Po, uint32_t* ido, long np, const float* phi,
                           int lgN, const PosArgs& pa, double ck, double pq, double cd, int cd_on, uint32_t* keys,
                           uint32_t* in_count, uint32_t* out_count, uint32_t* defer, int qpar, hipStream_t st) {
-    launch_gather_brick<PosFX, FXG_NT>(R, bstart, perm, X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck * (1.0 / pq), pq,
+    launch_gather_brick<PosFX, FXG_NT, CPM_GATHER_LSEG>(R, bstart, perm, X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck * (1.0 / pq), pq,
                                        cd, cd_on, keys, in_count, out_count, defer, qpar, st);
 }
 
@@ -1408,7 +1408,7 @@ void gather_kick_f64_brick(int R, const uint32_t* bstart, const uint32_t* perm, 
                            const uint32_t* idi, double* Xo, double* Po, uint32_t* ido, long np, const double* phi,
                            int lgN, const PosArgs& pa, double ck, double cd, int cd_on, uint32_t* keys,
                            uint32_t* in_count, uint32_t* out_count, uint32_t* defer, int qpar, hipStream_t st) {
-    launch_gather_brick<PosF64, F64G_NT>(R, bstart, perm, X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck, 0.0, cd,
+    launch_gather_brick<PosF64, F64G_NT, CPM_GATHER_LSEG64>(R, bstart, perm, X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck, 0.0, cd,
                                          cd_on, keys, in_count, out_count, defer, qpar, st);
 }
 
