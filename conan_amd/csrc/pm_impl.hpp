@@ -183,6 +183,7 @@ struct Impl : ImplBase {
     const uint32_t* bstart = nullptr;  // group starts of the last sort (row_start or bs2)
     bool resort_ok = false;            // storage order == output order of the last brick sort
     bool keys_ready = false;           // knew / cnt3 hold the keys of the current positions (fused gather)
+    bool skeys_valid = false;          // skeys holds the keys of the last sort (the re-sort does not write them)
     DevBuf<Mesh> mesh;
     DevBuf<Mesh> frac;  // FP64 deposit: fractional cell coordinates in sorted order
     DevBuf<Pos> xsort;     // FP64: positions in the deposit's sorted order
@@ -417,6 +418,12 @@ struct Impl : ImplBase {
         keys_ready = false;
         if (resort_ok) {
             zero_u32(cnt3.p, 3 * nsort_groups, st);
+            if (!skeys_valid) {  // the previous keys, from the group starts of the last sort
+                uint32_t* sk = (skeys == skb0.p) ? skb1.p : skb0.p;
+                group_keys(bstart, brick_count(pa), 1 << R, sk, st);
+                skeys = sk;
+                skeys_valid = true;
+            }
             return FxKeyArgs{skeys, knew.p, nullptr, cnt3.p, cnt3.p + nsort_groups};
         }
         return FxKeyArgs{nullptr, k0.p, v0.p, nullptr, nullptr};
@@ -425,17 +432,17 @@ struct Impl : ImplBase {
         keys_ready = false;
         if (resort_ok) {
             uint32_t* bnew = (bstart == row_start.p) ? bs2.p : row_start.p;
-            uint32_t* sk = (skeys == skb0.p) ? skb1.p : skb0.p;
             resort_counts(bstart, cnt3.p, cnt3.p + nsort_groups, nsort_groups, stay.p, tot.p, st);
             exclusive_scan(tot.p, bnew, nsort_groups + 1, scan_scr.p, st);
-            resort_place(knew.p, skeys, bstart, bnew, stay.p, cnt3.p + 2 * nsort_groups, brick_count(pa), 1 << R,
-                         permb.p, sk, st);
-            skeys = sk;
+            resort_place(knew.p, bstart, bnew, stay.p, cnt3.p + 2 * nsort_groups, brick_count(pa), 1 << R, permb.p,
+                         st);
+            skeys_valid = false;
             spids = permb.p;
             bstart = bnew;
         } else {
             const int where = sorter.sort(k0.p, v0.p, k1.p, v1.p, st);
             skeys = where ? k1.p : k0.p;
+            skeys_valid = true;
             spids = where ? v1.p : v0.p;
             row_bounds(skeys, Np, 0, nsort_groups, row_start.p, st);
             bstart = row_start.p;

@@ -696,42 +696,90 @@ __global__ __launch_bounds__(NT) void k_gather_kick_brick(
             }
         }
     };
-    // Particles are processed in rounds of NT (round r: sorted position
-    // s0 + r NT + tid); a thread's particle of round r + 1 is loaded at
-    // the end of round r, its first one of the next brick before the barrier
-    // that ends the brick.  The next brick's planes are issued at the start of
-    // the last round, after a wait for everything loaded so far: no load the
-    // compiler waits for is then younger than them, so they stay in flight
-    // through the last round instead of being drained by the first wait.
-    uint32_t s0 = bstart[bid << R], s1 = bstart[(bid + 1) << R];
+    // Particles are processed in rounds of NT (round r of a brick: sorted
+    // position s0 + r NT + tid; an empty brick still takes one round, which
+    // advances the ring).  The loads are pipelined two rounds deep: a round
+    // first issues the loads of the thread's particle of the next round (its
+    // storage index came with the round before) and the storage index of the
+    // particle of the round after that, then processes its own particle, so
+    // the loads overlap the round's work.  The two particle register sets are
+    // used in turn (a copy of a loaded set would wait for its loads at once),
+    // and every load is unconditional (invalid slots read the last particle
+    // again), so no value is merged from two paths.  The next brick's planes
+    // are issued at the start of the brick's last round, after a wait for
+    // everything loaded so far: no load the compiler waits for is then older
+    // than them, so they stay in flight through the last round instead of
+    // being drained by the first wait.
     const int nseg = 1 << lseg;
-    uint32_t nn0 = 0, nn1 = 0;
+    const uint32_t plast = (uint32_t)(np - 1);
+    // sorted ranges of the current brick and the next two along the segment
+    // (past the segment: empty)
+    uint32_t c0 = bstart[bid << R], c1 = bstart[(bid + 1) << R];
+    uint32_t n0 = c1, n1 = c1, m0 = c1, m1 = c1;
     if (nseg > 1) {
-        nn0 = bstart[(bid + cstride) << R];
-        nn1 = bstart[(bid + cstride + 1) << R];
+        n0 = bstart[(bid + cstride) << R];
+        n1 = bstart[(bid + cstride + 1) << R];
     }
-    Pos xs[3] = {0, 0, 0};
-    uint32_t idv = 0u;
-    double pv[3] = {0, 0, 0};
-    auto fetch = [&](uint32_t i) {
-        idv = idi[i];
+    if (nseg > 2) {
+        m0 = bstart[(bid + 2 * cstride) << R];
+        m1 = bstart[(bid + 2 * cstride + 1) << R];
+    }
+    auto rounds = [](uint32_t a, uint32_t b) -> int {
+        const int n = (int)((b - a + NT - 1) / NT);
+        return n > 0 ? n : 1;
+    };
+    struct PSet {
+        Pos x[3];
+        double p[3];
+        uint32_t id;
+    };
+    auto fetch = [&](PSet& d, uint32_t i) {
+        d.id = idi[i];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            xs[k] = X[k * np + i];
-            pv[k] = P[k * np + i];
+            d.x[k] = X[k * np + i];
+            d.p[k] = P[k * np + i];
         }
     };
-    if (s0 + tid < s1) fetch(perm[s0 + tid]);
+    int j = 0, r = 0, nrr = rounds(c0, c1);
+    // slot of the thread in round r + dr (dr = 1, 2) of the pipeline, counted
+    // over the current brick and the next two; ~0u: no such particle
+    auto slot_ahead = [&](int dr) -> uint32_t {
+        int rr = r + dr;
+        if (rr < nrr) {
+            const uint32_t s = c0 + (uint32_t)rr * NT + tid;
+            return s < c1 ? s : ~0u;
+        }
+        rr -= nrr;
+        if (j + 1 >= nseg) return ~0u;
+        const int nr1 = rounds(n0, n1);
+        if (rr < nr1) {
+            const uint32_t s = n0 + (uint32_t)rr * NT + tid;
+            return s < n1 ? s : ~0u;
+        }
+        rr -= nr1;
+        if (j + 2 >= nseg) return ~0u;
+        const uint32_t s = m0 + (uint32_t)rr * NT + tid;  // rr = 0 here (dr <= 2)
+        return s < m1 ? s : ~0u;
+    };
+    PSet pa0, pa1;
+    uint32_t q0, q1;  // storage index of the particle of the next round
+    {
+        const uint32_t s = c0 + tid;
+        fetch(pa0, perm[s < c1 ? s : plast]);
+        const uint32_t s1 = slot_ahead(1);
+        q0 = perm[s1 != ~0u ? s1 : plast];
+    }
     if (tid == 0) sm.dqn = 0u;
     load_planes(0, TP, bc.x0[LA], 0);
     __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
     __syncthreads();
-    auto process = [&](uint32_t s) {
-        ido[s] = idv;
+    auto process = [&](uint32_t s, const PSet& d) {
+        ido[s] = d.id;
         int o[3] = {0, 0, 0};
         T a1[3] = {0, 0, 0};
 #pragma unroll
-        for (int k = 0; k < R; ++k) PP::gat(xs[k], pa, o[k], a1[k]);
+        for (int k = 0; k < R; ++k) PP::gat(d.x[k], pa, o[k], a1[k]);
         // tile coordinates of the stencil origin (cell o - 2 along each axis)
         const int lx = (o[0] - bc.x0[0]) & msk, ly = (o[1] - bc.x0[1]) & msk;
         const int lz = (R == 3) ? ((o[2] - bc.x0[2]) & msk) : 0;
@@ -753,16 +801,16 @@ __global__ __launch_bounds__(NT) void k_gather_kick_brick(
         Pos xn[3] = {0, 0, 0};
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            const double p = brick_kick<PP>(pv[k], (double)F[k], ckq, pq);
+            const double p = brick_kick<PP>(d.p[k], (double)F[k], ckq, pq);
             Po[k * np + s] = p;
-            xn[k] = cd_on ? brick_drift<PP>(xs[k], p, cd) : xs[k];
+            xn[k] = cd_on ? brick_drift<PP>(d.x[k], p, cd) : d.x[k];
             Xo[k * np + s] = xn[k];
         }
         if (keys) {
             int co[3] = {0, 0, 0}, cn[3] = {0, 0, 0};
 #pragma unroll
             for (int k = 0; k < R; ++k) {
-                co[k] = brick_cell<PP>(xs[k], pa, msk);
+                co[k] = brick_cell<PP>(d.x[k], pa, msk);
                 cn[k] = brick_cell<PP>(xn[k], pa, msk);
             }
             const uint32_t ko = brick_key(co[0], co[1], co[2], R, pa), kn = brick_key(cn[0], cn[1], cn[2], R, pa);
@@ -773,50 +821,56 @@ __global__ __launch_bounds__(NT) void k_gather_kick_brick(
             }
         }
     };
-    const uint32_t plast = (uint32_t)(np - 1);
-    for (int j = 0;; ++j) {
+    // One round with the particle in cur (its storage index came with the
+    // round before); qc: storage index of the next round's particle.  Returns
+    // false after the segment's last round.
+    auto round = [&](const PSet& cur, PSet& nxt, uint32_t qc, uint32_t& qn) -> bool {
+        const bool last = r + 1 == nrr;
         const bool more = j + 1 < nseg;
-        const int nr = (int)((s1 - s0 + NT - 1) / NT);
-        uint32_t npi;
-        // (npi is loaded unconditionally: a value merged from two paths would
-        // be copied, and the copy would wait for the load at once)
-        auto issue_next = [&]() {
+        if (last) {
             __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
             if (more) {
                 int rn = rot + B;
                 rn = rn >= TR ? rn - TR : rn;
                 load_new_planes(bc.x0[LA] + B, rn);
             }
-            const uint32_t t = nn0 + tid;
-            npi = perm[t < plast ? t : plast];
-        };
-        // all rounds but the last: every thread has a particle
-        for (int r = 0; r + 1 < nr; ++r) {
-            const uint32_t s = s0 + (uint32_t)r * NT + tid;
-            process(s);
-            if (s + NT < s1) fetch(perm[s + NT]);
         }
-        // the last round, a separate copy of the body so that the compiler
-        // sees no outstanding load of its own in it
-        issue_next();
-        if (nr > 0) {
-            const uint32_t s = s0 + (uint32_t)(nr - 1) * NT + tid;
-            if (s < s1) process(s);
+        fetch(nxt, qc);
+        const uint32_t s2 = slot_ahead(2);
+        qn = perm[s2 != ~0u ? s2 : plast];
+        const uint32_t s = c0 + (uint32_t)r * NT + tid;
+        if (s < c1) process(s, cur);
+        if (!last) {
+            ++r;
+            return true;
         }
-        if (!more) break;
-        if (nn0 + tid < nn1) fetch(npi);
-        s0 = nn0;
-        s1 = nn1;
-        if (j + 2 < nseg) {
-            nn0 = bstart[(bid + 2 * cstride) << R];
-            nn1 = bstart[(bid + 2 * cstride + 1) << R];
+        if (!more) return false;
+        // the next brick: its planes have landed once every wave has waited
+        // for its own direct loads and passed the barrier
+        c0 = n0;
+        c1 = n1;
+        n0 = m0;
+        n1 = m1;
+        if (j + 3 < nseg) {
+            m0 = bstart[(bid + 3 * cstride) << R];
+            m1 = bstart[(bid + 3 * cstride + 1) << R];
+        } else {
+            m0 = m1 = n1;
         }
+        ++j;
+        r = 0;
+        nrr = rounds(c0, c1);
         __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);  // this wave's direct loads have landed
         __syncthreads();
         rot += B;
         rot = rot >= TR ? rot - TR : rot;
         bid += cstride;
         bc.x0[LA] += B;
+        return true;
+    };
+    for (;;) {
+        if (!round(pa0, pa1, q0, q1)) break;
+        if (!round(pa1, pa0, q1, q0)) break;
     }
     // the queued particles of the segment, from the mesh
     __syncthreads();

@@ -266,10 +266,10 @@ int RadixSorter::sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, hi
 
 // ------------------------------------------------------------ incremental re-sort
 // Keys that were sorted one step ago change for few elements (particles that
-// crossed into another brick or face class).  With prevkeys the sorted keys of
-// the last sort, in the current storage order, and bprev their group starts,
-// the stayers of group g keep their order at the front of the group and the
-// movers follow them.
+// crossed into another brick or face class).  The storage order is the order
+// of the last sort, whose group starts bprev give each position its previous
+// key; the stayers of group g keep their order at the front of the group and
+// the movers follow them.
 namespace cpm {
 
 __global__ __launch_bounds__(256) void k_resort_counts(const uint32_t* __restrict__ bprev,
@@ -292,21 +292,23 @@ __global__ __launch_bounds__(256) void k_resort_counts(const uint32_t* __restric
 // a pass in flight together; a block scan of the stayer flags (wave ballots)
 // gives each stayer its rank among the stayers of its group (they keep their
 // order); a mover takes the next free slot after the stayers of its new group.
+// The previous key of position s is the group of bprev that holds s (the
+// brick's G group starts, in LDS), so the previous keys are not read.
 static constexpr int RS_NT = 256;
 static constexpr int RS_E = 4;
 __global__ __launch_bounds__(RS_NT) void k_resort_place(const uint32_t* __restrict__ keys,
-                                                        const uint32_t* __restrict__ prevkeys,
                                                         const uint32_t* __restrict__ bprev,
                                                         const uint32_t* __restrict__ bnew,
                                                         const uint32_t* __restrict__ stay, uint32_t* __restrict__ fill,
-                                                        int G, uint32_t* __restrict__ perm,
-                                                        uint32_t* __restrict__ skeys) {
+                                                        int G, uint32_t* __restrict__ perm) {
     __shared__ uint32_t before[8];  // stayers of the brick's earlier groups
     __shared__ uint32_t nstart[8];  // new starts of the brick's groups
+    __shared__ uint32_t ostart[9];  // previous starts of the brick's groups (and the next brick's)
     __shared__ uint32_t wsum[RS_E][RS_NT / 64];
     const uint32_t g0 = blockIdx.x * (uint32_t)G;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (threadIdx.x < (unsigned)G) nstart[threadIdx.x] = bnew[g0 + threadIdx.x];
+    if (threadIdx.x <= (unsigned)G) ostart[threadIdx.x] = bprev[g0 + threadIdx.x];
     if (threadIdx.x == 0) {
         uint32_t a = 0;
         for (int c = 0; c < G; ++c) {
@@ -316,6 +318,12 @@ __global__ __launch_bounds__(RS_NT) void k_resort_place(const uint32_t* __restri
     }
     const uint32_t r0 = bprev[g0], r1 = bprev[g0 + G];
     __syncthreads();
+    // previous key of position s of the brick's range
+    auto prev_key = [&](uint32_t s) -> uint32_t {
+        uint32_t c = 0;
+        for (int k = 1; k < G; ++k) c += (s >= ostart[k]) ? 1u : 0u;
+        return g0 + c;
+    };
     uint32_t carry = 0;
     for (uint32_t base = r0; base < r1; base += RS_E * RS_NT) {
         uint32_t key[RS_E], pk[RS_E];
@@ -323,7 +331,7 @@ __global__ __launch_bounds__(RS_NT) void k_resort_place(const uint32_t* __restri
         for (int e = 0; e < RS_E; ++e) {
             const uint32_t s = base + e * RS_NT + threadIdx.x;
             key[e] = s < r1 ? keys[s] : 0u;
-            pk[e] = s < r1 ? prevkeys[s] : 1u;
+            pk[e] = s < r1 ? prev_key(s) : 1u;
         }
         uint32_t lanepre[RS_E];
 #pragma unroll
@@ -352,7 +360,6 @@ __global__ __launch_bounds__(RS_NT) void k_resort_place(const uint32_t* __restri
                 else
                     pos = bnew[key[e]] + stay[key[e]] + atomicAdd(&fill[key[e]], 1u);
                 perm[pos] = s;
-                skeys[pos] = key[e];
             }
             pre += total;
         }
@@ -373,11 +380,30 @@ void resort_counts(const uint32_t* bprev, const uint32_t* in_count, const uint32
     CPM_LAUNCH_CHECK();
 }
 
-void resort_place(const uint32_t* keys, const uint32_t* prevkeys, const uint32_t* bprev, const uint32_t* bnew,
-                  const uint32_t* stay, uint32_t* fill, long nbricks, int G, uint32_t* perm, uint32_t* skeys,
-                  hipStream_t st) {
-    hipLaunchKernelGGL(k_resort_place, dim3((unsigned)nbricks), dim3(RS_NT), 0, st, keys, prevkeys, bprev, bnew, stay,
-                       fill, G, perm, skeys);
+void resort_place(const uint32_t* keys, const uint32_t* bprev, const uint32_t* bnew, const uint32_t* stay,
+                  uint32_t* fill, long nbricks, int G, uint32_t* perm, hipStream_t st) {
+    hipLaunchKernelGGL(k_resort_place, dim3((unsigned)nbricks), dim3(RS_NT), 0, st, keys, bprev, bnew, stay, fill, G,
+                       perm);
+    CPM_LAUNCH_CHECK();
+}
+
+// skeys[s] = the group of bstart that holds position s, one workgroup per
+// brick of G groups.
+__global__ __launch_bounds__(256) void k_group_keys(const uint32_t* __restrict__ bstart, int G,
+                                                    uint32_t* __restrict__ skeys) {
+    __shared__ uint32_t gs[9];
+    const uint32_t g0 = blockIdx.x * (uint32_t)G;
+    if (threadIdx.x <= (unsigned)G) gs[threadIdx.x] = bstart[g0 + threadIdx.x];
+    __syncthreads();
+    for (uint32_t s = gs[0] + threadIdx.x; s < gs[G]; s += 256) {
+        uint32_t c = 0;
+        for (int k = 1; k < G; ++k) c += (s >= gs[k]) ? 1u : 0u;
+        skeys[s] = g0 + c;
+    }
+}
+
+void group_keys(const uint32_t* bstart, long nbricks, int G, uint32_t* skeys, hipStream_t st) {
+    hipLaunchKernelGGL(k_group_keys, dim3((unsigned)nbricks), dim3(256), 0, st, bstart, G, skeys);
     CPM_LAUNCH_CHECK();
 }
 

@@ -32,19 +32,21 @@ namespace cpm {
 
 // Incremental re-sort of keys sorted one step ago (the gather of the FX step
 // moves the particles into the sorted order, so the next step's keys are in
-// nearly sorted order).  prevkeys: the last sorted keys, indexed like keys;
-// bprev: their group starts (ngroups + 1); in_count / out_count: movers into
-// / out of each group (keys_fx).
+// nearly sorted order).  bprev: the group starts of the last sort (ngroups +
+// 1), so the previous key of position s is the group that holds s;
+// in_count / out_count: movers into / out of each group (keys_fx).
 // resort_counts: stay[g] = stayers of g, tot[g] = stay[g] + in_count[g]
 // (tot[ngroups] = 0; its exclusive scan gives the new group starts bnew).
-// resort_place, one workgroup per brick of G = 2^R groups: perm[pos] = s,
-// skeys[pos] = keys[s]; stayers keep their order at the front of their group,
-// movers follow in atomic order (fill: zeroed per-group counters).
+// resort_place, one workgroup per brick of G = 2^R groups: perm[pos] = s;
+// stayers keep their order at the front of their group, movers follow in
+// atomic order (fill: zeroed per-group counters).
+// group_keys: skeys[s] = the group of bstart that holds s (the sorted keys,
+// for the kernels that compare new keys with them).
 void resort_counts(const uint32_t* bprev, const uint32_t* in_count, const uint32_t* out_count, long ngroups,
                    uint32_t* stay, uint32_t* tot, hipStream_t st);
-void resort_place(const uint32_t* keys, const uint32_t* prevkeys, const uint32_t* bprev, const uint32_t* bnew,
-                  const uint32_t* stay, uint32_t* fill, long nbricks, int G, uint32_t* perm, uint32_t* skeys,
-                  hipStream_t st);
+void resort_place(const uint32_t* keys, const uint32_t* bprev, const uint32_t* bnew, const uint32_t* stay,
+                  uint32_t* fill, long nbricks, int G, uint32_t* perm, hipStream_t st);
+void group_keys(const uint32_t* bstart, long nbricks, int G, uint32_t* skeys, hipStream_t st);
 void zero_u32(uint32_t* p, long n, hipStream_t st);
 
 }  // namespace cpm
