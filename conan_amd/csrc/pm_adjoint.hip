@@ -179,6 +179,137 @@ __global__ __launch_bounds__(VEC_NT) void k_deposit_fx_vec(const uint32_t* __res
     }
 }
 
+// The vector deposit of lP and the mass deposit of the same positions in one
+// launch, for the backward pass that recomputes the potential: two
+// workgroups per brick, each with two accumulator tiles (64 KB of LDS, so two
+// 1024-thread workgroups share a CU where one k_deposit_fx_vec fills it).
+// Workgroup 2b deposits lP components 0 and 1 of brick b; workgroup 2b + 1
+// component 2 (3-D) and the mass.  The mass channel is k_deposit_fx_brick's:
+// the same weights and quanta, the same FP32 cell values, and the brick's
+// partial sum in the same order (threads 0..511 sum as that kernel's 512
+// threads do, the waves in order), so the mesh and the partials are
+// bit-identical to a separate mass deposit.
+template <int R>
+__global__ __launch_bounds__(VEC_NT) void k_deposit_fx_vm(const uint32_t* __restrict__ bstart,
+                                                          const uint32_t* __restrict__ perm,
+                                                          const uint32_t* __restrict__ Y,
+                                                          const float* __restrict__ lP, long np, int lgN,
+                                                          PosArgs pa, const uint32_t* __restrict__ lmax,
+                                                          float* __restrict__ s, long M, float mass,
+                                                          float* __restrict__ mesh, double* __restrict__ partial) {
+    static_assert(VEC_NT >= FXB_NT, "the mass channel's output uses FXB_NT threads");
+    __shared__ unsigned long long acc[2][FXB_CELLS];
+    __shared__ BrickRanges rg;
+    __shared__ double red[VEC_NT / 64];
+    const int N = 1 << lgN, msk = N - 1;
+    const uint32_t brick = blockIdx.x >> 1;
+    const int half = (int)(blockIdx.x & 1);
+    const BrickCoord bc = brick_coord(brick, pa);
+    const int ncell = bc.B[0] * bc.B[1] * bc.B[2];
+    const int tid = threadIdx.x;
+    // this workgroup's lP components: c0, c0 + 1, .. (nl of them); the mass in acc[1] (half 1)
+    const int c0 = 2 * half;
+    const int nl = half ? (R == 3 ? 1 : 0) : 2;
+    const bool hm = half == 1;
+    for (int e = tid; e < ncell; e += VEC_NT) {
+        acc[0][e] = 0ull;
+        acc[1][e] = 0ull;
+    }
+    const float mx = __uint_as_float(*lmax);
+    double scale = 0.0;
+    if (mx > 0.f) {
+        int ex = 0;
+        (void)frexpf(mx, &ex);  // mx < 2^ex
+        scale = ldexp(1.0, 40 - ex);
+    }
+    brick_ranges<R>(rg, bstart, bc, pa, tid);
+    const uint32_t total = rg.roff[rg.n];
+    for (uint32_t t = tid; t < total; t += VEC_NT) {
+        const uint32_t sp = brick_range_pos(rg, t);
+        const uint32_t i = perm[sp];
+        int o[3] = {0, 0, 0};
+        float f[3] = {0.f, 0.f, 0.f};
+        double lam[2] = {0, 0};
+#pragma unroll
+        for (int k = 0; k < R; ++k) PosFX::dep(Y[k * np + i], pa, o[k], f[k]);
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            if (q < nl) lam[q] = (double)lP[(c0 + q) * np + i] * scale;
+#pragma unroll
+        for (int j = 0; j < (1 << R); ++j) {
+            int l[3];
+            bool in = true;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                l[k] = (k < R) ? (((o[k] + ((j >> k) & 1)) - bc.x0[k]) & msk) : 0;
+                in &= l[k] < bc.B[k];
+            }
+            if (!in) continue;
+            const int cell = l[0] + bc.B[0] * (l[1] + bc.B[1] * l[2]);
+            float w = 1.f;  // as k_deposit_fx_vec
+#pragma unroll
+            for (int k = 0; k < R; ++k) w *= ((j >> k) & 1) ? f[k] : 1.f - f[k];
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+                if (q < nl) atomicAdd(&acc[q][cell], (unsigned long long)__double2ll_rn((double)w * lam[q]));
+            if (hm) {
+                float wm = mass;  // as k_deposit_fx_brick
+#pragma unroll
+                for (int k = 0; k < R; ++k) wm *= ((j >> k) & 1) ? f[k] : 1.f - f[k];
+                atomicAdd(&acc[1][cell], fx_quantum(wm));
+            }
+        }
+    }
+    __syncthreads();
+    const double inv = scale > 0 ? 1.0 / scale : 0.0;  // exact: scale is a power of two
+    auto cell_index = [&](int e) -> long {
+        const int lx = e & (bc.B[0] - 1), ly = (e >> pa.lb[0]) & (bc.B[1] - 1), lz = e >> (pa.lb[0] + pa.lb[1]);
+        return (long)(bc.x0[0] + lx) + (long)N * ((bc.x0[1] + ly) + (long)N * (bc.x0[2] + lz));
+    };
+    if (!hm) {
+        for (int e = tid; e < ncell; e += VEC_NT) {
+            const long idx = cell_index(e);
+#pragma unroll
+            for (int q = 0; q < 2; ++q) s[(c0 + q) * M + idx] = (float)((double)(long long)acc[q][e] * inv);
+        }
+        return;
+    }
+    // half 1: threads FXB_NT.. write lP component 2 (3-D); threads 0..FXB_NT-1
+    // write the mass channel exactly as k_deposit_fx_brick does
+    double rsum = 0;
+    if (tid >= FXB_NT) {
+        if (R == 3)
+            for (int e = tid - FXB_NT; e < ncell; e += VEC_NT - FXB_NT)
+                s[2 * M + cell_index(e)] = (float)((double)(long long)acc[0][e] * inv);
+    } else if (R == 3 && bc.B[0] * bc.B[1] == FXB_NT) {
+        const int lx = tid & (bc.B[0] - 1), ly = tid >> pa.lb[0];
+        float* __restrict__ out = mesh + (long)(bc.x0[0] + lx) + (long)N * ((bc.x0[1] + ly) + (long)N * bc.x0[2]);
+        const long plane = (long)N * N;
+        for (int lz = 0; lz < bc.B[2]; ++lz) {
+            const float v = (float)((double)(long long)acc[1][tid + lz * FXB_NT] * (1.0 / FX_ONE));
+            out[lz * plane] = v;
+            rsum += (double)v;
+        }
+    } else {
+        for (int e = tid; e < ncell; e += FXB_NT) {
+            const float v = (float)((double)(long long)acc[1][e] * (1.0 / FX_ONE));
+            mesh[cell_index(e)] = v;
+            rsum += (double)v;
+        }
+    }
+    // block_sum<FXB_NT> over threads 0..FXB_NT-1 (the other waves add 0 and are not read)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) rsum += __shfl_down(rsum, off);
+    if ((tid & 63) == 0) red[tid >> 6] = rsum;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0;
+#pragma unroll
+        for (int w = 0; w < FXB_NT / 64; ++w) t += red[w];
+        partial[brick] = t;
+    }
+}
+
 // ------------------------------------------------------------ -div (FP32 mesh)
 template <int R>
 __global__ __launch_bounds__(256) void k_neg_div_f32(const float* __restrict__ s, int lgN, long M,
@@ -589,6 +720,16 @@ void deposit_fx_vec(int R, const uint32_t* bstart, const uint32_t* perm, const u
     const long M = 1L << (R * lgN);
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit_fx_vec<RR>), dim3(nb), dim3(VEC_NT), 0, st, bstart, perm, Y,
                                          lP, np, lgN, pa, lmax, s, M));
+    CPM_LAUNCH_CHECK();
+}
+
+void deposit_fx_vm(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* Y, const float* lP, long np,
+                   int lgN, const PosArgs& pa, const uint32_t* lmax, float* s, float mass, float* mesh,
+                   double* partial, hipStream_t st) {
+    const unsigned nb = (unsigned)brick_count(pa);
+    const long M = 1L << (R * lgN);
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_deposit_fx_vm<RR>), dim3(2 * nb), dim3(VEC_NT), 0, st, bstart, perm, Y,
+                                         lP, np, lgN, pa, lmax, s, M, mass, mesh, partial));
     CPM_LAUNCH_CHECK();
 }
 

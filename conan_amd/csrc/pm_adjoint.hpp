@@ -25,6 +25,12 @@ void rev_drift_keys(int R, uint32_t* Y, const double* P, const float* lY, float*
 void deposit_fx_vec(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* Y, const float* lP,
                     long np, int lgN, const PosArgs& pa, const uint32_t* lmax, float* s, hipStream_t st);
 
+// deposit_fx_vec and, in the same launch, deposit_fx_brick's mass deposit of
+// the same positions (mesh and per-brick partials bit-identical to it).
+void deposit_fx_vm(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* Y, const float* lP, long np,
+                   int lgN, const PosArgs& pa, const uint32_t* lmax, float* s, float mass, float* mesh,
+                   double* partial, hipStream_t st);
+
 // psi = -sum_k D_k s_k with D_k the FP32 4-point derivative of the gather:
 // the adjoint of the FD gradient, applied to R meshes.
 void neg_div_f32(int R, const float* s, int lgN, double res, float* psi, hipStream_t st);

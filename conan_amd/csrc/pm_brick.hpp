@@ -27,6 +27,18 @@ static constexpr int FXB_CELLS = FXB_X * FXB_Y * FXB_Z;
 static constexpr int FXB_TILE = (FXB_X + 5) * (FXB_Y + 5) * (FXB_Z + 5);
 static constexpr double FX_ONE = 1099511627776.0;  // 2^40: fixed-point unit of the deposit accumulators
 
+// round(w 2^40) (ties to even) as the 64-bit deposit quantum, for
+// 0 <= w < 2^23, in FP32 operations: a = w 2^8 is exact, hi = floor(a) and
+// a - hi are exact, (a - hi) 2^32 < 2^32 is exact, and rounding it to an
+// integer rounds w 2^40, whose integer part hi 2^32 needs no rounding.
+// Equal to __double2ll_rn((double)w * 2^40), without FP64 operations.
+__device__ __forceinline__ unsigned long long fx_quantum(float w) {
+    const float a = w * 256.f;
+    const float hi = floorf(a);
+    const float lo = rintf((a - hi) * 4294967296.f);
+    return ((unsigned long long)(uint32_t)hi << 32) + (uint32_t)lo;
+}
+
 struct BrickCoord {
     int x0[3], B[3], nb[3], bi[3];
 };

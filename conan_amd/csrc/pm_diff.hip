@@ -347,14 +347,16 @@ struct DiffPM {
             const float* phi = nullptr;
             if (store_phi) {
                 phi = phis[i]->p;
+                deposit_fx_vec(R, sim.bstart, sim.spids, sim.X.p, lP.p, Np, sim.lgNf, sim.pa, lmax.p, svec.p, st);
             } else {
-                deposit_fx_brick<float>(R, sim.bstart, sim.spids, sim.X.p, Np, sim.lgNf, sim.pa, (float)m,
-                                        sim.mesh.p, sim.row_partial.p, st);
+                // the deposit of lP and the mass deposit of the same positions
+                // in one launch (the mesh and partials of deposit_fx_brick)
+                deposit_fx_vm(R, sim.bstart, sim.spids, sim.X.p, lP.p, Np, sim.lgNf, sim.pa, lmax.p, svec.p,
+                              (float)m, sim.mesh.p, sim.row_partial.p, st);
                 sim.npartials = brick_count(sim.pa);
                 sim.do_poisson(ak);  // phi_i, as in the forward
                 phi = sim.mesh.p;
             }
-            deposit_fx_vec(R, sim.bstart, sim.spids, sim.X.p, lP.p, Np, sim.lgNf, sim.pa, lmax.p, svec.p, st);
             neg_div_f32(R, svec.p, sim.lgNf, sim.pa.res, psi.p, st);
             poisson_into(psi.p, chi.p, ak);
             const AdjCoef coef{ck, -ck, -ck * m, sim.pq, 1.0 / sim.pq};

@@ -220,18 +220,6 @@ __device__ __forceinline__ double fx128_value(unsigned long long hi, unsigned lo
 // Particles move less than a cell per step, so `perm` (pm_brick.hpp) is close
 // to the identity and reads through it are nearly contiguous.
 
-// round(w 2^40) (ties to even) as the 64-bit deposit quantum, for
-// 0 <= w < 2^23, in FP32 operations: a = w 2^8 is exact, hi = floor(a) and
-// a - hi are exact, (a - hi) 2^32 < 2^32 is exact, and rounding it to an
-// integer rounds w 2^40, whose integer part hi 2^32 needs no rounding.
-// Equal to __double2ll_rn((double)w * 2^40), without FP64 operations.
-__device__ __forceinline__ unsigned long long fx_quantum(float w) {
-    const float a = w * 256.f;
-    const float hi = floorf(a);
-    const float lo = rintf((a - hi) * 4294967296.f);
-    return ((unsigned long long)(uint32_t)hi << 32) + (uint32_t)lo;
-}
-
 // Deposit, one workgroup per brick.  The brick's cells receive the CIC
 // weights of the particles of the brick and of its lower neighbours (a
 // particle reaches its origin cell and the next one along each axis); only
