@@ -342,8 +342,11 @@ __global__ __launch_bounds__(256) void k_neg_div_f32(const float* __restrict__ s
 static constexpr int FXB_CTILE = (FXB_X + 1) * (FXB_Y + 1) * (FXB_Z + 1);
 
 // HASH: dr.hash is set (a separate instance, so that the checksum's
-// registers do not weigh on the launches without it).
-template <int R, bool HASH>
+// registers do not weigh on the launches without it).  FULL: every brick has
+// the full FX size (the mesh is at least a brick along each axis), so the
+// tile strides are compile-time constants and the tile reads take their
+// offsets as immediates.
+template <int R, bool HASH, bool FULL>
 __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_gather_fx_brick(
     const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ Y,
     const double* __restrict__ P, const float* __restrict__ lY, const float* __restrict__ lP,
@@ -358,8 +361,10 @@ __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) voi
     const int N = 1 << lgN, msk = N - 1;
     const uint32_t bid = blockIdx.x;
     const BrickCoord bc = brick_coord(bid, pa);
-    const int TX = bc.B[0] + 5, TY = bc.B[1] + 5, TZ = (R == 3) ? bc.B[2] + 5 : 1;
-    const int CX = bc.B[0] + 1, CY = bc.B[1] + 1, CZ = (R == 3) ? bc.B[2] + 1 : 1;
+    constexpr int FB0 = FXB_X, FB1 = (R == 3) ? FXB_Y : FXB_X, FB2 = (R == 3) ? FXB_Z : 1;
+    const int B0 = FULL ? FB0 : bc.B[0], B1 = FULL ? FB1 : bc.B[1], B2 = FULL ? FB2 : bc.B[2];
+    const int TX = B0 + 5, TY = B1 + 5, TZ = (R == 3) ? B2 + 5 : 1;
+    const int CX = B0 + 1, CY = B1 + 1, CZ = (R == 3) ? B2 + 1 : 1;
     const int tid = threadIdx.x;
     const bool hp = phi != nullptr, hc = chi != nullptr;
     // A thread's first particle is loaded before the tiles are staged, so the
@@ -407,16 +412,13 @@ __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) voi
             for (int k = 0; k < R; ++k) w *= ((j >> k) & 1) ? a1[k] : 1.f - a1[k];
             z[j] = w;
         }
-        // dW(c_j)/dY_k, formed where it is used
-        auto dzf = [&](int k, int j) -> double {
-            double w = ((j >> k) & 1) ? ires : -ires;
-#pragma unroll
-            for (int m = 0; m < R; ++m)
-                if (m != k) w *= ((j >> m) & 1) ? (double)a1[m] : 1.0 - (double)a1[m];
-            return w;
-        };
+        // X[j] = cT h_j + cU chi(c_j): what multiplies dW(c_j)/dY_k in both
+        // terms of lY's update, T = sum_j dW(c_j)/dY_k h_j (h_j = lP . g(c_j))
+        // and U = sum_j dW(c_j)/dY_k chi(c_j)
         float F[3] = {0.f, 0.f, 0.f};
-        double T[3] = {0, 0, 0}, U[3] = {0, 0, 0};
+        double X[NC];
+#pragma unroll
+        for (int j = 0; j < NC; ++j) X[j] = 0;
         if (hp) {
             float g[3][NC];
             const int zc = (R == 3) ? 2 : 0;
@@ -467,8 +469,7 @@ __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) voi
                 double h = 0;
 #pragma unroll
                 for (int q = 0; q < R; ++q) h += lam[q] * (double)g[q][j];
-#pragma unroll
-                for (int k = 0; k < R; ++k) T[k] += dzf(k, j) * h;
+                X[j] = cf.cT * h;
             }
         }
         if (hc) {
@@ -476,16 +477,40 @@ __global__ __launch_bounds__(FXG_NT) __attribute__((amdgpu_waves_per_eu(4))) voi
             for (int j = 0; j < NC; ++j) {
                 const int bx = j & 1, by = (j >> 1) & 1, bz = (j >> 2) & 1;
                 const double cv = (double)ctile[(lx + bx) + CX * ((ly + by) + CY * (lz + (R == 3 ? bz : 0)))];
-#pragma unroll
-                for (int k = 0; k < R; ++k) U[k] += dzf(k, j) * cv;
+                X[j] += cf.cU * cv;
             }
+        }
+        // V[k] = sum_j dW(c_j)/dY_k X[j].  dW(c_j)/dY_k = +-(1/res) prod_{m != k}
+        // A_m(c_j), + where c_j is the upper corner along k: the corners pair
+        // up along k with one product, V[k] = (1/res) sum over the upper
+        // corners j of prod_{m != k} A_m(c_j) (X[j] - X[j - 2^k]).
+        double A[3][2];
+#pragma unroll
+        for (int m = 0; m < R; ++m) {
+            A[m][1] = (double)a1[m];
+            A[m][0] = 1.0 - (double)a1[m];
+        }
+        double V[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            double v = 0;
+#pragma unroll
+            for (int j = 0; j < NC; ++j) {
+                if (!((j >> k) & 1)) continue;
+                double w = 1.0;
+#pragma unroll
+                for (int m = 0; m < R; ++m)
+                    if (m != k) w *= A[m][(j >> m) & 1];
+                v += w * (X[j] - X[j ^ (1 << k)]);
+            }
+            V[k] = v * ires;
         }
         uint32_t yn[3] = {0u, 0u, 0u};
         unsigned long long h = hh ? hash_start(idv) : 0ull;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const double p = pin[k] + fx_kick_increment((double)F[k], cFq, cf.pq);
-            const float lyk = (float)((double)lyin[k] + cf.cT * T[k] + cf.cU * U[k]);
+            const float lyk = (float)((double)lyin[k] + V[k]);
             float lpk = (float)lam[k];
             if (hh) h = hash_axis(h, ys[k], p);
             yn[k] = ys[k];
@@ -747,15 +772,25 @@ void adj_gather_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, co
                          int lgN, const PosArgs& pa, AdjCoef c, AdjDrift dr, uint32_t* lmax, hipStream_t st) {
     const unsigned nb = (unsigned)brick_count(pa);
     if (dr.hash) zero_u32(reinterpret_cast<uint32_t*>(dr.hash), 2, st);
+    // every brick at the full FX size: compile-time tile strides
+    const bool full = (1 << pa.lb[0]) == FXB_X && (1 << pa.lb[1]) == (R == 3 ? FXB_Y : FXB_X) &&
+                      (R < 3 || (1 << pa.lb[2]) == FXB_Z);
+#define CPM_ADJ_LAUNCH(H, F)                                                                                       \
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_adj_gather_fx_brick<RR, H, F>), dim3(nb), dim3(FXG_NT), 0, st, bstart, \
+                                         perm, Y, P, lY, lP, idi, Yo, Po, lYo, lPo, ido, np, phi, chi, lgN, pa, c,  \
+                                         dr))
     if (dr.hash) {
-        CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_adj_gather_fx_brick<RR, true>), dim3(nb), dim3(FXG_NT), 0, st, bstart,
-                                             perm, Y, P, lY, lP, idi, Yo, Po, lYo, lPo, ido, np, phi, chi, lgN, pa, c,
-                                             dr));
+        if (full)
+            CPM_ADJ_LAUNCH(true, true);
+        else
+            CPM_ADJ_LAUNCH(true, false);
     } else {
-        CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_adj_gather_fx_brick<RR, false>), dim3(nb), dim3(FXG_NT), 0, st, bstart,
-                                             perm, Y, P, lY, lP, idi, Yo, Po, lYo, lPo, ido, np, phi, chi, lgN, pa, c,
-                                             dr));
+        if (full)
+            CPM_ADJ_LAUNCH(false, true);
+        else
+            CPM_ADJ_LAUNCH(false, false);
     }
+#undef CPM_ADJ_LAUNCH
     CPM_LAUNCH_CHECK();
     if (dr.bmax) {
         hipLaunchKernelGGL(k_max_u32, dim3(1), dim3(1024), 0, st, (const uint32_t*)dr.bmax, (long)nb, lmax);
