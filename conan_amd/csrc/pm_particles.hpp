@@ -81,13 +81,6 @@ inline bool divides_by_fma(double b) {
     while (m && !(m & 1ull)) m >>= 1;
     return m < (1ull << 50);
 }
-// fd4 with its divisions by 12 and 3 done by div3, and the one by res by
-// div_fma where that is exact: the same value, bit for bit (a divide costs
-// some ten instructions, div3 and div_fma four).  The FP64 force uses this one.
-__device__ __forceinline__ double fd4_div3(double fm2, double fm1, double fp1, double fp2, const PosArgs& a) {
-    double v = div3(fm2 * 0.25) - div3(2 * fm1) + div3(2 * fp1) - div3(fp2 * 0.25);
-    return a.fastdiv ? div_fma(v, a.res, a.rinv) : v / a.res;
-}
 // Same operator in FP32, arranged for fewer roundings: (f[-2] - f[+2] + 8 (f[+1] - f[-1])) / (12 res),
 // the sum fused with the multiplication by 8 (explicitly, so every kernel that
 // uses it rounds the same way whatever its contraction setting).

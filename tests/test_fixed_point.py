@@ -50,11 +50,12 @@ def test_fp32_rounding_special_values():
 
 
 # ---------------------------------------------------------------- div3
-# The FP64 force (pm_particles.hpp, div3 / fd4_div3) forms the reference's
-# f / 12 and 2 f / 3 (src/run/numeric.hh:98,101) as RN(x / 3) of x = f / 4 and
-# x = 2 f, from q0 = RN(x RN(1/3)), r = fma(-q0, 3, x), RN(q0 + r RN(1/3)),
-# with the sign of x.  Here fma is evaluated exactly with fractions and
-# rounded once (float(Fraction) rounds to nearest even).
+# The FP64 force (pm_particles.hip, force_f64) forms the reference's f / 12
+# and 2 f / 3 (src/run/numeric.hh:98,101) from one third per node,
+# t = RN(x / 3) of x = f / 4, with 2 f / 3 rounded = 8 t; RN(x / 3) is
+# q0 = RN(x RN(1/3)), r = fma(-q0, 3, x), RN(q0 + r RN(1/3)), with the sign
+# of x.  Here fma is evaluated exactly with fractions and rounded once
+# (float(Fraction) rounds to nearest even).
 from fractions import Fraction
 
 _C3 = float.fromhex("0x1.5555555555555p-2")  # RN(1/3)
@@ -96,6 +97,46 @@ def test_div3_is_the_rounded_quotient():
 def test_div3_without_its_correction_fails():
     # planted mistake: the bare product x RN(1/3) is off by an ulp for some x
     assert any(_div3(x, corrected=False) != x / 3.0 for x in _div3_cases())
+
+
+def test_one_third_per_node_gives_both_terms():
+    # RN(f / 12) = div3(f / 4), and RN(2 f / 3) = 8 of it exactly
+    for f in _div3_cases():
+        t = _div3(f * 0.25)
+        for got, ref in ((t, f / 12.0), (8.0 * t, 2.0 * f / 3.0)):
+            assert got == ref and np.signbit(got) == np.signbit(ref), (f, got, ref)
+
+
+def _fd4_reference(fm2, fm1, fp1, fp2):
+    # src/run/numeric.hh:98,101, left to right
+    return fm2 / 12. - 2 * fm1 / 3. + 2 * fp1 / 3. - fp2 / 12.
+
+
+def _fd4_thirds(fm2, fm1, fp1, fp2, scale=8.0):
+    t = [_div3(f * 0.25) for f in (fm2, fm1, fp1, fp2)]
+    return t[0] - scale * t[1] + scale * t[2] - t[3]
+
+
+def test_fd4_from_shared_thirds_is_the_reference_derivative():
+    r = np.random.default_rng(19)
+    # potentials of one magnitude (neighbouring nodes) and of mixed ones
+    for _ in range(4000):
+        e = int(r.integers(-60, 60))
+        f = r.standard_normal(4) * 2.0 ** e
+        if r.random() < 0.3:
+            f = f * 2.0 ** r.integers(-30, 30, 4)
+        f = [float(v) for v in f]
+        assert _fd4_thirds(*f) == _fd4_reference(*f), f
+    for f in ([0.0, -0.0, 0.0, -0.0], [1.0, 1.0, 1.0, 1.0], [3.0, -3.0, 3.0, -3.0]):
+        got, ref = _fd4_thirds(*f), _fd4_reference(*f)
+        assert got == ref and np.signbit(got) == np.signbit(ref)
+
+
+def test_fd4_from_thirds_with_a_wrong_scale_fails():
+    # planted mistake: 2 f / 3 taken as 4 thirds instead of 8
+    r = np.random.default_rng(23)
+    cases = [[float(v) for v in r.standard_normal(4)] for _ in range(100)]
+    assert any(_fd4_thirds(*f, scale=4.0) != _fd4_reference(*f) for f in cases)
 
 
 # ---------------------------------------------------------------- div_fma
