@@ -434,7 +434,7 @@ struct Impl : ImplBase {
             uint32_t* bnew = (bstart == row_start.p) ? bs2.p : row_start.p;
             resort_counts(bstart, cnt3.p, cnt3.p + nsort_groups, nsort_groups, stay.p, tot.p, st);
             exclusive_scan(tot.p, bnew, nsort_groups + 1, scan_scr.p, st);
-            resort_place(knew.p, bstart, bnew, stay.p, cnt3.p + 2 * nsort_groups, brick_count(pa), 1 << R, permb.p,
+            resort_place(knew.p, bstart, bnew, stay.p, cnt3.p + 2 * nsort_groups, brick_count(pa), 1 << R, Np, permb.p,
                          st);
             skeys_valid = false;
             spids = permb.p;

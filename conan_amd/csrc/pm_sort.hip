@@ -287,84 +287,73 @@ __global__ __launch_bounds__(256) void k_resort_counts(const uint32_t* __restric
     tot[g] = s + in_count[g];
 }
 
-// One workgroup per brick: the brick's previous range (its G groups, one
-// after the other) is walked in passes of RS_E x RS_NT elements, all loads of
-// a pass in flight together; a block scan of the stayer flags (wave ballots)
-// gives each stayer its rank among the stayers of its group (they keep their
-// order); a mover takes the next free slot after the stayers of its new group.
-// The previous key of position s is the group of bprev that holds s (the
-// brick's G group starts, in LDS), so the previous keys are not read.
-static constexpr int RS_NT = 256;
-static constexpr int RS_E = 4;
-__global__ __launch_bounds__(RS_NT) void k_resort_place(const uint32_t* __restrict__ keys,
-                                                        const uint32_t* __restrict__ bprev,
-                                                        const uint32_t* __restrict__ bnew,
-                                                        const uint32_t* __restrict__ stay, uint32_t* __restrict__ fill,
-                                                        int G, uint32_t* __restrict__ perm) {
-    __shared__ uint32_t before[8];  // stayers of the brick's earlier groups
-    __shared__ uint32_t nstart[8];  // new starts of the brick's groups
-    __shared__ uint32_t ostart[9];  // previous starts of the brick's groups (and the next brick's)
-    __shared__ uint32_t wsum[RS_E][RS_NT / 64];
-    const uint32_t g0 = blockIdx.x * (uint32_t)G;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (threadIdx.x < (unsigned)G) nstart[threadIdx.x] = bnew[g0 + threadIdx.x];
-    if (threadIdx.x <= (unsigned)G) ostart[threadIdx.x] = bprev[g0 + threadIdx.x];
-    if (threadIdx.x == 0) {
-        uint32_t a = 0;
-        for (int c = 0; c < G; ++c) {
-            before[c] = a;
-            a += stay[g0 + c];
-        }
+// One wave per brick (four bricks per workgroup): the brick's previous range
+// (its G groups, one after the other) is walked in passes of E x 64
+// elements, all loads of a pass in flight together; a scan of the stayer
+// flags (wave ballots, no barriers, no LDS) gives each stayer its rank among
+// the stayers of its group (they keep their order); a mover takes the next
+// free slot after the stayers of its new group.  The previous key of
+// position s is the group of bprev that holds s (the brick's G group starts,
+// in scalar registers), so the previous keys are not read.  Lane c <= G
+// reads the previous start of group g0 + c, and lanes c < G its new start and
+// its stayers; nb[c] = new start of group c minus the stayers of the brick's
+// groups before c, so a stayer's position is nb[c] plus its rank among the
+// brick's stayers.  E: elements per lane in flight per pass.
+template <int E>
+__global__ __launch_bounds__(256) void k_resort_place(const uint32_t* __restrict__ keys,
+                                                           const uint32_t* __restrict__ bprev,
+                                                           const uint32_t* __restrict__ bnew,
+                                                           const uint32_t* __restrict__ stay,
+                                                           uint32_t* __restrict__ fill, long nbricks, int G,
+                                                           uint32_t* __restrict__ perm) {
+    const int lane = threadIdx.x & 63;
+    const long b = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= nbricks) return;  // the whole wave
+    const uint32_t g0 = (uint32_t)b * (uint32_t)G;
+    const uint32_t vo = lane <= G ? bprev[g0 + lane] : 0u;
+    const uint32_t vn = lane < G ? bnew[g0 + lane] : 0u;
+    const uint32_t vs = lane < G ? stay[g0 + lane] : 0u;
+    uint32_t ost[8], nb[8];
+    uint32_t before = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        ost[c] = __builtin_amdgcn_readlane(vo, c);
+        nb[c] = __builtin_amdgcn_readlane(vn, c) - before;
+        before += __builtin_amdgcn_readlane(vs, c);
     }
-    const uint32_t r0 = bprev[g0], r1 = bprev[g0 + G];
-    __syncthreads();
-    // previous key of position s of the brick's range
-    auto prev_key = [&](uint32_t s) -> uint32_t {
-        uint32_t c = 0;
-        for (int k = 1; k < G; ++k) c += (s >= ostart[k]) ? 1u : 0u;
-        return g0 + c;
-    };
-    uint32_t carry = 0;
-    for (uint32_t base = r0; base < r1; base += RS_E * RS_NT) {
-        uint32_t key[RS_E], pk[RS_E];
+    const uint32_t r0 = ost[0], r1 = __builtin_amdgcn_readlane(vo, G);
+    const uint64_t below = (1ull << lane) - 1ull;  // the lanes before this one
+    uint32_t carry = 0;                            // stayers of the brick before the pass
+    for (uint32_t base = r0; base < r1; base += E * 64) {
+        uint32_t key[E];
 #pragma unroll
-        for (int e = 0; e < RS_E; ++e) {
-            const uint32_t s = base + e * RS_NT + threadIdx.x;
+        for (int e = 0; e < E; ++e) {
+            const uint32_t s = base + e * 64 + lane;
             key[e] = s < r1 ? keys[s] : 0u;
-            pk[e] = s < r1 ? prev_key(s) : 1u;
         }
-        uint32_t lanepre[RS_E];
 #pragma unroll
-        for (int e = 0; e < RS_E; ++e) {
-            const bool st = (base + e * RS_NT + threadIdx.x < r1) && key[e] == pk[e];
+        for (int e = 0; e < E; ++e) {
+            const uint32_t s = base + e * 64 + lane;
+            uint32_t c = 0;  // the group that held s
+#pragma unroll
+            for (int k = 1; k < 8; ++k) c += (k < G && s >= ost[k]) ? 1u : 0u;
+            const bool in = s < r1, st = in && key[e] == g0 + c;
             const uint64_t m = __ballot(st);
-            lanepre[e] = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-            if (lane == 0) wsum[e][wave] = (uint32_t)__popcll(m);
-        }
-        __syncthreads();
-        uint32_t pre = carry;  // stayers of the brick before sub-chunk e, wave `wave`
-#pragma unroll
-        for (int e = 0; e < RS_E; ++e) {
-            const uint32_t s = base + e * RS_NT + threadIdx.x;
-            uint32_t wpre = 0, total = 0;
-#pragma unroll
-            for (int w = 0; w < RS_NT / 64; ++w) {
-                const uint32_t c = wsum[e][w];
-                wpre += (w < wave) ? c : 0u;
-                total += c;
-            }
-            if (s < r1) {
+            const uint32_t rank = carry + (uint32_t)__popcll(m & below);
+            carry += (uint32_t)__popcll(m);
+            if (in) {
                 uint32_t pos;
-                if (key[e] == pk[e])
-                    pos = nstart[pk[e] - g0] + (pre + wpre + lanepre[e] - before[pk[e] - g0]);
-                else
+                if (st) {
+                    uint32_t n = nb[0];
+#pragma unroll
+                    for (int k = 1; k < 8; ++k) n = (c == (uint32_t)k) ? nb[k] : n;
+                    pos = n + rank;
+                } else {
                     pos = bnew[key[e]] + stay[key[e]] + atomicAdd(&fill[key[e]], 1u);
+                }
                 perm[pos] = s;
             }
-            pre += total;
         }
-        carry = pre;
-        __syncthreads();  // wsum is rewritten by the next pass
     }
 }
 
@@ -381,9 +370,15 @@ void resort_counts(const uint32_t* bprev, const uint32_t* in_count, const uint32
 }
 
 void resort_place(const uint32_t* keys, const uint32_t* bprev, const uint32_t* bnew, const uint32_t* stay,
-                  uint32_t* fill, long nbricks, int G, uint32_t* perm, hipStream_t st) {
-    hipLaunchKernelGGL(k_resort_place, dim3((unsigned)nbricks), dim3(RS_NT), 0, st, keys, bprev, bnew, stay, fill, G,
-                       perm);
+                  uint32_t* fill, long nbricks, int G, long np, uint32_t* perm, hipStream_t st) {
+    // passes of 128 or 512 elements: about one pass per brick on average
+    const unsigned nb = (unsigned)((nbricks + 3) / 4);
+    if (np / nbricks >= 256)
+        hipLaunchKernelGGL((k_resort_place<8>), dim3(nb), dim3(256), 0, st, keys, bprev, bnew, stay, fill,
+                           nbricks, G, perm);
+    else
+        hipLaunchKernelGGL((k_resort_place<2>), dim3(nb), dim3(256), 0, st, keys, bprev, bnew, stay, fill,
+                           nbricks, G, perm);
     CPM_LAUNCH_CHECK();
 }
 

@@ -37,7 +37,7 @@ namespace cpm {
 // in_count / out_count: movers into / out of each group (keys_fx).
 // resort_counts: stay[g] = stayers of g, tot[g] = stay[g] + in_count[g]
 // (tot[ngroups] = 0; its exclusive scan gives the new group starts bnew).
-// resort_place, one workgroup per brick of G = 2^R groups: perm[pos] = s;
+// resort_place, per brick of G = 2^R groups (np particles in all): perm[pos] = s;
 // stayers keep their order at the front of their group, movers follow in
 // atomic order (fill: zeroed per-group counters).
 // group_keys: skeys[s] = the group of bstart that holds s (the sorted keys,
@@ -45,7 +45,7 @@ namespace cpm {
 void resort_counts(const uint32_t* bprev, const uint32_t* in_count, const uint32_t* out_count, long ngroups,
                    uint32_t* stay, uint32_t* tot, hipStream_t st);
 void resort_place(const uint32_t* keys, const uint32_t* bprev, const uint32_t* bnew, const uint32_t* stay,
-                  uint32_t* fill, long nbricks, int G, uint32_t* perm, hipStream_t st);
+                  uint32_t* fill, long nbricks, int G, long np, uint32_t* perm, hipStream_t st);
 void group_keys(const uint32_t* bstart, long nbricks, int G, uint32_t* skeys, hipStream_t st);
 void zero_u32(uint32_t* p, long n, hipStream_t st);
 
