@@ -579,8 +579,11 @@ __device__ __forceinline__ void kick_from_mesh(uint32_t s, uint32_t i, const typ
 // particle's new position, and a particle whose key changed is counted into
 // its new group (in_count) and out of its old one (out_count): what keys_fx
 // computes for an incremental re-sort, whose previous keys are those of the
-// positions read here.
-template <typename PP, int R, int NT>
+// positions read here.  FULL: every brick has the full size of the form (the
+// mesh is at least a brick along each axis), so the tile strides are
+// compile-time constants and the stencil reads take their offsets as
+// immediates.
+template <typename PP, int R, int NT, bool FULL>
 __global__ __launch_bounds__(NT) void k_gather_kick_brick(
     const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ perm, const typename PP::Pos* __restrict__ X,
     const double* __restrict__ P, const uint32_t* __restrict__ idi, typename PP::Pos* __restrict__ Xo,
@@ -618,9 +621,11 @@ __global__ __launch_bounds__(NT) void k_gather_kick_brick(
     const uint32_t lw = (G % 8 == 0) ? (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3) : blockIdx.x;
     uint32_t bid = (lw & (cstride - 1)) | ((uint32_t)(lw >> lcol) << (lcol + lseg));
     BrickCoord bc = brick_coord(bid, pa);
-    const int B = bc.B[LA];
-    const int TX = bc.B[0] + 5, TP = B + 5, TR = TP + B;
-    const int PS = (R == 3) ? TX * (bc.B[1] + 5) : TX;  // cells of a plane
+    constexpr int FB0 = (R == 3) ? BD::X : BD::X2, FB1 = (R == 3) ? BD::Y : BD::X2, FB2 = (R == 3) ? BD::Z : 1;
+    const int B0 = FULL ? FB0 : bc.B[0], B1 = FULL ? FB1 : bc.B[1], B2 = FULL ? FB2 : bc.B[2];
+    const int B = (R == 3) ? B2 : B1;  // extent along the last axis
+    const int TX = B0 + 5, TP = B + 5, TR = TP + B;
+    const int PS = (R == 3) ? TX * (B1 + 5) : TX;  // cells of a plane
     // e / PS and r / TX by multiply-high: exact for the e, r < 2^14 used here
     const uint32_t psinv = (uint32_t)((0x100000000ull + PS - 1) / PS);
     const uint32_t txinv = (uint32_t)((0x100000000ull + TX - 1) / TX);
@@ -799,7 +804,7 @@ __global__ __launch_bounds__(NT) void k_gather_kick_brick(
         // unwrapped coordinate that differs from the folded one's) on the
         // brick's upper face needs one plane beyond the tile: it is queued,
         // and kicked from the mesh when the segment is done (kick_from_mesh).
-        if (lx >= bc.B[0] || ly >= bc.B[1] || (R == 3 && lz >= bc.B[2])) {
+        if (lx >= B0 || ly >= B1 || (R == 3 && lz >= B2)) {
             const uint32_t q = atomicAdd(&sm.dqn, 1u);
             if (q < (uint32_t)DQ)
                 sm.dq[q] = s;
@@ -1447,9 +1452,19 @@ static void launch_gather_brick(int R, const uint32_t* bstart, const uint32_t* p
     int lseg = 0;
     while (lseg < LSEG && lseg < pa.lnb[R - 1] && (nbr >> (lseg + 1)) >= minwg) ++lseg;
     const unsigned nb = (unsigned)(nbr >> lseg);
-    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_kick_brick<PP, RR, NT>), dim3(nb), dim3(NT), 0, st, bstart, perm,
-                                         X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ckq, pq, cd, cd_on, lseg, keys,
-                                         in_count, out_count, defer, qpar));
+    // every brick at the full size of the form: compile-time tile strides
+    using BD = BrickDims<PP>;
+    const bool full = (R == 3) ? ((1 << pa.lb[0]) == BD::X && (1 << pa.lb[1]) == BD::Y && (1 << pa.lb[2]) == BD::Z)
+                               : ((1 << pa.lb[0]) == BD::X2 && (1 << pa.lb[1]) == BD::X2);
+#define CPM_GATHER_LAUNCH(F)                                                                                        \
+    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_kick_brick<PP, RR, NT, F>), dim3(nb), dim3(NT), 0, st, bstart,    \
+                                         perm, X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ckq, pq, cd, cd_on, lseg,   \
+                                         keys, in_count, out_count, defer, qpar))
+    if (full)
+        CPM_GATHER_LAUNCH(true);
+    else
+        CPM_GATHER_LAUNCH(false);
+#undef CPM_GATHER_LAUNCH
     CPM_LAUNCH_CHECK();
     // folded coordinates: every origin is the deposit's cell, in its brick, and nothing is queued
     if (pa.wrap) return;
