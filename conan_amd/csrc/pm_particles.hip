@@ -676,7 +676,8 @@ __global__ __launch_bounds__(NT) void k_gather_kick_brick(
     // The next brick's B new planes: cell e = wbase + l NT + lane of the
     // run (plane e / PS) has the same in-plane mesh offset for every brick of
     // the segment, so it is computed once, packed as (plane << 27) | offset
-    // (~0u: no cell).
+    // (lanes with tid + l NT >= NLIM have no cell; for full bricks NLIM is a
+    // constant and the test vanishes from the loads that are always whole).
     constexpr int NLN = (BL * PSMAX * DW + NT - 1) / NT;
     const int wbase = __builtin_amdgcn_readfirstlane(tid & ~63), lane = tid & 63;
     const int lgP = (R == 3) ? 2 * lgN : lgN;  // log2 of the mesh stride between planes
@@ -689,26 +690,29 @@ __global__ __launch_bounds__(NT) void k_gather_kick_brick(
         const int tx = r - ty * TX;
         uint32_t xy = (uint32_t)((bc.x0[0] - 2 + tx) & msk);
         if (R == 3) xy += (uint32_t)N * (uint32_t)((bc.x0[1] - 2 + ty) & msk);
-        noff[l] = (d < B * PS * DW) ? ((uint32_t)q << 27) | xy : ~0u;
+        noff[l] = ((uint32_t)q << 27) | xy;
     }
+    const int NLIM = B * PS * DW;  // dwords of the B new planes
     // the next brick's new planes (last-axis origin zb) into slots
     // (p + rp) mod TR, p = 5 .. TP - 1
     auto load_new_planes = [&](int zb, int rp) {
         int start = 5 + rp;
         start = start >= TR ? start - TR : start;
         const int E = (TR - start) * PS * DW;  // first dword whose plane wraps to slot 0
+        // the wave's LDS bases before and after the wrap (load l adds l NT dwords)
+        const uint32_t ba = ring_lds + 4u * (uint32_t)(start * PS * DW + wbase);
+        const uint32_t bb = ring_lds + 4u * (uint32_t)(wbase - E);
 #pragma unroll
         for (int l = 0; l < NLN; ++l) {
-            const uint32_t v = noff[l];
-            if (v != ~0u) {
+            if (tid + l * NT < NLIM) {
+                const uint32_t v = noff[l];
                 const uint32_t gl = (uint32_t)((zb + 3 + (int)(v >> 27)) & msk);
                 const uint32_t off = (v & 0x7FFFFFFu) + (gl << lgP);
-                const int d0 = wbase + l * NT;
                 const float* src = reinterpret_cast<const float*>(phi + off) + (lane & (DW - 1));
-                if (d0 + lane < E)
-                    lds_dma_dword(src, ring_lds + 4u * (uint32_t)(start * PS * DW + d0));
+                if (tid + l * NT < E)
+                    lds_dma_dword(src, ba + 4u * (uint32_t)(l * NT));
                 else
-                    lds_dma_dword(src, ring_lds + 4u * (uint32_t)(d0 - E));
+                    lds_dma_dword(src, bb + 4u * (uint32_t)(l * NT));
             }
         }
     };
