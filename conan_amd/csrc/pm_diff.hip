@@ -248,6 +248,7 @@ struct DiffPM {
         AdjDrift dr{0.0, 0.0, 0, nullptr, nullptr, nullptr, nullptr, hash};
         if (fuse) {
             zero_u32(sim.cnt3.p, 3 * sim.nsort_groups, st);
+            sim.counts_clear = false;  // the adjoint gather counts into them
             dr = AdjDrift{cd_fx, cd, (cd_fx != 0.0 || cd != 0.0) ? 1 : 0, sim.knew.p, sim.cnt3.p,
                           sim.cnt3.p + sim.nsort_groups, bmax.p, hash};
         }

@@ -184,6 +184,9 @@ struct Impl : ImplBase {
     bool resort_ok = false;            // storage order == output order of the last brick sort
     bool keys_ready = false;           // knew / cnt3 hold the keys of the current positions (fused gather)
     bool skeys_valid = false;          // skeys holds the keys of the last sort (the re-sort does not write them)
+    // in_count / out_count / fill (cnt3) are all zero: set by a re-sort, which
+    // clears them, and reset by every kernel that counts into them
+    bool counts_clear = false;
     DevBuf<Mesh> mesh;
     DevBuf<Mesh> frac;  // FP64 deposit: fractional cell coordinates in sorted order
     DevBuf<Pos> xsort;     // FP64: positions in the deposit's sorted order
@@ -418,6 +421,7 @@ struct Impl : ImplBase {
         keys_ready = false;
         if (resort_ok) {
             zero_u32(cnt3.p, 3 * nsort_groups, st);
+            counts_clear = false;  // the keys kernel counts into them
             if (!skeys_valid) {  // the previous keys, from the group starts of the last sort
                 uint32_t* sk = (skeys == skb0.p) ? skb1.p : skb0.p;
                 group_keys(bstart, brick_count(pa), 1 << R, sk, st);
@@ -432,7 +436,9 @@ struct Impl : ImplBase {
         keys_ready = false;
         if (resort_ok) {
             uint32_t* bnew = (bstart == row_start.p) ? bs2.p : row_start.p;
-            resort_counts(bstart, cnt3.p, cnt3.p + nsort_groups, nsort_groups, stay.p, tot.p, st);
+            resort_counts(bstart, cnt3.p, cnt3.p + nsort_groups, cnt3.p + 2 * nsort_groups, nsort_groups, stay.p,
+                          tot.p, st);
+            counts_clear = true;
             exclusive_scan(tot.p, bnew, nsort_groups + 1, scan_scr.p, st);
             resort_place(knew.p, bstart, bnew, stay.p, cnt3.p + 2 * nsort_groups, brick_count(pa), 1 << R, Np, permb.p,
                          st);
@@ -535,7 +541,8 @@ struct Impl : ImplBase {
         if constexpr (FX) {
             // kick (+ drift) and move the particles into the deposit's brick
             // order; with the drift fused, also the keys of the next re-sort
-            if (fuse) zero_u32(cnt3.p, 3 * nsort_groups, st);
+            if (fuse && !counts_clear) zero_u32(cnt3.p, 3 * nsort_groups, st);
+            if (fuse) counts_clear = false;
             gather_kick_fx_brick(R, bstart, spids, reinterpret_cast<const uint32_t*>(X.p), P.p, id.p,
                                  reinterpret_cast<uint32_t*>(X2.p), P2.p, id2.p, Np,
                                  reinterpret_cast<const float*>(phi_dst()), lgNf, pa, ck, pq, cd, fuse ? 1 : 0,
@@ -548,7 +555,8 @@ struct Impl : ImplBase {
             keys_ready = fuse;
         } else if (bricks) {
             // the same with the FP64 tile and the reference's arithmetic
-            if (fuse) zero_u32(cnt3.p, 3 * nsort_groups, st);
+            if (fuse && !counts_clear) zero_u32(cnt3.p, 3 * nsort_groups, st);
+            if (fuse) counts_clear = false;
             gather_kick_f64_brick(R, bstart, spids, reinterpret_cast<const double*>(X.p), P.p, id.p,
                                   reinterpret_cast<double*>(X2.p), P2.p, id2.p, Np,
                                   reinterpret_cast<const double*>(phi_dst()), lgNf, pa, ck, cd, fuse ? 1 : 0,

@@ -273,8 +273,9 @@ int RadixSorter::sort(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, hi
 namespace cpm {
 
 __global__ __launch_bounds__(256) void k_resort_counts(const uint32_t* __restrict__ bprev,
-                                                       const uint32_t* __restrict__ in_count,
-                                                       const uint32_t* __restrict__ out_count, long ngroups,
+                                                       uint32_t* __restrict__ in_count,
+                                                       uint32_t* __restrict__ out_count,
+                                                       uint32_t* __restrict__ fill, long ngroups,
                                                        uint32_t* __restrict__ stay, uint32_t* __restrict__ tot) {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g > ngroups) return;
@@ -282,9 +283,15 @@ __global__ __launch_bounds__(256) void k_resort_counts(const uint32_t* __restric
         tot[g] = 0;
         return;
     }
-    const uint32_t s = (bprev[g + 1] - bprev[g]) - out_count[g];
+    const uint32_t in = in_count[g], out = out_count[g];
+    const uint32_t s = (bprev[g + 1] - bprev[g]) - out;
     stay[g] = s;
-    tot[g] = s + in_count[g];
+    tot[g] = s + in;
+    // the counts are consumed: cleared for the next gather, and the
+    // placement's slot counters cleared for this sort
+    in_count[g] = 0u;
+    out_count[g] = 0u;
+    fill[g] = 0u;
 }
 
 // One wave per brick (four bricks per workgroup): the brick's previous range
@@ -362,10 +369,10 @@ __global__ __launch_bounds__(256) void k_zero_u32(uint32_t* __restrict__ p, long
     if (i < n) p[i] = 0u;
 }
 
-void resort_counts(const uint32_t* bprev, const uint32_t* in_count, const uint32_t* out_count, long ngroups,
+void resort_counts(const uint32_t* bprev, uint32_t* in_count, uint32_t* out_count, uint32_t* fill, long ngroups,
                    uint32_t* stay, uint32_t* tot, hipStream_t st) {
     hipLaunchKernelGGL(k_resort_counts, dim3((unsigned)((ngroups + 256) / 256)), dim3(256), 0, st, bprev, in_count,
-                       out_count, ngroups, stay, tot);
+                       out_count, fill, ngroups, stay, tot);
     CPM_LAUNCH_CHECK();
 }
 

@@ -36,13 +36,15 @@ namespace cpm {
 // 1), so the previous key of position s is the group that holds s;
 // in_count / out_count: movers into / out of each group (keys_fx).
 // resort_counts: stay[g] = stayers of g, tot[g] = stay[g] + in_count[g]
-// (tot[ngroups] = 0; its exclusive scan gives the new group starts bnew).
+// (tot[ngroups] = 0; its exclusive scan gives the new group starts bnew);
+// it then clears in_count, out_count and fill (the placement's counters), so
+// after a re-sort the counts are zero for the next gather.
 // resort_place, per brick of G = 2^R groups (np particles in all): perm[pos] = s;
 // stayers keep their order at the front of their group, movers follow in
-// atomic order (fill: zeroed per-group counters).
+// atomic order (fill: per-group counters, zero on entry).
 // group_keys: skeys[s] = the group of bstart that holds s (the sorted keys,
 // for the kernels that compare new keys with them).
-void resort_counts(const uint32_t* bprev, const uint32_t* in_count, const uint32_t* out_count, long ngroups,
+void resort_counts(const uint32_t* bprev, uint32_t* in_count, uint32_t* out_count, uint32_t* fill, long ngroups,
                    uint32_t* stay, uint32_t* tot, hipStream_t st);
 void resort_place(const uint32_t* keys, const uint32_t* bprev, const uint32_t* bnew, const uint32_t* stay,
                   uint32_t* fill, long nbricks, int G, long np, uint32_t* perm, hipStream_t st);
