@@ -549,9 +549,14 @@ __global__ __launch_bounds__((ColCfg<T, N>::NT)) void k_cols(ColPart<T> pa, ColP
     for (int m = tid; m < N; m += C::NT) stw[m] = tw[m];
     if constexpr (Fused<N>::ok) {
         fill_mid_tw<T, N>(smt, tw, tid, C::NT);
-        // one tile per workgroup: with a single FFT per tile, prefetching
-        // the next tile costs more in registers than it hides
-        const ColTile<T, N> t = col_tile<T, N>(pa, pb);
+        // One tile per workgroup: with a single FFT per tile, prefetching
+        // the next tile costs more in registers than it hides.  The tiles are
+        // taken from the last slab backwards, against the row passes, which
+        // walk the slabs upwards: the forward sweep first reads what the row
+        // pass wrote last, still in the Infinity Cache (256 MB, half of a
+        // 512^3 spectrum), and the backward sweep writes last the slabs the
+        // inverse row pass reads first.
+        const ColTile<T, N> t = col_tile_at<T, N>(pa, pb, gridDim.x - 1 - blockIdx.x);
         cx<T> v[8];
         col_first_load<T, N>(t, tid, v);
         col_first_store<T, N, DIR>(sh, v, tid);
