@@ -224,7 +224,17 @@ __global__ __launch_bounds__(VEC_NT) void k_deposit_fx_vm(const uint32_t* __rest
     }
     brick_ranges<R>(rg, bstart, bc, pa, tid);
     const uint32_t total = rg.roff[rg.n];
-    for (uint32_t t = tid; t < total; t += VEC_NT) {
+    // A wave's lanes are spread over the workgroup's chunk of VEC_NT
+    // particles, 8 stretches of 8: neighbours in the sorted order, which
+    // often share cells, then fall in different waves, so fewer of one
+    // instruction's 64-bit LDS atomics go to the same address (4.44 -> 3.70
+    // ms per launch at 512^3 particles on 512^3; 16 or more stretches lose
+    // more to the scattered loads than they gain, 4 gain less).
+    constexpr int SP = 8;
+    const uint32_t tmap = (uint32_t)((tid & (SP - 1)) * (VEC_NT / SP) + (tid / SP));
+    for (uint32_t t0 = 0; t0 < total; t0 += VEC_NT) {
+        const uint32_t t = t0 + tmap;
+        if (t >= total) continue;
         const uint32_t sp = brick_range_pos(rg, t);
         const uint32_t i = perm[sp];
         int o[3] = {0, 0, 0};
