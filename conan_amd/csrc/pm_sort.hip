@@ -306,7 +306,8 @@ __global__ __launch_bounds__(256) void k_resort_counts(const uint32_t* __restric
 // its stayers; nb[c] = new start of group c minus the stayers of the brick's
 // groups before c, so a stayer's position is nb[c] plus its rank among the
 // brick's stayers.  E: elements per lane in flight per pass.
-template <int E>
+static constexpr int RS_AGG = 8;  // movers of a 64-element slice above which they reserve slots by group
+template <int E, bool AGG>
 __global__ __launch_bounds__(256) void k_resort_place(const uint32_t* __restrict__ keys,
                                                            const uint32_t* __restrict__ bprev,
                                                            const uint32_t* __restrict__ bnew,
@@ -348,17 +349,37 @@ __global__ __launch_bounds__(256) void k_resort_place(const uint32_t* __restrict
             const uint64_t m = __ballot(st);
             const uint32_t rank = carry + (uint32_t)__popcll(m & below);
             carry += (uint32_t)__popcll(m);
-            if (in) {
-                uint32_t pos;
-                if (st) {
-                    uint32_t n = nb[0];
+            uint32_t n = nb[0];
 #pragma unroll
-                    for (int k = 1; k < 8; ++k) n = (c == (uint32_t)k) ? nb[k] : n;
-                    pos = n + rank;
-                } else {
-                    pos = bnew[key[e]] + stay[key[e]] + atomicAdd(&fill[key[e]], 1u);
+            for (int k = 1; k < 8; ++k) n = (c == (uint32_t)k) ? nb[k] : n;
+            const bool mv = in && !st;
+            uint64_t pend = __ballot(mv);
+            if (!AGG || __popcll(pend) <= RS_AGG) {
+                // a few movers: each reserves its own slot
+                if (in) perm[st ? n + rank : bnew[key[e]] + stay[key[e]] + atomicAdd(&fill[key[e]], 1u)] = s;
+            } else {
+                // many (dense bricks, large steps): by new group, the first pending lane's
+                // group is taken, its movers ranked among themselves, and one
+                // lane (the leader) reserves their slots with one atomic, so
+                // the device-scope atomics number the slice's groups, not its
+                // movers
+                int ld = lane;
+                uint32_t rk = 0u, cnt = 0u;
+                while (pend) {
+                    const int l = (int)__builtin_ctzll(pend);
+                    const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)key[e], l);
+                    const uint64_t same = __ballot(mv && key[e] == k);
+                    if (mv && key[e] == k) {
+                        ld = l;
+                        rk = (uint32_t)__popcll(same & below);
+                        cnt = lane == l ? (uint32_t)__popcll(same) : 0u;
+                    }
+                    pend &= ~same;
                 }
-                perm[pos] = s;
+                uint32_t lb = 0u;
+                if (cnt) lb = bnew[key[e]] + stay[key[e]] + atomicAdd(&fill[key[e]], cnt);
+                const uint32_t b = (uint32_t)__shfl((int)lb, ld);
+                if (in) perm[st ? n + rank : b + rk] = s;
             }
         }
     }
@@ -378,13 +399,18 @@ void resort_counts(const uint32_t* bprev, uint32_t* in_count, uint32_t* out_coun
 
 void resort_place(const uint32_t* keys, const uint32_t* bprev, const uint32_t* bnew, const uint32_t* stay,
                   uint32_t* fill, long nbricks, int G, long np, uint32_t* perm, hipStream_t st) {
-    // passes of 128 or 512 elements: about one pass per brick on average
+    // passes of 128 or 512 elements: about one pass per brick on average;
+    // bricks of thousands of particles (whose large steps move many of them)
+    // reserve their movers' slots by group
     const unsigned nb = (unsigned)((nbricks + 3) / 4);
-    if (np / nbricks >= 256)
-        hipLaunchKernelGGL((k_resort_place<8>), dim3(nb), dim3(256), 0, st, keys, bprev, bnew, stay, fill,
+    if (np / nbricks >= 2048)
+        hipLaunchKernelGGL((k_resort_place<8, true>), dim3(nb), dim3(256), 0, st, keys, bprev, bnew, stay, fill,
+                           nbricks, G, perm);
+    else if (np / nbricks >= 256)
+        hipLaunchKernelGGL((k_resort_place<8, false>), dim3(nb), dim3(256), 0, st, keys, bprev, bnew, stay, fill,
                            nbricks, G, perm);
     else
-        hipLaunchKernelGGL((k_resort_place<2>), dim3(nb), dim3(256), 0, st, keys, bprev, bnew, stay, fill,
+        hipLaunchKernelGGL((k_resort_place<2, false>), dim3(nb), dim3(256), 0, st, keys, bprev, bnew, stay, fill,
                            nbricks, G, perm);
     CPM_LAUNCH_CHECK();
 }
