@@ -1,5 +1,5 @@
-"""GPU parity at the survey's measurement configurations, and of the
-initial-condition branches that feed them, against the oracle.
+"""GPU parity against the oracle over two whole runs small enough for the
+oracle to follow, and of the initial-condition branches that feed them.
 
 C1: 2-D, 128^2 particles on a 128^2 mesh, ns = -1 with the Nyquist cutoff,
     normalised to an rms Zel'dovich displacement of 0.1 mesh cell at a0,
