@@ -501,19 +501,18 @@ struct Impl : ImplBase {
     }
 
     Mesh* phi_dst() { return phi_out ? phi_out : mesh.p; }
-    // mesh (rho) -> phi_dst() (phi) at scale factor a.
-    // FX: scal[0] must hold the FP64 mean of the mesh (mean_ready), or the
-    // row partials of the last deposit provide it.
+    // mesh (rho) -> phi_dst() (phi) at scale factor a.  mean_ready (FX): the
+    // mesh is a caller's density, whose FP64 mean is in scal[0] and is
+    // subtracted instead of 1.
     void do_poisson(double a, bool mean_ready = false) {
         const double s = fft.M / cos.G * a;  // real_part(size / G * a), src/run/nbody.hh:110
         const double kf = 2 * M_PI / cfg.L;
-        const double* mean_ptr = nullptr;
-        Mesh shift = (Mesh)1.0;  // delta = rho - 1 (src/run/nbody.hh:103-104)
-        if (FX) {
-            // FP64 mean of the FP32 deposit, subtracted on the device
-            if (!mean_ready) sum_partials(row_partial.p, npartials, 1.0 / (double)M, red.p, 4096, scal.p, st);
-            mean_ptr = scal.p;
-        }
+        // delta = rho - 1 (src/run/nbody.hh:103-104), in both forms.  The
+        // FP32 deposit's mean is 1 to about 1e-11 (particle mass (Nf/Nm)^dim;
+        // each weight rounded to 2^-40), so its FP32 value is 1 exactly and
+        // no mesh sum is needed for it.
+        const double* mean_ptr = (FX && mean_ready) ? scal.p : nullptr;
+        const Mesh shift = (Mesh)1.0;
         for (int stage = 0; stage < 5; ++stage) {
             prof.begin(2 + stage, st);
             fft.poisson_stage(stage, 0, mesh.p, phi_dst(), spec.p, nyq.p, shift, mean_ptr, kf, s, st);
