@@ -470,7 +470,9 @@ struct Impl : ImplBase {
         fx_sort();
     }
 
-    void do_deposit(Mesh m) {
+    // partials: also the FP64 mesh-sum partials (row_partial), for the
+    // callers that sum the mesh; the step's Poisson solve does not.
+    void do_deposit(Mesh m, bool partials = true) {
         prof.begin(0, st);
         if (bricks) {
             fx_sort_current();
@@ -485,12 +487,12 @@ struct Impl : ImplBase {
         prof.end(0, st);
         prof.begin(1, st);
         if constexpr (FX) {
-            deposit_fx_brick(R, bstart, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p, st);
-            npartials = brick_count(pa);
+            deposit_fx_brick(R, bstart, spids, X.p, Np, lgNf, pa, m, mesh.p, partials ? row_partial.p : nullptr, st);
+            npartials = partials ? brick_count(pa) : 0;
         } else if (bricks) {
             deposit_f64_brick(R, bstart, spids, reinterpret_cast<const double*>(X.p), Np, lgNf, pa, (double)m,
-                              reinterpret_cast<double*>(mesh.p), row_partial.p, st);
-            npartials = brick_count(pa);
+                              reinterpret_cast<double*>(mesh.p), partials ? row_partial.p : nullptr, st);
+            npartials = partials ? brick_count(pa) : 0;
         } else {
             ParticleOps<PP>::deposit(R, row_start.p, skeys, spids, X.p, Np, lgNf, pa, m, mesh.p, row_partial.p,
                                      nrows, frac.p, xsort.p, inv.p, st);
@@ -531,7 +533,7 @@ struct Impl : ImplBase {
     }
 
     void kick_impl(double a, double da, bool fuse, double a_next, double da_next) {
-        do_deposit((Mesh)mass);
+        do_deposit((Mesh)mass, false);
         do_poisson(a_eval(a, da));
         const double adot = cos.adot(a_eval(a, da));
         const double ck = da / adot;  // src/run/nbody.hh:112-116

@@ -309,8 +309,10 @@ __global__ __launch_bounds__(FXB_NT) void k_deposit_fx_brick(const uint32_t* __r
             rsum += (double)v;
         }
     }
-    const double tot = block_sum<FXB_NT>(rsum, red, tid);
-    if (tid == 0) partial[bid] = tot;
+    if (partial) {  // (uniform)
+        const double tot = block_sum<FXB_NT>(rsum, red, tid);
+        if (tid == 0) partial[bid] = tot;
+    }
 }
 
 // FX force at one particle: the trilinear interpolation (weights from the
@@ -1029,8 +1031,10 @@ __global__ __launch_bounds__(F64B_NT) void k_deposit_f64_brick(const uint32_t* _
         mesh[idx] = v;
         rsum += v;
     }
-    const double tot = block_sum<F64B_NT>(rsum, red, tid);
-    if (tid == 0) partial[bid] = tot;
+    if (partial) {  // (uniform)
+        const double tot = block_sum<F64B_NT>(rsum, red, tid);
+        if (tid == 0) partial[bid] = tot;
+    }
 }
 
 // ------------------------------------------------------------ gather + kick (+ drift)
