@@ -307,6 +307,7 @@ __global__ __launch_bounds__(VEC_NT) void k_deposit_fx_vm(const uint32_t* __rest
             rsum += (double)v;
         }
     }
+    if (!partial) return;  // (uniform) no mesh sum wanted
     // block_sum<FXB_NT> over threads 0..FXB_NT-1 (the other waves add 0 and are not read)
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) rsum += __shfl_down(rsum, off);

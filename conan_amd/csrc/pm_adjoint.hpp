@@ -26,7 +26,8 @@ void deposit_fx_vec(int R, const uint32_t* bstart, const uint32_t* perm, const u
                     long np, int lgN, const PosArgs& pa, const uint32_t* lmax, float* s, hipStream_t st);
 
 // deposit_fx_vec and, in the same launch, deposit_fx_brick's mass deposit of
-// the same positions (mesh and per-brick partials bit-identical to it).
+// the same positions (mesh and per-brick partials bit-identical to it;
+// partial may be null, and then no partials are formed).
 void deposit_fx_vm(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* Y, const float* lP, long np,
                    int lgN, const PosArgs& pa, const uint32_t* lmax, float* s, float mass, float* mesh,
                    double* partial, hipStream_t st);

@@ -351,10 +351,11 @@ struct DiffPM {
                 deposit_fx_vec(R, sim.bstart, sim.spids, sim.X.p, lP.p, Np, sim.lgNf, sim.pa, lmax.p, svec.p, st);
             } else {
                 // the deposit of lP and the mass deposit of the same positions
-                // in one launch (the mesh and partials of deposit_fx_brick)
+                // in one launch (the mesh of deposit_fx_brick; the Poisson
+                // solve needs no mesh sum)
                 deposit_fx_vm(R, sim.bstart, sim.spids, sim.X.p, lP.p, Np, sim.lgNf, sim.pa, lmax.p, svec.p,
-                              (float)m, sim.mesh.p, sim.row_partial.p, st);
-                sim.npartials = brick_count(sim.pa);
+                              (float)m, sim.mesh.p, nullptr, st);
+                sim.npartials = 0;
                 sim.do_poisson(ak);  // phi_i, as in the forward
                 phi = sim.mesh.p;
             }
