@@ -175,11 +175,9 @@ struct Impl : ImplBase {
     // outputs, the other group-start buffer, per-group counters (movers in |
     // movers out | fill), stayers, group totals.
     DevBuf<uint32_t> knew, skb0, skb1, permb, bs2, cnt3, stay, tot, scan_scr;
-    // FX gather: two counts used in turn (dpar: the next launch's) and the
-    // queue of the particles whose truncated origin lies outside their
-    // brick's tile and overflow their workgroup's LDS queue
+    // brick gathers: one flag per sorted position for the particles that
+    // overflow a workgroup's LDS queue (all zero between launches)
     DevBuf<uint32_t> defer;
-    int dpar = 0;
     const uint32_t* bstart = nullptr;  // group starts of the last sort (row_start or bs2)
     bool resort_ok = false;            // storage order == output order of the last brick sort
     bool keys_ready = false;           // knew / cnt3 hold the keys of the current positions (fused gather)
@@ -273,8 +271,8 @@ struct Impl : ImplBase {
             stay.alloc(nsort_groups);
             tot.alloc(nsort_groups + 1);
             scan_scr.alloc(scan_scratch_elems(nsort_groups + 1));
-            defer.alloc(Np + 2);
-            zero_u32(defer.p, 2, st);
+            defer.alloc(Np);
+            zero_u32(defer.p, Np, st);
         }
         if (!bricks) {  // the serial-order deposit's and its kick's sorted-order copies
             frac.alloc((size_t)R * Np);
@@ -547,8 +545,7 @@ struct Impl : ImplBase {
             gather_kick_fx_brick(R, bstart, spids, reinterpret_cast<const uint32_t*>(X.p), P.p, id.p,
                                  reinterpret_cast<uint32_t*>(X2.p), P2.p, id2.p, Np,
                                  reinterpret_cast<const float*>(phi_dst()), lgNf, pa, ck, pq, cd, fuse ? 1 : 0,
-                                 fuse ? knew.p : nullptr, cnt3.p, cnt3.p + nsort_groups, defer.p, dpar, st);
-            dpar ^= 1;
+                                 fuse ? knew.p : nullptr, cnt3.p, cnt3.p + nsort_groups, defer.p, st);
             std::swap(X.p, X2.p);
             std::swap(P.p, P2.p);
             std::swap(id.p, id2.p);
@@ -561,8 +558,7 @@ struct Impl : ImplBase {
             gather_kick_f64_brick(R, bstart, spids, reinterpret_cast<const double*>(X.p), P.p, id.p,
                                   reinterpret_cast<double*>(X2.p), P2.p, id2.p, Np,
                                   reinterpret_cast<const double*>(phi_dst()), lgNf, pa, ck, cd, fuse ? 1 : 0,
-                                  fuse ? knew.p : nullptr, cnt3.p, cnt3.p + nsort_groups, defer.p, dpar, st);
-            dpar ^= 1;
+                                  fuse ? knew.p : nullptr, cnt3.p, cnt3.p + nsort_groups, defer.p, st);
             std::swap(X.p, X2.p);
             std::swap(P.p, P2.p);
             std::swap(id.p, id2.p);

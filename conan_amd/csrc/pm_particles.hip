@@ -591,7 +591,7 @@ __global__ __launch_bounds__(NT) void k_gather_kick_brick(
     const double* __restrict__ P, const uint32_t* __restrict__ idi, typename PP::Pos* __restrict__ Xo,
     double* __restrict__ Po, uint32_t* __restrict__ ido, long np, const typename PP::Mesh* __restrict__ phi, int lgN,
     PosArgs pa, double ckq, double pq, double cd, int cd_on, int lseg, uint32_t* __restrict__ keys,
-    uint32_t* __restrict__ in_count, uint32_t* __restrict__ out_count, uint32_t* __restrict__ defer, int qpar) {
+    uint32_t* __restrict__ in_count, uint32_t* __restrict__ out_count, uint32_t* __restrict__ defer) {
     using Pos = typename PP::Pos;
     using T = typename PP::Mesh;                  // potential (float or double)
     using BD = BrickDims<PP>;
@@ -736,6 +736,7 @@ __global__ __launch_bounds__(NT) void k_gather_kick_brick(
     const uint32_t plast = (uint32_t)(np - 1);
     // sorted ranges of the current brick and the next two along the segment
     // (past the segment: empty)
+    const uint32_t bid0 = bid;  // the segment's first brick
     uint32_t c0 = bstart[bid << R], c1 = bstart[(bid + 1) << R];
     uint32_t n0 = c1, n1 = c1, m0 = c1, m1 = c1;
     if (nseg > 1) {
@@ -815,7 +816,7 @@ __global__ __launch_bounds__(NT) void k_gather_kick_brick(
             if (q < (uint32_t)DQ)
                 sm.dq[q] = s;
             else
-                defer[2 + atomicAdd(&defer[qpar], 1u)] = s;  // for k_gather_deferred
+                defer[s] = 1u;  // beyond the LDS queue: flagged at its sorted position
             return;
         }
         T F[3];
@@ -902,30 +903,21 @@ __global__ __launch_bounds__(NT) void k_gather_kick_brick(
         kick_from_mesh<PP, R>(s, perm[s], X, P, Xo, Po, np, phi, lgN, pa, ckq, pq, cd, cd_on, keys, in_count,
                               out_count);
     }
-}
-
-// The particles that k_gather_kick_brick's workgroups could not queue in
-// LDS (defer[qpar] of them, sorted positions from defer[2]), kicked from the
-// mesh.  The two counts alternate between launches: this one clears the
-// other, which the next gather fills (the previous launch's queue is done
-// with by now).
-template <typename PP, int R>
-__global__ __launch_bounds__(256) void k_gather_deferred(uint32_t* __restrict__ defer, int qpar,
-                                                          const uint32_t* __restrict__ perm,
-                                                          const typename PP::Pos* __restrict__ X,
-                                                          const double* __restrict__ P,
-                                                          typename PP::Pos* __restrict__ Xo, double* __restrict__ Po,
-                                                          long np, const typename PP::Mesh* __restrict__ phi, int lgN,
-                                                          PosArgs pa,
-                                                          double ckq, double pq, double cd, int cd_on,
-                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ in_count,
-                                                          uint32_t* __restrict__ out_count) {
-    const uint32_t n = defer[qpar];
-    if (blockIdx.x == 0 && threadIdx.x == 0) defer[qpar ^ 1] = 0u;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        const uint32_t s = defer[2 + q];
-        kick_from_mesh<PP, R>(s, perm[s], X, P, Xo, Po, np, phi, lgN, pa, ckq, pq, cd, cd_on, keys, in_count,
-                              out_count);
+    // those beyond the LDS queue (rare): the segment's bricks are walked for
+    // their flags, each cleared after its particle is kicked, so the flags
+    // are all zero again when the kernel ends
+    if (sm.dqn > (uint32_t)DQ) {  // (uniform)
+        for (int jj = 0; jj < nseg; ++jj) {
+            const uint32_t b = bid0 + (uint32_t)jj * cstride;
+            const uint32_t r0 = bstart[b << R], r1 = bstart[(b + 1) << R];
+            for (uint32_t s = r0 + tid; s < r1; s += NT) {
+                if (defer[s]) {
+                    defer[s] = 0u;
+                    kick_from_mesh<PP, R>(s, perm[s], X, P, Xo, Po, np, phi, lgN, pa, ckq, pq, cd, cd_on, keys,
+                                          in_count, out_count);
+                }
+            }
+        }
     }
 }
 
@@ -1453,7 +1445,7 @@ static void launch_gather_brick(int R, const uint32_t* bstart, const uint32_t* p
                                 const double* P, const uint32_t* idi, typename PP::Pos* Xo, double* Po, uint32_t* ido,
                                 long np, const typename PP::Mesh* phi, int lgN, const PosArgs& pa, double ckq,
                                 double pq, double cd, int cd_on, uint32_t* keys, uint32_t* in_count,
-                                uint32_t* out_count, uint32_t* defer, int qpar, hipStream_t st) {
+                                uint32_t* out_count, uint32_t* defer, hipStream_t st) {
     // segments of up to 2^LSEG bricks along the last axis, as long as that
     // leaves at least 4 workgroups per compute unit
     const long nbr = brick_count(pa), minwg = 4 * device_cus();
@@ -1467,37 +1459,29 @@ static void launch_gather_brick(int R, const uint32_t* bstart, const uint32_t* p
 #define CPM_GATHER_LAUNCH(F)                                                                                        \
     CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_kick_brick<PP, RR, NT, F>), dim3(nb), dim3(NT), 0, st, bstart,    \
                                          perm, X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ckq, pq, cd, cd_on, lseg,   \
-                                         keys, in_count, out_count, defer, qpar))
+                                         keys, in_count, out_count, defer))
     if (full)
         CPM_GATHER_LAUNCH(true);
     else
         CPM_GATHER_LAUNCH(false);
 #undef CPM_GATHER_LAUNCH
     CPM_LAUNCH_CHECK();
-    // folded coordinates: every origin is the deposit's cell, in its brick, and nothing is queued
-    if (pa.wrap) return;
-    // the overflow of the workgroups' LDS queues: usually empty
-    const long nbd = std::min<long>(256, (np + 255) / 256);
-    CPM_R_DISPATCH(R, hipLaunchKernelGGL((k_gather_deferred<PP, RR>), dim3((unsigned)nbd), dim3(256), 0, st, defer,
-                                         qpar, perm, X, P, Xo, Po, np, phi, lgN, pa, ckq, pq, cd, cd_on, keys,
-                                         in_count, out_count));
-    CPM_LAUNCH_CHECK();
 }
 
 void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, const double* P,
                           const uint32_t* idi, uint32_t* Xo, double* Po, uint32_t* ido, long np, const float* phi,
                           int lgN, const PosArgs& pa, double ck, double pq, double cd, int cd_on, uint32_t* keys,
-                          uint32_t* in_count, uint32_t* out_count, uint32_t* defer, int qpar, hipStream_t st) {
+                          uint32_t* in_count, uint32_t* out_count, uint32_t* defer, hipStream_t st) {
     launch_gather_brick<PosFX, FXG_NT, CPM_GATHER_LSEG>(R, bstart, perm, X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck * (1.0 / pq), pq,
-                                       cd, cd_on, keys, in_count, out_count, defer, qpar, st);
+                                       cd, cd_on, keys, in_count, out_count, defer, st);
 }
 
 void gather_kick_f64_brick(int R, const uint32_t* bstart, const uint32_t* perm, const double* X, const double* P,
                            const uint32_t* idi, double* Xo, double* Po, uint32_t* ido, long np, const double* phi,
                            int lgN, const PosArgs& pa, double ck, double cd, int cd_on, uint32_t* keys,
-                           uint32_t* in_count, uint32_t* out_count, uint32_t* defer, int qpar, hipStream_t st) {
+                           uint32_t* in_count, uint32_t* out_count, uint32_t* defer, hipStream_t st) {
     launch_gather_brick<PosF64, F64G_NT, CPM_GATHER_LSEG64>(R, bstart, perm, X, P, idi, Xo, Po, ido, np, phi, lgN, pa, ck, 0.0, cd,
-                                         cd_on, keys, in_count, out_count, defer, qpar, st);
+                                         cd_on, keys, in_count, out_count, defer, st);
 }
 
 void keys_f64_brick(int R, const double* X, long np, int lgN, const PosArgs& pa, const uint32_t* prevkeys,

@@ -261,17 +261,17 @@ long brick_count(const PosArgs& pa);
 // writes particle s to position s; with keys != nullptr it also produces
 // what keys_fx would for the new positions against the keys of the old ones
 // (incremental re-sort: keys, in_count, out_count, all counters zeroed).
-// defer: np + 2 words: two counts used in turn (qpar = 0, 1, 0, ...; count
-// qpar zero on entry, the launch zeroes the other), then the queue of the
-// particles whose truncated origin lies outside their brick's tile and that
-// overflow a workgroup's LDS queue of them (k_gather_deferred).
+// defer: np words, all zero on entry and on return: a particle that
+// overflows a workgroup's LDS queue of particles kicked from the mesh is
+// flagged at its sorted position, and the workgroup kicks and clears its
+// flagged particles when its segment is done.
 template <typename MT>
 void deposit_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, long np, int lgN,
                       const PosArgs& pa, float mass, MT* mesh, double* partial, hipStream_t st);
 void gather_kick_fx_brick(int R, const uint32_t* bstart, const uint32_t* perm, const uint32_t* X, const double* P,
                           const uint32_t* idi, uint32_t* Xo, double* Po, uint32_t* ido, long np, const float* phi,
                           int lgN, const PosArgs& pa, double ck, double pq, double cd, int cd_on, uint32_t* keys,
-                          uint32_t* in_count, uint32_t* out_count, uint32_t* defer, int qpar, hipStream_t st);
+                          uint32_t* in_count, uint32_t* out_count, uint32_t* defer, hipStream_t st);
 // The FP64 brick pipeline: keys_f64_brick is keys_fx for FP64 positions
 // (the brick of the deposit's cell, PosF64::dep); deposit_f64_brick is the
 // FP64 fixed-point deposit (128-bit accumulators, the row deposit's cell sums
@@ -285,7 +285,7 @@ void deposit_f64_brick(int R, const uint32_t* bstart, const uint32_t* perm, cons
 void gather_kick_f64_brick(int R, const uint32_t* bstart, const uint32_t* perm, const double* X, const double* P,
                            const uint32_t* idi, double* Xo, double* Po, uint32_t* ido, long np, const double* phi,
                            int lgN, const PosArgs& pa, double ck, double cd, int cd_on, uint32_t* keys,
-                           uint32_t* in_count, uint32_t* out_count, uint32_t* defer, int qpar, hipStream_t st);
+                           uint32_t* in_count, uint32_t* out_count, uint32_t* defer, hipStream_t st);
 // max |a[i]| (deterministic), and a[i] := rint(a[i] / q) * q.
 void reduce_max_abs(const double* a, long n, double* partials, long max_partials, double* out, hipStream_t st);
 void quantize(double* a, long n, double q, hipStream_t st);
