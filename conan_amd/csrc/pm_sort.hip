@@ -190,6 +190,48 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_add(uint32_t* __restrict__ out
         if (base + i < n) out[base + i] += add;
 }
 
+// Two-pass form for up to SCAN_FLAT tiles: the tile totals, then each tile
+// scanned from the sum of the totals before it (read by the whole workgroup
+// from the small sums array), so the scan of the totals and the pass that
+// adds them need no launches of their own.
+static constexpr long SCAN_FLAT = 4096;
+__global__ __launch_bounds__(SCAN_NT) void k_scan_tile_sums(const uint32_t* __restrict__ in, long n,
+                                                            uint32_t* __restrict__ sums) {
+    __shared__ uint32_t sh[SCAN_NT];
+    const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
+    uint32_t loc = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) loc += (base + i < n) ? in[base + i] : 0u;
+    uint32_t tot;
+    (void)block_exclusive_scan(loc, sh, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+__global__ __launch_bounds__(SCAN_NT) void k_scan_tiles_from(const uint32_t* __restrict__ in,
+                                                             uint32_t* __restrict__ out, long n,
+                                                             const uint32_t* __restrict__ sums) {
+    __shared__ uint32_t sh[SCAN_NT];
+    uint32_t x = 0;
+    for (uint32_t j = threadIdx.x; j < blockIdx.x; j += SCAN_NT) x += sums[j];
+    uint32_t off;
+    (void)block_exclusive_scan(x, sh, &off);
+    __syncthreads();  // sh is reused
+    const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t loc = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        v[i] = (base + i < n) ? in[base + i] : 0u;
+        loc += v[i];
+    }
+    uint32_t tot;
+    uint32_t pre = off + block_exclusive_scan(loc, sh, &tot);
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        if (base + i < n) out[base + i] = pre;
+        pre += v[i];
+    }
+}
+
 size_t scan_scratch_elems(long n) {
     size_t tot = 0;
     long m = n;
@@ -208,6 +250,14 @@ void exclusive_scan(const uint32_t* in, uint32_t* out, long n, uint32_t* scratch
         return;
     }
     uint32_t* sums = scratch;
+    if (nt <= SCAN_FLAT) {
+        hipLaunchKernelGGL(k_scan_tile_sums, dim3((unsigned)nt), dim3(SCAN_NT), 0, st, in, n, sums);
+        CPM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_scan_tiles_from, dim3((unsigned)nt), dim3(SCAN_NT), 0, st, in, out, n,
+                           (const uint32_t*)sums);
+        CPM_LAUNCH_CHECK();
+        return;
+    }
     uint32_t* ssc = scratch + nt;
     hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)nt), dim3(SCAN_NT), 0, st, in, out, n, sums);
     CPM_LAUNCH_CHECK();
